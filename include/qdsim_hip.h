@@ -1,0 +1,146 @@
+/*
+ * qdsim_hip -- C-ABI of the MI355X (gfx950) FEM-Hamiltonian path.
+ *
+ * Conventions
+ *   - Every call returns 0 on success or a negative code (QDS_E*); the message of
+ *     the last failure on the calling thread is qds_last_error().
+ *   - Device pointers are plain HBM addresses (e.g. torch tensor data_ptr()); host
+ *     pointers are marked "host".  `stream` is a hipStream_t passed as void*.
+ *   - Calls never allocate device memory.  Entry points that need scratch take a
+ *     `workspace` pointer and size; the matching *_workspace() call gives the size.
+ *   - dtype codes: 1 = FP32, 2 = FP64, 3 = complex64, 4 = complex128
+ *     (complex values interleaved re, im).
+ *   - Blocks are row-major n x b arrays with a leading dimension (elements).
+ *
+ * The reference binds its FEM path through pybind11 (backend/src/bindings.cpp:948-985)
+ * and scipy; the Cython solvers call scipy.sparse / scipy.sparse.linalg directly.
+ * Each entry point below names the reference operation it replaces.
+ */
+#ifndef QDSIM_HIP_H
+#define QDSIM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QDS_OK 0
+#define QDS_EINVAL (-1)
+#define QDS_EHIP (-2)
+#define QDS_ENOMEM (-3)
+#define QDS_EWORKSPACE (-4)
+
+/* ---- library --------------------------------------------------------------- */
+const char* qds_last_error(void);
+int qds_version(void);
+int qds_device_info(char* arch /* host */, int32_t arch_len, int32_t* cu_count /* host */);
+
+/* ---- CSR pattern and scatter maps (host) -------------------------------------
+ * Replaces the COO->CSR conversion of the element triplets:
+ *   qdsim_cython/solvers/fem_solver.pyx:142-145 (sp.csr_matrix((data,(row,col))))
+ *   qdsim_cython/solvers/fixed_open_system_solver.pyx:259-267
+ *   backend/src/fem.cpp:148-150 (setFromTriplets)
+ * conn: host int32 [n_elem][npe].  seg_src entries index packed symmetric element
+ * matrices: e*nsym + s(min(i,j), max(i,j)), s(i,j) = i*npe - i*(i-1)/2 + (j-i).  */
+int qds_pattern_count(const int32_t* conn, int64_t n_elem, int32_t npe, int64_t n_dof,
+                      int64_t* nnz_out);
+int qds_pattern_fill(const int32_t* conn, int64_t n_elem, int32_t npe, int64_t n_dof, int64_t nnz,
+                     int32_t* indptr, int32_t* indices, int32_t* seg_ptr, int32_t* seg_src,
+                     int32_t* elem_map /* optional [n_elem*npe*npe] */);
+/* Submatrix on renumbered DOFs (new_of_old[old] = new id or -1).  Replaces the
+ * Dirichlet row replacement via LIL of schrodinger_solver.pyx:253-283 by a
+ * restriction to the free DOFs.  All arrays host.                               */
+int qds_pattern_restrict_count(const int32_t* indptr, const int32_t* indices, int64_t n_dof,
+                               const int32_t* new_of_old, int64_t n_new, int64_t* nnz_out);
+int qds_pattern_restrict_fill(const int32_t* indptr, const int32_t* indices, int64_t n_dof,
+                              const int32_t* new_of_old, int64_t n_new, int64_t nnz_r,
+                              int32_t* r_indptr, int32_t* r_indices, int32_t* r_src);
+
+/* ---- assembly (device) ---------------------------------------------------------
+ * Element matrices: replaces FEMSolver::assemble_element_matrix
+ * (backend/src/fem.cpp:736-989) and the P1 element formulas of
+ * fem_solver.pyx:277-320 / schrodinger_solver.pyx:201-251.
+ * tables = [w(nq) | phi(nq*npe) | dphi_dxi(nq*npe) | dphi_deta(nq*npe)].
+ * Coefficient c/v/r/w: pointer, element stride, quadrature stride (0: per element).
+ * out0 = K(c) + M(v), out1 = M(r) (r NULL: 1), out2 = M(w); each [n_elem*nsym].  */
+int qds_assemble_elements(int32_t npe, int32_t nq, int64_t n_elem, const int32_t* conn,
+                          const double* x, const double* y, const double* tables,
+                          const double* c_q, int64_t c_se, int32_t c_sq,
+                          const double* v_q, int64_t v_se, int32_t v_sq,
+                          const double* r_q, int64_t r_se, int32_t r_sq,
+                          const double* w_q, int64_t w_se, int32_t w_sq,
+                          double* out0, double* out1, double* out2, void* stream);
+/* CSR values = segmented sums of packed element entries (duplicate summation of
+ * the COO->CSR conversion, fem_solver.pyx:142-145), deterministic order.          */
+int qds_assemble_gather(int64_t nnz, const int32_t* seg_ptr, const int32_t* seg_src,
+                        const double* e0, const double* e1, const double* e2,
+                        double* v0, double* v1, double* v2, void* stream);
+/* Physical quadrature points (coefficient sampling; replaces the per-element
+ * centroid callbacks of fem_solver.pyx:126-128 and per-point calls of
+ * fem.cpp:926-943).  bary: [nq][3].  xq, yq: [n_elem][nq].                        */
+int qds_quad_points(int32_t npe, int32_t nq, int64_t n_elem, const int32_t* conn, const double* x,
+                    const double* y, const double* bary, double* xq, double* yq, void* stream);
+/* out[k] = re[src[k]] * scale (real dtypes) or re + i*scale*im (complex dtypes).  */
+int qds_csr_take(int32_t dtype, int64_t n, const int32_t* src, const double* re, const double* im,
+                 double scale, void* out, void* stream);
+
+/* ---- sparse products (device) --------------------------------------------------
+ * Y_a = A_a X (and Y_b = A_b X on the same pattern).  Replaces scipy csr @ x inside
+ * ARPACK (eigen_solver.pyx:225-235) and the unused fusedSpMV
+ * (backend/src/fused_gpu_kernels.cu:97-125).  Complex products do not conjugate.  */
+int qds_spmm(int32_t dtype, int64_t n_rows, int64_t nnz, int32_t b, const int32_t* indptr,
+             const int32_t* indices, const void* vals_a, const void* vals_b, const void* X,
+             int64_t ldx, void* Y_a, int64_t ldy_a, void* Y_b, int64_t ldy_b, void* stream);
+
+/* ---- dense block kernels for the block eigensolver (device) --------------------
+ * Replace the dense Gram / projection steps inside scipy's lobpcg
+ * (eigen_solver.pyx:254-266) and ARPACK's orthogonalisation.                      */
+int64_t qds_gram_workspace(int64_t n, int32_t m, int32_t k1, int32_t k2);
+int qds_gram(int32_t dtype, int64_t n, int32_t m, const void* A, int64_t lda, int32_t k1,
+             const void* B1, int64_t ldb1, int32_t k2, const void* B2, int64_t ldb2, int32_t sym,
+             double* C, void* workspace, int64_t ws_bytes, void* stream);
+int qds_block_update(int32_t dtype, int64_t n, int32_t m, int32_t k, const void* S, int64_t lds,
+                     const void* C1, void* Y1, int64_t ldy1, const void* C2, void* Y2, int64_t ldy2,
+                     void* stream);
+int qds_block_residual(int32_t dtype, int64_t n, int32_t b, const void* HX, int64_t ldh,
+                       const void* MX, int64_t ldm, const double* lam, void* R, int64_t ldr,
+                       void* stream);
+int64_t qds_coldot_workspace(int32_t b);
+int qds_coldot(int32_t dtype, int64_t n, int32_t b, const void* X, int64_t ldx, const void* Y,
+               int64_t ldy, int32_t conj, double* out, void* workspace, int64_t ws_bytes,
+               void* stream);
+
+/* ---- lattice multigrid preconditioner (device) ---------------------------------
+ * No reference counterpart: replaces the sparse LU of ARPACK shift-invert
+ * (qdsim_cython/advanced_eigenvalue_solvers.py:106-122) as the spectral
+ * accelerator of the eigensolver.                                                  */
+typedef struct {
+  int64_t nx, ny;
+  const void* d;
+  const void* we;
+  const void* wn;
+  void* u;
+  void* f;
+  void* t;
+  int64_t ldu, ldf; /* per-point strides of u and f (0 means b) */
+} qds_mg_level;
+int qds_mg_vcycle(int32_t dtype, int32_t n_levels, const qds_mg_level* levels, int32_t b,
+                  int32_t nu_pre, int32_t nu_post, int32_t coarse_sweeps, double omega,
+                  void* stream);
+int qds_mg_apply(int32_t dtype, const qds_mg_level* level, int32_t b, const void* u, void* out,
+                 void* stream);
+
+/* ---- Poisson CG (device) -------------------------------------------------------
+ * Replaces spla.spsolve of the Dirichlet-modified Poisson system
+ * (qdsim_cython/solvers/poisson_solver.pyx:241).                                   */
+int64_t qds_cg_workspace(int64_t n);
+int qds_cg(int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* indices, const double* vals,
+           const double* b, double* x, double rtol, int32_t max_iter, int32_t check_every,
+           void* workspace, int64_t ws_bytes, int32_t* iters_out /* host */,
+           double* relres_out /* host */, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QDSIM_HIP_H */

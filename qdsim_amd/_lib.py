@@ -1,0 +1,95 @@
+"""ctypes binding of libqdsim_hip.so (the C-ABI declared in include/qdsim_hip.h).
+
+The library is required: importing a solver without it raises immediately.  There
+is no CPU fallback anywhere in the package.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libqdsim_hip.so")
+
+F32, F64, C64, C128 = 1, 2, 3, 4
+
+_c = ctypes
+_p = _c.c_void_p
+_i32 = _c.c_int32
+_i64 = _c.c_int64
+_d = _c.c_double
+
+_SIGNATURES = {
+    "qds_last_error": ([], _c.c_char_p),
+    "qds_version": ([], _c.c_int),
+    "qds_device_info": ([_c.c_char_p, _i32, _c.POINTER(_i32)], _c.c_int),
+    "qds_pattern_count": ([_p, _i64, _i32, _i64, _c.POINTER(_i64)], _c.c_int),
+    "qds_pattern_fill": ([_p, _i64, _i32, _i64, _i64, _p, _p, _p, _p, _p], _c.c_int),
+    "qds_pattern_restrict_count": ([_p, _p, _i64, _p, _i64, _c.POINTER(_i64)], _c.c_int),
+    "qds_pattern_restrict_fill": ([_p, _p, _i64, _p, _i64, _i64, _p, _p, _p], _c.c_int),
+    "qds_assemble_elements": ([_i32, _i32, _i64, _p, _p, _p, _p,
+                               _p, _i64, _i32, _p, _i64, _i32, _p, _i64, _i32, _p, _i64, _i32,
+                               _p, _p, _p, _p], _c.c_int),
+    "qds_assemble_gather": ([_i64, _p, _p, _p, _p, _p, _p, _p, _p, _p], _c.c_int),
+    "qds_quad_points": ([_i32, _i32, _i64, _p, _p, _p, _p, _p, _p, _p], _c.c_int),
+    "qds_csr_take": ([_i32, _i64, _p, _p, _p, _d, _p, _p], _c.c_int),
+    "qds_spmm": ([_i32, _i64, _i64, _i32, _p, _p, _p, _p, _p, _i64, _p, _i64, _p, _i64, _p], _c.c_int),
+    "qds_gram_workspace": ([_i64, _i32, _i32, _i32], _i64),
+    "qds_gram": ([_i32, _i64, _i32, _p, _i64, _i32, _p, _i64, _i32, _p, _i64, _i32, _p, _p, _i64, _p],
+                 _c.c_int),
+    "qds_block_update": ([_i32, _i64, _i32, _i32, _p, _i64, _p, _p, _i64, _p, _p, _i64, _p], _c.c_int),
+    "qds_block_residual": ([_i32, _i64, _i32, _p, _i64, _p, _i64, _p, _p, _i64, _p], _c.c_int),
+    "qds_coldot_workspace": ([_i32], _i64),
+    "qds_coldot": ([_i32, _i64, _i32, _p, _i64, _p, _i64, _i32, _p, _p, _i64, _p], _c.c_int),
+    "qds_mg_vcycle": ([_i32, _i32, _p, _i32, _i32, _i32, _i32, _d, _p], _c.c_int),
+    "qds_mg_apply": ([_i32, _p, _i32, _p, _p, _p], _c.c_int),
+    "qds_cg_workspace": ([_i64], _i64),
+    "qds_cg": ([_i64, _i64, _p, _p, _p, _p, _p, _d, _i32, _i32, _p, _i64, _c.POINTER(_i32),
+                _c.POINTER(_d), _p], _c.c_int),
+}
+
+
+class MGLevel(ctypes.Structure):
+    """Mirror of qds_mg_level."""
+    _fields_ = [("nx", _i64), ("ny", _i64), ("d", _p), ("we", _p), ("wn", _p),
+                ("u", _p), ("f", _p), ("t", _p), ("ldu", _i64), ("ldf", _i64)]
+
+
+class QDSError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"qdsim_amd: native library {LIB_PATH} is missing; build it with "
+            "`python -m qdsim_amd._build` (hipcc --offload-arch=gfx950)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (args, res) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    return lib
+
+
+lib = _load()
+
+
+def exported_symbols():
+    return sorted(_SIGNATURES)
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib.qds_last_error().decode(errors="replace")
+        raise QDSError(f"{what or 'qdsim_hip'} failed (rc={rc}): {msg}")
+
+
+def host_ptr(a):
+    """Pointer to a C-contiguous numpy array."""
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)

@@ -1,0 +1,516 @@
+// Tall-skinny dense block kernels for the block eigensolver (gfx950 MFMA).
+//
+// * qds_gram: C = A^T [B1 | B2] in FP64 with v_mfma_f64_16x16x4f64.  A and B are
+//   n x m row-major blocks stored in FP32 or FP64; rows are staged through LDS as
+//   FP64, so the Rayleigh-Ritz matrices are accumulated in FP64 whatever the
+//   storage precision.  Every workgroup reduces a contiguous slab of rows into a
+//   partial m x k tile set; a second kernel sums the partials in a fixed order
+//   (deterministic, no atomics).  With sym, only tiles (ti <= tj) of each half are
+//   computed and the rest are mirrored.
+// * qds_block_update: Y1 = S C1 (and Y2 = S C2) with S n x m, C m x k, on the
+//   16x16x4 MFMA of the storage type.  Each lane loads a 16-byte vector of one row
+//   of S and the contraction index is permuted accordingly (the same permutation
+//   is applied to the rows of C staged in LDS).
+// * qds_block_residual, qds_coldot: R = HX - MX diag(lambda); column dot products.
+#include "qds_common.h"
+
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kRB = 16;       // rows per LDS chunk in the Gram kernel
+constexpr int kMaxTiles = 24; // per wave
+
+__device__ __forceinline__ v4d mfma_f64(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f mfma_f32(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+struct GramShape {
+  int m, k1, k2;
+  int tm, tk1, tk2;  // tiles
+  int mpad, kpad;
+  int sym;
+};
+
+__device__ __forceinline__ bool tile_needed(const GramShape& g, int ti, int tj) {
+  if (!g.sym) return true;
+  const int tl = tj < g.tk1 ? tj : tj - g.tk1;
+  return ti <= tl;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, const T* __restrict__ A, int64_t lda,
+             const T* __restrict__ B1, int64_t ldb1, const T* __restrict__ B2, int64_t ldb2,
+             double* __restrict__ partial) {
+  extern __shared__ double smem[];
+  double* sA = smem;                 // kRB x mpad
+  double* sB = smem + kRB * g.mpad;  // kRB x kpad
+  __shared__ int s_tile[kWaves * kMaxTiles];
+  __shared__ int s_count[kWaves];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int tk = g.tk1 + g.tk2;
+  const int ntiles = g.tm * tk;
+  if (lane == 0) {
+    // tiles are dealt round-robin over the waves; slot lists keep the
+    // accumulator index a compile-time constant in the MFMA loop
+    int c = 0;
+    for (int t = wave; t < ntiles; t += kWaves) {
+      const int ti = t / tk, tj = t - ti * tk;
+      if (!tile_needed(g, ti, tj)) continue;
+      if (c < kMaxTiles) s_tile[wave * kMaxTiles + c] = (ti << 8) | tj;
+      ++c;
+    }
+    s_count[wave] = c;
+  }
+  __syncthreads();
+  const int nslots = s_count[wave];
+
+  v4d acc[kMaxTiles];
+#pragma unroll
+  for (int t = 0; t < kMaxTiles; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  const int64_t r_begin = (int64_t)blockIdx.x * rows_per_blk;
+  const int64_t r_end = min(n, r_begin + rows_per_blk);
+  const int k1pad = 16 * g.tk1;
+  for (int64_t r0 = r_begin; r0 < r_end; r0 += kRB) {
+    for (int t = threadIdx.x; t < kRB * g.mpad; t += kBlock) {
+      const int rr = t / g.mpad, c = t - rr * g.mpad;
+      const int64_t row = r0 + rr;
+      sA[t] = (row < r_end && c < g.m) ? (double)A[row * lda + c] : 0.0;
+    }
+    for (int t = threadIdx.x; t < kRB * g.kpad; t += kBlock) {
+      const int rr = t / g.kpad, c = t - rr * g.kpad;
+      const int64_t row = r0 + rr;
+      double v = 0.0;
+      if (row < r_end) {
+        if (c < k1pad) {
+          if (c < g.k1) v = (double)B1[row * ldb1 + c];
+        } else {
+          const int c2 = c - k1pad;
+          if (c2 < g.k2) v = (double)B2[row * ldb2 + c2];
+        }
+      }
+      sB[t] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kRB / 4; ++kk) {
+      const int row = kk * 4 + (lane >> 4);
+#pragma unroll
+      for (int slot = 0; slot < kMaxTiles; ++slot) {
+        if (slot < nslots) {
+          const int code = s_tile[wave * kMaxTiles + slot];
+          const int ti = code >> 8, tj = code & 255;
+          const double a = sA[row * g.mpad + ti * 16 + (lane & 15)];
+          const double b = sB[row * g.kpad + tj * 16 + (lane & 15)];
+          acc[slot] = mfma_f64(a, b, acc[slot]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  double* P = partial + (int64_t)blockIdx.x * g.mpad * g.kpad;
+#pragma unroll
+  for (int slot = 0; slot < kMaxTiles; ++slot) {
+    if (slot < nslots) {
+      const int code = s_tile[wave * kMaxTiles + slot];
+      const int ti = code >> 8, tj = code & 255;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = ti * 16 + 4 * (lane >> 4) + r;
+        const int j = tj * 16 + (lane & 15);
+        P[i * g.kpad + j] = acc[slot][r];
+      }
+    }
+  }
+}
+
+__global__ void gram_reduce(int nblk, GramShape g, const double* __restrict__ partial,
+                            double* __restrict__ C) {
+  const int ktot = g.k1 + g.k2;
+  const int k1pad = 16 * g.tk1;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < g.m * ktot; t += gridDim.x * blockDim.x) {
+    const int i = t / ktot, j = t - i * ktot;
+    int jp = j < g.k1 ? j : k1pad + (j - g.k1);
+    int ip = i;
+    const int half_off = j < g.k1 ? 0 : k1pad;
+    if (g.sym) {
+      const int jl = jp - half_off;
+      if ((ip >> 4) > (jl >> 4)) {  // mirrored tile
+        const int tmp = ip;
+        ip = jl;
+        jp = tmp + half_off;
+      }
+    }
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * g.mpad * g.kpad + ip * g.kpad + jp];
+    C[t] = s;
+  }
+}
+
+// ---------------------------------------------------------------- block update
+template <typename T> struct MT;
+template <> struct MT<float> {
+  static constexpr int VEC = 4;
+  typedef float4 vec_t;
+  typedef v4f acc_t;
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) { return mfma_f32(a, b, c); }
+};
+template <> struct MT<double> {
+  static constexpr int VEC = 2;
+  typedef double2 vec_t;
+  typedef v4d acc_t;
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) { return mfma_f64(a, b, c); }
+};
+
+constexpr int kMaxOutTiles = 4;  // k <= 64
+
+template <typename T, int NOUT>
+__global__ void __launch_bounds__(kBlock)
+block_update_kernel(int64_t n, int m, int k, const T* __restrict__ S, int64_t lds,
+                    const T* __restrict__ C1, const T* __restrict__ C2, T* __restrict__ Y1,
+                    int64_t ldy1, T* __restrict__ Y2, int64_t ldy2) {
+  using M = MT<T>;
+  constexpr int VEC = M::VEC;
+  constexpr int KB = 4 * VEC;  // contraction columns covered by one vector load per lane
+  extern __shared__ unsigned char smem_raw[];
+  T* sC = reinterpret_cast<T*>(smem_raw);
+  const int nkb = (m + KB - 1) / KB;
+  const int mp = nkb * KB;
+  const int kt = (k + 15) / 16;
+  const int kp = kt * 16;
+  for (int t = threadIdx.x; t < NOUT * mp * kp; t += kBlock) {
+    const int o = t / (mp * kp);
+    const int rem = t - o * mp * kp;
+    const int i = rem / kp, j = rem - i * kp;
+    const T* C = o == 0 ? C1 : C2;
+    sC[t] = (i < m && j < k) ? C[i * k + j] : T(0);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t tiles = (n + 15) / 16;
+  const int q = lane >> 4;      // 0..3
+  const int rl = lane & 15;
+  for (int64_t tile = (int64_t)blockIdx.x * kWaves + wave; tile < tiles;
+       tile += (int64_t)gridDim.x * kWaves) {
+    const int64_t r0 = tile * 16;
+    const int64_t row = r0 + rl;
+    typename M::acc_t acc[NOUT][kMaxOutTiles];
+#pragma unroll
+    for (int o = 0; o < NOUT; ++o)
+#pragma unroll
+      for (int t = 0; t < kMaxOutTiles; ++t) acc[o][t] = typename M::acc_t{0, 0, 0, 0};
+    for (int cb = 0; cb < nkb; ++cb) {
+      const int kbase = cb * KB + q * VEC;
+      T sv[VEC];
+      if (row < n && kbase < m) {
+        typename M::vec_t v = *reinterpret_cast<const typename M::vec_t*>(S + row * lds + kbase);
+        const T* vp = reinterpret_cast<const T*>(&v);
+#pragma unroll
+        for (int s = 0; s < VEC; ++s) sv[s] = vp[s];
+      } else {
+#pragma unroll
+        for (int s = 0; s < VEC; ++s) sv[s] = T(0);
+      }
+#pragma unroll
+      for (int s = 0; s < VEC; ++s) {
+        const int kidx = kbase + s;
+#pragma unroll
+        for (int t = 0; t < kMaxOutTiles; ++t) {
+          if (t < kt) {
+#pragma unroll
+            for (int o = 0; o < NOUT; ++o) {
+              const T bv = sC[o * mp * kp + kidx * kp + t * 16 + rl];
+              acc[o][t] = M::mma(sv[s], bv, acc[o][t]);
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < kMaxOutTiles; ++t) {
+      if (t >= kt) continue;
+      const int col = t * 16 + rl;
+      if (col >= k) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t orow = r0 + 4 * q + r;
+        if (orow < n) {
+          Y1[orow * ldy1 + col] = acc[0][t][r];
+          if (NOUT == 2) Y2[orow * ldy2 + col] = acc[NOUT - 1][t][r];
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- residual / dots
+template <typename T>
+__global__ void residual_kernel(int64_t n, int b, const T* __restrict__ HX, int64_t ldh,
+                                const T* __restrict__ MX, int64_t ldm, const double* __restrict__ lam,
+                                T* __restrict__ R, int64_t ldr) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * b;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = t / b;
+    const int c = (int)(t - r * b);
+    R[r * ldr + c] = HX[r * ldh + c] - (T)lam[c] * MX[r * ldm + c];
+  }
+}
+
+// complex residual with complex lambda (interleaved); bilinear (no conjugation)
+template <typename T>
+__global__ void residual_cplx_kernel(int64_t n, int b, const T* __restrict__ HX, int64_t ldh,
+                                     const T* __restrict__ MX, int64_t ldm,
+                                     const double* __restrict__ lam, T* __restrict__ R, int64_t ldr) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * b;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = t / b;
+    const int c = (int)(t - r * b);
+    const T lr = (T)lam[2 * c], li = (T)lam[2 * c + 1];
+    const T mr = MX[2 * (r * ldm + c)], mi = MX[2 * (r * ldm + c) + 1];
+    R[2 * (r * ldr + c)] = HX[2 * (r * ldh + c)] - (lr * mr - li * mi);
+    R[2 * (r * ldr + c) + 1] = HX[2 * (r * ldh + c) + 1] - (lr * mi + li * mr);
+  }
+}
+
+// partial column dots: mode 0 real x.y ; mode 1 complex conj(x).y (re part + im part);
+// mode 2 complex bilinear x.y
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kBlock)
+coldot_partial(int64_t n, int b, const T* __restrict__ X, int64_t ldx, const T* __restrict__ Y,
+               int64_t ldy, int64_t rows_per_blk, double* __restrict__ partial) {
+  // thread layout: threads cover columns (b <= 256 / groups), rows strided
+  const int cols = MODE == 0 ? b : b;
+  const int groups = kBlock / cols;
+  const int tc = threadIdx.x % cols;
+  const int tg = threadIdx.x / cols;
+  __shared__ double red[2 * kBlock];
+  double s0 = 0.0, s1 = 0.0;
+  const int64_t r_begin = (int64_t)blockIdx.x * rows_per_blk;
+  const int64_t r_end = min(n, r_begin + rows_per_blk);
+  if (tg < groups) {
+    for (int64_t r = r_begin + tg; r < r_end; r += groups) {
+      if (MODE == 0) {
+        s0 += (double)X[r * ldx + tc] * (double)Y[r * ldy + tc];
+      } else {
+        const double xr = X[2 * (r * ldx + tc)], xi = X[2 * (r * ldx + tc) + 1];
+        const double yr = Y[2 * (r * ldy + tc)], yi = Y[2 * (r * ldy + tc) + 1];
+        if (MODE == 1) { s0 += xr * yr + xi * yi; s1 += xr * yi - xi * yr; }
+        else { s0 += xr * yr - xi * yi; s1 += xr * yi + xi * yr; }
+      }
+    }
+  }
+  red[threadIdx.x] = s0;
+  red[kBlock + threadIdx.x] = s1;
+  __syncthreads();
+  if (threadIdx.x < cols) {
+    double a0 = 0.0, a1 = 0.0;
+    for (int g = 0; g < groups; ++g) {
+      a0 += red[g * cols + threadIdx.x];
+      a1 += red[kBlock + g * cols + threadIdx.x];
+    }
+    partial[(int64_t)blockIdx.x * 2 * b + threadIdx.x] = a0;
+    partial[(int64_t)blockIdx.x * 2 * b + b + threadIdx.x] = a1;
+  }
+}
+
+__global__ void coldot_reduce(int nblk, int b, const double* __restrict__ partial, double* __restrict__ out,
+                              int cplx) {
+  const int t = threadIdx.x;
+  if (t >= b) return;
+  double a0 = 0.0, a1 = 0.0;
+  for (int k = 0; k < nblk; ++k) {
+    a0 += partial[(int64_t)k * 2 * b + t];
+    a1 += partial[(int64_t)k * 2 * b + b + t];
+  }
+  if (cplx) { out[2 * t] = a0; out[2 * t + 1] = a1; }
+  else out[t] = a0;
+}
+
+int gram_blocks(int64_t n) {
+  int64_t chunks = (n + kRB - 1) / kRB;
+  int64_t g = chunks < 512 ? chunks : 512;
+  return (int)(g < 1 ? 1 : g);
+}
+
+GramShape make_shape(int m, int k1, int k2, int sym) {
+  GramShape g;
+  g.m = m; g.k1 = k1; g.k2 = k2;
+  g.tm = (m + 15) / 16; g.tk1 = (k1 + 15) / 16; g.tk2 = (k2 + 15) / 16;
+  g.mpad = 16 * g.tm; g.kpad = 16 * (g.tk1 + g.tk2);
+  g.sym = sym;
+  return g;
+}
+
+constexpr int kColdotBlocks = 512;
+
+}  // namespace
+
+QDS_API int64_t qds_gram_workspace(int64_t n, int32_t m, int32_t k1, int32_t k2) {
+  GramShape g = make_shape(m, k1, k2, 0);
+  return (int64_t)gram_blocks(n) * g.mpad * g.kpad * (int64_t)sizeof(double);
+}
+
+// C (m x (k1+k2), row-major FP64) = A^T [B1 | B2].  B2 may be NULL (k2 = 0).
+// sym = 1 asserts each half is symmetric (m == k1 == k2 layout) and computes only
+// its upper tiles.  dtype F32 or F64 gives the storage type of A and B.
+QDS_API int qds_gram(int32_t dtype, int64_t n, int32_t m, const void* A, int64_t lda, int32_t k1,
+                     const void* B1, int64_t ldb1, int32_t k2, const void* B2, int64_t ldb2,
+                     int32_t sym, double* C, void* workspace, int64_t ws_bytes, void* stream) {
+  QDS_CHECK_ARG(A && B1 && C && workspace, "qds_gram: null pointer");
+  QDS_CHECK_ARG(m > 0 && k1 > 0 && k2 >= 0 && (k2 == 0 || B2), "qds_gram: bad shape");
+  QDS_CHECK_ARG(!sym || (m == k1 && (k2 == 0 || k2 == m)), "qds_gram: sym needs square halves");
+  GramShape g = make_shape(m, k1, k2, sym);
+  int ntiles = 0;
+  for (int ti = 0; ti < g.tm; ++ti)
+    for (int tj = 0; tj < g.tk1 + g.tk2; ++tj) {
+      const int tl = tj < g.tk1 ? tj : tj - g.tk1;
+      if (!sym || ti <= tl) ++ntiles;
+    }
+  // per-wave slot count: tiles are dealt round-robin over the 4 waves of a block
+  int max_slots = 0;
+  for (int w = 0; w < kWaves; ++w) {
+    int s = 0, tk = g.tk1 + g.tk2;
+    for (int t = w; t < g.tm * tk; t += kWaves) {
+      const int ti = t / tk, tj = t - ti * tk;
+      const int tl = tj < g.tk1 ? tj : tj - g.tk1;
+      if (!sym || ti <= tl) ++s;
+    }
+    max_slots = s > max_slots ? s : max_slots;
+  }
+  QDS_CHECK_ARG(max_slots <= kMaxTiles, "qds_gram: %d tiles per wave exceed %d (m=%d k=%d)",
+                max_slots, kMaxTiles, m, k1 + k2);
+  const int nblk = gram_blocks(n);
+  QDS_CHECK_ARG(ws_bytes >= qds_gram_workspace(n, m, k1, k2), "qds_gram: workspace too small");
+  const int64_t rows_per_blk = (((n + nblk - 1) / nblk) + kRB - 1) / kRB * kRB;
+  const size_t shm = (size_t)kRB * (g.mpad + g.kpad) * sizeof(double);
+  QDS_CHECK_ARG(shm <= 64 * 1024, "qds_gram: block too wide for LDS staging");
+  hipStream_t st = qds_stream(stream);
+  double* partial = static_cast<double*>(workspace);
+  if (dtype == qds::F32)
+    hipLaunchKernelGGL(gram_partial<float>, dim3(nblk), dim3(kBlock), shm, st, n, rows_per_blk, g,
+                       (const float*)A, lda, (const float*)B1, ldb1, (const float*)B2, ldb2, partial);
+  else if (dtype == qds::F64)
+    hipLaunchKernelGGL(gram_partial<double>, dim3(nblk), dim3(kBlock), shm, st, n, rows_per_blk, g,
+                       (const double*)A, lda, (const double*)B1, ldb1, (const double*)B2, ldb2, partial);
+  else
+    QDS_CHECK_ARG(false, "qds_gram: dtype must be F32 or F64");
+  QDS_LAUNCH_CHECK();
+  const int tot = m * (k1 + k2);
+  hipLaunchKernelGGL(gram_reduce, dim3(qds_blocks(tot, 256, 1024)), dim3(256), 0, st, nblk, g,
+                     partial, C);
+  QDS_LAUNCH_CHECK();
+  return QDS_OK;
+}
+
+// Y1 = S C1 (n x k), and Y2 = S C2 when C2 != NULL.  C1, C2: m x k row-major in the
+// storage type.  dtype F32 (m % 4 == 0) or F64 (m % 2 == 0); k <= 64.
+QDS_API int qds_block_update(int32_t dtype, int64_t n, int32_t m, int32_t k, const void* S, int64_t lds,
+                             const void* C1, void* Y1, int64_t ldy1, const void* C2, void* Y2,
+                             int64_t ldy2, void* stream) {
+  QDS_CHECK_ARG(S && C1 && Y1 && (!C2 || Y2), "qds_block_update: null pointer");
+  QDS_CHECK_ARG(k >= 1 && k <= 16 * kMaxOutTiles && m >= 1, "qds_block_update: bad shape");
+  if (n == 0) return QDS_OK;
+  hipStream_t st = qds_stream(stream);
+  const int64_t tiles = (n + 15) / 16;
+  const unsigned blocks = qds_blocks(tiles, kWaves, 8192);
+  const int kp = (k + 15) / 16 * 16;
+  const int nout = C2 ? 2 : 1;
+  if (dtype == qds::F32) {
+    QDS_CHECK_ARG(m % 4 == 0 && lds % 4 == 0 && (reinterpret_cast<uintptr_t>(S) % 16) == 0,
+                  "qds_block_update: F32 needs m, lds multiple of 4 and 16-byte aligned S");
+    const int mp = (m + 15) / 16 * 16;
+    const size_t shm = (size_t)nout * mp * kp * sizeof(float);
+    QDS_CHECK_ARG(shm <= 64 * 1024, "qds_block_update: C too large");
+    if (C2)
+      hipLaunchKernelGGL((block_update_kernel<float, 2>), dim3(blocks), dim3(kBlock), shm, st, n, m, k,
+                         (const float*)S, lds, (const float*)C1, (const float*)C2, (float*)Y1, ldy1,
+                         (float*)Y2, ldy2);
+    else
+      hipLaunchKernelGGL((block_update_kernel<float, 1>), dim3(blocks), dim3(kBlock), shm, st, n, m, k,
+                         (const float*)S, lds, (const float*)C1, (const float*)nullptr, (float*)Y1,
+                         ldy1, (float*)nullptr, ldy2);
+  } else if (dtype == qds::F64) {
+    QDS_CHECK_ARG(m % 2 == 0 && lds % 2 == 0 && (reinterpret_cast<uintptr_t>(S) % 16) == 0,
+                  "qds_block_update: F64 needs m, lds even and 16-byte aligned S");
+    const int mp = (m + 7) / 8 * 8;
+    const size_t shm = (size_t)nout * mp * kp * sizeof(double);
+    QDS_CHECK_ARG(shm <= 64 * 1024, "qds_block_update: C too large");
+    if (C2)
+      hipLaunchKernelGGL((block_update_kernel<double, 2>), dim3(blocks), dim3(kBlock), shm, st, n, m, k,
+                         (const double*)S, lds, (const double*)C1, (const double*)C2, (double*)Y1,
+                         ldy1, (double*)Y2, ldy2);
+    else
+      hipLaunchKernelGGL((block_update_kernel<double, 1>), dim3(blocks), dim3(kBlock), shm, st, n, m, k,
+                         (const double*)S, lds, (const double*)C1, (const double*)nullptr,
+                         (double*)Y1, ldy1, (double*)nullptr, ldy2);
+  } else {
+    QDS_CHECK_ARG(false, "qds_block_update: dtype must be F32 or F64");
+  }
+  QDS_LAUNCH_CHECK();
+  return QDS_OK;
+}
+
+// R = HX - MX diag(lambda).  Real dtypes: lambda has b doubles.  Complex dtypes:
+// lambda has b interleaved complex doubles, products without conjugation.
+QDS_API int qds_block_residual(int32_t dtype, int64_t n, int32_t b, const void* HX, int64_t ldh,
+                               const void* MX, int64_t ldm, const double* lam, void* R, int64_t ldr,
+                               void* stream) {
+  QDS_CHECK_ARG(HX && MX && lam && R && b > 0, "qds_block_residual: bad arguments");
+  if (n == 0) return QDS_OK;
+  hipStream_t st = qds_stream(stream);
+  const dim3 g(qds_blocks(n * b, 256, 131072)), blk(256);
+  switch (dtype) {
+    case qds::F32: hipLaunchKernelGGL(residual_kernel<float>, g, blk, 0, st, n, b, (const float*)HX, ldh, (const float*)MX, ldm, lam, (float*)R, ldr); break;
+    case qds::F64: hipLaunchKernelGGL(residual_kernel<double>, g, blk, 0, st, n, b, (const double*)HX, ldh, (const double*)MX, ldm, lam, (double*)R, ldr); break;
+    case qds::C64: hipLaunchKernelGGL(residual_cplx_kernel<float>, g, blk, 0, st, n, b, (const float*)HX, ldh, (const float*)MX, ldm, lam, (float*)R, ldr); break;
+    case qds::C128: hipLaunchKernelGGL(residual_cplx_kernel<double>, g, blk, 0, st, n, b, (const double*)HX, ldh, (const double*)MX, ldm, lam, (double*)R, ldr); break;
+    default: QDS_CHECK_ARG(false, "qds_block_residual: bad dtype");
+  }
+  QDS_LAUNCH_CHECK();
+  return QDS_OK;
+}
+
+QDS_API int64_t qds_coldot_workspace(int32_t b) { return (int64_t)kColdotBlocks * 2 * b * sizeof(double); }
+
+// out[j] = sum_r X[r, j] * Y[r, j] in FP64.  Real dtypes: out has b doubles.
+// Complex dtypes: out has b complex doubles; conj = 1 conjugates X.
+QDS_API int qds_coldot(int32_t dtype, int64_t n, int32_t b, const void* X, int64_t ldx, const void* Y,
+                       int64_t ldy, int32_t conj, double* out, void* workspace, int64_t ws_bytes,
+                       void* stream) {
+  QDS_CHECK_ARG(X && Y && out && workspace && b > 0 && b <= 256, "qds_coldot: bad arguments");
+  QDS_CHECK_ARG(ws_bytes >= qds_coldot_workspace(b), "qds_coldot: workspace too small");
+  hipStream_t st = qds_stream(stream);
+  int64_t nblk = (n + 255) / 256;
+  if (nblk > kColdotBlocks) nblk = kColdotBlocks;
+  if (nblk < 1) nblk = 1;
+  const int64_t rpb = (n + nblk - 1) / nblk;
+  double* part = static_cast<double*>(workspace);
+  const bool cplx = dtype == qds::C64 || dtype == qds::C128;
+  switch (dtype) {
+    case qds::F32: hipLaunchKernelGGL((coldot_partial<float, 0>), dim3(nblk), dim3(kBlock), 0, st, n, b, (const float*)X, ldx, (const float*)Y, ldy, rpb, part); break;
+    case qds::F64: hipLaunchKernelGGL((coldot_partial<double, 0>), dim3(nblk), dim3(kBlock), 0, st, n, b, (const double*)X, ldx, (const double*)Y, ldy, rpb, part); break;
+    case qds::C64:
+      if (conj) hipLaunchKernelGGL((coldot_partial<float, 1>), dim3(nblk), dim3(kBlock), 0, st, n, b, (const float*)X, ldx, (const float*)Y, ldy, rpb, part);
+      else hipLaunchKernelGGL((coldot_partial<float, 2>), dim3(nblk), dim3(kBlock), 0, st, n, b, (const float*)X, ldx, (const float*)Y, ldy, rpb, part);
+      break;
+    case qds::C128:
+      if (conj) hipLaunchKernelGGL((coldot_partial<double, 1>), dim3(nblk), dim3(kBlock), 0, st, n, b, (const double*)X, ldx, (const double*)Y, ldy, rpb, part);
+      else hipLaunchKernelGGL((coldot_partial<double, 2>), dim3(nblk), dim3(kBlock), 0, st, n, b, (const double*)X, ldx, (const double*)Y, ldy, rpb, part);
+      break;
+    default: QDS_CHECK_ARG(false, "qds_coldot: bad dtype");
+  }
+  QDS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(coldot_reduce, dim3(1), dim3(256), 0, st, (int)nblk, b, part, out, cplx ? 1 : 0);
+  QDS_LAUNCH_CHECK();
+  return QDS_OK;
+}

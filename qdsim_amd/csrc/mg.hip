@@ -1,0 +1,272 @@
+// Geometric multigrid on the DOF lattice, used as the block preconditioner of the
+// eigensolver (no reference counterpart: the reference relies on ARPACK
+// shift-invert with a sparse direct factorisation,
+// qdsim_cython/advanced_eigenvalue_solvers.py:106-122).
+//
+// Every DOF of a P1/P2/P3 structured mesh sits on the lattice of the p-times
+// refined grid, so the interior DOFs (homogeneous Dirichlet) form an nx x ny array.
+// The preconditioner is the low-order-refined operator on that lattice: the P1
+// stiffness of the refined triangulation, which for right triangles is exactly a
+// five-point stencil with edge weights w = c * h_perp / h_par, plus a lumped mass
+// term.  Per level: d[p] (diagonal, including the couplings to Dirichlet
+// neighbours), we[p] (edge p -> east), wn[p] (edge p -> north), zero on edges that
+// leave the interior.  Coarse levels come from vertex-centred coarsening
+// (fine interior 2*nc + 1), bilinear prolongation P and restriction R = P^T.
+//
+// Vectors hold b columns per lattice point (row-major point x column), matching
+// the eigensolver's block layout.  One V(nu1, nu2)-cycle with damped Jacobi
+// smoothing is launched by qds_mg_vcycle; all kernels run on the caller's stream.
+#include "qds_common.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+
+template <typename T>
+struct Level {
+  int64_t nx, ny;
+  const T* d;
+  const T* we;
+  const T* wn;
+  T* u;
+  T* f;
+  T* t;
+  int64_t ldu, ldf;  // per-point strides of u and f (t is dense, stride b)
+};
+
+template <typename T>
+__device__ __forceinline__ T apply_at(const Level<T>& L, int64_t ld, int64_t p, int64_t ix, int64_t iy,
+                                      int c, const T* __restrict__ u) {
+  T s = L.d[p] * u[p * ld + c];
+  if (ix + 1 < L.nx) s -= L.we[p] * u[(p + 1) * ld + c];
+  if (ix > 0) s -= L.we[p - 1] * u[(p - 1) * ld + c];
+  if (iy + 1 < L.ny) s -= L.wn[p] * u[(p + L.nx) * ld + c];
+  if (iy > 0) s -= L.wn[p - L.nx] * u[(p - L.nx) * ld + c];
+  return s;
+}
+
+// out = u + omega D^-1 (f - A u); zero_init: u = 0 so out = omega D^-1 f
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+jacobi_kernel(Level<T> L, int b, const T* __restrict__ u, int64_t ldu_in, T* __restrict__ out,
+              int64_t ldo, T omega, int zero_init) {
+  const int64_t np = L.nx * L.ny;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np * b;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = t / b;
+    const int c = (int)(t - p * b);
+    const T inv = T(1) / L.d[p];
+    const T fv = L.f[p * L.ldf + c];
+    if (zero_init) {
+      out[p * ldo + c] = omega * inv * fv;
+    } else {
+      const int64_t iy = p / L.nx, ix = p - iy * L.nx;
+      const T r = fv - apply_at(L, ldu_in, p, ix, iy, c, u);
+      out[p * ldo + c] = u[p * ldu_in + c] + omega * inv * r;
+    }
+  }
+}
+
+// coarse f = R (f - A u), R = P^T of bilinear interpolation
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+restrict_residual_kernel(Level<T> F, Level<T> C, int b) {
+  const int64_t np = C.nx * C.ny;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np * b;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t pc = t / b;
+    const int c = (int)(t - pc * b);
+    const int64_t jy = pc / C.nx, jx = pc - jy * C.nx;
+    const int64_t fx = 2 * jx + 1, fy = 2 * jy + 1;
+    T acc = T(0);
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int64_t x = fx + dx, y = fy + dy;
+        const T w = (dx == 0 ? T(1) : T(0.5)) * (dy == 0 ? T(1) : T(0.5));
+        const int64_t p = y * F.nx + x;
+        const T r = F.f[p * F.ldf + c] - apply_at(F, F.ldu, p, x, y, c, F.u);
+        acc += w * r;
+      }
+    }
+    C.f[pc * C.ldf + c] = acc;
+  }
+}
+
+// fine u += P e (bilinear); coarse point (jx, jy) sits at fine (2jx+1, 2jy+1)
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+prolong_add_kernel(Level<T> F, Level<T> C, int b) {
+  const int64_t np = F.nx * F.ny;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np * b;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = t / b;
+    const int c = (int)(t - p * b);
+    const int64_t iy = p / F.nx, ix = p - iy * F.nx;
+    // 1-D weights: odd fine index -> one coarse point, even -> two neighbours
+    int64_t xs[2], ys[2];
+    T wx[2], wy[2];
+    int nxp, nyp;
+    if (ix & 1) { xs[0] = (ix - 1) / 2; wx[0] = T(1); nxp = 1; }
+    else { xs[0] = ix / 2 - 1; xs[1] = ix / 2; wx[0] = wx[1] = T(0.5); nxp = 2; }
+    if (iy & 1) { ys[0] = (iy - 1) / 2; wy[0] = T(1); nyp = 1; }
+    else { ys[0] = iy / 2 - 1; ys[1] = iy / 2; wy[0] = wy[1] = T(0.5); nyp = 2; }
+    T acc = T(0);
+    for (int a = 0; a < nyp; ++a) {
+      if (ys[a] < 0 || ys[a] >= C.ny) continue;
+      for (int q = 0; q < nxp; ++q) {
+        if (xs[q] < 0 || xs[q] >= C.nx) continue;
+        acc += wx[q] * wy[a] * C.u[(ys[a] * C.nx + xs[q]) * C.ldu + c];
+      }
+    }
+    F.u[p * F.ldu + c] += acc;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+apply_kernel(Level<T> L, int b, const T* __restrict__ u, T* __restrict__ out) {
+  const int64_t np = L.nx * L.ny;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np * b;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = t / b;
+    const int c = (int)(t - p * b);
+    const int64_t iy = p / L.nx, ix = p - iy * L.nx;
+    out[t] = apply_at(L, (int64_t)b, p, ix, iy, c, u);
+  }
+}
+
+template <typename T>
+__global__ void copy_kernel(int64_t np, int b, const T* __restrict__ src, int64_t lds, T* __restrict__ dst) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np * b;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = t / b;
+    const int c = (int)(t - p * b);
+    dst[t] = src[p * lds + c];
+  }
+}
+
+template <typename T>
+int smooth(const Level<T>& L, int b, int sweeps, T omega, bool zero_first, hipStream_t st) {
+  // Jacobi sweeps ping-pong between u and t; the result always ends in u
+  const int64_t work = L.nx * L.ny * b;
+  const dim3 g(qds_blocks(work, kBlock, 131072)), blk(kBlock);
+  // with an odd count the first sweep writes straight to u, so no final copy
+  T* src = L.u;
+  T* dst = (sweeps % 2 == 0) ? L.t : L.u;
+  auto ld_of = [&](const T* q) { return q == L.u ? L.ldu : (int64_t)b; };
+  for (int s = 0; s < sweeps; ++s) {
+    const bool zi = zero_first && s == 0;
+    if (!zi && src == dst) {
+      // first sweep of an odd count on a live u: copy u into t (dense) first
+      hipLaunchKernelGGL(copy_kernel<T>, g, blk, 0, st, L.nx * L.ny, b, (const T*)L.u, L.ldu, L.t);
+      QDS_LAUNCH_CHECK();
+      src = L.t;
+    }
+    hipLaunchKernelGGL(jacobi_kernel<T>, g, blk, 0, st, L, b, (const T*)src, ld_of(src), dst, ld_of(dst),
+                       omega, zi ? 1 : 0);
+    QDS_LAUNCH_CHECK();
+    src = dst;
+    dst = (dst == L.u) ? L.t : L.u;
+  }
+  return QDS_OK;
+}
+
+template <typename T>
+int vcycle(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, hipStream_t st) {
+  for (int l = 0; l < nl - 1; ++l) {
+    int rc = smooth(L[l], b, nu1, omega, true, st);
+    if (rc) return rc;
+    const int64_t work = L[l + 1].nx * L[l + 1].ny * b;
+    hipLaunchKernelGGL(restrict_residual_kernel<T>, dim3(qds_blocks(work, kBlock, 131072)),
+                       dim3(kBlock), 0, st, L[l], L[l + 1], b);
+    QDS_LAUNCH_CHECK();
+  }
+  int rc = smooth(L[nl - 1], b, ncoarse, omega, true, st);
+  if (rc) return rc;
+  for (int l = nl - 2; l >= 0; --l) {
+    const int64_t work = L[l].nx * L[l].ny * b;
+    hipLaunchKernelGGL(prolong_add_kernel<T>, dim3(qds_blocks(work, kBlock, 131072)), dim3(kBlock),
+                       0, st, L[l], L[l + 1], b);
+    QDS_LAUNCH_CHECK();
+    rc = smooth(L[l], b, nu2, omega, false, st);
+    if (rc) return rc;
+  }
+  return QDS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+// Level descriptor shared with the host side (all pointers are device memory of the
+// level's dtype).  u, f, t hold nx*ny*b values.
+typedef struct {
+  int64_t nx, ny;
+  const void* d;
+  const void* we;
+  const void* wn;
+  void* u;
+  void* f;
+  void* t;
+  int64_t ldu, ldf;  // per-point strides of u and f (0 means b)
+} qds_mg_level;
+}
+
+// One V-cycle for A z = r on level 0.  levels[0].f must hold r and levels[0].u
+// receives z (callers may point them at their own buffers).  Requires
+// levels[l].nx == 2 * levels[l+1].nx + 1 (same for ny).
+QDS_API int qds_mg_vcycle(int32_t dtype, int32_t n_levels, const qds_mg_level* levels, int32_t b,
+                          int32_t nu_pre, int32_t nu_post, int32_t coarse_sweeps, double omega,
+                          void* stream) {
+  QDS_CHECK_ARG(levels && n_levels >= 1 && n_levels <= 32 && b >= 1, "qds_mg_vcycle: bad arguments");
+  for (int l = 0; l + 1 < n_levels; ++l)
+    QDS_CHECK_ARG(levels[l].nx == 2 * levels[l + 1].nx + 1 && levels[l].ny == 2 * levels[l + 1].ny + 1,
+                  "qds_mg_vcycle: level %d is not a vertex-centred coarsening of level %d", l + 1, l);
+  for (int l = 0; l < n_levels; ++l)
+    QDS_CHECK_ARG(levels[l].d && levels[l].we && levels[l].wn && levels[l].u && levels[l].f && levels[l].t,
+                  "qds_mg_vcycle: null pointer on level %d", l);
+  hipStream_t st = qds_stream(stream);
+  if (dtype == qds::F32) {
+    Level<float> L[32];
+    for (int l = 0; l < n_levels; ++l)
+      L[l] = Level<float>{levels[l].nx, levels[l].ny, (const float*)levels[l].d, (const float*)levels[l].we,
+                          (const float*)levels[l].wn, (float*)levels[l].u, (float*)levels[l].f,
+                          (float*)levels[l].t, levels[l].ldu ? levels[l].ldu : b,
+                          levels[l].ldf ? levels[l].ldf : b};
+    return vcycle<float>(n_levels, L, b, nu_pre, nu_post, coarse_sweeps, (float)omega, st);
+  }
+  if (dtype == qds::F64) {
+    Level<double> L[32];
+    for (int l = 0; l < n_levels; ++l)
+      L[l] = Level<double>{levels[l].nx, levels[l].ny, (const double*)levels[l].d,
+                           (const double*)levels[l].we, (const double*)levels[l].wn,
+                           (double*)levels[l].u, (double*)levels[l].f, (double*)levels[l].t,
+                           levels[l].ldu ? levels[l].ldu : b, levels[l].ldf ? levels[l].ldf : b};
+    return vcycle<double>(n_levels, L, b, nu_pre, nu_post, coarse_sweeps, omega, st);
+  }
+  QDS_CHECK_ARG(false, "qds_mg_vcycle: dtype must be F32 or F64");
+  return QDS_EINVAL;
+}
+
+// out = A u on one level (tests of the lattice operator).
+QDS_API int qds_mg_apply(int32_t dtype, const qds_mg_level* level, int32_t b, const void* u, void* out,
+                         void* stream) {
+  QDS_CHECK_ARG(level && u && out && level->d && level->we && level->wn, "qds_mg_apply: null");
+  hipStream_t st = qds_stream(stream);
+  const int64_t work = level->nx * level->ny * b;
+  const dim3 g(qds_blocks(work, kBlock, 131072)), blk(kBlock);
+  if (dtype == qds::F64) {
+    Level<double> L{level->nx, level->ny, (const double*)level->d, (const double*)level->we,
+                    (const double*)level->wn, nullptr, nullptr, nullptr, b, b};
+    hipLaunchKernelGGL(apply_kernel<double>, g, blk, 0, st, L, b, (const double*)u, (double*)out);
+  } else if (dtype == qds::F32) {
+    Level<float> L{level->nx, level->ny, (const float*)level->d, (const float*)level->we,
+                   (const float*)level->wn, nullptr, nullptr, nullptr, b, b};
+    hipLaunchKernelGGL(apply_kernel<float>, g, blk, 0, st, L, b, (const float*)u, (float*)out);
+  } else {
+    QDS_CHECK_ARG(false, "qds_mg_apply: bad dtype");
+  }
+  QDS_LAUNCH_CHECK();
+  return QDS_OK;
+}

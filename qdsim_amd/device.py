@@ -1,0 +1,63 @@
+"""Device plumbing: torch tensors as device memory, streams, workspaces.
+
+PyTorch-ROCm provides allocation and the current HIP stream; every computation
+on those buffers is a qdsim_hip kernel.  A GPU is required.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+
+DTYPE_CODE = {torch.float32: _lib.F32, torch.float64: _lib.F64,
+              torch.complex64: _lib.C64, torch.complex128: _lib.C128}
+
+
+def device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("qdsim_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_ptr():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def code(t):
+    return DTYPE_CODE[t.dtype]
+
+
+def real_dtype(dt):
+    return {torch.complex64: torch.float32, torch.complex128: torch.float64}.get(dt, dt)
+
+
+class Workspace:
+    """Grow-only scratch buffer (bytes) reused across calls on one device."""
+
+    def __init__(self):
+        self._buf = None
+
+    def get(self, nbytes):
+        nbytes = max(int(nbytes), 256)
+        if self._buf is None or self._buf.numel() < nbytes:
+            self._buf = torch.empty(nbytes, dtype=torch.uint8, device=device())
+        return self._buf
+
+    def ptr(self, nbytes):
+        b = self.get(nbytes)
+        return ctypes.c_void_p(b.data_ptr()), b.numel()
+
+
+_WS = {}
+
+
+def workspace(name="default"):
+    if name not in _WS:
+        _WS[name] = Workspace()
+    return _WS[name]

@@ -1,0 +1,194 @@
+"""Thin Python wrappers over the qdsim_hip C-ABI (one function per entry point).
+
+Blocks are torch tensors of shape (n, b) with unit column stride; a column slice
+of a wider buffer is passed with its row stride as the leading dimension.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, host_ptr, i32, lib
+from .device import code, device, ptr, stream_ptr, workspace
+
+
+# ------------------------------------------------------------------ host pattern
+def pattern_build(conn, n_dof, with_elem_map=False):
+    """CSR pattern + scatter maps of the element cliques (host, native C++)."""
+    conn = i32(conn)
+    ne, npe = conn.shape
+    nnz = ctypes.c_int64()
+    check(lib.qds_pattern_count(host_ptr(conn), ne, npe, n_dof, ctypes.byref(nnz)), "qds_pattern_count")
+    nnz = nnz.value
+    indptr = np.empty(n_dof + 1, np.int32)
+    indices = np.empty(nnz, np.int32)
+    seg_ptr = np.empty(nnz + 1, np.int32)
+    seg_src = np.empty(ne * npe * npe, np.int32)
+    emap = np.empty(ne * npe * npe, np.int32) if with_elem_map else None
+    check(lib.qds_pattern_fill(host_ptr(conn), ne, npe, n_dof, nnz, host_ptr(indptr), host_ptr(indices),
+                               host_ptr(seg_ptr), host_ptr(seg_src),
+                               host_ptr(emap) if emap is not None else None), "qds_pattern_fill")
+    out = {"indptr": indptr, "indices": indices, "seg_ptr": seg_ptr, "seg_src": seg_src, "nnz": nnz}
+    if emap is not None:
+        out["elem_map"] = emap.reshape(ne, npe, npe)
+    return out
+
+
+def pattern_restrict(indptr, indices, new_of_old, n_new):
+    indptr = i32(indptr); indices = i32(indices); new_of_old = i32(new_of_old)
+    n = len(indptr) - 1
+    nnz = ctypes.c_int64()
+    check(lib.qds_pattern_restrict_count(host_ptr(indptr), host_ptr(indices), n, host_ptr(new_of_old),
+                                         n_new, ctypes.byref(nnz)), "qds_pattern_restrict_count")
+    nnz = nnz.value
+    r_ip = np.empty(n_new + 1, np.int32)
+    r_ix = np.empty(nnz, np.int32)
+    r_src = np.empty(nnz, np.int32)
+    check(lib.qds_pattern_restrict_fill(host_ptr(indptr), host_ptr(indices), n, host_ptr(new_of_old), n_new,
+                                        nnz, host_ptr(r_ip), host_ptr(r_ix), host_ptr(r_src)),
+          "qds_pattern_restrict_fill")
+    return r_ip, r_ix, r_src
+
+
+# ------------------------------------------------------------------ assembly
+def _coef(t):
+    """(pointer, element stride, quadrature stride) of a coefficient tensor.
+
+    Shapes: (E, Q) per quadrature point, (E,) per element, or None."""
+    if t is None:
+        return None, 0, 0
+    assert t.dtype == torch.float64 and t.is_contiguous()
+    if t.dim() == 1:
+        return ptr(t), 1, 0
+    return ptr(t), t.shape[1], 1
+
+
+def assemble_elements(npe, nq, conn, x, y, tables, c=None, v=None, r=None, w=None,
+                      out0=True, out1=False, out2=False):
+    ne = conn.shape[0]
+    nsym = npe * (npe + 1) // 2
+    dev = conn.device
+    o0 = torch.empty(ne * nsym, dtype=torch.float64, device=dev) if out0 else None
+    o1 = torch.empty(ne * nsym, dtype=torch.float64, device=dev) if out1 else None
+    o2 = torch.empty(ne * nsym, dtype=torch.float64, device=dev) if out2 else None
+    cp, cse, csq = _coef(c)
+    vp, vse, vsq = _coef(v)
+    rp, rse, rsq = _coef(r)
+    wp, wse, wsq = _coef(w)
+    check(lib.qds_assemble_elements(npe, nq, ne, ptr(conn), ptr(x), ptr(y), ptr(tables),
+                                    cp, cse, csq, vp, vse, vsq, rp, rse, rsq, wp, wse, wsq,
+                                    ptr(o0), ptr(o1), ptr(o2), stream_ptr()), "qds_assemble_elements")
+    return o0, o1, o2
+
+
+def assemble_gather(nnz, seg_ptr, seg_src, e0=None, e1=None, e2=None):
+    dev = seg_ptr.device
+    v = [torch.empty(nnz, dtype=torch.float64, device=dev) if e is not None else None for e in (e0, e1, e2)]
+    check(lib.qds_assemble_gather(nnz, ptr(seg_ptr), ptr(seg_src), ptr(e0), ptr(e1), ptr(e2),
+                                  ptr(v[0]), ptr(v[1]), ptr(v[2]), stream_ptr()), "qds_assemble_gather")
+    return v
+
+
+def quad_points(npe, conn, x, y, bary):
+    ne = conn.shape[0]
+    nq = bary.shape[0]
+    xq = torch.empty((ne, nq), dtype=torch.float64, device=conn.device)
+    yq = torch.empty_like(xq)
+    check(lib.qds_quad_points(npe, nq, ne, ptr(conn), ptr(x), ptr(y), ptr(bary), ptr(xq), ptr(yq),
+                              stream_ptr()), "qds_quad_points")
+    return xq, yq
+
+
+def csr_take(src, re, im=None, scale=1.0, dtype=torch.float64):
+    n = src.numel()
+    out = torch.empty(n, dtype=dtype, device=src.device)
+    check(lib.qds_csr_take(code(out), n, ptr(src), ptr(re), ptr(im), float(scale), ptr(out), stream_ptr()),
+          "qds_csr_take")
+    return out
+
+
+# ------------------------------------------------------------------ blocks
+def _ld(t):
+    if t.dim() == 1:
+        return 1
+    assert t.stride(1) == 1, "blocks need unit column stride"
+    return t.stride(0)
+
+
+def _width(t):
+    return 1 if t.dim() == 1 else t.shape[1]
+
+
+def spmm(indptr, indices, n_rows, nnz, vals_a, X, Ya, vals_b=None, Yb=None):
+    assert vals_a.dtype == X.dtype == Ya.dtype
+    b = _width(X)
+    check(lib.qds_spmm(code(X), n_rows, nnz, b, ptr(indptr), ptr(indices), ptr(vals_a), ptr(vals_b),
+                       ptr(X), _ld(X), ptr(Ya), _ld(Ya), ptr(Yb), _ld(Yb) if Yb is not None else 1,
+                       stream_ptr()), "qds_spmm")
+
+
+def gram(A, B1, B2=None, sym=False, out=None):
+    """A^T [B1 | B2] in FP64, shape (m, k1 + k2)."""
+    n, m = A.shape
+    k1 = B1.shape[1]
+    k2 = B2.shape[1] if B2 is not None else 0
+    if out is None:
+        out = torch.empty((m, k1 + k2), dtype=torch.float64, device=A.device)
+    wsb = lib.qds_gram_workspace(n, m, k1, k2)
+    wp, wn = workspace("gram").ptr(wsb)
+    check(lib.qds_gram(code(A), n, m, ptr(A), _ld(A), k1, ptr(B1), _ld(B1), k2, ptr(B2),
+                       _ld(B2) if B2 is not None else 1, 1 if sym else 0, ptr(out), wp, wn, stream_ptr()),
+          "qds_gram")
+    return out
+
+
+def block_update(S, C1, Y1, C2=None, Y2=None):
+    """Y1 = S C1 (and Y2 = S C2); C in the storage dtype of S."""
+    n, m = S.shape
+    k = C1.shape[1]
+    C1 = C1.to(S.dtype).contiguous()
+    C2 = C2.to(S.dtype).contiguous() if C2 is not None else None
+    check(lib.qds_block_update(code(S), n, m, k, ptr(S), _ld(S), ptr(C1), ptr(Y1), _ld(Y1),
+                               ptr(C2), ptr(Y2), _ld(Y2) if Y2 is not None else 1, stream_ptr()),
+          "qds_block_update")
+
+
+def block_residual(HX, MX, lam, R):
+    """R = HX - MX diag(lam); lam float64 (b,) or complex128 (b,)."""
+    n, b = HX.shape
+    lam = lam.contiguous()
+    check(lib.qds_block_residual(code(HX), n, b, ptr(HX), _ld(HX), ptr(MX), _ld(MX), ptr(lam),
+                                 ptr(R), _ld(R), stream_ptr()), "qds_block_residual")
+
+
+def coldot(X, Y, conj=True):
+    n, b = X.shape
+    cplx = X.is_complex()
+    out = torch.empty(b, dtype=torch.complex128 if cplx else torch.float64, device=X.device)
+    wsb = lib.qds_coldot_workspace(b)
+    wp, wn = workspace("coldot").ptr(wsb)
+    check(lib.qds_coldot(code(X), n, b, ptr(X), _ld(X), ptr(Y), _ld(Y), 1 if conj else 0, ptr(out),
+                         wp, wn, stream_ptr()), "qds_coldot")
+    return out
+
+
+def mg_vcycle(dtype_code, levels_struct, n_levels, b, nu_pre, nu_post, coarse, omega):
+    check(lib.qds_mg_vcycle(dtype_code, n_levels, ctypes.cast(levels_struct, ctypes.c_void_p), b,
+                            nu_pre, nu_post, coarse, float(omega), stream_ptr()), "qds_mg_vcycle")
+
+
+def mg_apply(dtype_code, level_struct, b, u, out):
+    check(lib.qds_mg_apply(dtype_code, ctypes.cast(ctypes.byref(level_struct), ctypes.c_void_p), b,
+                           ptr(u), ptr(out), stream_ptr()), "qds_mg_apply")
+
+
+def cg(indptr, indices, vals, n, nnz, b, x, rtol=1e-10, max_iter=10000, check_every=10):
+    wsb = lib.qds_cg_workspace(n)
+    wp, wn = workspace("cg").ptr(wsb)
+    it = ctypes.c_int32()
+    rr = ctypes.c_double()
+    check(lib.qds_cg(n, nnz, ptr(indptr), ptr(indices), ptr(vals), ptr(b), ptr(x), float(rtol),
+                     int(max_iter), int(check_every), wp, wn, ctypes.byref(it), ctypes.byref(rr),
+                     stream_ptr()), "qds_cg")
+    return it.value, rr.value

@@ -1,0 +1,166 @@
+"""Lattice view of a structured FE mesh and the multigrid block preconditioner.
+
+Every DOF of an order-p SimpleMesh sits on the (p*cx+1) x (p*cy+1) lattice of the
+p-times refined grid (cx, cy cells).  The eigensolver numbers the free
+(interior) DOFs in lattice row-major order, so a block vector is a plain
+(ny_i * nx_i, b) array and the preconditioner can work on 2-D stencils.
+
+Preconditioner: one multigrid V-cycle (qds_mg_vcycle) for the low-order-refined
+operator K_lor + (V - sigma) M_lump on the lattice: five-point P1 stiffness of the
+refined right-triangle mesh (edge weight c * h_perp / h_par, c = hbar^2/2m*
+averaged over the FE elements touching the edge), lumped P1 mass h_x h_y, shift
+sigma below min(V) so the operator is SPD.  Coarse levels are rediscretised:
+coarse edge weight = harmonic mean of the two fine edges on it, coarse mass term
+= R m with R = P^T of bilinear interpolation.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib, kernels
+from .device import device
+
+
+class Lattice:
+    """Lattice geometry of a SimpleMesh (lengths in nm)."""
+
+    def __init__(self, mesh):
+        self.p = int(getattr(mesh, "element_order", 1))
+        self.cx, self.cy = mesh.nx - 1, mesh.ny - 1
+        self.NX = self.p * self.cx + 1
+        self.NY = self.p * self.cy + 1
+        self.nx_i, self.ny_i = self.NX - 2, self.NY - 2
+        self.Lx = mesh.Lx * 1e9
+        self.Ly = mesh.Ly * 1e9
+        self.hx = self.Lx / (self.p * self.cx)
+        self.hy = self.Ly / (self.p * self.cy)
+        self.dxc = self.Lx / self.cx
+        self.dyc = self.Ly / self.cy
+        lx = np.asarray(mesh.lattice_x)
+        ly = np.asarray(mesh.lattice_y)
+        conn = mesh.get_elements()
+        used = np.zeros(mesh.num_nodes, dtype=bool)
+        used[conn.ravel()] = True
+        interior = (lx > 0) & (lx < self.NX - 1) & (ly > 0) & (ly < self.NY - 1) & used
+        new = np.full(mesh.num_nodes, -1, dtype=np.int64)
+        new[interior] = (ly[interior] - 1) * self.nx_i + (lx[interior] - 1)
+        self.new_of_old = new
+        self.n_free = self.nx_i * self.ny_i
+        if interior.sum() != self.n_free:
+            raise ValueError("mesh DOFs do not cover the interior lattice one-to-one")
+        self.free_dofs = np.nonzero(interior)[0]
+        # old id of each lattice-ordered free DOF
+        old = np.empty(self.n_free, dtype=np.int64)
+        old[new[interior]] = np.nonzero(interior)[0]
+        self.old_of_new = old
+
+    def element_at(self, x, y):
+        """FE element index containing points (x, y) in nm (device tensors)."""
+        fx = x / self.dxc
+        fy = y / self.dyc
+        i = torch.clamp(torch.floor(fx), 0, self.cx - 1)
+        j = torch.clamp(torch.floor(fy), 0, self.cy - 1)
+        u = fx - i
+        v = fy - j
+        upper = (u + v > 1.0).to(torch.int64)
+        return 2 * (j.to(torch.int64) * self.cx + i.to(torch.int64)) + upper
+
+
+class LatticeMG:
+    """Block multigrid preconditioner W = V-cycle(R) on the interior lattice."""
+
+    def __init__(self, lattice, c_elem, v_elem, sigma, b, dtype=torch.float32, nu_pre=2, nu_post=2,
+                 coarse_sweeps=40, omega=0.8, min_coarse=3):
+        self.lat = lattice
+        self.b = int(b)
+        self.dtype = dtype
+        self.nu_pre, self.nu_post, self.coarse_sweeps, self.omega = nu_pre, nu_post, coarse_sweeps, omega
+        dev = device()
+        L = lattice
+        c_elem = c_elem.to(dev, torch.float64)
+        v_elem = v_elem.to(dev, torch.float64)
+        f64 = dict(dtype=torch.float64, device=dev)
+        # fine full-lattice edge weights and mass term
+        ix = torch.arange(L.NX, **f64)
+        iy = torch.arange(L.NY, **f64)
+        Y, X = torch.meshgrid(iy, ix, indexing="ij")
+        # horizontal edges (NY, NX-1)
+        xm = (X[:, :-1] + 0.5) * L.hx
+        yy = Y[:, :-1] * L.hy
+        ca = c_elem[L.element_at(xm, yy + 0.25 * L.hy)]
+        cb = c_elem[L.element_at(xm, yy - 0.25 * L.hy)]
+        WE = 0.5 * (ca + cb) * (L.hy / L.hx)
+        # vertical edges (NY-1, NX)
+        ym = (Y[:-1, :] + 0.5) * L.hy
+        xx = X[:-1, :] * L.hx
+        ca = c_elem[L.element_at(xx + 0.25 * L.hx, ym)]
+        cb = c_elem[L.element_at(xx - 0.25 * L.hx, ym)]
+        WN = 0.5 * (ca + cb) * (L.hx / L.hy)
+        xp = X * L.hx
+        yp = Y * L.hy
+        vs = sum(v_elem[L.element_at(xp + sx * L.hx, yp + sy * L.hy)]
+                 for sx in (-0.25, 0.25) for sy in (-0.25, 0.25)) * 0.25
+        Mterm = (vs - sigma) * (L.hx * L.hy)
+        self.levels_data = []
+        NX, NY = L.NX, L.NY
+        while True:
+            self.levels_data.append(self._interior(WE, WN, Mterm, NX, NY, dev))
+            nxi, nyi = NX - 2, NY - 2
+            if not (nxi % 2 == 1 and nyi % 2 == 1 and (nxi - 1) // 2 >= min_coarse
+                    and (nyi - 1) // 2 >= min_coarse):
+                break
+            WE, WN, Mterm, NX, NY = self._coarsen(WE, WN, Mterm, NX, NY)
+        self.n_levels = len(self.levels_data)
+        self._structs = (_lib.MGLevel * self.n_levels)()
+        for l, lv in enumerate(self.levels_data):
+            n = lv["nx"] * lv["ny"]
+            lv["u"] = torch.zeros((n, self.b), dtype=dtype, device=dev)
+            lv["f"] = torch.zeros((n, self.b), dtype=dtype, device=dev)
+            lv["t"] = torch.zeros((n, self.b), dtype=dtype, device=dev)
+            s = self._structs[l]
+            s.nx, s.ny = lv["nx"], lv["ny"]
+            s.d, s.we, s.wn = lv["d"].data_ptr(), lv["we"].data_ptr(), lv["wn"].data_ptr()
+            s.u, s.f, s.t = lv["u"].data_ptr(), lv["f"].data_ptr(), lv["t"].data_ptr()
+            s.ldu = s.ldf = 0
+
+    def _interior(self, WE, WN, Mterm, NX, NY, dev):
+        d = (WE[1:-1, :-1] + WE[1:-1, 1:] + WN[:-1, 1:-1] + WN[1:, 1:-1] + Mterm[1:-1, 1:-1])
+        we = WE[1:-1, 1:].clone()
+        we[:, -1] = 0.0
+        wn = WN[1:, 1:-1].clone()
+        wn[-1, :] = 0.0
+        return {"nx": NX - 2, "ny": NY - 2,
+                "d": d.contiguous().to(self.dtype).reshape(-1),
+                "we": we.contiguous().to(self.dtype).reshape(-1),
+                "wn": wn.contiguous().to(self.dtype).reshape(-1),
+                "d64": d}
+
+    @staticmethod
+    def _coarsen(WE, WN, Mterm, NX, NY):
+        NXc, NYc = (NX + 1) // 2, (NY + 1) // 2
+
+        def hmean(a, b):
+            return 2.0 * a * b / (a + b)
+
+        WEc = hmean(WE[0::2, 0::2], WE[0::2, 1::2])
+        WNc = hmean(WN[0::2, 0::2], WN[1::2, 0::2])
+        # R m with bilinear weights; pad with zeros outside
+        Mp = torch.nn.functional.pad(Mterm[None, None], (1, 1, 1, 1))[0, 0]
+        w = torch.tensor([0.5, 1.0, 0.5], dtype=Mterm.dtype, device=Mterm.device)
+        Mc = torch.zeros((NYc, NXc), dtype=Mterm.dtype, device=Mterm.device)
+        for a in range(3):
+            for c in range(3):
+                Mc += w[a] * w[c] * Mp[a:a + 2 * NYc - 1:2, c:c + 2 * NXc - 1:2]
+        return WEc, WNc, Mc, NXc, NYc
+
+    def __call__(self, R, W):
+        """W = V-cycle(R); R dense (n, b), W may be a column slice (row stride >= b)."""
+        assert R.dtype == self.dtype and W.dtype == self.dtype
+        s0 = self._structs[0]
+        s0.f = R.data_ptr()
+        s0.ldf = R.stride(0)
+        s0.u = W.data_ptr()
+        s0.ldu = W.stride(0)
+        kernels.mg_vcycle(_lib.F32 if self.dtype == torch.float32 else _lib.F64, self._structs,
+                          self.n_levels, self.b, self.nu_pre, self.nu_post, self.coarse_sweeps, self.omega)

@@ -1,0 +1,201 @@
+"""Block LOBPCG for the lowest eigenpairs of H x = lambda M x on the GPU.
+
+Reference: the reference solves this problem with ARPACK (scipy eigsh, which='SM',
+qdsim_cython/solvers/eigen_solver.pyx:225-235; shift-invert for large N,
+qdsim_cython/advanced_eigenvalue_solvers.py:106-122) and with an unseeded scipy
+lobpcg (eigen_solver.pyx:254-266).  This is a fresh block solver built on the
+qdsim_hip kernels:
+
+* the operator apply is the fused CSR SpMM (H and M share the pattern, one pass);
+* the block iterate is S = [X | W | P] stored as one row-major (n, 3b) buffer, with
+  H S and M S alongside, so the Rayleigh-Ritz Gram matrices S^T [HS | MS] come
+  out of one MFMA FP64 kernel call (upper tiles only);
+* Rayleigh-Ritz uses SVQB on the small FP64 Gram pair: scale, drop directions whose
+  M-Gram eigenvalue falls below a threshold, solve the standard problem, which is
+  robust when W and P become nearly dependent close to convergence;
+* X, P and their H/M images are updated by the MFMA block-update kernel, writing
+  into the second half of a double buffer.
+
+Mixed precision: blocks and matrices in FP32 for the bulk of the iterations, Gram
+matrices and the small eigenproblem always in FP64, then a refinement in FP64
+storage started from the FP32 subspace (``solve_mixed``).
+"""
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import kernels
+from .device import device
+
+
+class CSROperator:
+    """H and M on one CSR pattern in HBM, values in one storage dtype."""
+
+    def __init__(self, indptr, indices, n, H, M, dtype):
+        self.indptr = indptr
+        self.indices = indices
+        self.n = int(n)
+        self.nnz = int(indices.numel())
+        self.H = H.to(dtype).contiguous()
+        self.M = M.to(dtype).contiguous()
+        self.dtype = dtype
+
+    def apply(self, X, HX, MX):
+        kernels.spmm(self.indptr, self.indices, self.n, self.nnz, self.H, X, HX, self.M, MX)
+
+    def apply_h(self, X, HX):
+        kernels.spmm(self.indptr, self.indices, self.n, self.nnz, self.H, X, HX)
+
+    def apply_m(self, X, MX):
+        kernels.spmm(self.indptr, self.indices, self.n, self.nnz, self.M, X, MX)
+
+    def astype(self, dtype):
+        return CSROperator(self.indptr, self.indices, self.n, self.H, self.M, dtype)
+
+
+def _svqb_rr(G, m, nb, drop_tol):
+    """Rayleigh-Ritz on S with Gram G = S^T [HS | MS] (FP64, device).
+
+    Returns (theta[:nb], C (m x nb), n_kept)."""
+    GH = G[:, :m]
+    GM = G[:, m:2 * m]
+    GH = 0.5 * (GH + GH.T)
+    GM = 0.5 * (GM + GM.T)
+    d = torch.diagonal(GM).clamp_min(1e-300).rsqrt()
+    GMs = GM * d[:, None] * d[None, :]
+    GHs = GH * d[:, None] * d[None, :]
+    mu, Q = torch.linalg.eigh(GMs)
+    keep = mu > drop_tol * mu[-1]
+    Qk = Q[:, keep] * mu[keep].rsqrt()[None, :]
+    Hk = Qk.T @ GHs @ Qk
+    Hk = 0.5 * (Hk + Hk.T)
+    theta, Y = torch.linalg.eigh(Hk)
+    nk = int(keep.sum().item()) if isinstance(keep, torch.Tensor) else int(keep.sum())
+    C = (d[:, None] * (Qk @ Y[:, :nb]))
+    return theta[:nb], C, nk
+
+
+class LOBPCGResult:
+    def __init__(self, lam, X, res, iters, history, converged, timings):
+        self.eigenvalues = lam
+        self.X = X
+        self.residuals = res
+        self.iterations = iters
+        self.history = history
+        self.converged = converged
+        self.timings = timings
+
+
+def lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=None, seed=0,
+           verbose=False, stall_iters=None):
+    """Lowest k eigenpairs of (op.H, op.M) with block size b >= k.
+
+    precond(R, W) writes W = T R (R, W are (n, b) blocks of op.dtype).
+    Stops when the relative residual ||H x - lambda M x|| / (|lambda| ||M x||) of the
+    first k columns is below tol, or after maxiter iterations, or (stall_iters)
+    when the worst residual has not improved for that many iterations.
+    """
+    dev = op.H.device
+    dt = op.dtype
+    n = op.n
+    assert b % 4 == 0 and b >= k and 3 * b <= n, (n, b, k)
+    if drop_tol is None:
+        drop_tol = 1e-7 if dt == torch.float32 else 1e-13
+    t_start = time.perf_counter()
+    bufs = [dict(S=torch.zeros((n, 3 * b), dtype=dt, device=dev),
+                 HS=torch.zeros((n, 3 * b), dtype=dt, device=dev),
+                 MS=torch.zeros((n, 3 * b), dtype=dt, device=dev)) for _ in range(2)]
+    R = torch.empty((n, b), dtype=dt, device=dev)
+    cur = bufs[0]
+    X = cur["S"][:, :b]
+    if X0 is None:
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        Xr = torch.rand((n, b), generator=g, dtype=torch.float64) - 0.5
+        X.copy_(Xr.to(dev, dt))
+        if precond is not None:
+            R.copy_(X)
+            precond(R, X)
+    else:
+        X.copy_(X0[:, :b].to(dt))
+    # initial Rayleigh-Ritz on X alone
+    op.apply(X, cur["HS"][:, :b], cur["MS"][:, :b])
+    G = kernels.gram(X, cur["HS"][:, :b], cur["MS"][:, :b], sym=True)
+    theta, C, _ = _svqb_rr(G, b, b, drop_tol)
+    nxt = bufs[1]
+    for key in ("S", "HS", "MS"):
+        kernels.block_update(cur[key][:, :b].contiguous() if False else cur[key][:, :b], C,
+                             nxt[key][:, :b])
+    cur, nxt = nxt, cur
+    lam = theta
+    have_p = False
+    history = []
+    converged = False
+    it = 0
+    best = math.inf
+    best_it = 0
+    res = None
+    t_iter = time.perf_counter()
+    for it in range(1, maxiter + 1):
+        X = cur["S"][:, :b]
+        HX = cur["HS"][:, :b]
+        MX = cur["MS"][:, :b]
+        kernels.block_residual(HX, MX, lam.contiguous(), R)
+        rn = kernels.coldot(R, R)
+        mn = kernels.coldot(MX, MX)
+        res = (rn.clamp_min(0).sqrt() / (lam.abs() * mn.clamp_min(1e-300).sqrt()).clamp_min(1e-300))
+        worst = float(res[:k].max().item())
+        history.append(worst)
+        if verbose:
+            print(f"  lobpcg[{str(dt)[6:]}] it {it:4d} worst rel. residual {worst:.3e}")
+        if worst < tol:
+            converged = True
+            break
+        if stall_iters is not None:
+            if worst < 0.7 * best:
+                best, best_it = worst, it
+            elif it - best_it >= stall_iters:
+                break
+        W = cur["S"][:, b:2 * b]
+        if precond is not None:
+            precond(R, W)
+        else:
+            W.copy_(R)
+        op.apply(W, cur["HS"][:, b:2 * b], cur["MS"][:, b:2 * b])
+        m = 3 * b if have_p else 2 * b
+        S = cur["S"][:, :m]
+        HS = cur["HS"][:, :m]
+        MS = cur["MS"][:, :m]
+        G = kernels.gram(S, HS, MS, sym=True)
+        theta, C, nk = _svqb_rr(G, m, b, drop_tol)
+        CP = C.clone()
+        CP[:b] = 0.0
+        for key in ("S", "HS", "MS"):
+            src = cur[key][:, :m]
+            kernels.block_update(src, C, nxt[key][:, :b], CP, nxt[key][:, 2 * b:3 * b])
+        cur, nxt = nxt, cur
+        lam = theta
+        have_p = True
+    torch.cuda.synchronize()
+    t_end = time.perf_counter()
+    X = cur["S"][:, :b]
+    timings = {"setup_s": t_iter - t_start, "iterate_s": t_end - t_iter}
+    return LOBPCGResult(lam, X, res, it, history, converged, timings)
+
+
+def solve_mixed(op64, k, b, precond32=None, precond64=None, tol=1e-7, tol32=3e-4, maxiter=400,
+                seed=0, verbose=False):
+    """FP32 LOBPCG to tol32 (or stagnation), then FP64 LOBPCG to tol."""
+    op32 = op64.astype(torch.float32)
+    t0 = time.perf_counter()
+    r32 = lobpcg(op32, k, b, precond32, tol=tol32, maxiter=maxiter, seed=seed, verbose=verbose,
+                 stall_iters=8)
+    t1 = time.perf_counter()
+    X0 = r32.X.to(torch.float64)
+    r64 = lobpcg(op64, k, b, precond64, X0=X0, tol=tol, maxiter=maxiter, seed=seed, verbose=verbose)
+    t2 = time.perf_counter()
+    r64.timings.update({"fp32_s": t1 - t0, "fp64_s": t2 - t1, "fp32_iters": r32.iterations,
+                        "fp64_iters": r64.iterations})
+    r64.history = {"fp32": r32.history, "fp64": r64.history}
+    return r64

@@ -1,0 +1,145 @@
+"""Closed-system Schroedinger eigenproblem on the GPU: assembly -> Dirichlet
+restriction -> mixed-precision block LOBPCG.
+
+Reference path: CythonSchrodingerSolver._assemble_matrices + solve
+(qdsim_cython/solvers/schrodinger_solver.pyx:122-199, 417-526) and the P1-P3
+element formulas of backend/src/fem.cpp:924-989.  Differences (kept on purpose):
+
+* consistent FEM potential matrix int V phi_i phi_j at the quadrature points (the
+  Cython solver adds V A/3 to the diagonal, twice on two of the nodes,
+  schrodinger_solver.pyx:224-236);
+* homogeneous Dirichlet by restriction to the interior DOFs (the reference
+  replaces boundary rows by identity rows, which leaves a non-symmetric pencil and
+  spurious eigenvalues at 1.0 J, schrodinger_solver.pyx:253-283);
+* the lowest eigenvalues are returned (which='SA'); the reference asks ARPACK for
+  which='SM', which coincides for positive spectra.
+
+Units: SI at the API, nm / eV on the device.  Eigenvectors are M-normalised in SI
+(psi^T M psi = 1 with M in m^2), as ARPACK returns them for the SI pencil.
+"""
+import time
+
+import numpy as np
+import torch
+
+from . import kernels
+from .fem import FEMSpace
+from .lattice import Lattice, LatticeMG
+from .lobpcg import CSROperator, lobpcg, solve_mixed
+from .units import EV_TO_J, HBAR, NM
+
+KIN_SCALE = HBAR * HBAR / 2.0 / EV_TO_J / (NM * NM)   # hbar^2/2 in eV nm^2 kg
+
+
+def default_block(k, n):
+    b = ((k + 8) + 3) // 4 * 4
+    b = max(b, 8)
+    while 3 * b > n and b > 4:
+        b -= 4
+    return b
+
+
+class SchrodingerProblem:
+    """H = K(hbar^2/2m*) + M(V) [- i M(W)], M on a structured SimpleMesh."""
+
+    def __init__(self, mesh, m_star_func=None, potential_func=None, sampling="quadrature"):
+        t0 = time.perf_counter()
+        self.mesh = mesh
+        self.space = FEMSpace(mesh)
+        self.lat = Lattice(mesh)
+        r_ip, r_ix, r_src = kernels.pattern_restrict(self.space.indptr_host, self.space.indices_host,
+                                                     self.lat.new_of_old, self.lat.n_free)
+        dev = self.space.dev
+        self.r_indptr = torch.from_numpy(r_ip).to(dev)
+        self.r_indices = torch.from_numpy(r_ix).to(dev)
+        self.r_src = torch.from_numpy(r_src).to(dev)
+        self.old_of_new = torch.from_numpy(self.lat.old_of_new).to(dev)
+        self.n_free = self.lat.n_free
+        self.sampling = sampling
+        self.c_q = self.v_q = self.w_q = None
+        self.H = self.M = self.W = None
+        self.setup_time = time.perf_counter() - t0
+        if m_star_func is not None:
+            self.set_mass(m_star_func)
+        if potential_func is not None:
+            self.set_potential(potential_func)
+
+    # -- coefficients at quadrature points (device, eV / eV nm^2) ----------------
+    def _where(self):
+        return "centroid" if self.sampling == "centroid" else "quadrature"
+
+    def set_mass(self, m_star_func):
+        m = self.space.sample(m_star_func, self._where())
+        self.c_q = (KIN_SCALE / m).contiguous()
+
+    def set_potential(self, potential_func, extra_ev=None):
+        v = self.space.sample(potential_func, self._where(), 1.0 / EV_TO_J)
+        if extra_ev is not None:
+            v = v + extra_ev
+        self.v_q = v.contiguous()
+
+    def set_potential_ev(self, v_q):
+        self.v_q = v_q.contiguous()
+
+    def set_cap_ev(self, w_q):
+        self.w_q = None if w_q is None else w_q.contiguous()
+
+    # -- assembly ---------------------------------------------------------------
+    def assemble(self):
+        out = self.space.assemble(c=self.c_q, v=self.v_q, stiffness_mass=True, mass=True,
+                                  w=self.w_q, cap=self.w_q is not None)
+        self.H, self.M, self.W = out
+        return out
+
+    def operator(self, dtype=torch.float64):
+        H = kernels.csr_take(self.r_src, self.H, dtype=dtype)
+        M = kernels.csr_take(self.r_src, self.M, dtype=dtype)
+        return CSROperator(self.r_indptr, self.r_indices, self.n_free, H, M, dtype)
+
+    def preconditioner(self, b, dtype=torch.float32, sigma=None):
+        c_e = self.c_q.mean(dim=1) if self.c_q.dim() == 2 else self.c_q
+        v_e = self.v_q.mean(dim=1) if self.v_q.dim() == 2 else self.v_q
+        if sigma is None:
+            vmin = float(v_e.min().item())
+            vmax = float(v_e.max().item())
+            sigma = vmin - 0.05 * (vmax - vmin) - 1e-3
+        return LatticeMG(self.lat, c_e, v_e, sigma, b, dtype=dtype)
+
+    # -- solve ------------------------------------------------------------------
+    def solve(self, k, b=None, mode="mixed", tol=1e-7, tol32=3e-4, maxiter=400, seed=0, verbose=False):
+        """Lowest k eigenpairs.  Returns (eigenvalues eV (np), full-numbering
+        eigenvectors (n, k) float64 device tensor, M_nm-normalised, info)."""
+        if self.H is None:
+            self.assemble()
+        if b is None:
+            b = default_block(k, self.n_free)
+        t0 = time.perf_counter()
+        op64 = self.operator(torch.float64)
+        if mode == "mixed":
+            pre32 = self.preconditioner(b, torch.float32)
+
+            def pre64(R, W, _pre=pre32, _buf={}):
+                if "r" not in _buf:
+                    _buf["r"] = torch.empty((R.shape[0], b), dtype=torch.float32, device=R.device)
+                    _buf["w"] = torch.empty((R.shape[0], b), dtype=torch.float32, device=R.device)
+                _buf["r"].copy_(R)
+                _pre(_buf["r"], _buf["w"])
+                W.copy_(_buf["w"])
+
+            res = solve_mixed(op64, k, b, pre32, pre64, tol=tol, tol32=tol32, maxiter=maxiter, seed=seed,
+                              verbose=verbose)
+        elif mode == "fp64":
+            pre = self.preconditioner(b, torch.float64)
+            res = lobpcg(op64, k, b, pre, tol=tol, maxiter=maxiter, seed=seed, verbose=verbose)
+        else:
+            raise ValueError(f"unknown mode {mode}")
+        lam = res.eigenvalues[:k].double()
+        X = res.X[:, :k].double()
+        full = torch.zeros((self.space.n, k), dtype=torch.float64, device=X.device)
+        full[self.old_of_new] = X
+        torch.cuda.synchronize()
+        info = {"iterations": res.iterations, "converged": res.converged,
+                "residuals": res.residuals[:k].cpu().numpy() if res.residuals is not None else None,
+                "timings": res.timings, "history": res.history, "block": b,
+                "solve_s": time.perf_counter() - t0}
+        return lam.cpu().numpy(), full, info

@@ -1,0 +1,184 @@
+"""Pins the CPU oracle against what the reference tests and the mathematics fix.
+
+A plausible slip in the oracle (dropped term, wrong index, transposed operand,
+wrong quadrature weight) fails at least one of these checks."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import eig_ref, fem_ref, mesh_ref, poisson_ref
+
+
+# ---- mesh (mesh_minimal.pyx:236-261 asserts) -------------------------------
+def test_simple_mesh_reference_asserts():
+    x, y, el = mesh_ref.simple_mesh_p1(5, 4, 10e-9, 8e-9)
+    assert len(x) == 20 and el.shape == (24, 3)
+    assert x[0] == 0.0 and y[0] == 0.0
+    d = (x - 0.0) ** 2 + (y - 0.0) ** 2
+    assert int(np.argmin(d)) == 0
+
+
+@pytest.mark.parametrize("order,cells,ndof", [(1, (63, 47), 3072), (2, (384, 288), 443713),
+                                              (3, (16, 16), 49 * 49)])
+def test_dof_counts(order, cells, ndof):
+    # closed form: vertices + (p-1) per edge + interior; true P3 = (3c+1)^2 lattice
+    cx, cy = cells
+    nv = (cx + 1) * (cy + 1)
+    ne = cx * (cy + 1) + (cx + 1) * cy + cx * cy
+    expect = nv + (order - 1) * ne + (2 * cx * cy if order == 3 else 0)
+    assert expect == ndof
+    if cx * cy <= 20000:
+        x, y, el = mesh_ref.reference_mesh(cx + 1, cy + 1, 1.0, 1.0, order)
+        assert len(x) == ndof
+
+
+def test_p2_orphans_match_reference_count():
+    x, y, el = mesh_ref.mesh_cpp_p2(4, 3, 1.0, 1.0, orphan_nodes=True)
+    x2, _, _ = mesh_ref.mesh_cpp_p2(4, 3, 1.0, 1.0, orphan_nodes=False)
+    assert len(x) - len(x2) == 3 * 2          # one orphan n0-n3 midpoint per cell
+    used = np.zeros(len(x), bool); used[el.ravel()] = True
+    assert (~used).sum() == 6
+
+
+# ---- quadrature ------------------------------------------------------------
+@pytest.mark.parametrize("deg", [1, 2, 4, 6, 8])
+def test_quadrature_exact_on_monomials(deg):
+    from math import factorial
+    P, W = fem_ref.rule(deg)
+    assert abs(W.sum() - 1) < 1e-14
+    for i in range(deg + 1):
+        for j in range(deg + 1 - i):
+            exact = factorial(i) * factorial(j) / factorial(i + j + 2) * 2
+            got = np.sum(W * P[:, 1] ** i * P[:, 2] ** j)
+            assert abs(got - exact) < 1e-14, (deg, i, j)
+
+
+# ---- shape functions (fe_interpolator.cpp formulas) --------------------------
+def _lattice_points(order):
+    pts = []
+    for a in range(order + 1):
+        for b in range(order + 1 - a):
+            pts.append((1 - (a + b) / order, a / order, b / order))
+    return np.array(pts)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_shape_partition_and_kronecker(order):
+    rng = np.random.default_rng(0)
+    l = rng.dirichlet([1, 1, 1], size=50)
+    phi = np.stack(fem_ref.shape_values(order, l[:, 0], l[:, 1], l[:, 2]), -1)
+    assert np.abs(phi.sum(1) - 1).max() < 1e-13
+    g0 = np.tile([-1.0, -1.0], (50, 1)); g1 = np.tile([1.0, 0.0], (50, 1)); g2 = np.tile([0.0, 1.0], (50, 1))
+    grad = np.stack(fem_ref.shape_gradients(order, l[:, 0], l[:, 1], l[:, 2], g0, g1, g2), 1)
+    assert np.abs(grad.sum(1)).max() < 1e-12
+    # Kronecker at nodes: each lattice point is a node of exactly one shape function
+    L = _lattice_points(order)
+    ph = np.stack(fem_ref.shape_values(order, L[:, 0], L[:, 1], L[:, 2]), -1)
+    assert np.allclose(np.sort(ph.max(1)), 1.0) and np.allclose(np.sort(ph, 1)[:, :-1], 0, atol=1e-13)
+    assert np.linalg.matrix_rank(ph) == ph.shape[1]
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_shape_gradient_matches_finite_difference(order):
+    # directional derivative along (xi, eta) on the reference triangle
+    rng = np.random.default_rng(1)
+    l = rng.dirichlet([2, 2, 2], size=20)
+    h = 1e-6
+    g0 = np.tile([-1.0, -1.0], (20, 1)); g1 = np.tile([1.0, 0.0], (20, 1)); g2 = np.tile([0.0, 1.0], (20, 1))
+    grad = np.stack(fem_ref.shape_gradients(order, l[:, 0], l[:, 1], l[:, 2], g0, g1, g2), 1)
+    f = lambda a, b, c: np.stack(fem_ref.shape_values(order, a, b, c), -1)
+    dxi = (f(l[:, 0] - h, l[:, 1] + h, l[:, 2]) - f(l[:, 0] + h, l[:, 1] - h, l[:, 2])) / (2 * h)
+    assert np.abs(dxi - grad[:, :, 0]).max() < 1e-6
+
+
+# ---- element matrices ---------------------------------------------------------
+def _tri(seed=0):
+    rng = np.random.default_rng(seed)
+    vx = rng.uniform(0, 3, (5, 3)); vy = rng.uniform(0, 2, (5, 3))
+    return vx, vy
+
+
+def test_p1_closed_forms():
+    vx, vy = _tri()
+    K, _, M = fem_ref.element_matrices(1, vx, vy, 2)
+    A = 0.5 * np.abs((vx[:, 1] - vx[:, 0]) * (vy[:, 2] - vy[:, 0]) - (vx[:, 2] - vx[:, 0]) * (vy[:, 1] - vy[:, 0]))
+    bb = np.stack([vy[:, 1] - vy[:, 2], vy[:, 2] - vy[:, 0], vy[:, 0] - vy[:, 1]], 1)
+    cc = np.stack([vx[:, 2] - vx[:, 1], vx[:, 0] - vx[:, 2], vx[:, 1] - vx[:, 0]], 1)
+    Kc = (bb[:, :, None] * bb[:, None, :] + cc[:, :, None] * cc[:, None, :]) / (4 * A)[:, None, None]
+    Mc = A[:, None, None] / 12 * (np.ones((3, 3)) + np.eye(3))
+    assert np.allclose(K, Kc, rtol=1e-13, atol=1e-13)
+    assert np.allclose(M, Mc, rtol=1e-13, atol=1e-15)
+    # Cython centroid formulas (fem_solver.pyx:277-320) agree with quadrature for constants
+    Kz, Mz = fem_ref.p1_centroid_matrices(vx, vy, np.ones(5), np.ones(5))
+    assert np.allclose(Kz, Kc, rtol=1e-13) and np.allclose(Mz, Mc, rtol=1e-13)
+
+
+def test_p2_mass_closed_form():
+    vx, vy = _tri(2)
+    _, _, M = fem_ref.element_matrices(2, vx, vy, 4)
+    A = 0.5 * np.abs((vx[:, 1] - vx[:, 0]) * (vy[:, 2] - vy[:, 0]) - (vx[:, 2] - vx[:, 0]) * (vy[:, 1] - vy[:, 0]))
+    ref = np.array([[6, -1, -1, 0, -4, 0], [-1, 6, -1, 0, 0, -4], [-1, -1, 6, -4, 0, 0],
+                    [0, 0, -4, 32, 16, 16], [-4, 0, 0, 16, 32, 16], [0, -4, 0, 16, 16, 32]]) / 180.0
+    assert np.allclose(M, A[:, None, None] * ref[None], rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_element_invariants(order):
+    vx, vy = _tri(3)
+    K, _, M = fem_ref.element_matrices(order, vx, vy, 2 * order + 2)
+    A = 0.5 * np.abs((vx[:, 1] - vx[:, 0]) * (vy[:, 2] - vy[:, 0]) - (vx[:, 2] - vx[:, 0]) * (vy[:, 1] - vy[:, 0]))
+    assert np.abs(K.sum(2)).max() < 1e-11                      # K 1 = 0
+    assert np.allclose(M.sum((1, 2)), A, rtol=1e-13)           # 1^T M 1 = area
+    assert np.allclose(K, K.transpose(0, 2, 1)) and np.allclose(M, M.transpose(0, 2, 1))
+    # u = 2x + 3y interpolated at nodes: u^T K u = A |grad u|^2 = 13 A
+    from qdsim_amd.quadrature import dof_positions
+    pos = dof_positions(order)
+    xn = vx[:, :1] + (vx[:, 1:2] - vx[:, :1]) * pos[None, :, 0] + (vx[:, 2:3] - vx[:, :1]) * pos[None, :, 1]
+    yn = vy[:, :1] + (vy[:, 1:2] - vy[:, :1]) * pos[None, :, 0] + (vy[:, 2:3] - vy[:, :1]) * pos[None, :, 1]
+    u = 2 * xn + 3 * yn
+    assert np.allclose(np.einsum('ei,eij,ej->e', u, K, u), 13 * A, rtol=1e-11)
+
+
+def test_assembly_canonical_and_invariants():
+    x, y, el = mesh_ref.reference_mesh(6, 5, 5.0, 4.0, 2)
+    vx, vy = x[el[:, :3]], y[el[:, :3]]
+    K, _, M = fem_ref.element_matrices(2, vx, vy, 6)
+    Kg = fem_ref.assemble(el, len(x), K)
+    Mg = fem_ref.assemble(el, len(x), M)
+    assert Kg.has_canonical_format
+    assert abs(Mg.sum() - 20.0) < 1e-12
+    assert np.abs(Kg @ np.ones(len(x))).max() < 1e-11
+
+
+def test_config1_pattern_nnz():
+    # survey Table A: FixedOpenSystemSolver(nx=64, ny=48) P1 pattern nnz 21,058
+    x, y, el = mesh_ref.simple_mesh_p1(64, 48, 25e-9, 20e-9)
+    A = fem_ref.assemble(el, len(x), np.ones((len(el), 3, 3)))
+    assert A.nnz == 21058
+
+
+# ---- eigen oracle ---------------------------------------------------------------
+def test_tridiag_spectrum():
+    n = 50
+    T = sp.diags([-np.ones(n - 1), 2 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1], format='csr')
+    vals, _ = eig_ref.lowest(T, sp.identity(n, format='csr'), 5, sigma=-0.1)
+    exact = 2 - 2 * np.cos(np.arange(1, 6) * np.pi / 51)
+    assert np.allclose(vals, exact, rtol=1e-12)
+
+
+def test_particle_in_box_convergence():
+    # E_nm = hbar^2 pi^2 / 2m (n^2/Lx^2 + m^2/Ly^2); P2 FEM error O(h^4)
+    Lx, Ly = 20.0, 15.0
+    e1 = eig_ref.KIN / (0.067 * eig_ref.M_E) * np.pi ** 2 * (1 / Lx ** 2 + 1 / Ly ** 2)
+    errs = []
+    for c in (8, 16):
+        r = eig_ref.solve_closed(c + 1, c + 1, Lx, Ly, 2, lambda x, y: 0.067 + 0 * x, lambda x, y: 0 * x, 1)
+        errs.append(abs(r["vals"][0] - e1) / e1)
+    assert errs[1] < 1e-4 and errs[0] / errs[1] > 12          # ~2^4 convergence
+
+
+# ---- Poisson oracle -----------------------------------------------------------
+def test_poisson_linear_solution():
+    nx, ny, Lx, Ly = 9, 6, 50e-9, 20e-9
+    phi, x, y = poisson_ref.solve(nx, ny, Lx, Ly, lambda x, y: 13.9 + 0 * x, np.zeros(nx * ny), 0.0, 0.7)
+    assert np.allclose(phi, 0.7 * x / Lx, atol=1e-12)
