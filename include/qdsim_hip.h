@@ -85,6 +85,21 @@ int qds_quad_points(int32_t npe, int32_t nq, int64_t n_elem, const int32_t* conn
 int qds_csr_take(int32_t dtype, int64_t n, const int32_t* src, const double* re, const double* im,
                  double scale, void* out, void* stream);
 
+/* FE interpolation of a nodal field at the quadrature points (replaces the
+ * nearest-node lookup of self_consistent_solver.pyx:119-128,225-236):
+ * out[e][q] = sum_a phi[q][a] u[conn[e][a]].                                      */
+int qds_interpolate_q(int32_t npe, int32_t nq, int64_t n_elem, const int32_t* conn,
+                      const double* phi, const double* u, double* out, void* stream);
+/* Hellmann-Feynman weights out[e][q][s] = w_q |J_e|/2 psi_s(x_q)^2 (no reference
+ * counterpart: adjoint of the eigenvalues w.r.t. the potential at x_q).           */
+int qds_state_density_q(int32_t npe, int32_t nq, int64_t n_elem, const int32_t* conn,
+                        const double* x, const double* y, const double* tables, const double* psi,
+                        int64_t ldpsi, int32_t k, double* out, void* stream);
+/* Electron density n[i] = sum_s occ[s] psi[i][s]^2 (fusedElectronDensity,
+ * backend/src/fused_gpu_kernels.cu:127-157; self_consistent_solver.pyx:262-274).  */
+int qds_density(int64_t n, int32_t k, const double* psi, int64_t ld, const double* occ, double* out,
+                void* stream);
+
 /* ---- sparse products (device) --------------------------------------------------
  * Y_a = A_a X (and Y_b = A_b X on the same pattern).  Replaces scipy csr @ x inside
  * ARPACK (eigen_solver.pyx:225-235) and the unused fusedSpMV
@@ -92,6 +107,15 @@ int qds_csr_take(int32_t dtype, int64_t n, const int32_t* src, const double* re,
 int qds_spmm(int32_t dtype, int64_t n_rows, int64_t nnz, int32_t b, const int32_t* indptr,
              const int32_t* indices, const void* vals_a, const void* vals_b, const void* X,
              int64_t ldx, void* Y_a, int64_t ldy_a, void* Y_b, int64_t ldy_b, void* stream);
+
+/* CAP Hamiltonian on complex blocks: Y_h = (H - iW) X, Y_m = M X with real value
+ * arrays h, w, m on one pattern (dtype C64: FP32 values; C128: FP64).  Replaces
+ * the complex V + i eta of backend/src/fem.cpp:942,974 and the constant lifetime
+ * shift of fixed_open_system_solver.pyx:321-328.                                  */
+int qds_spmm_cap(int32_t dtype, int64_t n_rows, int32_t b, const int32_t* indptr,
+                 const int32_t* indices, const void* vals_h, const void* vals_w, const void* vals_m,
+                 const void* X, int64_t ldx, void* Y_h, int64_t ldy_h, void* Y_m, int64_t ldy_m,
+                 void* stream);
 
 /* ---- dense block kernels for the block eigensolver (device) --------------------
  * Replace the dense Gram / projection steps inside scipy's lobpcg

@@ -34,6 +34,7 @@ _SIGNATURES = {
     "qds_quad_points": ([_i32, _i32, _i64, _p, _p, _p, _p, _p, _p, _p], _c.c_int),
     "qds_csr_take": ([_i32, _i64, _p, _p, _p, _d, _p, _p], _c.c_int),
     "qds_spmm": ([_i32, _i64, _i64, _i32, _p, _p, _p, _p, _p, _i64, _p, _i64, _p, _i64, _p], _c.c_int),
+    "qds_spmm_cap": ([_i32, _i64, _i32, _p, _p, _p, _p, _p, _p, _i64, _p, _i64, _p, _i64, _p], _c.c_int),
     "qds_gram_workspace": ([_i64, _i32, _i32, _i32], _i64),
     "qds_gram": ([_i32, _i64, _i32, _p, _i64, _i32, _p, _i64, _i32, _p, _i64, _i32, _p, _p, _i64, _p],
                  _c.c_int),
@@ -43,6 +44,9 @@ _SIGNATURES = {
     "qds_coldot": ([_i32, _i64, _i32, _p, _i64, _p, _i64, _i32, _p, _p, _i64, _p], _c.c_int),
     "qds_mg_vcycle": ([_i32, _i32, _p, _i32, _i32, _i32, _i32, _d, _p], _c.c_int),
     "qds_mg_apply": ([_i32, _p, _i32, _p, _p, _p], _c.c_int),
+    "qds_interpolate_q": ([_i32, _i32, _i64, _p, _p, _p, _p, _p], _c.c_int),
+    "qds_state_density_q": ([_i32, _i32, _i64, _p, _p, _p, _p, _p, _i64, _i32, _p, _p], _c.c_int),
+    "qds_density": ([_i64, _i32, _p, _i64, _p, _p, _p], _c.c_int),
     "qds_cg_workspace": ([_i64], _i64),
     "qds_cg": ([_i64, _i64, _p, _p, _p, _p, _p, _d, _i32, _i32, _p, _i64, _c.POINTER(_i32),
                 _c.POINTER(_d), _p], _c.c_int),

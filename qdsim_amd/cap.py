@@ -1,0 +1,224 @@
+"""Complex-symmetric eigensolver for open systems with a complex absorbing potential.
+
+Problem: (K + M_V - i W) psi = E M psi with real symmetric K, M_V, W and SPD M:
+the pencil is complex symmetric (A^T = A, not Hermitian), the eigenvalues are
+E = E_r - i Gamma/2 and tau = hbar / (2 |Im E|) (README.md:104-110).  The
+reference never solves this problem: it adds |Im CAP| to the real potential,
+solves a real eigsh and subtracts a constant i 0.1 cap_strength afterwards
+(schrodinger_solver.pyx:160-164, 548-557; fixed_open_system_solver.pyx:321-328).
+
+Method: block preconditioned iteration on S = [X | W | P] with the bilinear
+(complex-symmetric) Rayleigh-Ritz projection S^T A S y = theta S^T M S y, the
+natural Galerkin condition for complex symmetric pencils.  The basis is first
+conditioned with the Hermitian Gram S^H M S (SVQB), and Ritz pairs are selected
+by smallest real part (or by distance to a target).  All n-sized work runs in
+qdsim_hip kernels on the interleaved (re, im) layout:
+  * fused CAP SpMM (qds_spmm_cap): (H - iW) X and M X from one pass over the
+    pattern with three real value arrays;
+  * every complex block is also a real (n, 2m) block, so the MFMA FP64 Gram of
+    the real views yields both the bilinear and the Hermitian Gram matrices, and
+    the complex update X S C is the real MFMA update with the 2x2-real form of C;
+  * the preconditioner is the real lattice multigrid applied to the real and
+    imaginary parts (a block of 2b real columns).
+complex64 storage for the bulk of the iterations, complex128 refinement.
+"""
+import numpy as np
+import torch
+
+from . import kernels
+from .units import EV_TO_J
+
+
+def _rv(T):
+    """Real (n, 2m) view of a complex (n, m) block."""
+    return torch.view_as_real(T).reshape(T.shape[0], 2 * T.shape[1])
+
+
+def _bilinear(G):
+    return torch.complex(G[0::2, 0::2] - G[1::2, 1::2], G[0::2, 1::2] + G[1::2, 0::2])
+
+
+def _hermitian(G):
+    return torch.complex(G[0::2, 0::2] + G[1::2, 1::2], G[0::2, 1::2] - G[1::2, 0::2])
+
+
+def _real_form(C):
+    """2x2-real form of a complex (m, k) matrix acting on interleaved columns."""
+    m, k = C.shape
+    R = torch.empty((2 * m, 2 * k), dtype=C.real.dtype, device=C.device)
+    R[0::2, 0::2] = C.real
+    R[1::2, 0::2] = -C.imag
+    R[0::2, 1::2] = C.imag
+    R[1::2, 1::2] = C.real
+    return R
+
+
+class CAPOperator:
+    def __init__(self, indptr, indices, n, H, W, M, cdtype):
+        rdt = torch.float32 if cdtype == torch.complex64 else torch.float64
+        self.indptr, self.indices, self.n = indptr, indices, int(n)
+        self.H = H.to(rdt).contiguous()
+        self.W = W.to(rdt).contiguous()
+        self.M = M.to(rdt).contiguous()
+        self.dtype = cdtype
+
+    def apply(self, X, HX, MX):
+        kernels.spmm_cap(self.indptr, self.indices, self.n, self.H, self.W, self.M, X, HX, MX)
+
+    def astype(self, cdtype):
+        return CAPOperator(self.indptr, self.indices, self.n, self.H, self.W, self.M, cdtype)
+
+
+def _rr(G, m, nb, drop_tol, target):
+    Ab = _bilinear(G[:, : 2 * m])
+    Mb = _bilinear(G[:, 2 * m:])
+    Mh = _hermitian(G[:, 2 * m:])
+    Mh = 0.5 * (Mh + Mh.conj().T)
+    d = torch.diagonal(Mh).real.clamp_min(1e-300).rsqrt()
+    Mhs = Mh * d[:, None] * d[None, :]
+    mu, Q = torch.linalg.eigh(Mhs)
+    n_ok = int((mu > drop_tol * mu[-1]).sum().item())
+    nk = max(n_ok, nb)
+    Bm = (d[:, None].to(Q.dtype) * Q[:, -nk:]) * mu[-nk:].clamp_min(1e-300).rsqrt()[None, :].to(Q.dtype)
+    Ah = Bm.T @ Ab @ Bm
+    Mhat = Bm.T @ Mb @ Bm
+    theta, Y = torch.linalg.eig(torch.linalg.solve(Mhat, Ah))
+    if target is None:
+        order = torch.argsort(theta.real)
+    else:
+        order = torch.argsort((theta - target).abs())
+    sel = order[:nb]
+    Ys = Y[:, sel]
+    nrm = torch.sqrt(torch.diagonal(Ys.T @ Mhat @ Ys))
+    Ys = Ys / nrm[None, :]
+    C = Bm @ Ys
+    th = theta[sel]
+    # present the block sorted by real part
+    o2 = torch.argsort(th.real)
+    return th[o2], C[:, o2]
+
+
+def cap_lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, seed=0, target=None,
+               drop_tol=None, stall_iters=None, verbose=False):
+    dev = op.H.device
+    dt = op.dtype
+    n = op.n
+    assert b % 2 == 0 and 2 * b <= 64 and 3 * b <= n
+    if drop_tol is None:
+        drop_tol = 1e-7 if dt == torch.complex64 else 1e-13
+    bufs = [dict(S=torch.zeros((n, 3 * b), dtype=dt, device=dev),
+                 HS=torch.zeros((n, 3 * b), dtype=dt, device=dev),
+                 MS=torch.zeros((n, 3 * b), dtype=dt, device=dev)) for _ in range(2)]
+    R = torch.empty((n, b), dtype=dt, device=dev)
+    cur, nxt = bufs
+    X = cur["S"][:, :b]
+    if X0 is None:
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        Xr = torch.rand((n, b), generator=g, dtype=torch.float64) - 0.5
+        X.copy_(Xr.to(dev).to(dt))
+        if precond is not None:
+            R.copy_(X)
+            precond(_rv(R), _rv(X))
+    else:
+        X.copy_(X0[:, :b].to(dt))
+    op.apply(X, cur["HS"][:, :b], cur["MS"][:, :b])
+    G = kernels.gram(_rv(X), _rv(cur["HS"][:, :b]), _rv(cur["MS"][:, :b]))
+    lam, C = _rr(G, b, b, drop_tol, target)
+    Cr = _real_form(C)
+    for key in ("S", "HS", "MS"):
+        kernels.block_update(_rv(cur[key][:, :b]), Cr, _rv(nxt[key][:, :b]))
+    cur, nxt = nxt, cur
+    have_p = False
+    history = []
+    converged = False
+    res = None
+    best, best_it = float("inf"), 0
+    it = 0
+    for it in range(1, maxiter + 1):
+        HX = cur["HS"][:, :b]
+        MX = cur["MS"][:, :b]
+        kernels.block_residual(HX, MX, lam.to(torch.complex128).contiguous(), R)
+        rn = kernels.coldot(R, R, conj=True).real
+        mn = kernels.coldot(MX, MX, conj=True).real
+        res = rn.clamp_min(0).sqrt() / (lam.abs() * mn.clamp_min(1e-300).sqrt()).clamp_min(1e-300)
+        worst = float(res[:k].max().item())
+        history.append(worst)
+        if verbose:
+            print(f"  cap[{str(dt)[6:]}] it {it:4d} worst rel. residual {worst:.3e}")
+        if worst < tol:
+            converged = True
+            break
+        if stall_iters is not None:
+            if worst < 0.7 * best:
+                best, best_it = worst, it
+            elif it - best_it >= stall_iters:
+                break
+        W = cur["S"][:, b:2 * b]
+        if precond is not None:
+            precond(_rv(R), _rv(W))
+        else:
+            W.copy_(R)
+        op.apply(W, cur["HS"][:, b:2 * b], cur["MS"][:, b:2 * b])
+        m = 3 * b if have_p else 2 * b
+        G = kernels.gram(_rv(cur["S"][:, :m]), _rv(cur["HS"][:, :m]), _rv(cur["MS"][:, :m]))
+        lam, C = _rr(G, m, b, drop_tol, target)
+        CP = C.clone()
+        CP[:b] = 0
+        Cr, CPr = _real_form(C), _real_form(CP)
+        for key in ("S", "HS", "MS"):
+            kernels.block_update(_rv(cur[key][:, :m]), Cr, _rv(nxt[key][:, :b]), CPr,
+                                 _rv(nxt[key][:, 2 * b:3 * b]))
+        cur, nxt = nxt, cur
+        have_p = True
+    torch.cuda.synchronize()
+    return lam, cur["S"][:, :b], res, it, converged, history
+
+
+def solve_cap(prob, k, b=None, target=None, tol=1e-8, tol32=3e-4, maxiter=600, seed=0, verbose=False):
+    """Lowest-real-part (or nearest-target, eV) eigenpairs of the CAP pencil of a
+    SchrodingerProblem whose CAP coefficient is set.  Returns (E complex eV (np),
+    eigenvectors (n_full, k) complex128 np, Hermitian M_nm-normalised)."""
+    if prob.H is None or prob.W is None:
+        prob.assemble()
+    if b is None:
+        b = min(32, max(8, ((k + 8) + 1) // 2 * 2))
+    H = kernels.csr_take(prob.r_src, prob.H)
+    W = kernels.csr_take(prob.r_src, prob.W)
+    M = kernels.csr_take(prob.r_src, prob.M)
+    op64 = CAPOperator(prob.r_indptr, prob.r_indices, prob.n_free, H, W, M, torch.complex128)
+    op32 = op64.astype(torch.complex64)
+    pre32 = prob.preconditioner(2 * b, torch.float32)
+
+    def pre64(Rr, Wr, _buf={}):
+        if "r" not in _buf:
+            _buf["r"] = torch.empty(Rr.shape, dtype=torch.float32, device=Rr.device)
+            _buf["w"] = torch.empty(Rr.shape, dtype=torch.float32, device=Rr.device)
+        _buf["r"].copy_(Rr)
+        pre32(_buf["r"], _buf["w"])
+        Wr.copy_(_buf["w"])
+
+    lam32, X32, _, it32, _, h32 = cap_lobpcg(op32, k, b, pre32, tol=tol32, maxiter=maxiter, seed=seed,
+                                            target=target, stall_iters=10, verbose=verbose)
+    lam, X, res, it64, conv, h64 = cap_lobpcg(op64, k, b, pre64, X0=X32.to(torch.complex128), tol=tol,
+                                             maxiter=maxiter, seed=seed, target=target, verbose=verbose)
+    if target is not None:
+        sel = torch.argsort((lam - target).abs())[:k]
+        sel = sel[torch.argsort(lam[sel].real)]
+    else:
+        sel = torch.arange(k, device=lam.device)
+    lam = lam[sel]
+    Xk = X[:, sel].to(torch.complex128)
+    # Hermitian M-normalisation with the phase making psi^T M psi real positive
+    MX = torch.empty_like(Xk)
+    HX = torch.empty_like(Xk)
+    op64.apply(Xk.contiguous(), HX, MX)
+    bil = (Xk * MX).sum(0)
+    Xk = Xk * torch.exp(-0.5j * torch.angle(bil))[None, :]
+    herm = (Xk.conj() * MX * torch.exp(-0.5j * torch.angle(bil))[None, :]).sum(0).real
+    Xk = Xk / herm.clamp_min(1e-300).sqrt()[None, :]
+    full = torch.zeros((prob.space.n, k), dtype=torch.complex128, device=Xk.device)
+    full[prob.old_of_new] = Xk
+    prob.last_cap_info = {"fp32_iters": it32, "fp64_iters": it64, "converged": conv,
+                          "residuals": res[sel].cpu().numpy() if res is not None else None,
+                          "history": {"c64": h32, "c128": h64}}
+    return lam.cpu().numpy(), full.cpu().numpy()

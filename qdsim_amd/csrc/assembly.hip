@@ -239,3 +239,69 @@ QDS_API int qds_csr_take(int32_t dtype, int64_t n, const int32_t* src, const dou
   QDS_LAUNCH_CHECK();
   return QDS_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Quadrature-point fields used by the self-consistent loop and the adjoint.
+namespace {
+
+// u_q[e, q] = sum_a phi_a(q) u[conn[e, a]]  (FE interpolation of a nodal field)
+__global__ void interp_q_kernel(int npe, int nq, int64_t n_elem, const int32_t* __restrict__ conn,
+                                const double* __restrict__ phi, const double* __restrict__ u,
+                                double* __restrict__ out) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_elem * nq;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e = t / nq;
+    const int q = (int)(t - e * nq);
+    double s = 0.0;
+    for (int a = 0; a < npe; ++a) s += phi[q * npe + a] * u[conn[e * npe + a]];
+    out[t] = s;
+  }
+}
+
+// rho[e, q, s] = w_q |det J_e| / 2 * psi_s(x_q)^2, psi_s = sum_a phi_a(q) Psi[conn[e,a], s]
+// (the Hellmann-Feynman weight: dE_s / dV(x_q) for M-normalised real eigenvectors)
+__global__ void state_density_q_kernel(int npe, int nq, int64_t n_elem, const int32_t* __restrict__ conn,
+                                       const double* __restrict__ x, const double* __restrict__ y,
+                                       const double* __restrict__ tables, const double* __restrict__ psi,
+                                       int64_t ldpsi, int k, double* __restrict__ out) {
+  const double* w = tables;
+  const double* phi = tables + nq;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_elem * nq * k;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t eq = t / k;
+    const int s = (int)(t - eq * k);
+    const int64_t e = eq / nq;
+    const int q = (int)(eq - e * nq);
+    const int32_t* ce = conn + e * npe;
+    const double x0 = x[ce[0]], y0 = y[ce[0]];
+    const double det = (x[ce[1]] - x0) * (y[ce[2]] - y0) - (x[ce[2]] - x0) * (y[ce[1]] - y0);
+    double v = 0.0;
+    for (int a = 0; a < npe; ++a) v += phi[q * npe + a] * psi[(int64_t)ce[a] * ldpsi + s];
+    out[t] = w[q] * 0.5 * fabs(det) * v * v;
+  }
+}
+
+}  // namespace
+
+QDS_API int qds_interpolate_q(int32_t npe, int32_t nq, int64_t n_elem, const int32_t* conn,
+                              const double* phi, const double* u, double* out, void* stream) {
+  QDS_CHECK_ARG(conn && phi && u && out, "qds_interpolate_q: null");
+  if (n_elem == 0) return QDS_OK;
+  hipLaunchKernelGGL(interp_q_kernel, dim3(qds_blocks(n_elem * nq, kBlock, 131072)), dim3(kBlock), 0,
+                     qds_stream(stream), npe, nq, n_elem, conn, phi, u, out);
+  QDS_LAUNCH_CHECK();
+  return QDS_OK;
+}
+
+QDS_API int qds_state_density_q(int32_t npe, int32_t nq, int64_t n_elem, const int32_t* conn,
+                                const double* x, const double* y, const double* tables,
+                                const double* psi, int64_t ldpsi, int32_t k, double* out, void* stream) {
+  QDS_CHECK_ARG(conn && x && y && tables && psi && out && k >= 1 && ldpsi >= k,
+                "qds_state_density_q: bad arguments");
+  if (n_elem == 0) return QDS_OK;
+  hipLaunchKernelGGL(state_density_q_kernel, dim3(qds_blocks(n_elem * nq * k, kBlock, 131072)),
+                     dim3(kBlock), 0, qds_stream(stream), npe, nq, n_elem, conn, x, y, tables, psi,
+                     ldpsi, k, out);
+  QDS_LAUNCH_CHECK();
+  return QDS_OK;
+}

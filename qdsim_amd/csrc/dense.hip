@@ -44,26 +44,30 @@ __device__ __forceinline__ bool tile_needed(const GramShape& g, int ti, int tj) 
   return ti <= tl;
 }
 
+// grid: (row slabs, output column groups of tpg tiles)
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, const T* __restrict__ A, int64_t lda,
+gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, const T* __restrict__ A, int64_t lda,
              const T* __restrict__ B1, int64_t ldb1, const T* __restrict__ B2, int64_t ldb2,
              double* __restrict__ partial) {
   extern __shared__ double smem[];
+  const int tk = g.tk1 + g.tk2;
+  const int tj0 = blockIdx.y * tpg;
+  const int tj1 = min(tk, tj0 + tpg);
+  const int cw = (tj1 - tj0) * 16;   // B columns staged by this block
   double* sA = smem;                 // kRB x mpad
-  double* sB = smem + kRB * g.mpad;  // kRB x kpad
+  double* sB = smem + kRB * g.mpad;  // kRB x cw
   __shared__ int s_tile[kWaves * kMaxTiles];
   __shared__ int s_count[kWaves];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int tk = g.tk1 + g.tk2;
-  const int ntiles = g.tm * tk;
+  const int ntl = g.tm * (tj1 - tj0);
   if (lane == 0) {
     // tiles are dealt round-robin over the waves; slot lists keep the
     // accumulator index a compile-time constant in the MFMA loop
     int c = 0;
-    for (int t = wave; t < ntiles; t += kWaves) {
-      const int ti = t / tk, tj = t - ti * tk;
+    for (int t = wave; t < ntl; t += kWaves) {
+      const int ti = t / (tj1 - tj0), tj = tj0 + (t - ti * (tj1 - tj0));
       if (!tile_needed(g, ti, tj)) continue;
       if (c < kMaxTiles) s_tile[wave * kMaxTiles + c] = (ti << 8) | tj;
       ++c;
@@ -86,8 +90,8 @@ gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, const T* __restrict__
       const int64_t row = r0 + rr;
       sA[t] = (row < r_end && c < g.m) ? (double)A[row * lda + c] : 0.0;
     }
-    for (int t = threadIdx.x; t < kRB * g.kpad; t += kBlock) {
-      const int rr = t / g.kpad, c = t - rr * g.kpad;
+    for (int t = threadIdx.x; t < kRB * cw; t += kBlock) {
+      const int rr = t / cw, c = tj0 * 16 + (t - rr * cw);
       const int64_t row = r0 + rr;
       double v = 0.0;
       if (row < r_end) {
@@ -110,7 +114,7 @@ gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, const T* __restrict__
           const int code = s_tile[wave * kMaxTiles + slot];
           const int ti = code >> 8, tj = code & 255;
           const double a = sA[row * g.mpad + ti * 16 + (lane & 15)];
-          const double b = sB[row * g.kpad + tj * 16 + (lane & 15)];
+          const double b = sB[row * cw + (tj - tj0) * 16 + (lane & 15)];
           acc[slot] = mfma_f64(a, b, acc[slot]);
         }
       }
@@ -172,7 +176,11 @@ template <> struct MT<double> {
 };
 
 constexpr int kMaxOutTiles = 4;  // k <= 64
+constexpr int kRowTiles = 2;     // 16-row tiles per wave and pass
+constexpr int kKC = 32;          // rows of C staged in LDS per chunk
 
+// One pass of a block covers kWaves * kRowTiles * 16 rows; C is staged through LDS
+// kKC rows at a time, so m is unbounded.
 template <typename T, int NOUT>
 __global__ void __launch_bounds__(kBlock)
 block_update_kernel(int64_t n, int m, int k, const T* __restrict__ S, int64_t lds,
@@ -180,73 +188,83 @@ block_update_kernel(int64_t n, int m, int k, const T* __restrict__ S, int64_t ld
                     int64_t ldy1, T* __restrict__ Y2, int64_t ldy2) {
   using M = MT<T>;
   constexpr int VEC = M::VEC;
-  constexpr int KB = 4 * VEC;  // contraction columns covered by one vector load per lane
-  extern __shared__ unsigned char smem_raw[];
-  T* sC = reinterpret_cast<T*>(smem_raw);
-  const int nkb = (m + KB - 1) / KB;
-  const int mp = nkb * KB;
+  constexpr int KB = 4 * VEC;  // contraction indices covered by one vector load per lane
+  __shared__ T sC[NOUT * kKC * 16 * kMaxOutTiles];
   const int kt = (k + 15) / 16;
   const int kp = kt * 16;
-  for (int t = threadIdx.x; t < NOUT * mp * kp; t += kBlock) {
-    const int o = t / (mp * kp);
-    const int rem = t - o * mp * kp;
-    const int i = rem / kp, j = rem - i * kp;
-    const T* C = o == 0 ? C1 : C2;
-    sC[t] = (i < m && j < k) ? C[i * k + j] : T(0);
-  }
-  __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int64_t tiles = (n + 15) / 16;
   const int q = lane >> 4;      // 0..3
   const int rl = lane & 15;
-  for (int64_t tile = (int64_t)blockIdx.x * kWaves + wave; tile < tiles;
-       tile += (int64_t)gridDim.x * kWaves) {
-    const int64_t r0 = tile * 16;
-    const int64_t row = r0 + rl;
-    typename M::acc_t acc[NOUT][kMaxOutTiles];
+  const int64_t rows_per_pass = (int64_t)kWaves * kRowTiles * 16;
+  for (int64_t base = (int64_t)blockIdx.x * rows_per_pass; base < n;
+       base += (int64_t)gridDim.x * rows_per_pass) {
+    typename M::acc_t acc[NOUT][kRowTiles][kMaxOutTiles];
 #pragma unroll
     for (int o = 0; o < NOUT; ++o)
 #pragma unroll
-      for (int t = 0; t < kMaxOutTiles; ++t) acc[o][t] = typename M::acc_t{0, 0, 0, 0};
-    for (int cb = 0; cb < nkb; ++cb) {
-      const int kbase = cb * KB + q * VEC;
-      T sv[VEC];
-      if (row < n && kbase < m) {
-        typename M::vec_t v = *reinterpret_cast<const typename M::vec_t*>(S + row * lds + kbase);
-        const T* vp = reinterpret_cast<const T*>(&v);
+      for (int r = 0; r < kRowTiles; ++r)
 #pragma unroll
-        for (int s = 0; s < VEC; ++s) sv[s] = vp[s];
-      } else {
-#pragma unroll
-        for (int s = 0; s < VEC; ++s) sv[s] = T(0);
+        for (int t = 0; t < kMaxOutTiles; ++t) acc[o][r][t] = typename M::acc_t{0, 0, 0, 0};
+    for (int kc = 0; kc < m; kc += kKC) {
+      __syncthreads();
+      for (int t = threadIdx.x; t < NOUT * kKC * kp; t += kBlock) {
+        const int o = t / (kKC * kp);
+        const int rem = t - o * kKC * kp;
+        const int i = rem / kp, j = rem - i * kp;
+        const T* C = o == 0 ? C1 : C2;
+        const int gi = kc + i;
+        sC[t] = (gi < m && j < k) ? C[gi * k + j] : T(0);
       }
+      __syncthreads();
 #pragma unroll
-      for (int s = 0; s < VEC; ++s) {
-        const int kidx = kbase + s;
+      for (int r = 0; r < kRowTiles; ++r) {
+        const int64_t row = base + ((int64_t)wave * kRowTiles + r) * 16 + rl;
 #pragma unroll
-        for (int t = 0; t < kMaxOutTiles; ++t) {
-          if (t < kt) {
+        for (int cb = 0; cb < kKC / KB; ++cb) {
+          const int kloc = cb * KB + q * VEC;
+          const int kbase = kc + kloc;
+          T sv[VEC];
+          if (row < n && kbase < m) {
+            typename M::vec_t v = *reinterpret_cast<const typename M::vec_t*>(S + row * lds + kbase);
+            const T* vp = reinterpret_cast<const T*>(&v);
 #pragma unroll
-            for (int o = 0; o < NOUT; ++o) {
-              const T bv = sC[o * mp * kp + kidx * kp + t * 16 + rl];
-              acc[o][t] = M::mma(sv[s], bv, acc[o][t]);
+            for (int e = 0; e < VEC; ++e) sv[e] = vp[e];
+          } else {
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) sv[e] = T(0);
+          }
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) {
+#pragma unroll
+            for (int t = 0; t < kMaxOutTiles; ++t) {
+              if (t < kt) {
+#pragma unroll
+                for (int o = 0; o < NOUT; ++o) {
+                  const T bv = sC[o * kKC * kp + (kloc + e) * kp + t * 16 + rl];
+                  acc[o][r][t] = M::mma(sv[e], bv, acc[o][r][t]);
+                }
+              }
             }
           }
         }
       }
     }
 #pragma unroll
-    for (int t = 0; t < kMaxOutTiles; ++t) {
-      if (t >= kt) continue;
-      const int col = t * 16 + rl;
-      if (col >= k) continue;
+    for (int r = 0; r < kRowTiles; ++r) {
+      const int64_t r0 = base + ((int64_t)wave * kRowTiles + r) * 16;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t orow = r0 + 4 * q + r;
-        if (orow < n) {
-          Y1[orow * ldy1 + col] = acc[0][t][r];
-          if (NOUT == 2) Y2[orow * ldy2 + col] = acc[NOUT - 1][t][r];
+      for (int t = 0; t < kMaxOutTiles; ++t) {
+        if (t >= kt) continue;
+        const int col = t * 16 + rl;
+        if (col >= k) continue;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int64_t orow = r0 + 4 * q + e;
+          if (orow < n) {
+            Y1[orow * ldy1 + col] = acc[0][r][t][e];
+            if (NOUT == 2) Y2[orow * ldy2 + col] = acc[NOUT - 1][r][t][e];
+          }
         }
       }
     }
@@ -370,38 +388,42 @@ QDS_API int qds_gram(int32_t dtype, int64_t n, int32_t m, const void* A, int64_t
   QDS_CHECK_ARG(m > 0 && k1 > 0 && k2 >= 0 && (k2 == 0 || B2), "qds_gram: bad shape");
   QDS_CHECK_ARG(!sym || (m == k1 && (k2 == 0 || k2 == m)), "qds_gram: sym needs square halves");
   GramShape g = make_shape(m, k1, k2, sym);
-  int ntiles = 0;
-  for (int ti = 0; ti < g.tm; ++ti)
-    for (int tj = 0; tj < g.tk1 + g.tk2; ++tj) {
-      const int tl = tj < g.tk1 ? tj : tj - g.tk1;
-      if (!sym || ti <= tl) ++ntiles;
+  QDS_CHECK_ARG(g.tm <= 255 && g.tk1 + g.tk2 <= 255, "qds_gram: block too wide");
+  const int tk = g.tk1 + g.tk2;
+  // column groups: every wave holds at most kMaxTiles accumulators and the staged
+  // rows fit in 64 KiB of LDS
+  int tpg = tk;
+  auto slots_ok = [&](int tp) {
+    for (int j0 = 0; j0 < tk; j0 += tp) {
+      const int j1 = j0 + tp < tk ? j0 + tp : tk;
+      for (int w = 0; w < kWaves; ++w) {
+        int c = 0;
+        const int nt = g.tm * (j1 - j0);
+        for (int t = w; t < nt; t += kWaves) {
+          const int ti = t / (j1 - j0), tj = j0 + (t - ti * (j1 - j0));
+          const int tl = tj < g.tk1 ? tj : tj - g.tk1;
+          if (!sym || ti <= tl) ++c;
+        }
+        if (c > kMaxTiles) return false;
+      }
     }
-  // per-wave slot count: tiles are dealt round-robin over the 4 waves of a block
-  int max_slots = 0;
-  for (int w = 0; w < kWaves; ++w) {
-    int s = 0, tk = g.tk1 + g.tk2;
-    for (int t = w; t < g.tm * tk; t += kWaves) {
-      const int ti = t / tk, tj = t - ti * tk;
-      const int tl = tj < g.tk1 ? tj : tj - g.tk1;
-      if (!sym || ti <= tl) ++s;
-    }
-    max_slots = s > max_slots ? s : max_slots;
-  }
-  QDS_CHECK_ARG(max_slots <= kMaxTiles, "qds_gram: %d tiles per wave exceed %d (m=%d k=%d)",
-                max_slots, kMaxTiles, m, k1 + k2);
+    return (size_t)kRB * (g.mpad + 16 * tp) * sizeof(double) <= 64 * 1024;
+  };
+  while (tpg > 1 && !slots_ok(tpg)) --tpg;
+  QDS_CHECK_ARG(slots_ok(tpg), "qds_gram: A block of %d columns too wide", m);
+  const int ngroups = (tk + tpg - 1) / tpg;
   const int nblk = gram_blocks(n);
   QDS_CHECK_ARG(ws_bytes >= qds_gram_workspace(n, m, k1, k2), "qds_gram: workspace too small");
   const int64_t rows_per_blk = (((n + nblk - 1) / nblk) + kRB - 1) / kRB * kRB;
-  const size_t shm = (size_t)kRB * (g.mpad + g.kpad) * sizeof(double);
-  QDS_CHECK_ARG(shm <= 64 * 1024, "qds_gram: block too wide for LDS staging");
+  const size_t shm = (size_t)kRB * (g.mpad + 16 * tpg) * sizeof(double);
   hipStream_t st = qds_stream(stream);
   double* partial = static_cast<double*>(workspace);
   if (dtype == qds::F32)
-    hipLaunchKernelGGL(gram_partial<float>, dim3(nblk), dim3(kBlock), shm, st, n, rows_per_blk, g,
-                       (const float*)A, lda, (const float*)B1, ldb1, (const float*)B2, ldb2, partial);
+    hipLaunchKernelGGL(gram_partial<float>, dim3(nblk, ngroups), dim3(kBlock), shm, st, n, rows_per_blk, g,
+                       tpg, (const float*)A, lda, (const float*)B1, ldb1, (const float*)B2, ldb2, partial);
   else if (dtype == qds::F64)
-    hipLaunchKernelGGL(gram_partial<double>, dim3(nblk), dim3(kBlock), shm, st, n, rows_per_blk, g,
-                       (const double*)A, lda, (const double*)B1, ldb1, (const double*)B2, ldb2, partial);
+    hipLaunchKernelGGL(gram_partial<double>, dim3(nblk, ngroups), dim3(kBlock), shm, st, n, rows_per_blk, g,
+                       tpg, (const double*)A, lda, (const double*)B1, ldb1, (const double*)B2, ldb2, partial);
   else
     QDS_CHECK_ARG(false, "qds_gram: dtype must be F32 or F64");
   QDS_LAUNCH_CHECK();
@@ -421,36 +443,28 @@ QDS_API int qds_block_update(int32_t dtype, int64_t n, int32_t m, int32_t k, con
   QDS_CHECK_ARG(k >= 1 && k <= 16 * kMaxOutTiles && m >= 1, "qds_block_update: bad shape");
   if (n == 0) return QDS_OK;
   hipStream_t st = qds_stream(stream);
-  const int64_t tiles = (n + 15) / 16;
-  const unsigned blocks = qds_blocks(tiles, kWaves, 8192);
-  const int kp = (k + 15) / 16 * 16;
-  const int nout = C2 ? 2 : 1;
+  const int64_t passes = (n + kWaves * kRowTiles * 16 - 1) / (kWaves * kRowTiles * 16);
+  const unsigned blocks = qds_blocks(passes, 1, 4096);
   if (dtype == qds::F32) {
     QDS_CHECK_ARG(m % 4 == 0 && lds % 4 == 0 && (reinterpret_cast<uintptr_t>(S) % 16) == 0,
                   "qds_block_update: F32 needs m, lds multiple of 4 and 16-byte aligned S");
-    const int mp = (m + 15) / 16 * 16;
-    const size_t shm = (size_t)nout * mp * kp * sizeof(float);
-    QDS_CHECK_ARG(shm <= 64 * 1024, "qds_block_update: C too large");
     if (C2)
-      hipLaunchKernelGGL((block_update_kernel<float, 2>), dim3(blocks), dim3(kBlock), shm, st, n, m, k,
+      hipLaunchKernelGGL((block_update_kernel<float, 2>), dim3(blocks), dim3(kBlock), 0, st, n, m, k,
                          (const float*)S, lds, (const float*)C1, (const float*)C2, (float*)Y1, ldy1,
                          (float*)Y2, ldy2);
     else
-      hipLaunchKernelGGL((block_update_kernel<float, 1>), dim3(blocks), dim3(kBlock), shm, st, n, m, k,
+      hipLaunchKernelGGL((block_update_kernel<float, 1>), dim3(blocks), dim3(kBlock), 0, st, n, m, k,
                          (const float*)S, lds, (const float*)C1, (const float*)nullptr, (float*)Y1,
                          ldy1, (float*)nullptr, ldy2);
   } else if (dtype == qds::F64) {
     QDS_CHECK_ARG(m % 2 == 0 && lds % 2 == 0 && (reinterpret_cast<uintptr_t>(S) % 16) == 0,
                   "qds_block_update: F64 needs m, lds even and 16-byte aligned S");
-    const int mp = (m + 7) / 8 * 8;
-    const size_t shm = (size_t)nout * mp * kp * sizeof(double);
-    QDS_CHECK_ARG(shm <= 64 * 1024, "qds_block_update: C too large");
     if (C2)
-      hipLaunchKernelGGL((block_update_kernel<double, 2>), dim3(blocks), dim3(kBlock), shm, st, n, m, k,
+      hipLaunchKernelGGL((block_update_kernel<double, 2>), dim3(blocks), dim3(kBlock), 0, st, n, m, k,
                          (const double*)S, lds, (const double*)C1, (const double*)C2, (double*)Y1,
                          ldy1, (double*)Y2, ldy2);
     else
-      hipLaunchKernelGGL((block_update_kernel<double, 1>), dim3(blocks), dim3(kBlock), shm, st, n, m, k,
+      hipLaunchKernelGGL((block_update_kernel<double, 1>), dim3(blocks), dim3(kBlock), 0, st, n, m, k,
                          (const double*)S, lds, (const double*)C1, (const double*)nullptr,
                          (double*)Y1, ldy1, (double*)nullptr, ldy2);
   } else {
@@ -511,6 +525,36 @@ QDS_API int qds_coldot(int32_t dtype, int64_t n, int32_t b, const void* X, int64
   }
   QDS_LAUNCH_CHECK();
   hipLaunchKernelGGL(coldot_reduce, dim3(1), dim3(256), 0, st, (int)nblk, b, part, out, cplx ? 1 : 0);
+  QDS_LAUNCH_CHECK();
+  return QDS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Electron density of the self-consistent loop.
+// Reference: fusedElectronDensity (backend/src/fused_gpu_kernels.cu:127-157, unused)
+// and the per-node loop of self_consistent_solver.pyx:262-274:
+//   n[i] = sum_s occ[s] * |psi[i, s]|^2
+namespace {
+__global__ void density_kernel(int64_t n, int k, const double* __restrict__ psi, int64_t ld,
+                               const double* __restrict__ occ, double* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int j = 0; j < k; ++j) {
+      const double v = psi[i * ld + j];
+      s += occ[j] * v * v;
+    }
+    out[i] = s;
+  }
+}
+}  // namespace
+
+QDS_API int qds_density(int64_t n, int32_t k, const double* psi, int64_t ld, const double* occ,
+                        double* out, void* stream) {
+  QDS_CHECK_ARG(psi && occ && out && k >= 0 && ld >= k, "qds_density: bad arguments");
+  if (n == 0) return QDS_OK;
+  hipLaunchKernelGGL(density_kernel, dim3(qds_blocks(n, 256, 65536)), dim3(256), 0, qds_stream(stream),
+                     n, k, psi, ld, occ, out);
   QDS_LAUNCH_CHECK();
   return QDS_OK;
 }

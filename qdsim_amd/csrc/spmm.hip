@@ -16,6 +16,8 @@
 // (no conjugation), as needed for the CAP Hamiltonian H - iW.
 #include "qds_common.h"
 
+#include <type_traits>
+
 namespace {
 
 constexpr int kBlock = 256;
@@ -289,4 +291,80 @@ QDS_API int qds_spmm(int32_t dtype, int64_t n_rows, int64_t nnz, int32_t b, cons
       QDS_CHECK_ARG(false, "qds_spmm: bad dtype %d", dtype);
   }
   return QDS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// CAP Hamiltonian product on complex blocks with real value arrays sharing one
+// pattern: Y_H = (H - i W) X, Y_M = M X  (complex-symmetric, no conjugation).
+// Reference: the CAP enters as V + i eta in backend/src/fem.cpp:942,974 (and is
+// faked by a constant imaginary shift in the Cython solvers,
+// schrodinger_solver.pyx:548-557); here -i W is applied exactly.
+namespace {
+template <typename T, int VEC>
+__global__ void __launch_bounds__(kBlock)
+spmm_cap_kernel(int64_t n, int b, int lpr, const int32_t* __restrict__ indptr,
+                const int32_t* __restrict__ indices, const T* __restrict__ vh, const T* __restrict__ vw,
+                const T* __restrict__ vm, const T* __restrict__ X, int64_t ldx, T* __restrict__ YH,
+                int64_t ldh, T* __restrict__ YM, int64_t ldm) {
+  constexpr int R = 2 * VEC;
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int li = (int)(gtid % lpr);
+  const int64_t groups = (int64_t)gridDim.x * blockDim.x / lpr;
+  const int c0 = li * VEC;
+  if (c0 >= b) return;
+  for (int64_t r = gtid / lpr; r < n; r += groups) {
+    T ah[R], am[R];
+#pragma unroll
+    for (int v = 0; v < R; ++v) { ah[v] = T(0); am[v] = T(0); }
+    const int32_t ke = indptr[r + 1];
+    for (int32_t k = indptr[r]; k < ke; ++k) {
+      const int32_t c = indices[k];
+      const T h = vh[k], w = vw[k], m = vm[k];
+      T x[R];
+      load_vec<T, R>(X + 2 * ((int64_t)c * ldx + c0), x);
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        const T xr = x[2 * v], xi = x[2 * v + 1];
+        ah[2 * v] = fma(h, xr, fma(w, xi, ah[2 * v]));          // (h - iw)(xr + i xi)
+        ah[2 * v + 1] = fma(h, xi, fma(-w, xr, ah[2 * v + 1]));
+        am[2 * v] = fma(m, xr, am[2 * v]);
+        am[2 * v + 1] = fma(m, xi, am[2 * v + 1]);
+      }
+    }
+    store_vec<T, R>(YH + 2 * (r * ldh + c0), ah);
+    store_vec<T, R>(YM + 2 * (r * ldm + c0), am);
+  }
+}
+}  // namespace
+
+// dtype C64 (real arrays FP32) or C128 (FP64); X, Y complex interleaved blocks.
+QDS_API int qds_spmm_cap(int32_t dtype, int64_t n_rows, int32_t b, const int32_t* indptr,
+                         const int32_t* indices, const void* vals_h, const void* vals_w,
+                         const void* vals_m, const void* X, int64_t ldx, void* Y_h, int64_t ldy_h,
+                         void* Y_m, int64_t ldy_m, void* stream) {
+  QDS_CHECK_ARG(indptr && indices && vals_h && vals_w && vals_m && X && Y_h && Y_m && b >= 1,
+                "qds_spmm_cap: bad arguments");
+  if (n_rows == 0) return QDS_OK;
+  hipStream_t st = qds_stream(stream);
+  const int es = qds::dtype_size(dtype);
+  const bool v2 = (b % 2 == 0) && aligned(X, 16) && aligned(Y_h, 16) && aligned(Y_m, 16) &&
+                  (ldx * es) % 16 == 0 && (ldy_h * es) % 16 == 0 && (ldy_m * es) % 16 == 0;
+  auto launch = [&](auto tag, auto vec) -> int {
+    using T = decltype(tag);
+    constexpr int VEC = decltype(vec)::value;
+    int lpr = 1;
+    while (lpr * VEC < b) lpr <<= 1;
+    QDS_CHECK_ARG(lpr <= 64, "qds_spmm_cap: block width %d too large", b);
+    hipLaunchKernelGGL((spmm_cap_kernel<T, VEC>), dim3(qds_blocks(n_rows, kBlock / lpr, 262144)),
+                       dim3(kBlock), 0, st, n_rows, b, lpr, indptr, indices, (const T*)vals_h,
+                       (const T*)vals_w, (const T*)vals_m, (const T*)X, ldx, (T*)Y_h, ldy_h, (T*)Y_m, ldy_m);
+    QDS_LAUNCH_CHECK();
+    return QDS_OK;
+  };
+  if (dtype == qds::C64)
+    return v2 ? launch(float(0), std::integral_constant<int, 2>()) : launch(float(0), std::integral_constant<int, 1>());
+  if (dtype == qds::C128)
+    return launch(double(0), std::integral_constant<int, 1>());
+  QDS_CHECK_ARG(false, "qds_spmm_cap: dtype must be C64 or C128");
+  return QDS_EINVAL;
 }

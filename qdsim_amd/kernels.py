@@ -128,6 +128,13 @@ def spmm(indptr, indices, n_rows, nnz, vals_a, X, Ya, vals_b=None, Yb=None):
                        stream_ptr()), "qds_spmm")
 
 
+def spmm_cap(indptr, indices, n_rows, h, w, m, X, YH, YM):
+    """YH = (H - iW) X, YM = M X for complex blocks, real value arrays."""
+    b = _width(X)
+    check(lib.qds_spmm_cap(code(X), n_rows, b, ptr(indptr), ptr(indices), ptr(h), ptr(w), ptr(m),
+                           ptr(X), _ld(X), ptr(YH), _ld(YH), ptr(YM), _ld(YM), stream_ptr()), "qds_spmm_cap")
+
+
 def gram(A, B1, B2=None, sym=False, out=None):
     """A^T [B1 | B2] in FP64, shape (m, k1 + k2)."""
     n, m = A.shape

@@ -67,12 +67,14 @@ def _svqb_rr(G, m, nb, drop_tol):
     GMs = GM * d[:, None] * d[None, :]
     GHs = GH * d[:, None] * d[None, :]
     mu, Q = torch.linalg.eigh(GMs)
-    keep = mu > drop_tol * mu[-1]
-    Qk = Q[:, keep] * mu[keep].rsqrt()[None, :]
+    # keep well-conditioned directions, but never fewer than nb (eigh sorts ascending)
+    n_ok = int((mu > drop_tol * mu[-1]).sum().item())
+    nk = max(n_ok, nb)
+    mu_k = mu[-nk:].clamp_min(1e-300)
+    Qk = Q[:, -nk:] * mu_k.rsqrt()[None, :]
     Hk = Qk.T @ GHs @ Qk
     Hk = 0.5 * (Hk + Hk.T)
     theta, Y = torch.linalg.eigh(Hk)
-    nk = int(keep.sum().item()) if isinstance(keep, torch.Tensor) else int(keep.sum())
     C = (d[:, None] * (Qk @ Y[:, :nb]))
     return theta[:nb], C, nk
 

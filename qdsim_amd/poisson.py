@@ -1,0 +1,150 @@
+"""Poisson equation on the GPU: assembly + Dirichlet lifting + FP64 CG.
+
+Call surface of CythonPoissonSolver (qdsim_cython/solvers/poisson_solver.pyx:31-340):
+``PoissonSolver(mesh, epsilon_r_func, rho_func)``, ``solve(V_p, V_n, n=None, p=None)``,
+``get_potential()``, public ``stiffness_matrix``/``mass_matrix``.  Discretisation as
+in the reference: stiffness with epsilon = eps_r(centroid) EPSILON_0 (:130-140),
+consistent mass, right-hand side M (-rho_nodal) (:254-274; the reference's sign
+convention is kept), Dirichlet V_p on x = 0 and V_n on x = Lx (:276-306), natural
+boundary elsewhere.  The reference factorises the row-replaced matrix with spsolve
+(:241); here the Dirichlet values are lifted into the right-hand side, the
+remaining SPD system is solved with the device CG (qds_cg), and the solution of
+the row-replaced system is identical in exact arithmetic.
+"""
+import time
+
+import numpy as np
+import torch
+
+from . import kernels
+from .coefficients import evaluate
+from .fem import FEMSpace, NM
+from .units import EPSILON_0, Q_E
+
+
+class PoissonProblem:
+    """Device-resident Poisson operator of one mesh (values in SI)."""
+
+    def __init__(self, mesh, epsilon_r_func, sampling="centroid"):
+        self.mesh = mesh
+        self.space = FEMSpace(mesh)
+        sp_ = self.space
+        where = "centroid" if sampling == "centroid" else "quadrature"
+        eps = sp_.sample(epsilon_r_func, where, EPSILON_0)
+        K, M, _ = sp_.assemble(c=eps, stiffness_mass=True, mass=True)
+        self.K = K                       # SI (scale free in 2-D)
+        self.M = M * (NM * NM)           # m^2
+        ry, rx = mesh.lattice_shape
+        lx = np.asarray(mesh.lattice_x)
+        conn = mesh.get_elements()
+        used = np.zeros(mesh.num_nodes, dtype=bool)
+        used[conn.ravel()] = True
+        self.left = np.nonzero((lx == 0) & used)[0]
+        self.right = np.nonzero((lx == rx - 1) & used)[0]
+        fixed = np.zeros(mesh.num_nodes, dtype=bool)
+        fixed[self.left] = True
+        fixed[self.right] = True
+        free = np.nonzero(~fixed & used)[0]
+        self.free = free
+        new = np.full(mesh.num_nodes, -1, dtype=np.int64)
+        new[free] = np.arange(free.size)
+        r_ip, r_ix, r_src = kernels.pattern_restrict(sp_.indptr_host, sp_.indices_host, new, free.size)
+        dev = sp_.dev
+        self.r_indptr = torch.from_numpy(r_ip).to(dev)
+        self.r_indices = torch.from_numpy(r_ix).to(dev)
+        self.K_ff = kernels.csr_take(torch.from_numpy(r_src).to(dev), K)
+        self.free_d = torch.from_numpy(free).to(dev)
+        self.left_d = torch.from_numpy(self.left).to(dev)
+        self.right_d = torch.from_numpy(self.right).to(dev)
+        self.n = mesh.num_nodes
+        self.last_iterations = 0
+        self.last_relres = 0.0
+
+    def spmv(self, vals, x):
+        y = torch.empty_like(x)
+        kernels.spmm(self.space.indptr, self.space.indices, self.n, self.space.nnz, vals, x, y)
+        return y
+
+    def solve(self, rho_nodal, V_p, V_n, x0=None, rtol=1e-12):
+        """phi (device float64, SI volts) for nodal charge density rho (C/m^3)."""
+        rhs = self.spmv(self.M, -rho_nodal.contiguous())
+        g = torch.zeros(self.n, dtype=torch.float64, device=rhs.device)
+        g[self.left_d] = V_p
+        g[self.right_d] = V_n
+        t = self.spmv(self.K, g)
+        b = (rhs - t)[self.free_d].contiguous()
+        x = torch.zeros_like(b) if x0 is None else x0[self.free_d].contiguous()
+        it, rr = kernels.cg(self.r_indptr, self.r_indices, self.K_ff, self.free.size,
+                            int(self.r_indices.numel()), b, x, rtol=rtol, max_iter=20000)
+        self.last_iterations, self.last_relres = it, rr
+        phi = g.clone()
+        phi[self.free_d] = x
+        return phi
+
+
+class PoissonSolver:
+    """Drop-in for CythonPoissonSolver (poisson_solver.pyx:31)."""
+
+    def __init__(self, mesh, epsilon_r_func, rho_func):
+        self.num_nodes = mesh.num_nodes
+        self.num_elements = mesh.num_elements
+        self.Lx, self.Ly = mesh.Lx, mesh.Ly
+        self.nx, self.ny = mesh.nx, mesh.ny
+        self.mesh = mesh
+        self.nodes_x, self.nodes_y = mesh.get_nodes()
+        self.epsilon_r_func = epsilon_r_func
+        self.rho_func = rho_func
+        self.problem = PoissonProblem(mesh, epsilon_r_func)
+        self.stiffness_matrix = self.problem.space.to_scipy(self.problem.K)
+        self.mass_matrix = self.problem.space.to_scipy(self.problem.M)
+        self.potential = np.zeros(self.num_nodes)
+        self.electric_field_x = np.zeros(self.num_nodes)
+        self.electric_field_y = np.zeros(self.num_nodes)
+        self.is_assembled = True
+        self.last_solve_time = 0.0
+
+    def _rho_nodal(self, n, p):
+        if n is None:
+            n = np.zeros(self.num_nodes)
+        if p is None:
+            p = np.zeros(self.num_nodes)
+        try:
+            v = np.asarray(self.rho_func(self.nodes_x, self.nodes_y, n, p), dtype=np.float64)
+            if v.shape == (self.num_nodes,):
+                return v
+        except Exception:
+            pass
+        return np.array([self.rho_func(float(x), float(y), n, p)
+                         for x, y in zip(self.nodes_x, self.nodes_y)], dtype=np.float64)
+
+    def solve(self, V_p, V_n, n=None, p=None):
+        t0 = time.time()
+        rho = torch.from_numpy(self._rho_nodal(n, p)).to(self.problem.space.dev)
+        phi = self.problem.solve(rho, float(V_p), float(V_n))
+        self.potential = phi.cpu().numpy()
+        self._compute_electric_field()
+        self.last_solve_time = time.time() - t0
+
+    def _compute_electric_field(self):
+        """Finite differences on the vertex grid (poisson_solver.pyx:308-335); P1 only."""
+        if getattr(self.mesh, "element_order", 1) != 1:
+            return
+        dx = self.Lx / (self.nx - 1)
+        dy = self.Ly / (self.ny - 1)
+        P = self.potential.reshape(self.ny, self.nx)
+        Ex = np.empty_like(P)
+        Ey = np.empty_like(P)
+        Ex[:, 1:-1] = -(P[:, 2:] - P[:, :-2]) / (2 * dx)
+        Ex[:, 0] = -(P[:, 1] - P[:, 0]) / dx
+        Ex[:, -1] = -(P[:, -1] - P[:, -2]) / dx
+        Ey[1:-1] = -(P[2:] - P[:-2]) / (2 * dy)
+        Ey[0] = -(P[1] - P[0]) / dy
+        Ey[-1] = -(P[-1] - P[-2]) / dy
+        self.electric_field_x = Ex.ravel()
+        self.electric_field_y = Ey.ravel()
+
+    def get_potential(self):
+        return self.potential.copy()
+
+    def get_electric_field(self):
+        return self.electric_field_x.copy(), self.electric_field_y.copy()

@@ -1,0 +1,159 @@
+"""Schroedinger-type call surfaces: FixedOpenSystemSolver and SchrodingerSolver.
+
+FixedOpenSystemSolver reproduces the constructor, public attributes and methods of
+qdsim_cython/solvers/fixed_open_system_solver.pyx:28-376 (the README entry point).
+``solve(num_eigenvalues)`` returns ``(eigenvalues, [eigenvector_k])`` in SI: a
+float64 array for a closed system, complex128 for an open one, and a list of
+float64 (complex128 when open) vectors over all nodes, M-normalised.
+
+Formulations:
+* ``formulation='fem'`` (default): P1..P3 FEM with the coefficients at the
+  quadrature points, homogeneous Dirichlet on the whole boundary (the closed
+  system of the configuration being benchmarked), and for open systems a genuine
+  complex absorbing potential -i W(x) with the reference profile
+  cap_strength (1 - d / cap_length)^2 inside strips of width
+  cap_length_ratio * min(Lx, Ly) on the left and right contacts
+  (:124-142), giving complex eigenvalues E - i Gamma/2 and lifetimes hbar/(2|Im E|).
+* ``formulation='reference'``: the reference's lumped P1 formulas
+  (kinetic hbar^2/2m A/3 on the diagonal and half of it off the diagonal,
+  potential V A/3 on the diagonal, M = A/12 (1 + delta); :230-247), the two
+  corner nodes pinned by identity rows (:273-291) and the constant lifetime
+  broadening -i 0.1 cap_strength (:321-328).  Solved on the GPU as well.
+"""
+import time
+
+import numpy as np
+import torch
+
+from .mesh import SimpleMesh
+from .schrodinger import SchrodingerProblem
+from .units import EV_TO_J, HBAR, M_E
+
+
+class FixedOpenSystemSolver:
+    def __init__(self, nx, ny, Lx, Ly, m_star_func, potential_func, use_open_boundaries=False,
+                 element_order=1, formulation="fem"):
+        self.nx, self.ny = int(nx), int(ny)
+        self.Lx, self.Ly = float(Lx), float(Ly)
+        self.num_nodes = self.nx * self.ny
+        self.num_elements = (self.nx - 1) * (self.ny - 1) * 2
+        self.m_star_func = m_star_func
+        self.potential_func = potential_func
+        self.use_open_boundaries = bool(use_open_boundaries)
+        self.cap_strength = 0.01 * EV_TO_J
+        self.cap_length_ratio = 0.2
+        self.dirac_normalization = bool(use_open_boundaries)
+        self.device_type = "generic"
+        self.formulation = formulation
+        self.element_order = int(element_order)
+        self.mesh = SimpleMesh(self.nx, self.ny, self.Lx, self.Ly, self.element_order)
+        self.nodes_x, self.nodes_y = self.mesh.get_nodes()
+        self.elements = self.mesh.get_elements()
+        self.num_nodes = self.mesh.num_nodes
+        self.eigenvalues = None
+        self.eigenvectors = None
+        self.solve_time = 0.0
+        self._problem = None
+        print(f"Fixed solver created: {self.num_nodes} nodes, {self.num_elements} elements")
+
+    # -- CAP profile (fixed_open_system_solver.pyx:124-142), SI in, joules out ----
+    def cap_absorption(self, x, y):
+        cap_length = self.cap_length_ratio * min(self.Lx, self.Ly)
+        lib = torch if isinstance(x, torch.Tensor) else np
+        left = x < cap_length
+        right = x > (self.Lx - cap_length)
+        dl = (1.0 - x / cap_length) ** 2
+        dr = (1.0 - (self.Lx - x) / cap_length) ** 2
+        zero = x * 0.0
+        return self.cap_strength * lib.where(left, dl, lib.where(right, dr, zero))
+
+    def _get_problem(self):
+        if self._problem is None:
+            self._problem = SchrodingerProblem(self.mesh, self.m_star_func, self.potential_func)
+        return self._problem
+
+    def solve(self, num_eigenvalues=None, num_states=None):
+        k = int(num_eigenvalues if num_eigenvalues is not None else num_states)
+        t0 = time.time()
+        try:
+            if self.formulation == "reference":
+                vals, vecs = self._solve_reference(k)
+            elif self.use_open_boundaries:
+                vals, vecs = self._solve_open(k)
+            else:
+                prob = self._get_problem()
+                prob.assemble()
+                lam, X, info = prob.solve(k)
+                vals = lam * EV_TO_J
+                vecs = X.cpu().numpy() * 1e9          # SI M-normalisation
+            self.eigenvalues = vals
+            self.eigenvectors = [vecs[:, i].copy() for i in range(vecs.shape[1])]
+            self.solve_time = time.time() - t0
+            print(f"Solved in {self.solve_time:.3f}s: {len(vals)} eigenvalues")
+            return np.array(vals).copy(), [v.copy() for v in self.eigenvectors]
+        except Exception as e:  # reference behaviour: report and return empty results
+            print(f"Solve failed: {e}")
+            self.eigenvalues = np.array([])
+            self.eigenvectors = []
+            self.solve_time = time.time() - t0
+            return self.eigenvalues, self.eigenvectors
+
+    def _solve_open(self, k):
+        from .cap import solve_cap
+        prob = self._get_problem()
+        w_q = prob.space.sample(self.cap_absorption, "quadrature", 1.0 / EV_TO_J)
+        prob.set_cap_ev(w_q)
+        prob.assemble()
+        lam, X = solve_cap(prob, k, target=None)
+        return lam * EV_TO_J, X * 1e9
+
+    def _solve_reference(self, k):
+        from .reference_formulation import solve_reference_formulation
+        return solve_reference_formulation(self, k)
+
+    # -- configuration methods (fixed_open_system_solver.pyx:330-376) -------------
+    def apply_open_system_boundary_conditions(self):
+        self.use_open_boundaries = True
+        print("Open system boundary conditions applied")
+        print(f"   CAP strength: {self.cap_strength/EV_TO_J:.1f} meV")
+
+    def apply_dirac_delta_normalization(self):
+        self.dirac_normalization = True
+        print("Dirac delta normalization applied")
+
+    def configure_device_specific_solver(self, device_type, parameters=None):
+        if parameters is None:
+            parameters = {}
+        self.device_type = device_type
+        if device_type == "pn_junction":
+            self.cap_strength = parameters.get('cap_strength', 0.005 * EV_TO_J)
+            self.cap_length_ratio = parameters.get('cap_length_ratio', 0.15)
+        elif device_type == "quantum_well":
+            self.cap_strength = parameters.get('cap_strength', 0.02 * EV_TO_J)
+            self.cap_length_ratio = parameters.get('cap_length_ratio', 0.25)
+        elif device_type == "quantum_dot":
+            self.cap_strength = parameters.get('cap_strength', 0.001 * EV_TO_J)
+            self.cap_length_ratio = parameters.get('cap_length_ratio', 0.1)
+        print(f"Configured for {device_type}")
+
+    def apply_conservative_boundary_conditions(self):
+        self.use_open_boundaries = True
+        self.cap_strength = 0.001 * EV_TO_J
+        self.cap_length_ratio = 0.05
+
+    def apply_minimal_cap_boundaries(self):
+        self.use_open_boundaries = True
+        self.cap_strength = 0.002 * EV_TO_J
+        self.cap_length_ratio = 0.1
+
+    def get_eigenvalues_eV(self):
+        if self.eigenvalues is not None:
+            return np.asarray(self.eigenvalues) / EV_TO_J
+        return np.array([])
+
+
+def lifetimes_fs(eigenvalues_J):
+    """tau = hbar / (2 |Im E|) in femtoseconds (README.md:104-110)."""
+    im = np.abs(np.imag(np.asarray(eigenvalues_J)))
+    with np.errstate(divide="ignore"):
+        return np.where(im > 0, HBAR / (2 * im) * 1e15, np.inf)
