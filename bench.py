@@ -120,6 +120,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="only launch the fused SpMM (20x) and a 1 GiB device copy (5x), for rocprofv3")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -140,6 +142,17 @@ def main():
     setup_s = time.perf_counter() - t0
     print(f"[bench] setup {setup_s:.2f} s: N={mesh.num_nodes} free={prob.n_free} nnz={prob.space.nnz}",
           flush=True)
+
+    if args.profile_kernels:
+        prob.assemble()
+        roof = _spmm_roofline(prob, 24, iters=20)
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")   # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"spmm": roof, "copy_bytes_read": src.numel() * 4, "copy_launches": 5}))
+        return
 
     def step(mode="mixed"):
         prob.assemble()

@@ -130,6 +130,12 @@ int qds_block_update(int32_t dtype, int64_t n, int32_t m, int32_t k, const void*
 int qds_block_residual(int32_t dtype, int64_t n, int32_t b, const void* HX, int64_t ldh,
                        const void* MX, int64_t ldm, const double* lam, void* R, int64_t ldr,
                        void* stream);
+/* Fused R = HX - MX diag(lambda) with out[j] = ||R_j||^2, out[b+j] = ||MX_j||^2
+ * (FP64 sums, real dtypes): the residual step of each block-eigensolver iteration. */
+int64_t qds_residual_norms_workspace(int32_t b);
+int qds_residual_norms(int32_t dtype, int64_t n, int32_t b, const void* HX, int64_t ldh,
+                       const void* MX, int64_t ldm, const double* lam, void* R, int64_t ldr,
+                       double* out, void* workspace, int64_t ws_bytes, void* stream);
 int64_t qds_coldot_workspace(int32_t b);
 int qds_coldot(int32_t dtype, int64_t n, int32_t b, const void* X, int64_t ldx, const void* Y,
                int64_t ldy, int32_t conj, double* out, void* workspace, int64_t ws_bytes,

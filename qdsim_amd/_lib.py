@@ -41,6 +41,8 @@ _SIGNATURES = {
     "qds_block_update": ([_i32, _i64, _i32, _i32, _p, _i64, _p, _p, _i64, _p, _p, _i64, _p], _c.c_int),
     "qds_block_residual": ([_i32, _i64, _i32, _p, _i64, _p, _i64, _p, _p, _i64, _p], _c.c_int),
     "qds_coldot_workspace": ([_i32], _i64),
+    "qds_residual_norms_workspace": ([_i32], _i64),
+    "qds_residual_norms": ([_i32, _i64, _i32, _p, _i64, _p, _i64, _p, _p, _i64, _p, _p, _i64, _p], _c.c_int),
     "qds_coldot": ([_i32, _i64, _i32, _p, _i64, _p, _i64, _i32, _p, _p, _i64, _p], _c.c_int),
     "qds_mg_vcycle": ([_i32, _i32, _p, _i32, _i32, _i32, _i32, _d, _p], _c.c_int),
     "qds_mg_apply": ([_i32, _p, _i32, _p, _p, _p], _c.c_int),
@@ -67,7 +69,7 @@ def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"qdsim_amd: native library {LIB_PATH} is missing; build it with "
-            "`python -m qdsim_amd._build` (hipcc --offload-arch=gfx950)")
+            "`python qdsim_amd/_build.py` (hipcc --offload-arch=gfx950)")
     lib = ctypes.CDLL(LIB_PATH)
     for name, (args, res) in _SIGNATURES.items():
         fn = getattr(lib, name)

@@ -35,11 +35,13 @@ def _rv(T):
 
 
 def _bilinear(G):
-    return torch.complex(G[0::2, 0::2] - G[1::2, 1::2], G[0::2, 1::2] + G[1::2, 0::2])
+    """S^T B from the real Gram of the interleaved views (numpy or torch)."""
+    return (G[0::2, 0::2] - G[1::2, 1::2]) + 1j * (G[0::2, 1::2] + G[1::2, 0::2])
 
 
 def _hermitian(G):
-    return torch.complex(G[0::2, 0::2] + G[1::2, 1::2], G[0::2, 1::2] - G[1::2, 0::2])
+    """S^H B from the real Gram of the interleaved views (numpy or torch)."""
+    return (G[0::2, 0::2] + G[1::2, 1::2]) + 1j * (G[0::2, 1::2] - G[1::2, 0::2])
 
 
 def _real_form(C):
@@ -70,32 +72,34 @@ class CAPOperator:
 
 
 def _rr(G, m, nb, drop_tol, target):
-    Ab = _bilinear(G[:, : 2 * m])
-    Mb = _bilinear(G[:, 2 * m:])
-    Mh = _hermitian(G[:, 2 * m:])
+    """Orthogonal Rayleigh-Ritz: S^H A S y = theta S^H M S y.  Its Ritz values lie
+    in the field of values of the pencil (Re >= lambda_min(H, M), Im <= 0 for W >= 0),
+    so no spurious values compete in the selection; at convergence the Ritz pairs
+    are eigenpairs of the complex-symmetric pencil."""
+    dev = G.device
+    g = G.detach().cpu().numpy()
+    Ah = _hermitian(g[:, : 2 * m])
+    Mh = _hermitian(g[:, 2 * m:])
     Mh = 0.5 * (Mh + Mh.conj().T)
-    d = torch.diagonal(Mh).real.clamp_min(1e-300).rsqrt()
+    dg = np.diag(Mh).real.copy()
+    ok = dg > 1e-30 * dg.max()
+    d = np.where(ok, 1.0 / np.sqrt(np.where(ok, dg, 1.0)), 0.0)
     Mhs = Mh * d[:, None] * d[None, :]
-    mu, Q = torch.linalg.eigh(Mhs)
-    n_ok = int((mu > drop_tol * mu[-1]).sum().item())
+    mu, Q = np.linalg.eigh(Mhs)
+    n_ok = int((mu > drop_tol * mu[-1]).sum())
     nk = max(n_ok, nb)
-    Bm = (d[:, None].to(Q.dtype) * Q[:, -nk:]) * mu[-nk:].clamp_min(1e-300).rsqrt()[None, :].to(Q.dtype)
-    Ah = Bm.T @ Ab @ Bm
-    Mhat = Bm.T @ Mb @ Bm
-    theta, Y = torch.linalg.eig(torch.linalg.solve(Mhat, Ah))
+    Bm = (d[:, None] * Q[:, -nk:]) / np.sqrt(np.maximum(mu[-nk:], 1e-300))[None, :]
+    Ahat = Bm.conj().T @ Ah @ Bm
+    theta, Y = np.linalg.eig(Ahat)
     if target is None:
-        order = torch.argsort(theta.real)
+        order = np.argsort(theta.real)
     else:
-        order = torch.argsort((theta - target).abs())
+        order = np.argsort(np.abs(theta - target))
     sel = order[:nb]
-    Ys = Y[:, sel]
-    nrm = torch.sqrt(torch.diagonal(Ys.T @ Mhat @ Ys))
-    Ys = Ys / nrm[None, :]
-    C = Bm @ Ys
+    C = Bm @ Y[:, sel]          # unit Hermitian M-norm columns
     th = theta[sel]
-    # present the block sorted by real part
-    o2 = torch.argsort(th.real)
-    return th[o2], C[:, o2]
+    o2 = np.argsort(th.real)
+    return (torch.from_numpy(th[o2].copy()).to(dev), torch.from_numpy(C[:, o2].copy()).to(dev))
 
 
 def cap_lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, seed=0, target=None,

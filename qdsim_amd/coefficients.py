@@ -16,7 +16,7 @@ def _as_tensor(val, like):
     if isinstance(val, torch.Tensor):
         t = val.to(device=like.device, dtype=torch.float64)
     else:
-        t = torch.as_tensor(np.asarray(val, dtype=np.float64), device=like.device)
+        t = torch.as_tensor(np.array(val, dtype=np.float64), device=like.device)
     if t.shape != like.shape:
         t = torch.broadcast_to(t, like.shape)
     return t.contiguous()

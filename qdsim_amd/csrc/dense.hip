@@ -14,6 +14,8 @@
 // * qds_block_residual, qds_coldot: R = HX - MX diag(lambda); column dot products.
 #include "qds_common.h"
 
+#include <type_traits>
+
 namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -44,12 +46,26 @@ __device__ __forceinline__ bool tile_needed(const GramShape& g, int ti, int tj) 
   return ti <= tl;
 }
 
-// grid: (row slabs, output column groups of tpg tiles)
-template <typename T>
+// Vector width of the global loads of the Gram kernel (elements of T per load).
+template <typename T, bool VEC> struct GLoad;
+template <> struct GLoad<float, true> { static constexpr int W = 4; typedef float4 v; };
+template <> struct GLoad<float, false> { static constexpr int W = 1; typedef float v; };
+template <> struct GLoad<double, true> { static constexpr int W = 2; typedef double2 v; };
+template <> struct GLoad<double, false> { static constexpr int W = 1; typedef double v; };
+
+constexpr int kGramSlots = 6;  // vector loads per thread per staged chunk (A and B each)
+
+// grid: (row slabs, output column groups of tpg tiles).  Each block streams its slab
+// kRB rows at a time: every thread owns up to kGramSlots vector loads of A and of B
+// (positions fixed for the whole slab), loads the next chunk into registers while
+// the MFMAs of the current chunk run from LDS (FP64 copies of the rows).
+template <typename T, bool VEC, int MAXT>
 __global__ void __launch_bounds__(kBlock)
 gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, const T* __restrict__ A, int64_t lda,
              const T* __restrict__ B1, int64_t ldb1, const T* __restrict__ B2, int64_t ldb2,
              double* __restrict__ partial) {
+  using L = GLoad<T, VEC>;
+  constexpr int VW = L::W;
   extern __shared__ double smem[];
   const int tk = g.tk1 + g.tk2;
   const int tj0 = blockIdx.y * tpg;
@@ -57,7 +73,7 @@ gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, const T* __r
   const int cw = (tj1 - tj0) * 16;   // B columns staged by this block
   double* sA = smem;                 // kRB x mpad
   double* sB = smem + kRB * g.mpad;  // kRB x cw
-  __shared__ int s_tile[kWaves * kMaxTiles];
+  __shared__ int s_tile[kWaves * MAXT];
   __shared__ int s_count[kWaves];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -69,49 +85,90 @@ gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, const T* __r
     for (int t = wave; t < ntl; t += kWaves) {
       const int ti = t / (tj1 - tj0), tj = tj0 + (t - ti * (tj1 - tj0));
       if (!tile_needed(g, ti, tj)) continue;
-      if (c < kMaxTiles) s_tile[wave * kMaxTiles + c] = (ti << 8) | tj;
+      if (c < MAXT) s_tile[wave * MAXT + c] = (ti << 8) | tj;
       ++c;
     }
     s_count[wave] = c;
   }
+  // load slots: (row in chunk, column) of each vector this thread stages
+  const int k1pad = 16 * g.tk1;
+  const int apr = g.mpad / VW;  // vectors per staged A row
+  const int bpr = cw / VW;
+  int a_rr[kGramSlots], a_c[kGramSlots], b_rr[kGramSlots], b_c[kGramSlots];
+  const T* b_src[kGramSlots];
+  int64_t b_ld[kGramSlots];
+  bool a_ok[kGramSlots], b_ok[kGramSlots];
+#pragma unroll
+  for (int s = 0; s < kGramSlots; ++s) {
+    const int ta = threadIdx.x + s * kBlock;
+    a_rr[s] = ta / apr;
+    a_c[s] = (ta - a_rr[s] * apr) * VW;
+    a_ok[s] = a_rr[s] < kRB && a_c[s] < g.m;
+    const int tb = threadIdx.x + s * kBlock;
+    b_rr[s] = tb / bpr;
+    const int cl = (tb - b_rr[s] * bpr) * VW;   // local column
+    b_c[s] = cl;
+    const int c = tj0 * 16 + cl;                 // global padded column
+    if (c < k1pad) {
+      b_src[s] = B1; b_ld[s] = ldb1;
+      b_ok[s] = b_rr[s] < kRB && c < g.k1;
+      b_c[s] = cl | (c << 16);
+    } else {
+      b_src[s] = B2; b_ld[s] = ldb2;
+      b_ok[s] = b_rr[s] < kRB && (c - k1pad) < g.k2;
+      b_c[s] = cl | ((c - k1pad) << 16);
+    }
+  }
   __syncthreads();
   const int nslots = s_count[wave];
 
-  v4d acc[kMaxTiles];
+  v4d acc[MAXT];
 #pragma unroll
-  for (int t = 0; t < kMaxTiles; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int t = 0; t < MAXT; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
 
   const int64_t r_begin = (int64_t)blockIdx.x * rows_per_blk;
   const int64_t r_end = min(n, r_begin + rows_per_blk);
-  const int k1pad = 16 * g.tk1;
-  for (int64_t r0 = r_begin; r0 < r_end; r0 += kRB) {
-    for (int t = threadIdx.x; t < kRB * g.mpad; t += kBlock) {
-      const int rr = t / g.mpad, c = t - rr * g.mpad;
-      const int64_t row = r0 + rr;
-      sA[t] = (row < r_end && c < g.m) ? (double)A[row * lda + c] : 0.0;
+  typename L::v ra[kGramSlots], rb[kGramSlots];
+  auto load_chunk = [&](int64_t r0) {
+#pragma unroll
+    for (int s = 0; s < kGramSlots; ++s) {
+      const int64_t row = r0 + a_rr[s];
+      if (a_ok[s] && row < r_end)
+        ra[s] = *reinterpret_cast<const typename L::v*>(A + row * lda + a_c[s]);
+      else
+        ra[s] = typename L::v{};
+      const int64_t rowb = r0 + b_rr[s];
+      if (b_ok[s] && rowb < r_end)
+        rb[s] = *reinterpret_cast<const typename L::v*>(b_src[s] + rowb * b_ld[s] + (b_c[s] >> 16));
+      else
+        rb[s] = typename L::v{};
     }
-    for (int t = threadIdx.x; t < kRB * cw; t += kBlock) {
-      const int rr = t / cw, c = tj0 * 16 + (t - rr * cw);
-      const int64_t row = r0 + rr;
-      double v = 0.0;
-      if (row < r_end) {
-        if (c < k1pad) {
-          if (c < g.k1) v = (double)B1[row * ldb1 + c];
-        } else {
-          const int c2 = c - k1pad;
-          if (c2 < g.k2) v = (double)B2[row * ldb2 + c2];
-        }
+  };
+  if (r_begin < r_end) load_chunk(r_begin);
+  for (int64_t r0 = r_begin; r0 < r_end; r0 += kRB) {
+    __syncthreads();  // previous chunk's MFMAs are done with LDS
+#pragma unroll
+    for (int s = 0; s < kGramSlots; ++s) {
+      if (a_rr[s] < kRB) {
+        const T* v = reinterpret_cast<const T*>(&ra[s]);
+#pragma unroll
+        for (int e = 0; e < VW; ++e) sA[a_rr[s] * g.mpad + a_c[s] + e] = (double)v[e];
       }
-      sB[t] = v;
+      if (b_rr[s] < kRB) {
+        const T* v = reinterpret_cast<const T*>(&rb[s]);
+#pragma unroll
+        for (int e = 0; e < VW; ++e) sB[b_rr[s] * cw + (b_c[s] & 0xffff) + e] = (double)v[e];
+      }
     }
     __syncthreads();
+    if (r0 + kRB < r_end) load_chunk(r0 + kRB);   // in flight during the MFMAs below
 #pragma unroll
     for (int kk = 0; kk < kRB / 4; ++kk) {
       const int row = kk * 4 + (lane >> 4);
 #pragma unroll
-      for (int slot = 0; slot < kMaxTiles; ++slot) {
+      for (int slot = 0; slot < MAXT; ++slot) {
         if (slot < nslots) {
-          const int code = s_tile[wave * kMaxTiles + slot];
+          const int code = s_tile[wave * MAXT + slot];
           const int ti = code >> 8, tj = code & 255;
           const double a = sA[row * g.mpad + ti * 16 + (lane & 15)];
           const double b = sB[row * cw + (tj - tj0) * 16 + (lane & 15)];
@@ -119,17 +176,17 @@ gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, const T* __r
         }
       }
     }
-    __syncthreads();
   }
   double* P = partial + (int64_t)blockIdx.x * g.mpad * g.kpad;
 #pragma unroll
-  for (int slot = 0; slot < kMaxTiles; ++slot) {
+  for (int slot = 0; slot < MAXT; ++slot) {
     if (slot < nslots) {
-      const int code = s_tile[wave * kMaxTiles + slot];
+      const int code = s_tile[wave * MAXT + slot];
       const int ti = code >> 8, tj = code & 255;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = ti * 16 + 4 * (lane >> 4) + r;
+        // v_mfma_f64_16x16x4f64 result: lane l, register r holds D[l/16 + 4r][l%16]
+        const int i = ti * 16 + (lane >> 4) + 4 * r;
         const int j = tj * 16 + (lane & 15);
         P[i * g.kpad + j] = acc[slot][r];
       }
@@ -162,87 +219,116 @@ __global__ void gram_reduce(int nblk, GramShape g, const double* __restrict__ pa
 
 // ---------------------------------------------------------------- block update
 template <typename T> struct MT;
+// Operand layouts of both 16x16x4 MFMAs: A lane l = A[l%16][l/16], B lane l =
+// B[l/16][l%16].  Result rows differ: FP32 lane l register e holds D[4(l/16)+e][l%16],
+// FP64 holds D[l/16 + 4e][l%16].
 template <> struct MT<float> {
   static constexpr int VEC = 4;
   typedef float4 vec_t;
   typedef v4f acc_t;
   static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) { return mfma_f32(a, b, c); }
+  static __device__ __forceinline__ int drow(int q, int e) { return 4 * q + e; }
 };
 template <> struct MT<double> {
   static constexpr int VEC = 2;
   typedef double2 vec_t;
   typedef v4d acc_t;
   static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) { return mfma_f64(a, b, c); }
+  static __device__ __forceinline__ int drow(int q, int e) { return q + 4 * e; }
 };
 
 constexpr int kMaxOutTiles = 4;  // k <= 64
 constexpr int kRowTiles = 2;     // 16-row tiles per wave and pass
-constexpr int kKC = 32;          // rows of C staged in LDS per chunk
+constexpr int kKC = 32;          // rows of C staged per chunk when C does not fit in LDS
+constexpr int kUpdLds = 48 * 1024;
 
-// One pass of a block covers kWaves * kRowTiles * 16 rows; C is staged through LDS
-// kKC rows at a time, so m is unbounded.
-template <typename T, int NOUT>
+// Y1 = S C1 (and Y2 = S C2).  One pass of a block covers kWaves * kRowTiles * 16
+// rows.  When NOUT * m * k fits in LDS, C is staged once per block and stays
+// resident (no barriers inside the pass loop); otherwise C is streamed kKC rows at
+// a time.  MAXKB bounds the contraction blocks whose S vectors a lane loads
+// before its MFMAs (all loads of a row tile in flight together).
+template <typename T, int NOUT, int MAXOT>
 __global__ void __launch_bounds__(kBlock)
 block_update_kernel(int64_t n, int m, int k, const T* __restrict__ S, int64_t lds,
                     const T* __restrict__ C1, const T* __restrict__ C2, T* __restrict__ Y1,
-                    int64_t ldy1, T* __restrict__ Y2, int64_t ldy2) {
+                    int64_t ldy1, T* __restrict__ Y2, int64_t ldy2, int chunk) {
   using M = MT<T>;
   constexpr int VEC = M::VEC;
   constexpr int KB = 4 * VEC;  // contraction indices covered by one vector load per lane
-  __shared__ T sC[NOUT * kKC * 16 * kMaxOutTiles];
+  constexpr int MAXKB = 8;     // vector loads kept in flight per row tile
+  extern __shared__ unsigned char upd_smem[];
+  T* sC = reinterpret_cast<T*>(upd_smem);
   const int kt = (k + 15) / 16;
   const int kp = kt * 16;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int q = lane >> 4;      // 0..3
   const int rl = lane & 15;
+  const bool resident = chunk >= m;
+  auto stage = [&](int kc) {
+    for (int t = threadIdx.x; t < NOUT * chunk * kp; t += kBlock) {
+      const int o = t / (chunk * kp);
+      const int rem = t - o * chunk * kp;
+      const int i = rem / kp, j = rem - i * kp;
+      const T* C = o == 0 ? C1 : C2;
+      const int gi = kc + i;
+      sC[t] = (gi < m && j < k) ? C[gi * k + j] : T(0);
+    }
+  };
+  if (resident) {
+    stage(0);
+    __syncthreads();
+  }
   const int64_t rows_per_pass = (int64_t)kWaves * kRowTiles * 16;
   for (int64_t base = (int64_t)blockIdx.x * rows_per_pass; base < n;
        base += (int64_t)gridDim.x * rows_per_pass) {
-    typename M::acc_t acc[NOUT][kRowTiles][kMaxOutTiles];
+    typename M::acc_t acc[NOUT][kRowTiles][MAXOT];
 #pragma unroll
     for (int o = 0; o < NOUT; ++o)
 #pragma unroll
       for (int r = 0; r < kRowTiles; ++r)
 #pragma unroll
-        for (int t = 0; t < kMaxOutTiles; ++t) acc[o][r][t] = typename M::acc_t{0, 0, 0, 0};
-    for (int kc = 0; kc < m; kc += kKC) {
-      __syncthreads();
-      for (int t = threadIdx.x; t < NOUT * kKC * kp; t += kBlock) {
-        const int o = t / (kKC * kp);
-        const int rem = t - o * kKC * kp;
-        const int i = rem / kp, j = rem - i * kp;
-        const T* C = o == 0 ? C1 : C2;
-        const int gi = kc + i;
-        sC[t] = (gi < m && j < k) ? C[gi * k + j] : T(0);
+        for (int t = 0; t < MAXOT; ++t) acc[o][r][t] = typename M::acc_t{0, 0, 0, 0};
+    for (int kc = 0; kc < m; kc += chunk) {
+      if (!resident) {
+        __syncthreads();
+        stage(kc);
+        __syncthreads();
       }
-      __syncthreads();
+      const int kend = min(chunk, m - kc);
 #pragma unroll
       for (int r = 0; r < kRowTiles; ++r) {
         const int64_t row = base + ((int64_t)wave * kRowTiles + r) * 16 + rl;
+        for (int cb0 = 0; cb0 < kend; cb0 += MAXKB * KB) {
+          T sv[MAXKB][VEC];
 #pragma unroll
-        for (int cb = 0; cb < kKC / KB; ++cb) {
-          const int kloc = cb * KB + q * VEC;
-          const int kbase = kc + kloc;
-          T sv[VEC];
-          if (row < n && kbase < m) {
-            typename M::vec_t v = *reinterpret_cast<const typename M::vec_t*>(S + row * lds + kbase);
-            const T* vp = reinterpret_cast<const T*>(&v);
+          for (int j = 0; j < MAXKB; ++j) {
+            const int kloc = cb0 + j * KB + q * VEC;
+            if (row < n && kloc < kend) {
+              typename M::vec_t v = *reinterpret_cast<const typename M::vec_t*>(S + row * lds + kc + kloc);
+              const T* vp = reinterpret_cast<const T*>(&v);
 #pragma unroll
-            for (int e = 0; e < VEC; ++e) sv[e] = vp[e];
-          } else {
+              for (int e = 0; e < VEC; ++e) sv[j][e] = vp[e];
+            } else {
 #pragma unroll
-            for (int e = 0; e < VEC; ++e) sv[e] = T(0);
+              for (int e = 0; e < VEC; ++e) sv[j][e] = T(0);
+            }
           }
 #pragma unroll
-          for (int e = 0; e < VEC; ++e) {
+          for (int j = 0; j < MAXKB; ++j) {
+            const int kb = cb0 + j * KB;
+            if (kb >= kend) break;
 #pragma unroll
-            for (int t = 0; t < kMaxOutTiles; ++t) {
-              if (t < kt) {
+            for (int e = 0; e < VEC; ++e) {
+              const int kloc = kb + q * VEC + e;   // row of the staged C (< chunk: zero padded)
 #pragma unroll
-                for (int o = 0; o < NOUT; ++o) {
-                  const T bv = sC[o * kKC * kp + (kloc + e) * kp + t * 16 + rl];
-                  acc[o][r][t] = M::mma(sv[e], bv, acc[o][r][t]);
+              for (int t = 0; t < MAXOT; ++t) {
+                if (t < kt) {
+#pragma unroll
+                  for (int o = 0; o < NOUT; ++o) {
+                    const T bv = sC[o * chunk * kp + kloc * kp + t * 16 + rl];
+                    acc[o][r][t] = M::mma(sv[j][e], bv, acc[o][r][t]);
+                  }
                 }
               }
             }
@@ -254,13 +340,13 @@ block_update_kernel(int64_t n, int m, int k, const T* __restrict__ S, int64_t ld
     for (int r = 0; r < kRowTiles; ++r) {
       const int64_t r0 = base + ((int64_t)wave * kRowTiles + r) * 16;
 #pragma unroll
-      for (int t = 0; t < kMaxOutTiles; ++t) {
+      for (int t = 0; t < MAXOT; ++t) {
         if (t >= kt) continue;
         const int col = t * 16 + rl;
         if (col >= k) continue;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          const int64_t orow = r0 + 4 * q + e;
+          const int64_t orow = r0 + M::drow(q, e);
           if (orow < n) {
             Y1[orow * ldy1 + col] = acc[0][r][t][e];
             if (NOUT == 2) Y2[orow * ldy2 + col] = acc[NOUT - 1][r][t][e];
@@ -269,6 +355,30 @@ block_update_kernel(int64_t n, int m, int k, const T* __restrict__ S, int64_t ld
       }
     }
   }
+}
+
+template <typename T, int NOUT>
+int launch_update(int64_t n, int m, int k, const T* S, int64_t lds, const T* C1, const T* C2, T* Y1,
+                  int64_t ldy1, T* Y2, int64_t ldy2, hipStream_t st) {
+  constexpr int KB = 4 * MT<T>::VEC;
+  const int kp = (k + 15) / 16 * 16;
+  const int mpad = (m + KB - 1) / KB * KB;
+  // resident C when it fits, otherwise kKC-row chunks (both multiples of KB, zero padded)
+  const int chunk = (size_t)NOUT * mpad * kp * sizeof(T) <= (size_t)kUpdLds ? mpad : kKC;
+  const size_t shm = (size_t)NOUT * chunk * kp * sizeof(T);
+  const int64_t passes = (n + kWaves * kRowTiles * 16 - 1) / (kWaves * kRowTiles * 16);
+  const unsigned blocks = qds_blocks(passes, 1, 4096);
+  if (kp <= 16)
+    hipLaunchKernelGGL((block_update_kernel<T, NOUT, 1>), dim3(blocks), dim3(kBlock), shm, st, n, m, k, S, lds,
+                       C1, C2, Y1, ldy1, Y2, ldy2, chunk);
+  else if (kp <= 32)
+    hipLaunchKernelGGL((block_update_kernel<T, NOUT, 2>), dim3(blocks), dim3(kBlock), shm, st, n, m, k, S, lds,
+                       C1, C2, Y1, ldy1, Y2, ldy2, chunk);
+  else
+    hipLaunchKernelGGL((block_update_kernel<T, NOUT, 4>), dim3(blocks), dim3(kBlock), shm, st, n, m, k, S, lds,
+                       C1, C2, Y1, ldy1, Y2, ldy2, chunk);
+  QDS_LAUNCH_CHECK();
+  return QDS_OK;
 }
 
 // ---------------------------------------------------------------- residual / dots
@@ -389,6 +499,16 @@ QDS_API int qds_gram(int32_t dtype, int64_t n, int32_t m, const void* A, int64_t
   QDS_CHECK_ARG(!sym || (m == k1 && (k2 == 0 || k2 == m)), "qds_gram: sym needs square halves");
   GramShape g = make_shape(m, k1, k2, sym);
   QDS_CHECK_ARG(g.tm <= 255 && g.tk1 + g.tk2 <= 255, "qds_gram: block too wide");
+  // 16-byte loads when every row start of A, B1, B2 is 16-byte aligned and the column
+  // counts are multiples of the vector width (padding columns are masked per vector)
+  const int es = dtype == qds::F32 ? 4 : 8;
+  const int vw = 16 / es;
+  auto al = [&](const void* p, int64_t ld) {
+    return p == nullptr || ((reinterpret_cast<uintptr_t>(p) % 16) == 0 && (ld * es) % 16 == 0);
+  };
+  const bool vec = al(A, lda) && al(B1, ldb1) && al(B2, ldb2) && m % vw == 0 && k1 % vw == 0 &&
+                   k2 % vw == 0;
+  const int vw_used = vec ? vw : 1;
   const int tk = g.tk1 + g.tk2;
   // column groups: every wave holds at most kMaxTiles accumulators and the staged
   // rows fit in 64 KiB of LDS
@@ -407,23 +527,51 @@ QDS_API int qds_gram(int32_t dtype, int64_t n, int32_t m, const void* A, int64_t
         if (c > kMaxTiles) return false;
       }
     }
+    // every staged row chunk must fit the per-thread load slots
+    if ((kRB * g.mpad / vw_used + kBlock - 1) / kBlock > kGramSlots) return false;
+    if ((kRB * 16 * tp / vw_used + kBlock - 1) / kBlock > kGramSlots) return false;
     return (size_t)kRB * (g.mpad + 16 * tp) * sizeof(double) <= 64 * 1024;
   };
   while (tpg > 1 && !slots_ok(tpg)) --tpg;
   QDS_CHECK_ARG(slots_ok(tpg), "qds_gram: A block of %d columns too wide", m);
   const int ngroups = (tk + tpg - 1) / tpg;
+  int max_slots = 0;
+  for (int j0 = 0; j0 < tk; j0 += tpg) {
+    const int j1 = j0 + tpg < tk ? j0 + tpg : tk;
+    for (int w = 0; w < kWaves; ++w) {
+      int c = 0;
+      const int nt = g.tm * (j1 - j0);
+      for (int t = w; t < nt; t += kWaves) {
+        const int ti = t / (j1 - j0), tj = j0 + (t - ti * (j1 - j0));
+        const int tl = tj < g.tk1 ? tj : tj - g.tk1;
+        if (!sym || ti <= tl) ++c;
+      }
+      max_slots = c > max_slots ? c : max_slots;
+    }
+  }
   const int nblk = gram_blocks(n);
   QDS_CHECK_ARG(ws_bytes >= qds_gram_workspace(n, m, k1, k2), "qds_gram: workspace too small");
   const int64_t rows_per_blk = (((n + nblk - 1) / nblk) + kRB - 1) / kRB * kRB;
   const size_t shm = (size_t)kRB * (g.mpad + 16 * tpg) * sizeof(double);
   hipStream_t st = qds_stream(stream);
   double* partial = static_cast<double*>(workspace);
-  if (dtype == qds::F32)
-    hipLaunchKernelGGL(gram_partial<float>, dim3(nblk, ngroups), dim3(kBlock), shm, st, n, rows_per_blk, g,
-                       tpg, (const float*)A, lda, (const float*)B1, ldb1, (const float*)B2, ldb2, partial);
-  else if (dtype == qds::F64)
-    hipLaunchKernelGGL(gram_partial<double>, dim3(nblk, ngroups), dim3(kBlock), shm, st, n, rows_per_blk, g,
-                       tpg, (const double*)A, lda, (const double*)B1, ldb1, (const double*)B2, ldb2, partial);
+  auto launch = [&](auto tag, auto vecc, auto maxt) {
+    using T = decltype(tag);
+    constexpr bool V = decltype(vecc)::value;
+    constexpr int MT_ = decltype(maxt)::value;
+    hipLaunchKernelGGL((gram_partial<T, V, MT_>), dim3(nblk, ngroups), dim3(kBlock), shm, st, n,
+                       rows_per_blk, g, tpg, (const T*)A, lda, (const T*)B1, ldb1, (const T*)B2, ldb2,
+                       partial);
+  };
+  auto by_tiles = [&](auto tag, auto vecc) {
+    if (max_slots <= 8) launch(tag, vecc, std::integral_constant<int, 8>());
+    else if (max_slots <= 12) launch(tag, vecc, std::integral_constant<int, 12>());
+    else launch(tag, vecc, std::integral_constant<int, 24>());
+  };
+  if (dtype == qds::F32 && vec) by_tiles(float(0), std::true_type());
+  else if (dtype == qds::F32) by_tiles(float(0), std::false_type());
+  else if (dtype == qds::F64 && vec) by_tiles(double(0), std::true_type());
+  else if (dtype == qds::F64) by_tiles(double(0), std::false_type());
   else
     QDS_CHECK_ARG(false, "qds_gram: dtype must be F32 or F64");
   QDS_LAUNCH_CHECK();
@@ -443,33 +591,22 @@ QDS_API int qds_block_update(int32_t dtype, int64_t n, int32_t m, int32_t k, con
   QDS_CHECK_ARG(k >= 1 && k <= 16 * kMaxOutTiles && m >= 1, "qds_block_update: bad shape");
   if (n == 0) return QDS_OK;
   hipStream_t st = qds_stream(stream);
-  const int64_t passes = (n + kWaves * kRowTiles * 16 - 1) / (kWaves * kRowTiles * 16);
-  const unsigned blocks = qds_blocks(passes, 1, 4096);
   if (dtype == qds::F32) {
     QDS_CHECK_ARG(m % 4 == 0 && lds % 4 == 0 && (reinterpret_cast<uintptr_t>(S) % 16) == 0,
                   "qds_block_update: F32 needs m, lds multiple of 4 and 16-byte aligned S");
-    if (C2)
-      hipLaunchKernelGGL((block_update_kernel<float, 2>), dim3(blocks), dim3(kBlock), 0, st, n, m, k,
-                         (const float*)S, lds, (const float*)C1, (const float*)C2, (float*)Y1, ldy1,
-                         (float*)Y2, ldy2);
-    else
-      hipLaunchKernelGGL((block_update_kernel<float, 1>), dim3(blocks), dim3(kBlock), 0, st, n, m, k,
-                         (const float*)S, lds, (const float*)C1, (const float*)nullptr, (float*)Y1,
-                         ldy1, (float*)nullptr, ldy2);
+    return C2 ? launch_update<float, 2>(n, m, k, (const float*)S, lds, (const float*)C1, (const float*)C2,
+                                        (float*)Y1, ldy1, (float*)Y2, ldy2, st)
+              : launch_update<float, 1>(n, m, k, (const float*)S, lds, (const float*)C1, nullptr,
+                                        (float*)Y1, ldy1, nullptr, ldy2, st);
   } else if (dtype == qds::F64) {
     QDS_CHECK_ARG(m % 2 == 0 && lds % 2 == 0 && (reinterpret_cast<uintptr_t>(S) % 16) == 0,
                   "qds_block_update: F64 needs m, lds even and 16-byte aligned S");
-    if (C2)
-      hipLaunchKernelGGL((block_update_kernel<double, 2>), dim3(blocks), dim3(kBlock), 0, st, n, m, k,
-                         (const double*)S, lds, (const double*)C1, (const double*)C2, (double*)Y1,
-                         ldy1, (double*)Y2, ldy2);
-    else
-      hipLaunchKernelGGL((block_update_kernel<double, 1>), dim3(blocks), dim3(kBlock), 0, st, n, m, k,
-                         (const double*)S, lds, (const double*)C1, (const double*)nullptr,
-                         (double*)Y1, ldy1, (double*)nullptr, ldy2);
-  } else {
-    QDS_CHECK_ARG(false, "qds_block_update: dtype must be F32 or F64");
+    return C2 ? launch_update<double, 2>(n, m, k, (const double*)S, lds, (const double*)C1,
+                                         (const double*)C2, (double*)Y1, ldy1, (double*)Y2, ldy2, st)
+              : launch_update<double, 1>(n, m, k, (const double*)S, lds, (const double*)C1, nullptr,
+                                         (double*)Y1, ldy1, nullptr, ldy2, st);
   }
+  QDS_CHECK_ARG(false, "qds_block_update: dtype must be F32 or F64");
   QDS_LAUNCH_CHECK();
   return QDS_OK;
 }
@@ -555,6 +692,90 @@ QDS_API int qds_density(int64_t n, int32_t k, const double* psi, int64_t ld, con
   if (n == 0) return QDS_OK;
   hipLaunchKernelGGL(density_kernel, dim3(qds_blocks(n, 256, 65536)), dim3(256), 0, qds_stream(stream),
                      n, k, psi, ld, occ, out);
+  QDS_LAUNCH_CHECK();
+  return QDS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Fused residual and column norms for the eigensolver loop:
+//   R = HX - MX diag(lambda),  out[j] = ||R_j||^2,  out[b + j] = ||MX_j||^2  (FP64)
+// Real dtypes only.  Threads of a block form groups of b lanes; lane c of every
+// group keeps the running sums of column c over the rows of its group.
+namespace {
+constexpr int kResBlocks = 2048;
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+residual_norms_partial(int64_t n, int b, const T* __restrict__ HX, int64_t ldh, const T* __restrict__ MX,
+                       int64_t ldm, const double* __restrict__ lam, T* __restrict__ R, int64_t ldr,
+                       int64_t rows_per_blk, double* __restrict__ partial) {
+  const int groups = kBlock / b;
+  const int c = threadIdx.x % b;
+  const int gidx = threadIdx.x / b;
+  __shared__ double red[2 * kBlock];
+  double sr = 0.0, sm = 0.0;
+  const int64_t r_begin = (int64_t)blockIdx.x * rows_per_blk;
+  const int64_t r_end = min(n, r_begin + rows_per_blk);
+  if (gidx < groups) {
+    const T l = (T)lam[c];
+    for (int64_t r = r_begin + gidx; r < r_end; r += groups) {
+      const T m = MX[r * ldm + c];
+      const T v = HX[r * ldh + c] - l * m;
+      R[r * ldr + c] = v;
+      sr += (double)v * (double)v;
+      sm += (double)m * (double)m;
+    }
+  }
+  red[threadIdx.x] = sr;
+  red[kBlock + threadIdx.x] = sm;
+  __syncthreads();
+  if (threadIdx.x < b) {
+    double a0 = 0.0, a1 = 0.0;
+    for (int g = 0; g < groups; ++g) {
+      a0 += red[g * b + threadIdx.x];
+      a1 += red[kBlock + g * b + threadIdx.x];
+    }
+    partial[(int64_t)blockIdx.x * 2 * b + threadIdx.x] = a0;
+    partial[(int64_t)blockIdx.x * 2 * b + b + threadIdx.x] = a1;
+  }
+}
+
+__global__ void residual_norms_reduce(int nblk, int b, const double* __restrict__ partial,
+                                      double* __restrict__ out) {
+  const int t = threadIdx.x;  // 2b threads
+  if (t >= 2 * b) return;
+  double a = 0.0;
+  for (int k = 0; k < nblk; ++k) a += partial[(int64_t)k * 2 * b + t];
+  out[t] = a;
+}
+}  // namespace
+
+QDS_API int64_t qds_residual_norms_workspace(int32_t b) {
+  return (int64_t)kResBlocks * 2 * b * sizeof(double);
+}
+
+QDS_API int qds_residual_norms(int32_t dtype, int64_t n, int32_t b, const void* HX, int64_t ldh,
+                               const void* MX, int64_t ldm, const double* lam, void* R, int64_t ldr,
+                               double* out, void* workspace, int64_t ws_bytes, void* stream) {
+  QDS_CHECK_ARG(HX && MX && lam && R && out && workspace && b >= 1 && b <= 128,
+                "qds_residual_norms: bad arguments");
+  QDS_CHECK_ARG(ws_bytes >= qds_residual_norms_workspace(b), "qds_residual_norms: workspace too small");
+  hipStream_t st = qds_stream(stream);
+  int64_t nblk = (n + 511) / 512;
+  if (nblk > kResBlocks) nblk = kResBlocks;
+  if (nblk < 1) nblk = 1;
+  const int64_t rpb = (n + nblk - 1) / nblk;
+  double* part = static_cast<double*>(workspace);
+  if (dtype == qds::F32)
+    hipLaunchKernelGGL(residual_norms_partial<float>, dim3(nblk), dim3(kBlock), 0, st, n, b, (const float*)HX,
+                       ldh, (const float*)MX, ldm, lam, (float*)R, ldr, rpb, part);
+  else if (dtype == qds::F64)
+    hipLaunchKernelGGL(residual_norms_partial<double>, dim3(nblk), dim3(kBlock), 0, st, n, b,
+                       (const double*)HX, ldh, (const double*)MX, ldm, lam, (double*)R, ldr, rpb, part);
+  else
+    QDS_CHECK_ARG(false, "qds_residual_norms: dtype must be F32 or F64");
+  QDS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(residual_norms_reduce, dim3(1), dim3(256), 0, st, (int)nblk, b, part, out);
   QDS_LAUNCH_CHECK();
   return QDS_OK;
 }

@@ -34,124 +34,196 @@ struct Level {
   int64_t ldu, ldf;  // per-point strides of u and f (t is dense, stride b)
 };
 
-template <typename T>
-__device__ __forceinline__ T apply_at(const Level<T>& L, int64_t ld, int64_t p, int64_t ix, int64_t iy,
-                                      int c, const T* __restrict__ u) {
-  T s = L.d[p] * u[p * ld + c];
-  if (ix + 1 < L.nx) s -= L.we[p] * u[(p + 1) * ld + c];
-  if (ix > 0) s -= L.we[p - 1] * u[(p - 1) * ld + c];
-  if (iy + 1 < L.ny) s -= L.wn[p] * u[(p + L.nx) * ld + c];
-  if (iy > 0) s -= L.wn[p - L.nx] * u[(p - L.nx) * ld + c];
+// Column vectors: each thread owns VEC consecutive columns of one lattice point,
+// so every stencil access is a 16-byte load (32-bit index math throughout; the
+// host guarantees np * b < 2^31).
+template <typename T, int VEC> struct V;
+template <> struct V<float, 4> { typedef float4 t; };
+template <> struct V<float, 1> { typedef float t; };
+template <> struct V<double, 2> { typedef double2 t; };
+template <> struct V<double, 1> { typedef double t; };
+
+template <typename T, int VEC>
+struct Vec {
+  T v[VEC];
+  __device__ __forceinline__ void load(const T* p) {
+    typename V<T, VEC>::t x = *reinterpret_cast<const typename V<T, VEC>::t*>(p);
+    const T* q = reinterpret_cast<const T*>(&x);
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) v[i] = q[i];
+  }
+  __device__ __forceinline__ void store(T* p) const {
+    typename V<T, VEC>::t x;
+    T* q = reinterpret_cast<T*>(&x);
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) q[i] = v[i];
+    *reinterpret_cast<typename V<T, VEC>::t*>(p) = x;
+  }
+};
+
+// (A u)[p, c0:c0+VEC]
+template <typename T, int VEC>
+__device__ __forceinline__ Vec<T, VEC> apply_at(const Level<T>& L, uint32_t ld, uint32_t p, uint32_t ix,
+                                                uint32_t iy, uint32_t c0, const T* __restrict__ u) {
+  Vec<T, VEC> s, x;
+  s.load(u + p * ld + c0);
+  const T d = L.d[p];
+#pragma unroll
+  for (int i = 0; i < VEC; ++i) s.v[i] *= d;
+  if (ix + 1 < (uint32_t)L.nx) {
+    const T w = L.we[p];
+    x.load(u + (p + 1) * ld + c0);
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) s.v[i] -= w * x.v[i];
+  }
+  if (ix > 0) {
+    const T w = L.we[p - 1];
+    x.load(u + (p - 1) * ld + c0);
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) s.v[i] -= w * x.v[i];
+  }
+  if (iy + 1 < (uint32_t)L.ny) {
+    const T w = L.wn[p];
+    x.load(u + (p + (uint32_t)L.nx) * ld + c0);
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) s.v[i] -= w * x.v[i];
+  }
+  if (iy > 0) {
+    const T w = L.wn[p - (uint32_t)L.nx];
+    x.load(u + (p - (uint32_t)L.nx) * ld + c0);
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) s.v[i] -= w * x.v[i];
+  }
   return s;
 }
 
 // out = u + omega D^-1 (f - A u); zero_init: u = 0 so out = omega D^-1 f
-template <typename T>
+template <typename T, int VEC>
 __global__ void __launch_bounds__(kBlock)
 jacobi_kernel(Level<T> L, int b, const T* __restrict__ u, int64_t ldu_in, T* __restrict__ out,
               int64_t ldo, T omega, int zero_init) {
-  const int64_t np = L.nx * L.ny;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np * b;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t p = t / b;
-    const int c = (int)(t - p * b);
-    const T inv = T(1) / L.d[p];
-    const T fv = L.f[p * L.ldf + c];
+  const uint32_t nv = (uint32_t)(b / VEC);
+  const uint32_t total = (uint32_t)(L.nx * L.ny) * nv;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const uint32_t p = t / nv;
+    const uint32_t c0 = (t - p * nv) * VEC;
+    const T sc = omega / L.d[p];
+    Vec<T, VEC> fv;
+    fv.load(L.f + p * (uint32_t)L.ldf + c0);
     if (zero_init) {
-      out[p * ldo + c] = omega * inv * fv;
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) fv.v[i] *= sc;
+      fv.store(out + p * (uint32_t)ldo + c0);
     } else {
-      const int64_t iy = p / L.nx, ix = p - iy * L.nx;
-      const T r = fv - apply_at(L, ldu_in, p, ix, iy, c, u);
-      out[p * ldo + c] = u[p * ldu_in + c] + omega * inv * r;
+      const uint32_t iy = p / (uint32_t)L.nx, ix = p - iy * (uint32_t)L.nx;
+      Vec<T, VEC> au = apply_at<T, VEC>(L, (uint32_t)ldu_in, p, ix, iy, c0, u);
+      Vec<T, VEC> uv;
+      uv.load(u + p * (uint32_t)ldu_in + c0);
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) uv.v[i] += sc * (fv.v[i] - au.v[i]);
+      uv.store(out + p * (uint32_t)ldo + c0);
     }
   }
 }
 
 // coarse f = R (f - A u), R = P^T of bilinear interpolation
-template <typename T>
+template <typename T, int VEC>
 __global__ void __launch_bounds__(kBlock)
 restrict_residual_kernel(Level<T> F, Level<T> C, int b) {
-  const int64_t np = C.nx * C.ny;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np * b;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t pc = t / b;
-    const int c = (int)(t - pc * b);
-    const int64_t jy = pc / C.nx, jx = pc - jy * C.nx;
-    const int64_t fx = 2 * jx + 1, fy = 2 * jy + 1;
-    T acc = T(0);
+  const uint32_t nv = (uint32_t)(b / VEC);
+  const uint32_t total = (uint32_t)(C.nx * C.ny) * nv;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const uint32_t pc = t / nv;
+    const uint32_t c0 = (t - pc * nv) * VEC;
+    const uint32_t jy = pc / (uint32_t)C.nx, jx = pc - jy * (uint32_t)C.nx;
+    const uint32_t fx = 2 * jx + 1, fy = 2 * jy + 1;
+    Vec<T, VEC> acc;
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) acc.v[i] = T(0);
 #pragma unroll
     for (int dy = -1; dy <= 1; ++dy) {
 #pragma unroll
       for (int dx = -1; dx <= 1; ++dx) {
-        const int64_t x = fx + dx, y = fy + dy;
+        const uint32_t x = fx + dx, y = fy + dy;
         const T w = (dx == 0 ? T(1) : T(0.5)) * (dy == 0 ? T(1) : T(0.5));
-        const int64_t p = y * F.nx + x;
-        const T r = F.f[p * F.ldf + c] - apply_at(F, F.ldu, p, x, y, c, F.u);
-        acc += w * r;
+        const uint32_t p = y * (uint32_t)F.nx + x;
+        Vec<T, VEC> fv;
+        fv.load(F.f + p * (uint32_t)F.ldf + c0);
+        Vec<T, VEC> au = apply_at<T, VEC>(F, (uint32_t)F.ldu, p, x, y, c0, F.u);
+#pragma unroll
+        for (int i = 0; i < VEC; ++i) acc.v[i] += w * (fv.v[i] - au.v[i]);
       }
     }
-    C.f[pc * C.ldf + c] = acc;
+    acc.store(C.f + pc * (uint32_t)C.ldf + c0);
   }
 }
 
 // fine u += P e (bilinear); coarse point (jx, jy) sits at fine (2jx+1, 2jy+1)
-template <typename T>
+template <typename T, int VEC>
 __global__ void __launch_bounds__(kBlock)
 prolong_add_kernel(Level<T> F, Level<T> C, int b) {
-  const int64_t np = F.nx * F.ny;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np * b;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t p = t / b;
-    const int c = (int)(t - p * b);
-    const int64_t iy = p / F.nx, ix = p - iy * F.nx;
+  const uint32_t nv = (uint32_t)(b / VEC);
+  const uint32_t total = (uint32_t)(F.nx * F.ny) * nv;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const uint32_t p = t / nv;
+    const uint32_t c0 = (t - p * nv) * VEC;
+    const uint32_t iy = p / (uint32_t)F.nx, ix = p - iy * (uint32_t)F.nx;
     // 1-D weights: odd fine index -> one coarse point, even -> two neighbours
-    int64_t xs[2], ys[2];
+    int xs[2], ys[2];
     T wx[2], wy[2];
     int nxp, nyp;
-    if (ix & 1) { xs[0] = (ix - 1) / 2; wx[0] = T(1); nxp = 1; }
-    else { xs[0] = ix / 2 - 1; xs[1] = ix / 2; wx[0] = wx[1] = T(0.5); nxp = 2; }
-    if (iy & 1) { ys[0] = (iy - 1) / 2; wy[0] = T(1); nyp = 1; }
-    else { ys[0] = iy / 2 - 1; ys[1] = iy / 2; wy[0] = wy[1] = T(0.5); nyp = 2; }
-    T acc = T(0);
+    if (ix & 1) { xs[0] = (int)(ix - 1) / 2; wx[0] = T(1); nxp = 1; }
+    else { xs[0] = (int)ix / 2 - 1; xs[1] = (int)ix / 2; wx[0] = wx[1] = T(0.5); nxp = 2; }
+    if (iy & 1) { ys[0] = (int)(iy - 1) / 2; wy[0] = T(1); nyp = 1; }
+    else { ys[0] = (int)iy / 2 - 1; ys[1] = (int)iy / 2; wy[0] = wy[1] = T(0.5); nyp = 2; }
+    Vec<T, VEC> acc, e;
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) acc.v[i] = T(0);
     for (int a = 0; a < nyp; ++a) {
-      if (ys[a] < 0 || ys[a] >= C.ny) continue;
+      if (ys[a] < 0 || ys[a] >= (int)C.ny) continue;
       for (int q = 0; q < nxp; ++q) {
-        if (xs[q] < 0 || xs[q] >= C.nx) continue;
-        acc += wx[q] * wy[a] * C.u[(ys[a] * C.nx + xs[q]) * C.ldu + c];
+        if (xs[q] < 0 || xs[q] >= (int)C.nx) continue;
+        e.load(C.u + ((uint32_t)ys[a] * (uint32_t)C.nx + (uint32_t)xs[q]) * (uint32_t)C.ldu + c0);
+        const T w = wx[q] * wy[a];
+#pragma unroll
+        for (int i = 0; i < VEC; ++i) acc.v[i] += w * e.v[i];
       }
     }
-    F.u[p * F.ldu + c] += acc;
+    Vec<T, VEC> uv;
+    uv.load(F.u + p * (uint32_t)F.ldu + c0);
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) uv.v[i] += acc.v[i];
+    uv.store(F.u + p * (uint32_t)F.ldu + c0);
   }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 apply_kernel(Level<T> L, int b, const T* __restrict__ u, T* __restrict__ out) {
-  const int64_t np = L.nx * L.ny;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np * b;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t p = t / b;
-    const int c = (int)(t - p * b);
-    const int64_t iy = p / L.nx, ix = p - iy * L.nx;
-    out[t] = apply_at(L, (int64_t)b, p, ix, iy, c, u);
+  const uint32_t total = (uint32_t)(L.nx * L.ny) * (uint32_t)b;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const uint32_t p = t / (uint32_t)b;
+    const uint32_t c = t - p * (uint32_t)b;
+    const uint32_t iy = p / (uint32_t)L.nx, ix = p - iy * (uint32_t)L.nx;
+    out[t] = apply_at<T, 1>(L, (uint32_t)b, p, ix, iy, c, u).v[0];
   }
 }
 
 template <typename T>
 __global__ void copy_kernel(int64_t np, int b, const T* __restrict__ src, int64_t lds, T* __restrict__ dst) {
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < np * b;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t p = t / b;
-    const int c = (int)(t - p * b);
-    dst[t] = src[p * lds + c];
+  const uint32_t total = (uint32_t)np * (uint32_t)b;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const uint32_t p = t / (uint32_t)b;
+    const uint32_t c = t - p * (uint32_t)b;
+    dst[t] = src[p * (uint32_t)lds + c];
   }
 }
 
-template <typename T>
+template <typename T, int VEC>
 int smooth(const Level<T>& L, int b, int sweeps, T omega, bool zero_first, hipStream_t st) {
   // Jacobi sweeps ping-pong between u and t; the result always ends in u
-  const int64_t work = L.nx * L.ny * b;
-  const dim3 g(qds_blocks(work, kBlock, 131072)), blk(kBlock);
+  const int64_t work = L.nx * L.ny * (b / VEC);
+  const dim3 g(qds_blocks(work, kBlock, 65536)), blk(kBlock);
   // with an odd count the first sweep writes straight to u, so no final copy
   T* src = L.u;
   T* dst = (sweeps % 2 == 0) ? L.t : L.u;
@@ -160,12 +232,13 @@ int smooth(const Level<T>& L, int b, int sweeps, T omega, bool zero_first, hipSt
     const bool zi = zero_first && s == 0;
     if (!zi && src == dst) {
       // first sweep of an odd count on a live u: copy u into t (dense) first
-      hipLaunchKernelGGL(copy_kernel<T>, g, blk, 0, st, L.nx * L.ny, b, (const T*)L.u, L.ldu, L.t);
+      hipLaunchKernelGGL(copy_kernel<T>, dim3(qds_blocks(L.nx * L.ny * b, kBlock, 65536)), blk, 0, st,
+                         L.nx * L.ny, b, (const T*)L.u, L.ldu, L.t);
       QDS_LAUNCH_CHECK();
       src = L.t;
     }
-    hipLaunchKernelGGL(jacobi_kernel<T>, g, blk, 0, st, L, b, (const T*)src, ld_of(src), dst, ld_of(dst),
-                       omega, zi ? 1 : 0);
+    hipLaunchKernelGGL((jacobi_kernel<T, VEC>), g, blk, 0, st, L, b, (const T*)src, ld_of(src), dst,
+                       ld_of(dst), omega, zi ? 1 : 0);
     QDS_LAUNCH_CHECK();
     src = dst;
     dst = (dst == L.u) ? L.t : L.u;
@@ -173,27 +246,40 @@ int smooth(const Level<T>& L, int b, int sweeps, T omega, bool zero_first, hipSt
   return QDS_OK;
 }
 
-template <typename T>
+template <typename T, int VEC>
 int vcycle(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, hipStream_t st) {
   for (int l = 0; l < nl - 1; ++l) {
-    int rc = smooth(L[l], b, nu1, omega, true, st);
+    int rc = smooth<T, VEC>(L[l], b, nu1, omega, true, st);
     if (rc) return rc;
-    const int64_t work = L[l + 1].nx * L[l + 1].ny * b;
-    hipLaunchKernelGGL(restrict_residual_kernel<T>, dim3(qds_blocks(work, kBlock, 131072)),
+    const int64_t work = L[l + 1].nx * L[l + 1].ny * (b / VEC);
+    hipLaunchKernelGGL((restrict_residual_kernel<T, VEC>), dim3(qds_blocks(work, kBlock, 65536)),
                        dim3(kBlock), 0, st, L[l], L[l + 1], b);
     QDS_LAUNCH_CHECK();
   }
-  int rc = smooth(L[nl - 1], b, ncoarse, omega, true, st);
+  int rc = smooth<T, VEC>(L[nl - 1], b, ncoarse, omega, true, st);
   if (rc) return rc;
   for (int l = nl - 2; l >= 0; --l) {
-    const int64_t work = L[l].nx * L[l].ny * b;
-    hipLaunchKernelGGL(prolong_add_kernel<T>, dim3(qds_blocks(work, kBlock, 131072)), dim3(kBlock),
+    const int64_t work = L[l].nx * L[l].ny * (b / VEC);
+    hipLaunchKernelGGL((prolong_add_kernel<T, VEC>), dim3(qds_blocks(work, kBlock, 65536)), dim3(kBlock),
                        0, st, L[l], L[l + 1], b);
     QDS_LAUNCH_CHECK();
-    rc = smooth(L[l], b, nu2, omega, false, st);
+    rc = smooth<T, VEC>(L[l], b, nu2, omega, false, st);
     if (rc) return rc;
   }
   return QDS_OK;
+}
+
+template <typename T>
+int vcycle_any(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, hipStream_t st) {
+  constexpr int VW = 16 / sizeof(T);
+  bool vec = b % VW == 0;
+  for (int l = 0; l < nl && vec; ++l) {
+    vec = vec && L[l].ldu % VW == 0 && L[l].ldf % VW == 0 &&
+          reinterpret_cast<uintptr_t>(L[l].u) % 16 == 0 && reinterpret_cast<uintptr_t>(L[l].f) % 16 == 0 &&
+          reinterpret_cast<uintptr_t>(L[l].t) % 16 == 0;
+  }
+  if (vec) return vcycle<T, VW>(nl, L, b, nu1, nu2, ncoarse, omega, st);
+  return vcycle<T, 1>(nl, L, b, nu1, nu2, ncoarse, omega, st);
 }
 
 }  // namespace
@@ -226,6 +312,10 @@ QDS_API int qds_mg_vcycle(int32_t dtype, int32_t n_levels, const qds_mg_level* l
   for (int l = 0; l < n_levels; ++l)
     QDS_CHECK_ARG(levels[l].d && levels[l].we && levels[l].wn && levels[l].u && levels[l].f && levels[l].t,
                   "qds_mg_vcycle: null pointer on level %d", l);
+  for (int l = 0; l < n_levels; ++l)
+    QDS_CHECK_ARG(levels[l].nx * levels[l].ny * (int64_t)(levels[l].ldu ? levels[l].ldu : b) < (int64_t(1) << 31) &&
+                  levels[l].nx * levels[l].ny * (int64_t)(levels[l].ldf ? levels[l].ldf : b) < (int64_t(1) << 31),
+                  "qds_mg_vcycle: level %d exceeds 32-bit indexing", l);
   hipStream_t st = qds_stream(stream);
   if (dtype == qds::F32) {
     Level<float> L[32];
@@ -234,7 +324,7 @@ QDS_API int qds_mg_vcycle(int32_t dtype, int32_t n_levels, const qds_mg_level* l
                           (const float*)levels[l].wn, (float*)levels[l].u, (float*)levels[l].f,
                           (float*)levels[l].t, levels[l].ldu ? levels[l].ldu : b,
                           levels[l].ldf ? levels[l].ldf : b};
-    return vcycle<float>(n_levels, L, b, nu_pre, nu_post, coarse_sweeps, (float)omega, st);
+    return vcycle_any<float>(n_levels, L, b, nu_pre, nu_post, coarse_sweeps, (float)omega, st);
   }
   if (dtype == qds::F64) {
     Level<double> L[32];
@@ -243,7 +333,7 @@ QDS_API int qds_mg_vcycle(int32_t dtype, int32_t n_levels, const qds_mg_level* l
                            (const double*)levels[l].we, (const double*)levels[l].wn,
                            (double*)levels[l].u, (double*)levels[l].f, (double*)levels[l].t,
                            levels[l].ldu ? levels[l].ldu : b, levels[l].ldf ? levels[l].ldf : b};
-    return vcycle<double>(n_levels, L, b, nu_pre, nu_post, coarse_sweeps, omega, st);
+    return vcycle_any<double>(n_levels, L, b, nu_pre, nu_post, coarse_sweeps, omega, st);
   }
   QDS_CHECK_ARG(false, "qds_mg_vcycle: dtype must be F32 or F64");
   return QDS_EINVAL;

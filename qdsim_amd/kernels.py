@@ -169,6 +169,17 @@ def block_residual(HX, MX, lam, R):
                                  ptr(R), _ld(R), stream_ptr()), "qds_block_residual")
 
 
+def residual_norms(HX, MX, lam, R):
+    """R = HX - MX diag(lam); returns (||R_j||^2, ||MX_j||^2) as FP64 device tensors."""
+    n, b = HX.shape
+    out = torch.empty(2 * b, dtype=torch.float64, device=HX.device)
+    wsb = lib.qds_residual_norms_workspace(b)
+    wp, wn = workspace("resnorm").ptr(wsb)
+    check(lib.qds_residual_norms(code(HX), n, b, ptr(HX), _ld(HX), ptr(MX), _ld(MX), ptr(lam.contiguous()),
+                                 ptr(R), _ld(R), ptr(out), wp, wn, stream_ptr()), "qds_residual_norms")
+    return out[:b], out[b:]
+
+
 def coldot(X, Y, conj=True):
     n, b = X.shape
     cplx = X.is_complex()

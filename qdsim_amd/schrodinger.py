@@ -34,7 +34,7 @@ KIN_SCALE = HBAR * HBAR / 2.0 / EV_TO_J / (NM * NM)   # hbar^2/2 in eV nm^2 kg
 def default_block(k, n):
     b = ((k + 8) + 3) // 4 * 4
     b = max(b, 8)
-    while 3 * b > n and b > 4:
+    while 4 * b > n and b > 4:
         b -= 4
     return b
 

@@ -128,3 +128,24 @@ def test_residual_and_coldot(dtype):
     if cplx:
         d2 = kernels.coldot(HX, MX, conj=False).cpu().numpy()
         assert np.allclose(d2, (h * mm).sum(0), rtol=1e-10, atol=1e-9)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("b", [4, 12, 24, 32])
+def test_residual_norms(dtype, b):
+    from qdsim_amd import kernels
+    dev = torch.device("cuda")
+    n = 70001
+    rng = np.random.default_rng(7)
+    buf = torch.from_numpy(rng.standard_normal((n, 3 * b))).to(dev, dtype)
+    HX, MX = buf[:, :b], buf[:, b:2 * b]
+    lam = torch.from_numpy(rng.standard_normal(b)).to(dev, torch.float64)
+    R = torch.empty((n, b), dtype=dtype, device=dev)
+    rn, mn = kernels.residual_norms(HX, MX, lam, R)
+    h = HX.double().cpu().numpy(); m = MX.double().cpu().numpy(); l = lam.cpu().numpy()
+    ref = h - m * l.astype(np.float32 if dtype == torch.float32 else np.float64)[None]
+    tol = 1e-6 if dtype == torch.float32 else 1e-13
+    assert np.abs(R.double().cpu().numpy() - ref).max() < tol * 10
+    Rd = R.double().cpu().numpy()
+    assert np.allclose(rn.cpu().numpy(), (Rd ** 2).sum(0), rtol=1e-10)
+    assert np.allclose(mn.cpu().numpy(), (m ** 2).sum(0), rtol=1e-10)

@@ -173,3 +173,22 @@ def test_cap_solver_matches_dense_oracle():
     Xf = X[ref["free"]]
     R = ref["A"] @ Xf - ref["Md"] @ Xf * lam[None]
     assert np.linalg.norm(R, axis=0).max() / np.abs(lam).max() < 1e-6
+
+
+def test_fem_solver_reference_workflow():
+    """The reference's own FEM workflow (fem_solver.pyx:459-505): assemble, load,
+    Dirichlet at nodes [0, nx-1], solve -- against scipy spsolve on the same system."""
+    import scipy.sparse.linalg as spla
+    from qdsim_amd.fem import FEMSolver
+    from qdsim_amd.mesh import SimpleMesh
+    mesh = SimpleMesh(10, 8, 20e-9, 16e-9)
+    fs = FEMSolver(mesh)
+    K = fs.assemble_stiffness_matrix()
+    fs.assemble_mass_matrix()
+    f = fs.assemble_load_vector(lambda x, y: 1.0)
+    Kbc, fbc = fs.apply_dirichlet_bc([0, mesh.nx - 1], [0.0, 1.0], K, f)
+    sol = fs.solve_linear_system(Kbc, fbc)
+    ref = spla.spsolve(Kbc.tocsc(), fbc)
+    assert np.allclose(sol, ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
+    q = fs.compute_element_quality()
+    assert q.shape == (mesh.num_elements,) and np.all((q > 0) & (q <= 1))
