@@ -127,6 +127,24 @@ int qds_gram(int32_t dtype, int64_t n, int32_t m, const void* A, int64_t lda, in
 int qds_block_update(int32_t dtype, int64_t n, int32_t m, int32_t k, const void* S, int64_t lds,
                      const void* C1, void* Y1, int64_t ldy1, const void* C2, void* Y2, int64_t ldy2,
                      void* stream);
+/* Column-block forms: the logical n x m operand is up to four separately stored
+ * row-major blocks side by side (block j: p[j], leading dimension ld[j], cols[j]
+ * columns), so iterate blocks X, W, P can live in their own contiguous buffers.
+ * Gram: C = A^T [B1 | B2] (B2 may be NULL).  Update: Y1 = S C1 (Y2 = S C2), C row-major
+ * m x k, k <= 64, each block 16-byte aligned with widths and leading dimensions a
+ * multiple of 4 (FP32) / 2 (FP64).                                                */
+typedef struct {
+  const void* p[4];
+  int64_t ld[4];
+  int32_t cols[4];
+  int32_t nb;
+} qds_colblocks;
+int qds_gram_blocks(int32_t dtype, int64_t n, const qds_colblocks* A /* host */,
+                    const qds_colblocks* B1 /* host */, const qds_colblocks* B2 /* host */,
+                    int32_t sym, double* C, void* workspace, int64_t ws_bytes, void* stream);
+int qds_block_update_blocks(int32_t dtype, int64_t n, const qds_colblocks* S /* host */, int32_t k,
+                            const void* C1, void* Y1, int64_t ldy1, const void* C2, void* Y2,
+                            int64_t ldy2, void* stream);
 int qds_block_residual(int32_t dtype, int64_t n, int32_t b, const void* HX, int64_t ldh,
                        const void* MX, int64_t ldm, const double* lam, void* R, int64_t ldr,
                        void* stream);
@@ -134,8 +152,10 @@ int qds_block_residual(int32_t dtype, int64_t n, int32_t b, const void* HX, int6
  * (FP64 sums, real dtypes): the residual step of each block-eigensolver iteration. */
 int64_t qds_residual_norms_workspace(int32_t b);
 int qds_residual_norms(int32_t dtype, int64_t n, int32_t b, const void* HX, int64_t ldh,
-                       const void* MX, int64_t ldm, const double* lam, void* R, int64_t ldr,
-                       double* out, void* workspace, int64_t ws_bytes, void* stream);
+                       const void* MX, int64_t ldm, const double* lam,
+                       const double* colscale /* optional: R columns scaled after the norms */,
+                       void* R, int64_t ldr, double* out, void* workspace, int64_t ws_bytes,
+                       void* stream);
 int64_t qds_coldot_workspace(int32_t b);
 int qds_coldot(int32_t dtype, int64_t n, int32_t b, const void* X, int64_t ldx, const void* Y,
                int64_t ldy, int32_t conj, double* out, void* workspace, int64_t ws_bytes,

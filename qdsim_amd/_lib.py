@@ -39,10 +39,13 @@ _SIGNATURES = {
     "qds_gram": ([_i32, _i64, _i32, _p, _i64, _i32, _p, _i64, _i32, _p, _i64, _i32, _p, _p, _i64, _p],
                  _c.c_int),
     "qds_block_update": ([_i32, _i64, _i32, _i32, _p, _i64, _p, _p, _i64, _p, _p, _i64, _p], _c.c_int),
+    "qds_gram_blocks": ([_i32, _i64, _p, _p, _p, _i32, _p, _p, _i64, _p], _c.c_int),
+    "qds_block_update_blocks": ([_i32, _i64, _p, _i32, _p, _p, _i64, _p, _p, _i64, _p], _c.c_int),
     "qds_block_residual": ([_i32, _i64, _i32, _p, _i64, _p, _i64, _p, _p, _i64, _p], _c.c_int),
     "qds_coldot_workspace": ([_i32], _i64),
     "qds_residual_norms_workspace": ([_i32], _i64),
-    "qds_residual_norms": ([_i32, _i64, _i32, _p, _i64, _p, _i64, _p, _p, _i64, _p, _p, _i64, _p], _c.c_int),
+    "qds_residual_norms": ([_i32, _i64, _i32, _p, _i64, _p, _i64, _p, _p, _p, _i64, _p, _p, _i64, _p],
+                           _c.c_int),
     "qds_coldot": ([_i32, _i64, _i32, _p, _i64, _p, _i64, _i32, _p, _p, _i64, _p], _c.c_int),
     "qds_mg_vcycle": ([_i32, _i32, _p, _i32, _i32, _i32, _i32, _d, _p], _c.c_int),
     "qds_mg_apply": ([_i32, _p, _i32, _p, _p, _p], _c.c_int),
@@ -53,6 +56,11 @@ _SIGNATURES = {
     "qds_cg": ([_i64, _i64, _p, _p, _p, _p, _p, _d, _i32, _i32, _p, _i64, _c.POINTER(_i32),
                 _c.POINTER(_d), _p], _c.c_int),
 }
+
+
+class ColBlocks(ctypes.Structure):
+    """Mirror of qds_colblocks."""
+    _fields_ = [("p", _p * 4), ("ld", _i64 * 4), ("cols", _i32 * 4), ("nb", _i32)]
 
 
 class MGLevel(ctypes.Structure):

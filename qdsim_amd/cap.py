@@ -7,12 +7,16 @@ reference never solves this problem: it adds |Im CAP| to the real potential,
 solves a real eigsh and subtracts a constant i 0.1 cap_strength afterwards
 (schrodinger_solver.pyx:160-164, 548-557; fixed_open_system_solver.pyx:321-328).
 
-Method: block preconditioned iteration on S = [X | W | P] with the bilinear
-(complex-symmetric) Rayleigh-Ritz projection S^T A S y = theta S^T M S y, the
-natural Galerkin condition for complex symmetric pencils.  The basis is first
-conditioned with the Hermitian Gram S^H M S (SVQB), and Ritz pairs are selected
+Method: block preconditioned iteration on S = [X | W | P] with the orthogonal
+(Hermitian) Rayleigh-Ritz projection S^H A S y = theta S^H M S y.  The bilinear
+projection S^T A S, the Galerkin condition native to complex-symmetric pencils,
+produced spurious Ritz values far from the spectrum on the small CAP test case;
+the orthogonal one keeps every Ritz value in the field of values of the pencil,
+and its fixed points are eigenpairs of the complex-symmetric problem.  The basis
+is conditioned with the Hermitian Gram S^H M S (SVQB), and Ritz pairs are selected
 by smallest real part (or by distance to a target).  All n-sized work runs in
-qdsim_hip kernels on the interleaved (re, im) layout:
+qdsim_hip kernels on the interleaved (re, im) layout, with X, W, P as separate
+contiguous (n, b) blocks:
   * fused CAP SpMM (qds_spmm_cap): (H - iW) X and M X from one pass over the
     pattern with three real value arrays;
   * every complex block is also a real (n, 2m) block, so the MFMA FP64 Gram of
@@ -109,29 +113,32 @@ def cap_lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, seed=0, t
     n = op.n
     assert b % 2 == 0 and 2 * b <= 64 and 3 * b <= n
     if drop_tol is None:
-        drop_tol = 1e-7 if dt == torch.complex64 else 1e-13
-    bufs = [dict(S=torch.zeros((n, 3 * b), dtype=dt, device=dev),
-                 HS=torch.zeros((n, 3 * b), dtype=dt, device=dev),
-                 MS=torch.zeros((n, 3 * b), dtype=dt, device=dev)) for _ in range(2)]
-    R = torch.empty((n, b), dtype=dt, device=dev)
-    cur, nxt = bufs
-    X = cur["S"][:, :b]
+        drop_tol = 1e-5 if dt == torch.complex64 else 1e-10
+
+    def blk():
+        return torch.zeros((n, b), dtype=dt, device=dev)
+
+    # contiguous (n, b) complex blocks; X, P and their images double-buffered
+    X, HX, MX = [blk(), blk()], [blk(), blk()], [blk(), blk()]
+    P, HP, MP = [blk(), blk()], [blk(), blk()], [blk(), blk()]
+    W, HW, MW, R = blk(), blk(), blk(), blk()
+    c = 0
     if X0 is None:
-        g = torch.Generator(device="cpu").manual_seed(seed)
-        Xr = torch.rand((n, b), generator=g, dtype=torch.float64) - 0.5
-        X.copy_(Xr.to(dev).to(dt))
+        g = torch.Generator(device=dev).manual_seed(seed)
+        rdt = torch.float32 if dt == torch.complex64 else torch.float64
+        X[c].copy_((torch.rand((n, b), generator=g, dtype=rdt, device=dev) - 0.5).to(dt))
         if precond is not None:
-            R.copy_(X)
-            precond(_rv(R), _rv(X))
+            R.copy_(X[c])
+            precond(_rv(R), _rv(X[c]))
     else:
-        X.copy_(X0[:, :b].to(dt))
-    op.apply(X, cur["HS"][:, :b], cur["MS"][:, :b])
-    G = kernels.gram(_rv(X), _rv(cur["HS"][:, :b]), _rv(cur["MS"][:, :b]))
+        X[c].copy_(X0[:, :b].to(dt))
+    op.apply(X[c], HX[c], MX[c])
+    G = kernels.gram_blocks([_rv(X[c])], [_rv(HX[c])], [_rv(MX[c])])
     lam, C = _rr(G, b, b, drop_tol, target)
     Cr = _real_form(C)
-    for key in ("S", "HS", "MS"):
-        kernels.block_update(_rv(cur[key][:, :b]), Cr, _rv(nxt[key][:, :b]))
-    cur, nxt = nxt, cur
+    for src in (X, HX, MX):
+        kernels.block_update_blocks([_rv(src[c])], Cr, _rv(src[1 - c]))
+    c = 1 - c
     have_p = False
     history = []
     converged = False
@@ -139,12 +146,11 @@ def cap_lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, seed=0, t
     best, best_it = float("inf"), 0
     it = 0
     for it in range(1, maxiter + 1):
-        HX = cur["HS"][:, :b]
-        MX = cur["MS"][:, :b]
-        kernels.block_residual(HX, MX, lam.to(torch.complex128).contiguous(), R)
+        kernels.block_residual(HX[c], MX[c], lam.to(torch.complex128).contiguous(), R)
         rn = kernels.coldot(R, R, conj=True).real
-        mn = kernels.coldot(MX, MX, conj=True).real
-        res = rn.clamp_min(0).sqrt() / (lam.abs() * mn.clamp_min(1e-300).sqrt()).clamp_min(1e-300)
+        mn = kernels.coldot(MX[c], MX[c], conj=True).real
+        scale = lam[:k].abs().max().clamp_min(1e-300)
+        res = rn.clamp_min(0).sqrt() / (torch.maximum(lam.abs(), scale) * mn.clamp_min(1e-300).sqrt())
         worst = float(res[:k].max().item())
         history.append(worst)
         if verbose:
@@ -157,28 +163,29 @@ def cap_lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, seed=0, t
                 best, best_it = worst, it
             elif it - best_it >= stall_iters:
                 break
-        W = cur["S"][:, b:2 * b]
         if precond is not None:
             precond(_rv(R), _rv(W))
         else:
             W.copy_(R)
-        op.apply(W, cur["HS"][:, b:2 * b], cur["MS"][:, b:2 * b])
-        m = 3 * b if have_p else 2 * b
-        G = kernels.gram(_rv(cur["S"][:, :m]), _rv(cur["HS"][:, :m]), _rv(cur["MS"][:, :m]))
+        op.apply(W, HW, MW)
+        S = [X[c], W] + ([P[c]] if have_p else [])
+        HS = [HX[c], HW] + ([HP[c]] if have_p else [])
+        MS = [MX[c], MW] + ([MP[c]] if have_p else [])
+        m = len(S) * b
+        G = kernels.gram_blocks([_rv(x) for x in S], [_rv(x) for x in HS], [_rv(x) for x in MS])
         lam, C = _rr(G, m, b, drop_tol, target)
         CP = C.clone()
         CP[:b] = 0
         Cr, CPr = _real_form(C), _real_form(CP)
-        for key in ("S", "HS", "MS"):
-            kernels.block_update(_rv(cur[key][:, :m]), Cr, _rv(nxt[key][:, :b]), CPr,
-                                 _rv(nxt[key][:, 2 * b:3 * b]))
-        cur, nxt = nxt, cur
+        for src, img, pimg in ((S, X, P), (HS, HX, HP), (MS, MX, MP)):
+            kernels.block_update_blocks([_rv(x) for x in src], Cr, _rv(img[1 - c]), CPr, _rv(pimg[1 - c]))
+        c = 1 - c
         have_p = True
     torch.cuda.synchronize()
-    return lam, cur["S"][:, :b], res, it, converged, history
+    return lam, X[c], res, it, converged, history
 
 
-def solve_cap(prob, k, b=None, target=None, tol=1e-8, tol32=3e-4, maxiter=600, seed=0, verbose=False):
+def solve_cap(prob, k, b=None, target=None, tol=1e-6, tol32=3e-4, maxiter=600, seed=0, verbose=False):
     """Lowest-real-part (or nearest-target, eV) eigenpairs of the CAP pencil of a
     SchrodingerProblem whose CAP coefficient is set.  Returns (E complex eV (np),
     eigenvectors (n_full, k) complex128 np, Hermitian M_nm-normalised)."""

@@ -40,6 +40,25 @@ struct GramShape {
   int sym;
 };
 
+// Up to four column blocks forming one logical n x cols matrix (each block its own
+// allocation and leading dimension).  start[j] is the first logical column of j.
+struct ColBlocks {
+  const void* p[4];
+  int64_t ld[4];
+  int32_t start[5];
+  int32_t nb;
+};
+
+template <typename T>
+__device__ __forceinline__ const T* col_ptr(const ColBlocks& B, int c, int64_t& ld) {
+  int j = 0;
+#pragma unroll
+  for (int t = 1; t < 4; ++t)
+    if (t < B.nb && c >= B.start[t]) j = t;
+  ld = B.ld[j];
+  return reinterpret_cast<const T*>(B.p[j]) + (c - B.start[j]);
+}
+
 __device__ __forceinline__ bool tile_needed(const GramShape& g, int ti, int tj) {
   if (!g.sym) return true;
   const int tl = tj < g.tk1 ? tj : tj - g.tk1;
@@ -61,9 +80,8 @@ constexpr int kGramSlots = 6;  // vector loads per thread per staged chunk (A an
 // the MFMAs of the current chunk run from LDS (FP64 copies of the rows).
 template <typename T, bool VEC, int MAXT>
 __global__ void __launch_bounds__(kBlock)
-gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, const T* __restrict__ A, int64_t lda,
-             const T* __restrict__ B1, int64_t ldb1, const T* __restrict__ B2, int64_t ldb2,
-             double* __restrict__ partial) {
+gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, ColBlocks A, ColBlocks B1,
+             ColBlocks B2, double* __restrict__ partial) {
   using L = GLoad<T, VEC>;
   constexpr int VW = L::W;
   extern __shared__ double smem[];
@@ -90,34 +108,34 @@ gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, const T* __r
     }
     s_count[wave] = c;
   }
-  // load slots: (row in chunk, column) of each vector this thread stages
+  // load slots: source pointer, leading dimension and LDS position of each vector
   const int k1pad = 16 * g.tk1;
   const int apr = g.mpad / VW;  // vectors per staged A row
   const int bpr = cw / VW;
-  int a_rr[kGramSlots], a_c[kGramSlots], b_rr[kGramSlots], b_c[kGramSlots];
+  int a_rr[kGramSlots], a_lc[kGramSlots], b_rr[kGramSlots], b_lc[kGramSlots];
+  const T* a_src[kGramSlots];
   const T* b_src[kGramSlots];
-  int64_t b_ld[kGramSlots];
+  int64_t a_ld[kGramSlots], b_ld[kGramSlots];
   bool a_ok[kGramSlots], b_ok[kGramSlots];
 #pragma unroll
   for (int s = 0; s < kGramSlots; ++s) {
-    const int ta = threadIdx.x + s * kBlock;
-    a_rr[s] = ta / apr;
-    a_c[s] = (ta - a_rr[s] * apr) * VW;
-    a_ok[s] = a_rr[s] < kRB && a_c[s] < g.m;
-    const int tb = threadIdx.x + s * kBlock;
-    b_rr[s] = tb / bpr;
-    const int cl = (tb - b_rr[s] * bpr) * VW;   // local column
-    b_c[s] = cl;
-    const int c = tj0 * 16 + cl;                 // global padded column
+    const int t = threadIdx.x + s * kBlock;
+    a_rr[s] = t / apr;
+    a_lc[s] = (t - a_rr[s] * apr) * VW;
+    a_ok[s] = a_rr[s] < kRB && a_lc[s] < g.m;
+    a_src[s] = a_ok[s] ? col_ptr<T>(A, a_lc[s], a_ld[s]) : nullptr;
+    if (!a_ok[s]) a_ld[s] = 0;
+    b_rr[s] = t / bpr;
+    b_lc[s] = (t - b_rr[s] * bpr) * VW;
+    const int c = tj0 * 16 + b_lc[s];   // global padded column
     if (c < k1pad) {
-      b_src[s] = B1; b_ld[s] = ldb1;
       b_ok[s] = b_rr[s] < kRB && c < g.k1;
-      b_c[s] = cl | (c << 16);
+      b_src[s] = b_ok[s] ? col_ptr<T>(B1, c, b_ld[s]) : nullptr;
     } else {
-      b_src[s] = B2; b_ld[s] = ldb2;
       b_ok[s] = b_rr[s] < kRB && (c - k1pad) < g.k2;
-      b_c[s] = cl | ((c - k1pad) << 16);
+      b_src[s] = b_ok[s] ? col_ptr<T>(B2, c - k1pad, b_ld[s]) : nullptr;
     }
+    if (!b_ok[s]) b_ld[s] = 0;
   }
   __syncthreads();
   const int nslots = s_count[wave];
@@ -134,12 +152,12 @@ gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, const T* __r
     for (int s = 0; s < kGramSlots; ++s) {
       const int64_t row = r0 + a_rr[s];
       if (a_ok[s] && row < r_end)
-        ra[s] = *reinterpret_cast<const typename L::v*>(A + row * lda + a_c[s]);
+        ra[s] = *reinterpret_cast<const typename L::v*>(a_src[s] + row * a_ld[s]);
       else
         ra[s] = typename L::v{};
       const int64_t rowb = r0 + b_rr[s];
       if (b_ok[s] && rowb < r_end)
-        rb[s] = *reinterpret_cast<const typename L::v*>(b_src[s] + rowb * b_ld[s] + (b_c[s] >> 16));
+        rb[s] = *reinterpret_cast<const typename L::v*>(b_src[s] + rowb * b_ld[s]);
       else
         rb[s] = typename L::v{};
     }
@@ -152,12 +170,12 @@ gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, const T* __r
       if (a_rr[s] < kRB) {
         const T* v = reinterpret_cast<const T*>(&ra[s]);
 #pragma unroll
-        for (int e = 0; e < VW; ++e) sA[a_rr[s] * g.mpad + a_c[s] + e] = (double)v[e];
+        for (int e = 0; e < VW; ++e) sA[a_rr[s] * g.mpad + a_lc[s] + e] = (double)v[e];
       }
       if (b_rr[s] < kRB) {
         const T* v = reinterpret_cast<const T*>(&rb[s]);
 #pragma unroll
-        for (int e = 0; e < VW; ++e) sB[b_rr[s] * cw + (b_c[s] & 0xffff) + e] = (double)v[e];
+        for (int e = 0; e < VW; ++e) sB[b_rr[s] * cw + b_lc[s] + e] = (double)v[e];
       }
     }
     __syncthreads();
@@ -194,11 +212,20 @@ gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, const T* __r
   }
 }
 
-__global__ void gram_reduce(int nblk, GramShape g, const double* __restrict__ partial,
-                            double* __restrict__ C) {
+// grid: one block per 32 outputs; the 8 waves-halves of a block split the partial
+// rows (thread group g sums rows g, g + 8, ...), then combine in a fixed order.
+constexpr int kRedOut = 32;
+constexpr int kRedGroups = kBlock / kRedOut;
+__global__ void __launch_bounds__(kBlock)
+gram_reduce(int nblk, GramShape g, const double* __restrict__ partial, double* __restrict__ C) {
+  __shared__ double red[kBlock];
   const int ktot = g.k1 + g.k2;
   const int k1pad = 16 * g.tk1;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < g.m * ktot; t += gridDim.x * blockDim.x) {
+  const int o = threadIdx.x % kRedOut;
+  const int grp = threadIdx.x / kRedOut;
+  const int t = blockIdx.x * kRedOut + o;
+  double a = 0.0;
+  if (t < g.m * ktot) {
     const int i = t / ktot, j = t - i * ktot;
     int jp = j < g.k1 ? j : k1pad + (j - g.k1);
     int ip = i;
@@ -211,17 +238,34 @@ __global__ void gram_reduce(int nblk, GramShape g, const double* __restrict__ pa
         jp = tmp + half_off;
       }
     }
+    const int64_t stride = (int64_t)g.mpad * g.kpad;
+    const double* src = partial + ip * g.kpad + jp;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int b = grp;
+    for (; b + 3 * kRedGroups < nblk; b += 4 * kRedGroups) {
+      a0 += src[(int64_t)b * stride];
+      a1 += src[(int64_t)(b + kRedGroups) * stride];
+      a2 += src[(int64_t)(b + 2 * kRedGroups) * stride];
+      a3 += src[(int64_t)(b + 3 * kRedGroups) * stride];
+    }
+    for (; b < nblk; b += kRedGroups) a0 += src[(int64_t)b * stride];
+    a = (a0 + a1) + (a2 + a3);
+  }
+  red[threadIdx.x] = a;
+  __syncthreads();
+  if (grp == 0 && t < g.m * ktot) {
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += partial[(int64_t)b * g.mpad * g.kpad + ip * g.kpad + jp];
+#pragma unroll
+    for (int q = 0; q < kRedGroups; ++q) s += red[q * kRedOut + o];
     C[t] = s;
   }
 }
 
 // ---------------------------------------------------------------- block update
-template <typename T> struct MT;
 // Operand layouts of both 16x16x4 MFMAs: A lane l = A[l%16][l/16], B lane l =
 // B[l/16][l%16].  Result rows differ: FP32 lane l register e holds D[4(l/16)+e][l%16],
 // FP64 holds D[l/16 + 4e][l%16].
+template <typename T> struct MT;
 template <> struct MT<float> {
   static constexpr int VEC = 4;
   typedef float4 vec_t;
@@ -242,16 +286,16 @@ constexpr int kRowTiles = 2;     // 16-row tiles per wave and pass
 constexpr int kKC = 32;          // rows of C staged per chunk when C does not fit in LDS
 constexpr int kUpdLds = 48 * 1024;
 
-// Y1 = S C1 (and Y2 = S C2).  One pass of a block covers kWaves * kRowTiles * 16
-// rows.  When NOUT * m * k fits in LDS, C is staged once per block and stays
-// resident (no barriers inside the pass loop); otherwise C is streamed kKC rows at
-// a time.  MAXKB bounds the contraction blocks whose S vectors a lane loads
-// before its MFMAs (all loads of a row tile in flight together).
+// Y1 = S C1 (and Y2 = S C2), S given as column blocks.  One pass of a block covers
+// kWaves * kRowTiles * 16 rows.  When NOUT * m * k fits in LDS, C is staged once per
+// block and stays resident (no barriers inside the pass loop); otherwise C is
+// streamed kKC rows at a time.  All S loads of a row tile's chunk are issued before
+// its MFMAs.
 template <typename T, int NOUT, int MAXOT>
 __global__ void __launch_bounds__(kBlock)
-block_update_kernel(int64_t n, int m, int k, const T* __restrict__ S, int64_t lds,
-                    const T* __restrict__ C1, const T* __restrict__ C2, T* __restrict__ Y1,
-                    int64_t ldy1, T* __restrict__ Y2, int64_t ldy2, int chunk) {
+block_update_kernel(int64_t n, int m, int k, ColBlocks S, const T* __restrict__ C1,
+                    const T* __restrict__ C2, T* __restrict__ Y1, int64_t ldy1, T* __restrict__ Y2,
+                    int64_t ldy2, int chunk) {
   using M = MT<T>;
   constexpr int VEC = M::VEC;
   constexpr int KB = 4 * VEC;  // contraction indices covered by one vector load per lane
@@ -305,7 +349,9 @@ block_update_kernel(int64_t n, int m, int k, const T* __restrict__ S, int64_t ld
           for (int j = 0; j < MAXKB; ++j) {
             const int kloc = cb0 + j * KB + q * VEC;
             if (row < n && kloc < kend) {
-              typename M::vec_t v = *reinterpret_cast<const typename M::vec_t*>(S + row * lds + kc + kloc);
+              int64_t ld;
+              const T* src = col_ptr<T>(S, kc + kloc, ld);
+              typename M::vec_t v = *reinterpret_cast<const typename M::vec_t*>(src + row * ld);
               const T* vp = reinterpret_cast<const T*>(&v);
 #pragma unroll
               for (int e = 0; e < VEC; ++e) sv[j][e] = vp[e];
@@ -320,7 +366,7 @@ block_update_kernel(int64_t n, int m, int k, const T* __restrict__ S, int64_t ld
             if (kb >= kend) break;
 #pragma unroll
             for (int e = 0; e < VEC; ++e) {
-              const int kloc = kb + q * VEC + e;   // row of the staged C (< chunk: zero padded)
+              const int kloc = kb + q * VEC + e;   // row of the staged C (zero padded)
 #pragma unroll
               for (int t = 0; t < MAXOT; ++t) {
                 if (t < kt) {
@@ -358,7 +404,7 @@ block_update_kernel(int64_t n, int m, int k, const T* __restrict__ S, int64_t ld
 }
 
 template <typename T, int NOUT>
-int launch_update(int64_t n, int m, int k, const T* S, int64_t lds, const T* C1, const T* C2, T* Y1,
+int launch_update(int64_t n, int m, int k, const ColBlocks& S, const T* C1, const T* C2, T* Y1,
                   int64_t ldy1, T* Y2, int64_t ldy2, hipStream_t st) {
   constexpr int KB = 4 * MT<T>::VEC;
   const int kp = (k + 15) / 16 * 16;
@@ -369,13 +415,13 @@ int launch_update(int64_t n, int m, int k, const T* S, int64_t lds, const T* C1,
   const int64_t passes = (n + kWaves * kRowTiles * 16 - 1) / (kWaves * kRowTiles * 16);
   const unsigned blocks = qds_blocks(passes, 1, 4096);
   if (kp <= 16)
-    hipLaunchKernelGGL((block_update_kernel<T, NOUT, 1>), dim3(blocks), dim3(kBlock), shm, st, n, m, k, S, lds,
+    hipLaunchKernelGGL((block_update_kernel<T, NOUT, 1>), dim3(blocks), dim3(kBlock), shm, st, n, m, k, S,
                        C1, C2, Y1, ldy1, Y2, ldy2, chunk);
   else if (kp <= 32)
-    hipLaunchKernelGGL((block_update_kernel<T, NOUT, 2>), dim3(blocks), dim3(kBlock), shm, st, n, m, k, S, lds,
+    hipLaunchKernelGGL((block_update_kernel<T, NOUT, 2>), dim3(blocks), dim3(kBlock), shm, st, n, m, k, S,
                        C1, C2, Y1, ldy1, Y2, ldy2, chunk);
   else
-    hipLaunchKernelGGL((block_update_kernel<T, NOUT, 4>), dim3(blocks), dim3(kBlock), shm, st, n, m, k, S, lds,
+    hipLaunchKernelGGL((block_update_kernel<T, NOUT, 4>), dim3(blocks), dim3(kBlock), shm, st, n, m, k, S,
                        C1, C2, Y1, ldy1, Y2, ldy2, chunk);
   QDS_LAUNCH_CHECK();
   return QDS_OK;
@@ -451,17 +497,27 @@ coldot_partial(int64_t n, int b, const T* __restrict__ X, int64_t ldx, const T* 
   }
 }
 
-__global__ void coldot_reduce(int nblk, int b, const double* __restrict__ partial, double* __restrict__ out,
-                              int cplx) {
-  const int t = threadIdx.x;
-  if (t >= b) return;
-  double a0 = 0.0, a1 = 0.0;
-  for (int k = 0; k < nblk; ++k) {
-    a0 += partial[(int64_t)k * 2 * b + t];
-    a1 += partial[(int64_t)k * 2 * b + b + t];
+// One block of 1024 threads; partial row k holds (re[b], im[b]) sums of block k.
+__global__ void __launch_bounds__(1024)
+coldot_reduce(int nblk, int b, const double* __restrict__ partial, double* __restrict__ out, int cplx) {
+  __shared__ double red[1024];
+  const int w = 2 * b;
+  const int groups = 1024 / w;
+  const int c = threadIdx.x % w;
+  const int g = threadIdx.x / w;
+  double a = 0.0;
+  if (g < groups)
+    for (int k = g; k < nblk; k += groups) a += partial[(int64_t)k * w + c];
+  red[threadIdx.x] = a;
+  __syncthreads();
+  if (threadIdx.x < w) {
+    double s = 0.0;
+    for (int q = 0; q < groups; ++q) s += red[q * w + threadIdx.x];
+    const int j = threadIdx.x % b;
+    const int part = threadIdx.x / b;   // 0: real sums, 1: imaginary sums
+    if (cplx) out[2 * j + part] = s;
+    else if (part == 0) out[j] = s;
   }
-  if (cplx) { out[2 * t] = a0; out[2 * t + 1] = a1; }
-  else out[t] = a0;
 }
 
 int gram_blocks(int64_t n) {
@@ -488,32 +544,81 @@ QDS_API int64_t qds_gram_workspace(int64_t n, int32_t m, int32_t k1, int32_t k2)
   return (int64_t)gram_blocks(n) * g.mpad * g.kpad * (int64_t)sizeof(double);
 }
 
-// C (m x (k1+k2), row-major FP64) = A^T [B1 | B2].  B2 may be NULL (k2 = 0).
-// sym = 1 asserts each half is symmetric (m == k1 == k2 layout) and computes only
-// its upper tiles.  dtype F32 or F64 gives the storage type of A and B.
-QDS_API int qds_gram(int32_t dtype, int64_t n, int32_t m, const void* A, int64_t lda, int32_t k1,
-                     const void* B1, int64_t ldb1, int32_t k2, const void* B2, int64_t ldb2,
-                     int32_t sym, double* C, void* workspace, int64_t ws_bytes, void* stream) {
-  QDS_CHECK_ARG(A && B1 && C && workspace, "qds_gram: null pointer");
-  QDS_CHECK_ARG(m > 0 && k1 > 0 && k2 >= 0 && (k2 == 0 || B2), "qds_gram: bad shape");
+extern "C" {
+// Host description of a logical matrix made of up to four column blocks.
+typedef struct {
+  const void* p[4];
+  int64_t ld[4];
+  int32_t cols[4];
+  int32_t nb;
+} qds_colblocks;
+}
+
+namespace {
+bool to_device_blocks(const qds_colblocks* h, ColBlocks& d, int& total) {
+  if (!h || h->nb < 1 || h->nb > 4) return false;
+  d.nb = h->nb;
+  int acc = 0;
+  for (int j = 0; j < 4; ++j) {
+    d.p[j] = j < h->nb ? h->p[j] : nullptr;
+    d.ld[j] = j < h->nb ? h->ld[j] : 0;
+    d.start[j] = acc;
+    if (j < h->nb) {
+      if (!h->p[j] || h->cols[j] < 1 || h->ld[j] < h->cols[j]) return false;
+      acc += h->cols[j];
+    }
+  }
+  d.start[4] = acc;
+  total = acc;
+  return true;
+}
+
+// 16-byte vector access is possible when every block starts 16-byte aligned, its
+// leading dimension and width are multiples of the vector width
+bool blocks_vec(const ColBlocks& d, const qds_colblocks* h, int es) {
+  const int vw = 16 / es;
+  for (int j = 0; j < d.nb; ++j)
+    if ((reinterpret_cast<uintptr_t>(d.p[j]) % 16) || (d.ld[j] * es) % 16 || h->cols[j] % vw) return false;
+  return true;
+}
+
+qds_colblocks single(const void* p, int64_t ld, int cols) {
+  qds_colblocks b;
+  for (int j = 0; j < 4; ++j) { b.p[j] = nullptr; b.ld[j] = 0; b.cols[j] = 0; }
+  b.p[0] = p; b.ld[0] = ld; b.cols[0] = cols; b.nb = 1;
+  return b;
+}
+}  // namespace
+
+// C (m x (k1+k2), row-major FP64) = A^T [B1 | B2] for column-block matrices.
+// B2 may be NULL.  sym = 1 asserts each half is symmetric (m == k1 == k2) and
+// computes only its upper tiles.  dtype F32 or F64 gives the storage type.
+QDS_API int qds_gram_blocks(int32_t dtype, int64_t n, const qds_colblocks* A, const qds_colblocks* B1,
+                            const qds_colblocks* B2, int32_t sym, double* C, void* workspace,
+                            int64_t ws_bytes, void* stream) {
+  ColBlocks dA, dB1, dB2;
+  int m = 0, k1 = 0, k2 = 0;
+  QDS_CHECK_ARG(to_device_blocks(A, dA, m) && to_device_blocks(B1, dB1, k1), "qds_gram: bad A or B1 blocks");
+  if (B2) {
+    QDS_CHECK_ARG(to_device_blocks(B2, dB2, k2), "qds_gram: bad B2 blocks");
+  } else {
+    dB2 = dB1;
+    dB2.nb = 1;
+    k2 = 0;
+  }
+  QDS_CHECK_ARG(C && workspace, "qds_gram: null pointer");
   QDS_CHECK_ARG(!sym || (m == k1 && (k2 == 0 || k2 == m)), "qds_gram: sym needs square halves");
+  QDS_CHECK_ARG(dtype == qds::F32 || dtype == qds::F64, "qds_gram: dtype must be F32 or F64");
   GramShape g = make_shape(m, k1, k2, sym);
   QDS_CHECK_ARG(g.tm <= 255 && g.tk1 + g.tk2 <= 255, "qds_gram: block too wide");
-  // 16-byte loads when every row start of A, B1, B2 is 16-byte aligned and the column
-  // counts are multiples of the vector width (padding columns are masked per vector)
   const int es = dtype == qds::F32 ? 4 : 8;
-  const int vw = 16 / es;
-  auto al = [&](const void* p, int64_t ld) {
-    return p == nullptr || ((reinterpret_cast<uintptr_t>(p) % 16) == 0 && (ld * es) % 16 == 0);
-  };
-  const bool vec = al(A, lda) && al(B1, ldb1) && al(B2, ldb2) && m % vw == 0 && k1 % vw == 0 &&
-                   k2 % vw == 0;
-  const int vw_used = vec ? vw : 1;
+  const bool vec = blocks_vec(dA, A, es) && blocks_vec(dB1, B1, es) && (!B2 || blocks_vec(dB2, B2, es));
+  const int vw_used = vec ? 16 / es : 1;
   const int tk = g.tk1 + g.tk2;
-  // column groups: every wave holds at most kMaxTiles accumulators and the staged
-  // rows fit in 64 KiB of LDS
-  int tpg = tk;
-  auto slots_ok = [&](int tp) {
+  // column groups: every wave holds at most 24 accumulators, every thread at most
+  // kGramSlots staged vectors per operand, and the staged rows fit 64 KiB of LDS
+  auto slots = [&](int tp) {
+    int mx = 0;
     for (int j0 = 0; j0 < tk; j0 += tp) {
       const int j1 = j0 + tp < tk ? j0 + tp : tk;
       for (int w = 0; w < kWaves; ++w) {
@@ -524,31 +629,22 @@ QDS_API int qds_gram(int32_t dtype, int64_t n, int32_t m, const void* A, int64_t
           const int tl = tj < g.tk1 ? tj : tj - g.tk1;
           if (!sym || ti <= tl) ++c;
         }
-        if (c > kMaxTiles) return false;
+        mx = c > mx ? c : mx;
       }
     }
-    // every staged row chunk must fit the per-thread load slots
+    return mx;
+  };
+  auto fits = [&](int tp) {
+    if (slots(tp) > 24) return false;
     if ((kRB * g.mpad / vw_used + kBlock - 1) / kBlock > kGramSlots) return false;
     if ((kRB * 16 * tp / vw_used + kBlock - 1) / kBlock > kGramSlots) return false;
     return (size_t)kRB * (g.mpad + 16 * tp) * sizeof(double) <= 64 * 1024;
   };
-  while (tpg > 1 && !slots_ok(tpg)) --tpg;
-  QDS_CHECK_ARG(slots_ok(tpg), "qds_gram: A block of %d columns too wide", m);
+  int tpg = tk;
+  while (tpg > 1 && !fits(tpg)) --tpg;
+  QDS_CHECK_ARG(fits(tpg), "qds_gram: A block of %d columns too wide", m);
   const int ngroups = (tk + tpg - 1) / tpg;
-  int max_slots = 0;
-  for (int j0 = 0; j0 < tk; j0 += tpg) {
-    const int j1 = j0 + tpg < tk ? j0 + tpg : tk;
-    for (int w = 0; w < kWaves; ++w) {
-      int c = 0;
-      const int nt = g.tm * (j1 - j0);
-      for (int t = w; t < nt; t += kWaves) {
-        const int ti = t / (j1 - j0), tj = j0 + (t - ti * (j1 - j0));
-        const int tl = tj < g.tk1 ? tj : tj - g.tk1;
-        if (!sym || ti <= tl) ++c;
-      }
-      max_slots = c > max_slots ? c : max_slots;
-    }
-  }
+  const int max_slots = slots(tpg);
   const int nblk = gram_blocks(n);
   QDS_CHECK_ARG(ws_bytes >= qds_gram_workspace(n, m, k1, k2), "qds_gram: workspace too small");
   const int64_t rows_per_blk = (((n + nblk - 1) / nblk) + kRB - 1) / kRB * kRB;
@@ -560,8 +656,7 @@ QDS_API int qds_gram(int32_t dtype, int64_t n, int32_t m, const void* A, int64_t
     constexpr bool V = decltype(vecc)::value;
     constexpr int MT_ = decltype(maxt)::value;
     hipLaunchKernelGGL((gram_partial<T, V, MT_>), dim3(nblk, ngroups), dim3(kBlock), shm, st, n,
-                       rows_per_blk, g, tpg, (const T*)A, lda, (const T*)B1, ldb1, (const T*)B2, ldb2,
-                       partial);
+                       rows_per_blk, g, tpg, dA, dB1, dB2, partial);
   };
   auto by_tiles = [&](auto tag, auto vecc) {
     if (max_slots <= 8) launch(tag, vecc, std::integral_constant<int, 8>());
@@ -570,45 +665,59 @@ QDS_API int qds_gram(int32_t dtype, int64_t n, int32_t m, const void* A, int64_t
   };
   if (dtype == qds::F32 && vec) by_tiles(float(0), std::true_type());
   else if (dtype == qds::F32) by_tiles(float(0), std::false_type());
-  else if (dtype == qds::F64 && vec) by_tiles(double(0), std::true_type());
-  else if (dtype == qds::F64) by_tiles(double(0), std::false_type());
-  else
-    QDS_CHECK_ARG(false, "qds_gram: dtype must be F32 or F64");
+  else if (vec) by_tiles(double(0), std::true_type());
+  else by_tiles(double(0), std::false_type());
   QDS_LAUNCH_CHECK();
   const int tot = m * (k1 + k2);
-  hipLaunchKernelGGL(gram_reduce, dim3(qds_blocks(tot, 256, 1024)), dim3(256), 0, st, nblk, g,
-                     partial, C);
+  hipLaunchKernelGGL(gram_reduce, dim3((tot + kRedOut - 1) / kRedOut), dim3(kBlock), 0, st, nblk, g, partial, C);
   QDS_LAUNCH_CHECK();
   return QDS_OK;
 }
 
-// Y1 = S C1 (n x k), and Y2 = S C2 when C2 != NULL.  C1, C2: m x k row-major in the
-// storage type.  dtype F32 (m % 4 == 0) or F64 (m % 2 == 0); k <= 64.
+// Single-block form: A n x m (lda), B1 n x k1, B2 n x k2 (or NULL).
+QDS_API int qds_gram(int32_t dtype, int64_t n, int32_t m, const void* A, int64_t lda, int32_t k1,
+                     const void* B1, int64_t ldb1, int32_t k2, const void* B2, int64_t ldb2,
+                     int32_t sym, double* C, void* workspace, int64_t ws_bytes, void* stream) {
+  QDS_CHECK_ARG(A && B1 && m > 0 && k1 > 0 && k2 >= 0 && (k2 == 0 || B2), "qds_gram: bad shape");
+  qds_colblocks a = single(A, lda, m), b1 = single(B1, ldb1, k1), b2 = single(B2, ldb2, k2);
+  return qds_gram_blocks(dtype, n, &a, &b1, k2 ? &b2 : nullptr, sym, C, workspace, ws_bytes, stream);
+}
+
+// Y1 = S C1 (n x k), and Y2 = S C2 when C2 != NULL; S given as column blocks.
+// C1, C2: m x k row-major in the storage type.  k <= 64.  Every block's width and
+// leading dimension must be multiples of 4 (F32) or 2 (F64), blocks 16-byte aligned.
+QDS_API int qds_block_update_blocks(int32_t dtype, int64_t n, const qds_colblocks* S, int32_t k,
+                                    const void* C1, void* Y1, int64_t ldy1, const void* C2, void* Y2,
+                                    int64_t ldy2, void* stream) {
+  ColBlocks dS;
+  int m = 0;
+  QDS_CHECK_ARG(to_device_blocks(S, dS, m), "qds_block_update: bad S blocks");
+  QDS_CHECK_ARG(C1 && Y1 && (!C2 || Y2), "qds_block_update: null pointer");
+  QDS_CHECK_ARG(k >= 1 && k <= 16 * kMaxOutTiles, "qds_block_update: bad shape");
+  if (n == 0) return QDS_OK;
+  hipStream_t st = qds_stream(stream);
+  QDS_CHECK_ARG(dtype == qds::F32 || dtype == qds::F64, "qds_block_update: dtype must be F32 or F64");
+  const int es = dtype == qds::F32 ? 4 : 8;
+  QDS_CHECK_ARG(blocks_vec(dS, S, es), "qds_block_update: S blocks need 16-byte alignment and widths, "
+                "leading dimensions multiple of %d", 16 / es);
+  if (dtype == qds::F32)
+    return C2 ? launch_update<float, 2>(n, m, k, dS, (const float*)C1, (const float*)C2, (float*)Y1, ldy1,
+                                        (float*)Y2, ldy2, st)
+              : launch_update<float, 1>(n, m, k, dS, (const float*)C1, nullptr, (float*)Y1, ldy1, nullptr,
+                                        ldy2, st);
+  return C2 ? launch_update<double, 2>(n, m, k, dS, (const double*)C1, (const double*)C2, (double*)Y1, ldy1,
+                                       (double*)Y2, ldy2, st)
+            : launch_update<double, 1>(n, m, k, dS, (const double*)C1, nullptr, (double*)Y1, ldy1, nullptr,
+                                       ldy2, st);
+}
+
+// Single-block form: S n x m with leading dimension lds.
 QDS_API int qds_block_update(int32_t dtype, int64_t n, int32_t m, int32_t k, const void* S, int64_t lds,
                              const void* C1, void* Y1, int64_t ldy1, const void* C2, void* Y2,
                              int64_t ldy2, void* stream) {
-  QDS_CHECK_ARG(S && C1 && Y1 && (!C2 || Y2), "qds_block_update: null pointer");
-  QDS_CHECK_ARG(k >= 1 && k <= 16 * kMaxOutTiles && m >= 1, "qds_block_update: bad shape");
-  if (n == 0) return QDS_OK;
-  hipStream_t st = qds_stream(stream);
-  if (dtype == qds::F32) {
-    QDS_CHECK_ARG(m % 4 == 0 && lds % 4 == 0 && (reinterpret_cast<uintptr_t>(S) % 16) == 0,
-                  "qds_block_update: F32 needs m, lds multiple of 4 and 16-byte aligned S");
-    return C2 ? launch_update<float, 2>(n, m, k, (const float*)S, lds, (const float*)C1, (const float*)C2,
-                                        (float*)Y1, ldy1, (float*)Y2, ldy2, st)
-              : launch_update<float, 1>(n, m, k, (const float*)S, lds, (const float*)C1, nullptr,
-                                        (float*)Y1, ldy1, nullptr, ldy2, st);
-  } else if (dtype == qds::F64) {
-    QDS_CHECK_ARG(m % 2 == 0 && lds % 2 == 0 && (reinterpret_cast<uintptr_t>(S) % 16) == 0,
-                  "qds_block_update: F64 needs m, lds even and 16-byte aligned S");
-    return C2 ? launch_update<double, 2>(n, m, k, (const double*)S, lds, (const double*)C1,
-                                         (const double*)C2, (double*)Y1, ldy1, (double*)Y2, ldy2, st)
-              : launch_update<double, 1>(n, m, k, (const double*)S, lds, (const double*)C1, nullptr,
-                                         (double*)Y1, ldy1, nullptr, ldy2, st);
-  }
-  QDS_CHECK_ARG(false, "qds_block_update: dtype must be F32 or F64");
-  QDS_LAUNCH_CHECK();
-  return QDS_OK;
+  QDS_CHECK_ARG(S && m >= 1, "qds_block_update: bad S");
+  qds_colblocks s = single(S, lds, m);
+  return qds_block_update_blocks(dtype, n, &s, k, C1, Y1, ldy1, C2, Y2, ldy2, stream);
 }
 
 // R = HX - MX diag(lambda).  Real dtypes: lambda has b doubles.  Complex dtypes:
@@ -661,7 +770,7 @@ QDS_API int qds_coldot(int32_t dtype, int64_t n, int32_t b, const void* X, int64
     default: QDS_CHECK_ARG(false, "qds_coldot: bad dtype");
   }
   QDS_LAUNCH_CHECK();
-  hipLaunchKernelGGL(coldot_reduce, dim3(1), dim3(256), 0, st, (int)nblk, b, part, out, cplx ? 1 : 0);
+  hipLaunchKernelGGL(coldot_reduce, dim3(1), dim3(1024), 0, st, (int)nblk, b, part, out, cplx ? 1 : 0);
   QDS_LAUNCH_CHECK();
   return QDS_OK;
 }
@@ -707,8 +816,8 @@ constexpr int kResBlocks = 2048;
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 residual_norms_partial(int64_t n, int b, const T* __restrict__ HX, int64_t ldh, const T* __restrict__ MX,
-                       int64_t ldm, const double* __restrict__ lam, T* __restrict__ R, int64_t ldr,
-                       int64_t rows_per_blk, double* __restrict__ partial) {
+                       int64_t ldm, const double* __restrict__ lam, const double* __restrict__ colscale,
+                       T* __restrict__ R, int64_t ldr, int64_t rows_per_blk, double* __restrict__ partial) {
   const int groups = kBlock / b;
   const int c = threadIdx.x % b;
   const int gidx = threadIdx.x / b;
@@ -718,10 +827,11 @@ residual_norms_partial(int64_t n, int b, const T* __restrict__ HX, int64_t ldh, 
   const int64_t r_end = min(n, r_begin + rows_per_blk);
   if (gidx < groups) {
     const T l = (T)lam[c];
+    const T cs = colscale ? (T)colscale[c] : T(1);
     for (int64_t r = r_begin + gidx; r < r_end; r += groups) {
       const T m = MX[r * ldm + c];
       const T v = HX[r * ldh + c] - l * m;
-      R[r * ldr + c] = v;
+      R[r * ldr + c] = v * cs;
       sr += (double)v * (double)v;
       sm += (double)m * (double)m;
     }
@@ -740,13 +850,25 @@ residual_norms_partial(int64_t n, int b, const T* __restrict__ HX, int64_t ldh, 
   }
 }
 
-__global__ void residual_norms_reduce(int nblk, int b, const double* __restrict__ partial,
-                                      double* __restrict__ out) {
-  const int t = threadIdx.x;  // 2b threads
-  if (t >= 2 * b) return;
+// One block of 1024 threads: thread t sums column t % 2b over partial rows
+// t / 2b, t / 2b + groups, ...; the group sums are then combined in a fixed order.
+__global__ void __launch_bounds__(1024)
+residual_norms_reduce(int nblk, int b, const double* __restrict__ partial, double* __restrict__ out) {
+  __shared__ double red[1024];
+  const int w = 2 * b;
+  const int groups = 1024 / w;
+  const int c = threadIdx.x % w;
+  const int g = threadIdx.x / w;
   double a = 0.0;
-  for (int k = 0; k < nblk; ++k) a += partial[(int64_t)k * 2 * b + t];
-  out[t] = a;
+  if (g < groups)
+    for (int k = g; k < nblk; k += groups) a += partial[(int64_t)k * w + c];
+  red[threadIdx.x] = a;
+  __syncthreads();
+  if (threadIdx.x < w) {
+    double s = 0.0;
+    for (int q = 0; q < groups; ++q) s += red[q * w + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
 }
 }  // namespace
 
@@ -754,9 +876,12 @@ QDS_API int64_t qds_residual_norms_workspace(int32_t b) {
   return (int64_t)kResBlocks * 2 * b * sizeof(double);
 }
 
+// colscale (optional, b doubles) multiplies the stored residual columns after the
+// norms are taken: the eigensolver zeroes the columns of converged pairs (soft locking).
 QDS_API int qds_residual_norms(int32_t dtype, int64_t n, int32_t b, const void* HX, int64_t ldh,
-                               const void* MX, int64_t ldm, const double* lam, void* R, int64_t ldr,
-                               double* out, void* workspace, int64_t ws_bytes, void* stream) {
+                               const void* MX, int64_t ldm, const double* lam, const double* colscale,
+                               void* R, int64_t ldr, double* out, void* workspace, int64_t ws_bytes,
+                               void* stream) {
   QDS_CHECK_ARG(HX && MX && lam && R && out && workspace && b >= 1 && b <= 128,
                 "qds_residual_norms: bad arguments");
   QDS_CHECK_ARG(ws_bytes >= qds_residual_norms_workspace(b), "qds_residual_norms: workspace too small");
@@ -768,14 +893,14 @@ QDS_API int qds_residual_norms(int32_t dtype, int64_t n, int32_t b, const void* 
   double* part = static_cast<double*>(workspace);
   if (dtype == qds::F32)
     hipLaunchKernelGGL(residual_norms_partial<float>, dim3(nblk), dim3(kBlock), 0, st, n, b, (const float*)HX,
-                       ldh, (const float*)MX, ldm, lam, (float*)R, ldr, rpb, part);
+                       ldh, (const float*)MX, ldm, lam, colscale, (float*)R, ldr, rpb, part);
   else if (dtype == qds::F64)
     hipLaunchKernelGGL(residual_norms_partial<double>, dim3(nblk), dim3(kBlock), 0, st, n, b,
-                       (const double*)HX, ldh, (const double*)MX, ldm, lam, (double*)R, ldr, rpb, part);
+                       (const double*)HX, ldh, (const double*)MX, ldm, lam, colscale, (double*)R, ldr, rpb, part);
   else
     QDS_CHECK_ARG(false, "qds_residual_norms: dtype must be F32 or F64");
   QDS_LAUNCH_CHECK();
-  hipLaunchKernelGGL(residual_norms_reduce, dim3(1), dim3(256), 0, st, (int)nblk, b, part, out);
+  hipLaunchKernelGGL(residual_norms_reduce, dim3(1), dim3(1024), 0, st, (int)nblk, b, part, out);
   QDS_LAUNCH_CHECK();
   return QDS_OK;
 }

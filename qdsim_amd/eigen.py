@@ -65,7 +65,10 @@ def gpu_eigsh(A, B, k, which="SM", tol=1e-8, maxiter=1000, seed=0):
     Mv = torch.from_numpy(Bs.data.astype(np.float64)).to(dev)
     op = CSROperator(ip, ix, n, Hv, Mv, torch.float64)
     b = default_block(k, n)
-    res = lobpcg(op, k, b, None, tol=max(tol, 1e-12), maxiter=maxiter, seed=seed)
+    # ARPACK's tol bounds the relative error of the Ritz values, which is about the
+    # square of the relative residual: ask the block solver for 0.1 sqrt(tol)
+    rtol = min(max(0.1 * np.sqrt(tol), 1e-9), 1e-3)
+    res = lobpcg(op, k, b, None, tol=rtol, maxiter=maxiter, seed=seed)
     lam = res.eigenvalues[:k].cpu().numpy() * sign * (sA / sB)
     X = res.X[:, :k].cpu().numpy() / np.sqrt(d)[:, None] / np.sqrt(sB)
     order = np.argsort(lam) if sign > 0 else np.argsort(-lam)

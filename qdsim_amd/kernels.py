@@ -161,6 +161,50 @@ def block_update(S, C1, Y1, C2=None, Y2=None):
           "qds_block_update")
 
 
+def _colblocks(blocks):
+    """qds_colblocks for a list of (n, c_j) row-major blocks (views with unit column stride)."""
+    assert 1 <= len(blocks) <= 4
+    cb = _lib.ColBlocks()
+    for j, B in enumerate(blocks):
+        assert B.stride(1) == 1, "column blocks need unit column stride"
+        cb.p[j] = B.data_ptr()
+        cb.ld[j] = _ld(B)
+        cb.cols[j] = B.shape[1]
+    cb.nb = len(blocks)
+    return cb
+
+
+def gram_blocks(A, B1, B2=None, sym=False, out=None):
+    """[A_0 | A_1 | ...]^T [B1 | B2] in FP64 for lists of column blocks."""
+    n = A[0].shape[0]
+    m = sum(a.shape[1] for a in A)
+    k1 = sum(x.shape[1] for x in B1)
+    k2 = sum(x.shape[1] for x in B2) if B2 else 0
+    if out is None:
+        out = torch.empty((m, k1 + k2), dtype=torch.float64, device=A[0].device)
+    ca, c1 = _colblocks(A), _colblocks(B1)
+    c2 = _colblocks(B2) if B2 else None
+    wsb = lib.qds_gram_workspace(n, m, k1, k2)
+    wp, wn = workspace("gram").ptr(wsb)
+    check(lib.qds_gram_blocks(code(A[0]), n, ctypes.byref(ca), ctypes.byref(c1),
+                              ctypes.byref(c2) if c2 is not None else None, 1 if sym else 0, ptr(out),
+                              wp, wn, stream_ptr()), "qds_gram_blocks")
+    return out
+
+
+def block_update_blocks(S, C1, Y1, C2=None, Y2=None):
+    """Y1 = [S_0 | S_1 | ...] C1 (and Y2 = ... C2); C in the storage dtype of S."""
+    n = S[0].shape[0]
+    k = C1.shape[1]
+    dt = S[0].dtype
+    C1 = C1.to(dt).contiguous()
+    C2 = C2.to(dt).contiguous() if C2 is not None else None
+    cs = _colblocks(S)
+    check(lib.qds_block_update_blocks(code(S[0]), n, ctypes.byref(cs), k, ptr(C1), ptr(Y1), _ld(Y1),
+                                      ptr(C2), ptr(Y2), _ld(Y2) if Y2 is not None else 1, stream_ptr()),
+          "qds_block_update_blocks")
+
+
 def block_residual(HX, MX, lam, R):
     """R = HX - MX diag(lam); lam float64 (b,) or complex128 (b,)."""
     n, b = HX.shape
@@ -169,14 +213,15 @@ def block_residual(HX, MX, lam, R):
                                  ptr(R), _ld(R), stream_ptr()), "qds_block_residual")
 
 
-def residual_norms(HX, MX, lam, R):
-    """R = HX - MX diag(lam); returns (||R_j||^2, ||MX_j||^2) as FP64 device tensors."""
+def residual_norms(HX, MX, lam, R, colscale=None):
+    """R = (HX - MX diag(lam)) diag(colscale); returns (||HX - MX lam||_j^2, ||MX_j||^2)."""
     n, b = HX.shape
     out = torch.empty(2 * b, dtype=torch.float64, device=HX.device)
     wsb = lib.qds_residual_norms_workspace(b)
     wp, wn = workspace("resnorm").ptr(wsb)
+    cs = colscale.contiguous() if colscale is not None else None
     check(lib.qds_residual_norms(code(HX), n, b, ptr(HX), _ld(HX), ptr(MX), _ld(MX), ptr(lam.contiguous()),
-                                 ptr(R), _ld(R), ptr(out), wp, wn, stream_ptr()), "qds_residual_norms")
+                                 ptr(cs), ptr(R), _ld(R), ptr(out), wp, wn, stream_ptr()), "qds_residual_norms")
     return out[:b], out[b:]
 
 

@@ -7,9 +7,10 @@ lobpcg (eigen_solver.pyx:254-266).  This is a fresh block solver built on the
 qdsim_hip kernels:
 
 * the operator apply is the fused CSR SpMM (H and M share the pattern, one pass);
-* the block iterate is S = [X | W | P] stored as one row-major (n, 3b) buffer, with
-  H S and M S alongside, so the Rayleigh-Ritz Gram matrices S^T [HS | MS] come
-  out of one MFMA FP64 kernel call (upper tiles only);
+* the block iterate is S = [X | W | P] kept as three contiguous row-major (n, b)
+  buffers, with H S and M S alongside; the Rayleigh-Ritz Gram matrices
+  S^T [HS | MS] come out of one MFMA FP64 kernel call over the column-block lists
+  (upper tiles only);
 * Rayleigh-Ritz uses SVQB on the small FP64 Gram pair: scale, drop directions whose
   M-Gram eigenvalue falls below a threshold, solve the standard problem, which is
   robust when W and P become nearly dependent close to convergence;
@@ -62,14 +63,16 @@ def _svqb_rr(G, m, nb, drop_tol):
     eigenproblems run in host LAPACK (dsyevd), which measured 3.5x faster than
     the device eigh for these sizes (0.72 vs 2.5 ms including the transfers) and is
     robust for the nearly rank-deficient pairs met close to convergence.
-    Returns (theta[:nb], C (m x nb), n_kept) as device FP64 tensors."""
+    Returns (theta[:nb], C (m x nb) as device FP64 tensors, number of dropped
+    non-zero directions)."""
     dev = G.device
     g = G.detach().cpu().numpy()
     GH = 0.5 * (g[:, :m] + g[:, :m].T)
     GM = 0.5 * (g[:, m:2 * m] + g[:, m:2 * m].T)
     dg = np.diag(GM).copy()
-    # columns that vanished (converged residuals) get weight 0 and are dropped below
+    # columns that vanished (locked residuals) get weight 0 and are dropped below
     ok = dg > 1e-30 * dg.max()
+    n_zero = int((~ok).sum())
     d = np.where(ok, 1.0 / np.sqrt(np.where(ok, dg, 1.0)), 0.0)
     GMs = GM * d[:, None] * d[None, :]
     GHs = GH * d[:, None] * d[None, :]
@@ -83,7 +86,9 @@ def _svqb_rr(G, m, nb, drop_tol):
     Hk = 0.5 * (Hk + Hk.T)
     theta, Y = np.linalg.eigh(Hk)
     C = d[:, None] * (Qk @ Y[:, :nb])
-    return (torch.from_numpy(theta[:nb].copy()).to(dev), torch.from_numpy(C).to(dev), nk)
+    # directions dropped beyond the zero columns: the basis was near-dependent
+    dropped = (m - n_zero) - n_ok
+    return (torch.from_numpy(theta[:nb].copy()).to(dev), torch.from_numpy(C).to(dev), dropped)
 
 
 class LOBPCGResult:
@@ -102,41 +107,49 @@ def lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=None
     """Lowest k eigenpairs of (op.H, op.M) with block size b >= k.
 
     precond(R, W) writes W = T R (R, W are (n, b) blocks of op.dtype).
-    Stops when the relative residual ||H x - lambda M x|| / (|lambda| ||M x||) of the
-    first k columns is below tol, or after maxiter iterations, or (stall_iters)
-    when the worst residual has not improved for that many iterations.
+    Stops when the relative residual ||H x - lambda M x|| / (s ||M x||),
+    s = max(|lambda|, max_j<k |lambda_j|), of the first k columns is below tol, or
+    after maxiter iterations, or (stall_iters) when the worst residual has not
+    improved for that many iterations.  Columns whose residual is below tol are
+    soft-locked (their preconditioned residual is zero, so they leave the search
+    space), and the P block is dropped for one iteration whenever the Rayleigh-Ritz
+    basis turned out numerically dependent.
     """
     dev = op.H.device
     dt = op.dtype
     n = op.n
     assert b % 4 == 0 and b >= k and 3 * b <= n, (n, b, k)
     if drop_tol is None:
-        drop_tol = 1e-7 if dt == torch.float32 else 1e-13
+        # directions whose M-Gram eigenvalue is below drop_tol carry rounding of the
+        # stored vectors amplified by drop_tol^-1/2 into the projected matrices
+        drop_tol = 1e-5 if dt == torch.float32 else 1e-10
     t_start = time.perf_counter()
-    bufs = [dict(S=torch.zeros((n, 3 * b), dtype=dt, device=dev),
-                 HS=torch.zeros((n, 3 * b), dtype=dt, device=dev),
-                 MS=torch.zeros((n, 3 * b), dtype=dt, device=dev)) for _ in range(2)]
-    R = torch.empty((n, b), dtype=dt, device=dev)
-    cur = bufs[0]
-    X = cur["S"][:, :b]
+
+    def blk():
+        return torch.zeros((n, b), dtype=dt, device=dev)
+
+    # every block is its own contiguous (n, b) buffer: a row of a block is b
+    # consecutive elements, so the block kernels read whole cache lines
+    # (X, P and their images are double-buffered: the update reads the old set)
+    X, HX, MX = [blk(), blk()], [blk(), blk()], [blk(), blk()]
+    P, HP, MP = [blk(), blk()], [blk(), blk()], [blk(), blk()]
+    W, HW, MW, R = blk(), blk(), blk(), blk()
+    c = 0
     if X0 is None:
-        g = torch.Generator(device="cpu").manual_seed(seed)
-        Xr = torch.rand((n, b), generator=g, dtype=torch.float64) - 0.5
-        X.copy_(Xr.to(dev, dt))
+        g = torch.Generator(device=dev).manual_seed(seed)
+        X[c].copy_(torch.rand((n, b), generator=g, dtype=dt, device=dev) - 0.5)
         if precond is not None:
-            R.copy_(X)
-            precond(R, X)
+            R.copy_(X[c])
+            precond(R, X[c])
     else:
-        X.copy_(X0[:, :b].to(dt))
+        X[c].copy_(X0[:, :b].to(dt))
     # initial Rayleigh-Ritz on X alone
-    op.apply(X, cur["HS"][:, :b], cur["MS"][:, :b])
-    G = kernels.gram(X, cur["HS"][:, :b], cur["MS"][:, :b], sym=True)
+    op.apply(X[c], HX[c], MX[c])
+    G = kernels.gram_blocks([X[c]], [HX[c]], [MX[c]], sym=True)
     theta, C, _ = _svqb_rr(G, b, b, drop_tol)
-    nxt = bufs[1]
-    for key in ("S", "HS", "MS"):
-        kernels.block_update(cur[key][:, :b].contiguous() if False else cur[key][:, :b], C,
-                             nxt[key][:, :b])
-    cur, nxt = nxt, cur
+    for src in (X, HX, MX):
+        kernels.block_update_blocks([src[c]], C, src[1 - c])
+    c = 1 - c
     lam = theta
     have_p = False
     history = []
@@ -146,17 +159,17 @@ def lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=None
     best_it = 0
     res = None
     t_iter = time.perf_counter()
+    colscale = torch.ones(b, dtype=torch.float64, device=dev)
     for it in range(1, maxiter + 1):
-        X = cur["S"][:, :b]
-        HX = cur["HS"][:, :b]
-        MX = cur["MS"][:, :b]
-        rn, mn = kernels.residual_norms(HX, MX, lam, R)
+        rn, mn = kernels.residual_norms(HX[c], MX[c], lam, R, colscale)
         # residual relative to the energy scale of the wanted window, so states with
         # eigenvalues close to zero are held to the same absolute accuracy as the rest
         scale = lam[:k].abs().max().clamp_min(1e-300)
         res = (rn.clamp_min(0).sqrt() / (torch.maximum(lam.abs(), scale) * mn.clamp_min(1e-300).sqrt()))
         worst = float(res[:k].max().item())
         history.append(worst)
+        # lock converged columns from the next residual on (no host sync needed)
+        colscale = (res >= tol).to(torch.float64)
         if verbose:
             print(f"  lobpcg[{str(dt)[6:]}] it {it:4d} worst rel. residual {worst:.3e}")
         if worst < tol:
@@ -167,34 +180,32 @@ def lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=None
                 best, best_it = worst, it
             elif it - best_it >= stall_iters:
                 break
-        W = cur["S"][:, b:2 * b]
         if precond is not None:
             precond(R, W)
         else:
             W.copy_(R)
-        op.apply(W, cur["HS"][:, b:2 * b], cur["MS"][:, b:2 * b])
-        m = 3 * b if have_p else 2 * b
-        S = cur["S"][:, :m]
-        HS = cur["HS"][:, :m]
-        MS = cur["MS"][:, :m]
-        G = kernels.gram(S, HS, MS, sym=True)
-        theta, C, nk = _svqb_rr(G, m, b, drop_tol)
+        op.apply(W, HW, MW)
+        S = [X[c], W] + ([P[c]] if have_p else [])
+        HS = [HX[c], HW] + ([HP[c]] if have_p else [])
+        MS = [MX[c], MW] + ([MP[c]] if have_p else [])
+        m = len(S) * b
+        G = kernels.gram_blocks(S, HS, MS, sym=True)
+        theta, C, dropped = _svqb_rr(G, m, b, drop_tol)
         CP = C.clone()
         CP[:b] = 0.0
-        for key in ("S", "HS", "MS"):
-            src = cur[key][:, :m]
-            kernels.block_update(src, C, nxt[key][:, :b], CP, nxt[key][:, 2 * b:3 * b])
-        cur, nxt = nxt, cur
+        for src, img, pimg in ((S, X, P), (HS, HX, HP), (MS, MX, MP)):
+            kernels.block_update_blocks(src, C, img[1 - c], CP, pimg[1 - c])
+        c = 1 - c
         lam = theta
-        have_p = True
+        have_p = dropped == 0
     torch.cuda.synchronize()
     t_end = time.perf_counter()
-    X = cur["S"][:, :b]
+    X = X[c]
     timings = {"setup_s": t_iter - t_start, "iterate_s": t_end - t_iter}
     return LOBPCGResult(lam, X, res, it, history, converged, timings)
 
 
-def solve_mixed(op64, k, b, precond32=None, precond64=None, tol=1e-7, tol32=3e-4, maxiter=400,
+def solve_mixed(op64, k, b, precond32=None, precond64=None, tol=1e-6, tol32=3e-4, maxiter=400,
                 seed=0, verbose=False):
     """FP32 LOBPCG to tol32 (or stagnation), then FP64 LOBPCG to tol."""
     op32 = op64.astype(torch.float32)

@@ -106,7 +106,7 @@ class SchrodingerProblem:
         return LatticeMG(self.lat, c_e, v_e, sigma, b, dtype=dtype)
 
     # -- solve ------------------------------------------------------------------
-    def solve(self, k, b=None, mode="mixed", tol=1e-7, tol32=3e-4, maxiter=400, seed=0, verbose=False):
+    def solve(self, k, b=None, mode="mixed", tol=1e-6, tol32=3e-4, maxiter=400, seed=0, verbose=False):
         """Lowest k eigenpairs.  Returns (eigenvalues eV (np), full-numbering
         eigenvectors (n, k) float64 device tensor, M_nm-normalised, info)."""
         if self.H is None:

@@ -105,6 +105,46 @@ def test_block_update(dtype, b, two):
         assert torch.all(out[:, 2 * b:] == 7.0)
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("b", [8, 24])
+@pytest.mark.parametrize("nblk", [1, 2, 3])
+def test_gram_and_update_column_blocks(dtype, b, nblk):
+    """Column-block forms against FP64 products of the concatenated matrices; the
+    blocks have different leading dimensions (one is a strided view)."""
+    from qdsim_amd import kernels
+    dev = torch.device("cuda")
+    n = 6007
+    rng = np.random.default_rng(11 + b + nblk)
+
+    def blocks(w_extra):
+        out = []
+        for j in range(nblk):
+            full = torch.from_numpy(rng.standard_normal((n, b + (w_extra if j == 1 else 0)))).to(dev, dtype)
+            out.append(full[:, :b])
+        return out
+    S, H, M = blocks(4), blocks(0), blocks(8)
+    cat = lambda L: torch.cat([x.double() for x in L], 1).cpu().numpy()
+    Sd, Hd, Md = cat(S), cat(H), cat(M)
+    G = kernels.gram_blocks(S, H, M).cpu().numpy()
+    ref = Sd.T @ np.concatenate([Hd, Md], 1)
+    assert np.allclose(G, ref, rtol=1e-12, atol=1e-10 * np.abs(ref).max())
+    # symmetric halves: A^T [A | 2A] computed from the upper tiles only
+    Gs = kernels.gram_blocks(S, S, [2.0 * x for x in S], sym=True).cpu().numpy()
+    refs = Sd.T @ np.concatenate([Sd, 2 * Sd], 1)
+    assert np.allclose(Gs, refs, rtol=1e-12, atol=1e-10 * np.abs(refs).max())
+    m = nblk * b
+    C1 = torch.from_numpy(rng.standard_normal((m, b))).to(dev, torch.float64)
+    C2 = torch.from_numpy(rng.standard_normal((m, b))).to(dev, torch.float64)
+    Y1 = torch.full((n, b), 7.0, dtype=dtype, device=dev)
+    Y2 = torch.full((n, b), 7.0, dtype=dtype, device=dev)
+    kernels.block_update_blocks(S, C1, Y1, C2, Y2)
+    tol = 2e-5 if dtype == torch.float32 else 1e-12
+    for Y, C in ((Y1, C1), (Y2, C2)):
+        Cd = C.to(dtype).double().cpu().numpy()
+        r = Sd @ Cd
+        assert np.abs(Y.double().cpu().numpy() - r).max() <= tol * np.abs(Sd).max() * np.abs(Cd).sum(0).max()
+
+
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64, torch.complex64, torch.complex128])
 def test_residual_and_coldot(dtype):
     from qdsim_amd import kernels
