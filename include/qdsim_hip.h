@@ -90,6 +90,19 @@ int qds_csr_take(int32_t dtype, int64_t n, const int32_t* src, const double* re,
  * out[e][q] = sum_a phi[q][a] u[conn[e][a]].                                      */
 int qds_interpolate_q(int32_t npe, int32_t nq, int64_t n_elem, const int32_t* conn,
                       const double* phi, const double* u, double* out, void* stream);
+/* FE field at arbitrary points of a structured order-p mesh (cx x cy cells of
+ * dx x dy; element 2 (j cx + i) + t).  Replaces FEInterpolator::interpolate /
+ * interpolateWithGradient / findElement (backend/src/fe_interpolator.cpp:54-410)
+ * and interpolate_field_kernel (backend/src/gpu_kernels.cu:352-423): same element
+ * choice (lowest index whose barycentrics are >= -1e-10), nearest-node value and
+ * inverse-distance gradient outside the mesh.  lattice_node[(p cy + 1)(p cx + 1)]:
+ * lowest node id at each refined-lattice position.  grad_x/grad_y (both or
+ * neither) and elem (-1 outside) are optional.                                    */
+int qds_interpolate_points(int32_t order, int64_t cx, int64_t cy, double dx, double dy,
+                           const int32_t* conn, const double* x, const double* y,
+                           const int32_t* lattice_node, const double* field, int64_t n_points,
+                           const double* px, const double* py, double* val, double* grad_x,
+                           double* grad_y, int32_t* elem, void* stream);
 /* Hellmann-Feynman weights out[e][q][s] = w_q |J_e|/2 psi_s(x_q)^2 (no reference
  * counterpart: adjoint of the eigenvalues w.r.t. the potential at x_q).           */
 int qds_state_density_q(int32_t npe, int32_t nq, int64_t n_elem, const int32_t* conn,
@@ -144,7 +157,8 @@ int qds_gram_blocks(int32_t dtype, int64_t n, const qds_colblocks* A /* host */,
                     int32_t sym, double* C, void* workspace, int64_t ws_bytes, void* stream);
 int qds_block_update_blocks(int32_t dtype, int64_t n, const qds_colblocks* S /* host */, int32_t k,
                             const void* C1, void* Y1, int64_t ldy1, const void* C2, void* Y2,
-                            int64_t ldy2, void* stream);
+                            int64_t ldy2, int32_t c2_first /* rows of C2 before it are zero */,
+                            void* stream);
 int qds_block_residual(int32_t dtype, int64_t n, int32_t b, const void* HX, int64_t ldh,
                        const void* MX, int64_t ldm, const double* lam, void* R, int64_t ldr,
                        void* stream);

@@ -9,7 +9,7 @@ from .mesh import SimpleMesh, create_simple_mesh
 
 __all__ = ["SimpleMesh", "create_simple_mesh", "FEMSolver", "EigenSolver", "SelfConsistentSolver",
            "FixedOpenSystemSolver", "PoissonSolver", "SchrodingerProblem", "create_fem_solver",
-           "create_eigen_solver", "lifetimes_fs"]
+           "create_eigen_solver", "lifetimes_fs", "FEInterpolator", "create_interpolator"]
 
 
 def __getattr__(name):
@@ -29,6 +29,9 @@ def __getattr__(name):
     if name == "PoissonSolver":
         from .poisson import PoissonSolver
         return PoissonSolver
+    if name in ("FEInterpolator", "create_interpolator"):
+        from . import interpolator
+        return getattr(interpolator, name)
     if name == "SchrodingerProblem":
         from .schrodinger import SchrodingerProblem
         return SchrodingerProblem

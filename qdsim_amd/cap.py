@@ -178,7 +178,8 @@ def cap_lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, seed=0, t
         CP[:b] = 0
         Cr, CPr = _real_form(C), _real_form(CP)
         for src, img, pimg in ((S, X, P), (HS, HX, HP), (MS, MX, MP)):
-            kernels.block_update_blocks([_rv(x) for x in src], Cr, _rv(img[1 - c]), CPr, _rv(pimg[1 - c]))
+            kernels.block_update_blocks([_rv(x) for x in src], Cr, _rv(img[1 - c]), CPr, _rv(pimg[1 - c]),
+                                        c2_first=2 * b)
         c = 1 - c
         have_p = True
     torch.cuda.synchronize()

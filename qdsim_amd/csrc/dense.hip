@@ -42,22 +42,15 @@ struct GramShape {
 
 // Up to four column blocks forming one logical n x cols matrix (each block its own
 // allocation and leading dimension).  start[j] is the first logical column of j.
+// Device form of a column-block operand: the caller's blocks split into pieces of
+// at most 16 vectors' width (kMaxPieces pieces).
+constexpr int kMaxPieces = 8;
 struct ColBlocks {
-  const void* p[4];
-  int64_t ld[4];
-  int32_t start[5];
+  const void* p[kMaxPieces];
+  int64_t ld[kMaxPieces];
+  int32_t start[kMaxPieces + 1];
   int32_t nb;
 };
-
-template <typename T>
-__device__ __forceinline__ const T* col_ptr(const ColBlocks& B, int c, int64_t& ld) {
-  int j = 0;
-#pragma unroll
-  for (int t = 1; t < 4; ++t)
-    if (t < B.nb && c >= B.start[t]) j = t;
-  ld = B.ld[j];
-  return reinterpret_cast<const T*>(B.p[j]) + (c - B.start[j]);
-}
 
 __device__ __forceinline__ bool tile_needed(const GramShape& g, int ti, int tj) {
   if (!g.sym) return true;
@@ -72,33 +65,46 @@ template <> struct GLoad<float, false> { static constexpr int W = 1; typedef flo
 template <> struct GLoad<double, true> { static constexpr int W = 2; typedef double2 v; };
 template <> struct GLoad<double, false> { static constexpr int W = 1; typedef double v; };
 
-constexpr int kGramSlots = 6;  // vector loads per thread per staged chunk (A and B each)
+// grid: (row slabs, output column groups of tpg tiles).  A block streams its slab
+// kRB rows at a time: the rows are staged in LDS as FP64, [A | this group's B
+// columns] per row, zero padded.  Operand pieces are at most 16 vectors wide, so
+// for kRB = 16 rows every thread owns at most one vector per piece; its offsets
+// (global, relative to the chunk's first row, and in LDS) are computed once, and
+// per chunk only uniform row bases advance.  The next chunk is loaded into
+// registers while the MFMAs of the current one run.  Each wave's MFMA loop reads
+// all operands of a k-step before its MFMAs, without branches (unused slots compute
+// into accumulators that are never stored).
+constexpr int kGramPieces = 3 * kMaxPieces;
 
-// grid: (row slabs, output column groups of tpg tiles).  Each block streams its slab
-// kRB rows at a time: every thread owns up to kGramSlots vector loads of A and of B
-// (positions fixed for the whole slab), loads the next chunk into registers while
-// the MFMAs of the current chunk run from LDS (FP64 copies of the rows).
-template <typename T, bool VEC, int MAXT>
-__global__ void __launch_bounds__(kBlock)
-gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, ColBlocks A, ColBlocks B1,
-             ColBlocks B2, double* __restrict__ partial) {
+struct GramStage {
+  int n;                           // pieces: A's, then B1's, then B2's
+  const void* src[kGramPieces];
+  int64_t ld[kGramPieces];
+  int w[kGramPieces];              // columns
+  int op[kGramPieces];             // 0 A, 1 B1, 2 B2
+  int col0[kGramPieces];           // first column of the piece inside its operand
+};
+
+template <typename T, bool VEC, int MAXP, int MAXT>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
+gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, GramStage stg,
+             double* __restrict__ partial) {
   using L = GLoad<T, VEC>;
+  using V = typename L::v;
   constexpr int VW = L::W;
-  extern __shared__ double smem[];
+  extern __shared__ double gsm[];
   const int tk = g.tk1 + g.tk2;
   const int tj0 = blockIdx.y * tpg;
   const int tj1 = min(tk, tj0 + tpg);
-  const int cw = (tj1 - tj0) * 16;   // B columns staged by this block
-  double* sA = smem;                 // kRB x mpad
-  double* sB = smem + kRB * g.mpad;  // kRB x cw
+  const int k1pad = 16 * g.tk1;
+  const int c_lo = tj0 * 16, c_hi = tj1 * 16;
+  const int ldsw = g.mpad + (c_hi - c_lo);     // LDS row: A (padded), then the group's B columns
   __shared__ int s_tile[kWaves * MAXT];
   __shared__ int s_count[kWaves];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int ntl = g.tm * (tj1 - tj0);
   if (lane == 0) {
-    // tiles are dealt round-robin over the waves; slot lists keep the
-    // accumulator index a compile-time constant in the MFMA loop
     int c = 0;
     for (int t = wave; t < ntl; t += kWaves) {
       const int ti = t / (tj1 - tj0), tj = tj0 + (t - ti * (tj1 - tj0));
@@ -108,91 +114,84 @@ gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, ColBlocks A,
     }
     s_count[wave] = c;
   }
-  // load slots: source pointer, leading dimension and LDS position of each vector
-  const int k1pad = 16 * g.tk1;
-  const int apr = g.mpad / VW;  // vectors per staged A row
-  const int bpr = cw / VW;
-  int a_rr[kGramSlots], a_lc[kGramSlots], b_rr[kGramSlots], b_lc[kGramSlots];
-  const T* a_src[kGramSlots];
-  const T* b_src[kGramSlots];
-  int64_t a_ld[kGramSlots], b_ld[kGramSlots];
-  bool a_ok[kGramSlots], b_ok[kGramSlots];
+  for (int t = threadIdx.x; t < kRB * ldsw; t += kBlock) gsm[t] = 0.0;
+  // this thread's vector of every piece: offset from the chunk's first row (global)
+  // and position in LDS (-1: none)
+  int goff[MAXP], loff[MAXP];
 #pragma unroll
-  for (int s = 0; s < kGramSlots; ++s) {
-    const int t = threadIdx.x + s * kBlock;
-    a_rr[s] = t / apr;
-    a_lc[s] = (t - a_rr[s] * apr) * VW;
-    a_ok[s] = a_rr[s] < kRB && a_lc[s] < g.m;
-    a_src[s] = a_ok[s] ? col_ptr<T>(A, a_lc[s], a_ld[s]) : nullptr;
-    if (!a_ok[s]) a_ld[s] = 0;
-    b_rr[s] = t / bpr;
-    b_lc[s] = (t - b_rr[s] * bpr) * VW;
-    const int c = tj0 * 16 + b_lc[s];   // global padded column
-    if (c < k1pad) {
-      b_ok[s] = b_rr[s] < kRB && c < g.k1;
-      b_src[s] = b_ok[s] ? col_ptr<T>(B1, c, b_ld[s]) : nullptr;
+  for (int q = 0; q < MAXP; ++q) {
+    goff[q] = -1;
+    loff[q] = -1;
+    if (q >= stg.n) continue;
+    const int w = stg.w[q];
+    const int vpr = w / VW;
+    const int v = threadIdx.x;
+    if (v >= kRB * vpr) continue;
+    const int rr = v / vpr, cc = (v - rr * vpr) * VW;
+    int lcol;
+    if (stg.op[q] == 0) {
+      lcol = stg.col0[q] + cc;
     } else {
-      b_ok[s] = b_rr[s] < kRB && (c - k1pad) < g.k2;
-      b_src[s] = b_ok[s] ? col_ptr<T>(B2, c - k1pad, b_ld[s]) : nullptr;
+      const int tcol = (stg.op[q] == 1 ? 0 : k1pad) + stg.col0[q] + cc;   // tile-space column
+      if (tcol < c_lo || tcol >= c_hi) continue;                          // vector inside one group
+      lcol = g.mpad + tcol - c_lo;
     }
-    if (!b_ok[s]) b_ld[s] = 0;
+    goff[q] = (int)(rr * stg.ld[q]) + cc;
+    loff[q] = rr * ldsw + lcol;
   }
   __syncthreads();
   const int nslots = s_count[wave];
-
   v4d acc[MAXT];
+  int a_off[MAXT], b_off[MAXT];
 #pragma unroll
-  for (int t = 0; t < MAXT; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
-
+  for (int t = 0; t < MAXT; ++t) {
+    acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+    const int code = t < nslots ? s_tile[wave * MAXT + t] : 0;
+    a_off[t] = (code >> 8) * 16 + (lane & 15);
+    b_off[t] = g.mpad + (code & 255) * 16 - c_lo + (lane & 15);
+    if (t >= nslots) b_off[t] = g.mpad + (lane & 15);   // valid LDS, result unused
+  }
   const int64_t r_begin = (int64_t)blockIdx.x * rows_per_blk;
   const int64_t r_end = min(n, r_begin + rows_per_blk);
-  typename L::v ra[kGramSlots], rb[kGramSlots];
-  auto load_chunk = [&](int64_t r0) {
+  V pre[MAXP];
+  auto load = [&](int64_t r0) {
+    const bool full = r0 + kRB <= r_end;
 #pragma unroll
-    for (int s = 0; s < kGramSlots; ++s) {
-      const int64_t row = r0 + a_rr[s];
-      if (a_ok[s] && row < r_end)
-        ra[s] = *reinterpret_cast<const typename L::v*>(a_src[s] + row * a_ld[s]);
-      else
-        ra[s] = typename L::v{};
-      const int64_t rowb = r0 + b_rr[s];
-      if (b_ok[s] && rowb < r_end)
-        rb[s] = *reinterpret_cast<const typename L::v*>(b_src[s] + rowb * b_ld[s]);
-      else
-        rb[s] = typename L::v{};
+    for (int q = 0; q < MAXP; ++q) {
+      pre[q] = V{};
+      if (goff[q] < 0) continue;
+      if (!full && r0 + loff[q] / ldsw >= r_end) continue;   // tail chunk: rows past the slab stay 0
+      const T* base = reinterpret_cast<const T*>(stg.src[q]) + r0 * stg.ld[q];
+      pre[q] = *reinterpret_cast<const V*>(base + goff[q]);
     }
   };
-  if (r_begin < r_end) load_chunk(r_begin);
-  for (int64_t r0 = r_begin; r0 < r_end; r0 += kRB) {
-    __syncthreads();  // previous chunk's MFMAs are done with LDS
+  auto store = [&]() {
 #pragma unroll
-    for (int s = 0; s < kGramSlots; ++s) {
-      if (a_rr[s] < kRB) {
-        const T* v = reinterpret_cast<const T*>(&ra[s]);
+    for (int q = 0; q < MAXP; ++q) {
+      if (loff[q] < 0) continue;
+      const T* v = reinterpret_cast<const T*>(&pre[q]);
 #pragma unroll
-        for (int e = 0; e < VW; ++e) sA[a_rr[s] * g.mpad + a_lc[s] + e] = (double)v[e];
-      }
-      if (b_rr[s] < kRB) {
-        const T* v = reinterpret_cast<const T*>(&rb[s]);
-#pragma unroll
-        for (int e = 0; e < VW; ++e) sB[b_rr[s] * cw + b_lc[s] + e] = (double)v[e];
-      }
+      for (int e = 0; e < VW; ++e) gsm[loff[q] + e] = (double)v[e];
     }
+  };
+  if (r_begin < r_end) load(r_begin);
+  for (int64_t r0 = r_begin; r0 < r_end; r0 += kRB) {
+    __syncthreads();  // the previous chunk's MFMAs are done with LDS
+    store();
     __syncthreads();
-    if (r0 + kRB < r_end) load_chunk(r0 + kRB);   // in flight during the MFMAs below
+    if (r0 + kRB < r_end) load(r0 + kRB);   // in flight during the MFMAs below
 #pragma unroll
     for (int kk = 0; kk < kRB / 4; ++kk) {
-      const int row = kk * 4 + (lane >> 4);
+      __builtin_amdgcn_sched_barrier(0);
+      const int row = (kk * 4 + (lane >> 4)) * ldsw;
+      double av[MAXT], bv[MAXT];
 #pragma unroll
       for (int slot = 0; slot < MAXT; ++slot) {
-        if (slot < nslots) {
-          const int code = s_tile[wave * MAXT + slot];
-          const int ti = code >> 8, tj = code & 255;
-          const double a = sA[row * g.mpad + ti * 16 + (lane & 15)];
-          const double b = sB[row * cw + (tj - tj0) * 16 + (lane & 15)];
-          acc[slot] = mfma_f64(a, b, acc[slot]);
-        }
+        av[slot] = gsm[row + a_off[slot]];
+        bv[slot] = gsm[row + b_off[slot]];
       }
+#pragma unroll
+      for (int slot = 0; slot < MAXT; ++slot) acc[slot] = mfma_f64(av[slot], bv[slot], acc[slot]);
     }
   }
   double* P = partial + (int64_t)blockIdx.x * g.mpad * g.kpad;
@@ -282,121 +281,130 @@ template <> struct MT<double> {
 };
 
 constexpr int kMaxOutTiles = 4;  // k <= 64
-constexpr int kRowTiles = 2;     // 16-row tiles per wave and pass
-constexpr int kKC = 32;          // rows of C staged per chunk when C does not fit in LDS
-constexpr int kUpdLds = 48 * 1024;
+constexpr int kUpdLds = 64 * 1024;
 
-// Y1 = S C1 (and Y2 = S C2), S given as column blocks.  One pass of a block covers
-// kWaves * kRowTiles * 16 rows.  When NOUT * m * k fits in LDS, C is staged once per
-// block and stays resident (no barriers inside the pass loop); otherwise C is
-// streamed kKC rows at a time.  All S loads of a row tile's chunk are issued before
-// its MFMAs.
-template <typename T, int NOUT, int MAXOT>
+// Y1 = S C1 (and Y2 = S C2), S given as column blocks.  A wave owns a tile of 16
+// rows; a block of 4 waves advances 64 rows per pass.  Within a pass the column
+// blocks of S are visited in order: the MAXJ 16-byte loads per lane of block j + 1
+// are issued before the MFMAs of block j (always MAXJ loads, with clamped
+// addresses, so the compiler can wait for exactly the older ones), and every wave
+// keeps one block in flight while it computes.  C lives in LDS: all of it when it
+// fits (resident), otherwise the rows of the current column block, staged with
+// barriers.  OT = ceil(k / 16) output tiles exactly.  Column blocks that end before
+// c2_first contribute nothing to Y2 (those rows of C2 are zero) and skip its MFMAs.
+template <typename T, int NOUT, int OT, int MAXJ>
 __global__ void __launch_bounds__(kBlock)
 block_update_kernel(int64_t n, int m, int k, ColBlocks S, const T* __restrict__ C1,
                     const T* __restrict__ C2, T* __restrict__ Y1, int64_t ldy1, T* __restrict__ Y2,
-                    int64_t ldy2, int chunk) {
+                    int64_t ldy2, int resident, int crows, int c2_first) {
   using M = MT<T>;
   constexpr int VEC = M::VEC;
   constexpr int KB = 4 * VEC;  // contraction indices covered by one vector load per lane
-  constexpr int MAXKB = 8;     // vector loads kept in flight per row tile
+  constexpr int KP = OT * 16;
+  using Vt = typename M::vec_t;
   extern __shared__ unsigned char upd_smem[];
-  T* sC = reinterpret_cast<T*>(upd_smem);
-  const int kt = (k + 15) / 16;
-  const int kp = kt * 16;
+  T* sC = reinterpret_cast<T*>(upd_smem);   // [NOUT][crows][KP], zero padded rows and columns
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int q = lane >> 4;      // 0..3
   const int rl = lane & 15;
-  const bool resident = chunk >= m;
-  auto stage = [&](int kc) {
-    for (int t = threadIdx.x; t < NOUT * chunk * kp; t += kBlock) {
-      const int o = t / (chunk * kp);
-      const int rem = t - o * chunk * kp;
-      const int i = rem / kp, j = rem - i * kp;
+  // copy C rows [r0, r0 + rows) into LDS rows [0, rows), zero elsewhere
+  auto stage = [&](int r0, int rows) {
+    for (int t = threadIdx.x; t < NOUT * crows * KP; t += kBlock) {
+      const int o = t / (crows * KP);
+      const int rem = t - o * crows * KP;
+      const int i = rem / KP, j = rem - i * KP;
       const T* C = o == 0 ? C1 : C2;
-      const int gi = kc + i;
-      sC[t] = (gi < m && j < k) ? C[gi * k + j] : T(0);
+      sC[t] = (i < rows && j < k) ? C[(r0 + i) * k + j] : T(0);
     }
   };
   if (resident) {
-    stage(0);
+    stage(0, m);
     __syncthreads();
   }
-  const int64_t rows_per_pass = (int64_t)kWaves * kRowTiles * 16;
-  for (int64_t base = (int64_t)blockIdx.x * rows_per_pass; base < n;
-       base += (int64_t)gridDim.x * rows_per_pass) {
-    typename M::acc_t acc[NOUT][kRowTiles][MAXOT];
+  const int64_t rows_per_pass = (int64_t)kWaves * 16;
+  const int64_t first = (int64_t)blockIdx.x * rows_per_pass;
+  const int64_t stride = (int64_t)gridDim.x * rows_per_pass;
+  Vt nxt[MAXJ];
+  // MAXJ 16-byte loads of column block j for this lane's row of the tile at `base`;
+  // out-of-range rows and columns load a valid address and are zeroed or unused later
+  auto fetch = [&](int64_t base, int j) {
+    int64_t row = base + wave * 16 + rl;
+    row = row < n ? row : n - 1;
+    const int w = S.start[j + 1] - S.start[j];
+    const T* rp = reinterpret_cast<const T*>(S.p[j]) + row * S.ld[j];
+#pragma unroll
+    for (int jj = 0; jj < MAXJ; ++jj) {
+      const int c = jj * KB + q * VEC;
+      nxt[jj] = *reinterpret_cast<const Vt*>(rp + (c < w ? c : 0));
+    }
+  };
+  if (first < n) fetch(first, 0);
+  for (int64_t base = first; base < n; base += stride) {
+    typename M::acc_t acc[NOUT][OT];
 #pragma unroll
     for (int o = 0; o < NOUT; ++o)
 #pragma unroll
-      for (int r = 0; r < kRowTiles; ++r)
+      for (int t = 0; t < OT; ++t) acc[o][t] = typename M::acc_t{0, 0, 0, 0};
+    for (int j = 0; j < S.nb; ++j) {
+      Vt cur[MAXJ];
 #pragma unroll
-        for (int t = 0; t < MAXOT; ++t) acc[o][r][t] = typename M::acc_t{0, 0, 0, 0};
-    for (int kc = 0; kc < m; kc += chunk) {
+      for (int jj = 0; jj < MAXJ; ++jj) cur[jj] = nxt[jj];
+      const int w = S.start[j + 1] - S.start[j];
+      const int cj = S.start[j];
       if (!resident) {
         __syncthreads();
-        stage(kc);
+        stage(cj, w);
         __syncthreads();
       }
-      const int kend = min(chunk, m - kc);
+      if (j + 1 < S.nb) fetch(base, j + 1);
+      else if (base + stride < n) fetch(base + stride, 0);
+      const int crow0 = resident ? cj : 0;
+      const bool two = NOUT == 2 && cj + w > c2_first;
 #pragma unroll
-      for (int r = 0; r < kRowTiles; ++r) {
-        const int64_t row = base + ((int64_t)wave * kRowTiles + r) * 16 + rl;
-        for (int cb0 = 0; cb0 < kend; cb0 += MAXKB * KB) {
-          T sv[MAXKB][VEC];
+      for (int jj = 0; jj < MAXJ; ++jj) {
+        if (jj * KB >= w) break;
+        __builtin_amdgcn_sched_barrier(0);
+        // columns past the block end carry zeros
+        const T keep = jj * KB + q * VEC < w ? T(1) : T(0);
+        const T* sv = reinterpret_cast<const T*>(&cur[jj]);
+        T bv[NOUT][VEC][OT];
 #pragma unroll
-          for (int j = 0; j < MAXKB; ++j) {
-            const int kloc = cb0 + j * KB + q * VEC;
-            if (row < n && kloc < kend) {
-              int64_t ld;
-              const T* src = col_ptr<T>(S, kc + kloc, ld);
-              typename M::vec_t v = *reinterpret_cast<const typename M::vec_t*>(src + row * ld);
-              const T* vp = reinterpret_cast<const T*>(&v);
+        for (int e = 0; e < VEC; ++e) {
+          const int kl = crow0 + jj * KB + q * VEC + e;   // staged C row (zero past the block)
 #pragma unroll
-              for (int e = 0; e < VEC; ++e) sv[j][e] = vp[e];
-            } else {
-#pragma unroll
-              for (int e = 0; e < VEC; ++e) sv[j][e] = T(0);
-            }
+          for (int t = 0; t < OT; ++t) {
+            bv[0][e][t] = sC[kl * KP + t * 16 + rl];
+            if (NOUT == 2) bv[NOUT - 1][e][t] = sC[(crows + kl) * KP + t * 16 + rl];
           }
+        }
 #pragma unroll
-          for (int j = 0; j < MAXKB; ++j) {
-            const int kb = cb0 + j * KB;
-            if (kb >= kend) break;
+        for (int e = 0; e < VEC; ++e) {
+          const T a = sv[e] * keep;
 #pragma unroll
-            for (int e = 0; e < VEC; ++e) {
-              const int kloc = kb + q * VEC + e;   // row of the staged C (zero padded)
+          for (int t = 0; t < OT; ++t) acc[0][t] = M::mma(a, bv[0][e][t], acc[0][t]);
+        }
+        if (two) {
 #pragma unroll
-              for (int t = 0; t < MAXOT; ++t) {
-                if (t < kt) {
+          for (int e = 0; e < VEC; ++e) {
+            const T a = sv[e] * keep;
 #pragma unroll
-                  for (int o = 0; o < NOUT; ++o) {
-                    const T bv = sC[o * chunk * kp + kloc * kp + t * 16 + rl];
-                    acc[o][r][t] = M::mma(sv[j][e], bv, acc[o][r][t]);
-                  }
-                }
-              }
-            }
+            for (int t = 0; t < OT; ++t) acc[NOUT - 1][t] = M::mma(a, bv[NOUT - 1][e][t], acc[NOUT - 1][t]);
           }
         }
       }
     }
+    const int64_t r0 = base + wave * 16;
 #pragma unroll
-    for (int r = 0; r < kRowTiles; ++r) {
-      const int64_t r0 = base + ((int64_t)wave * kRowTiles + r) * 16;
+    for (int t = 0; t < OT; ++t) {
+      const int col = t * 16 + rl;
+      if (col >= k) continue;
 #pragma unroll
-      for (int t = 0; t < MAXOT; ++t) {
-        if (t >= kt) continue;
-        const int col = t * 16 + rl;
-        if (col >= k) continue;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int64_t orow = r0 + M::drow(q, e);
-          if (orow < n) {
-            Y1[orow * ldy1 + col] = acc[0][r][t][e];
-            if (NOUT == 2) Y2[orow * ldy2 + col] = acc[NOUT - 1][r][t][e];
-          }
+      for (int e = 0; e < 4; ++e) {
+        const int64_t orow = r0 + M::drow(q, e);
+        if (orow < n) {
+          Y1[orow * ldy1 + col] = acc[0][t][e];
+          if (NOUT == 2) Y2[orow * ldy2 + col] = acc[NOUT - 1][t][e];
         }
       }
     }
@@ -404,25 +412,38 @@ block_update_kernel(int64_t n, int m, int k, ColBlocks S, const T* __restrict__ 
 }
 
 template <typename T, int NOUT>
-int launch_update(int64_t n, int m, int k, const ColBlocks& S, const T* C1, const T* C2, T* Y1,
+int launch_update(int64_t n, int m, int k, const ColBlocks& S, int c2_first, const T* C1, const T* C2, T* Y1,
                   int64_t ldy1, T* Y2, int64_t ldy2, hipStream_t st) {
   constexpr int KB = 4 * MT<T>::VEC;
-  const int kp = (k + 15) / 16 * 16;
-  const int mpad = (m + KB - 1) / KB * KB;
-  // resident C when it fits, otherwise kKC-row chunks (both multiples of KB, zero padded)
-  const int chunk = (size_t)NOUT * mpad * kp * sizeof(T) <= (size_t)kUpdLds ? mpad : kKC;
-  const size_t shm = (size_t)NOUT * chunk * kp * sizeof(T);
-  const int64_t passes = (n + kWaves * kRowTiles * 16 - 1) / (kWaves * kRowTiles * 16);
+  const int ot = (k + 15) / 16;
+  const int kp = ot * 16;
+  int wmax = 0;
+  for (int j = 0; j < S.nb; ++j) wmax = max(wmax, S.start[j + 1] - S.start[j]);
+  QDS_CHECK_ARG(wmax <= 4 * KB, "qds_block_update: column blocks wider than %d", 4 * KB);
+  // staged C rows: all m (+ one vector of zero padding) or one column block
+  int crows = m + KB;
+  int resident = 1;
+  if ((size_t)NOUT * crows * kp * sizeof(T) > (size_t)kUpdLds) {
+    resident = 0;
+    crows = wmax + KB;
+  }
+  QDS_CHECK_ARG((size_t)NOUT * crows * kp * sizeof(T) <= (size_t)kUpdLds, "qds_block_update: C too large");
+  const size_t shm = (size_t)NOUT * crows * kp * sizeof(T);
+  const int64_t passes = (n + kWaves * 16 - 1) / (kWaves * 16);
   const unsigned blocks = qds_blocks(passes, 1, 4096);
-  if (kp <= 16)
-    hipLaunchKernelGGL((block_update_kernel<T, NOUT, 1>), dim3(blocks), dim3(kBlock), shm, st, n, m, k, S,
-                       C1, C2, Y1, ldy1, Y2, ldy2, chunk);
-  else if (kp <= 32)
-    hipLaunchKernelGGL((block_update_kernel<T, NOUT, 2>), dim3(blocks), dim3(kBlock), shm, st, n, m, k, S,
-                       C1, C2, Y1, ldy1, Y2, ldy2, chunk);
-  else
-    hipLaunchKernelGGL((block_update_kernel<T, NOUT, 4>), dim3(blocks), dim3(kBlock), shm, st, n, m, k, S,
-                       C1, C2, Y1, ldy1, Y2, ldy2, chunk);
+#define QDS_UPD(OTV, J)                                                                                       \
+  hipLaunchKernelGGL((block_update_kernel<T, NOUT, OTV, J>), dim3(blocks), dim3(kBlock), shm, st, n, m, k, S, \
+                     C1, C2, Y1, ldy1, Y2, ldy2, resident, crows, c2_first)
+#define QDS_UPD_J(OTV) do { if (J <= 2) QDS_UPD(OTV, 2); else if (J == 3) QDS_UPD(OTV, 3); else QDS_UPD(OTV, 4); } while (0)
+  const int J = (wmax + KB - 1) / KB;
+  switch (ot) {
+    case 1: QDS_UPD_J(1); break;
+    case 2: QDS_UPD_J(2); break;
+    case 3: QDS_UPD_J(3); break;
+    default: QDS_UPD_J(4); break;
+  }
+#undef QDS_UPD_J
+#undef QDS_UPD
   QDS_LAUNCH_CHECK();
   return QDS_OK;
 }
@@ -555,30 +576,36 @@ typedef struct {
 }
 
 namespace {
-bool to_device_blocks(const qds_colblocks* h, ColBlocks& d, int& total) {
+// Split the caller's blocks into pieces of at most wcap columns.
+bool to_device_blocks(const qds_colblocks* h, ColBlocks& d, int& total, int es, int wcap) {
   if (!h || h->nb < 1 || h->nb > 4) return false;
-  d.nb = h->nb;
-  int acc = 0;
-  for (int j = 0; j < 4; ++j) {
-    d.p[j] = j < h->nb ? h->p[j] : nullptr;
-    d.ld[j] = j < h->nb ? h->ld[j] : 0;
-    d.start[j] = acc;
-    if (j < h->nb) {
-      if (!h->p[j] || h->cols[j] < 1 || h->ld[j] < h->cols[j]) return false;
-      acc += h->cols[j];
+  int np = 0, acc = 0;
+  for (int j = 0; j < h->nb; ++j) {
+    if (!h->p[j] || h->cols[j] < 1 || h->ld[j] < h->cols[j]) return false;
+    for (int c = 0; c < h->cols[j]; c += wcap) {
+      if (np == kMaxPieces) return false;
+      d.p[np] = static_cast<const char*>(h->p[j]) + (size_t)c * es;
+      d.ld[np] = h->ld[j];
+      d.start[np] = acc;
+      acc += h->cols[j] - c < wcap ? h->cols[j] - c : wcap;
+      ++np;
     }
   }
-  d.start[4] = acc;
+  for (int j = np; j <= kMaxPieces; ++j) {
+    if (j < kMaxPieces) { d.p[j] = nullptr; d.ld[j] = 0; }
+    d.start[j] = acc;
+  }
+  d.nb = np;
   total = acc;
   return true;
 }
 
-// 16-byte vector access is possible when every block starts 16-byte aligned, its
-// leading dimension and width are multiples of the vector width
-bool blocks_vec(const ColBlocks& d, const qds_colblocks* h, int es) {
+// 16-byte vector access is possible when every block starts 16-byte aligned and
+// its leading dimension and width are multiples of the vector width
+bool blocks_vec(const qds_colblocks* h, int es) {
   const int vw = 16 / es;
-  for (int j = 0; j < d.nb; ++j)
-    if ((reinterpret_cast<uintptr_t>(d.p[j]) % 16) || (d.ld[j] * es) % 16 || h->cols[j] % vw) return false;
+  for (int j = 0; j < h->nb; ++j)
+    if ((reinterpret_cast<uintptr_t>(h->p[j]) % 16) || (h->ld[j] * es) % 16 || h->cols[j] % vw) return false;
   return true;
 }
 
@@ -596,27 +623,35 @@ qds_colblocks single(const void* p, int64_t ld, int cols) {
 QDS_API int qds_gram_blocks(int32_t dtype, int64_t n, const qds_colblocks* A, const qds_colblocks* B1,
                             const qds_colblocks* B2, int32_t sym, double* C, void* workspace,
                             int64_t ws_bytes, void* stream) {
+  QDS_CHECK_ARG(dtype == qds::F32 || dtype == qds::F64, "qds_gram: dtype must be F32 or F64");
+  QDS_CHECK_ARG(C && workspace && A && B1, "qds_gram: null pointer");
+  const int es = dtype == qds::F32 ? 4 : 8;
+  const bool vec = blocks_vec(A, es) && blocks_vec(B1, es) && (!B2 || blocks_vec(B2, es));
+  const int wcap = vec ? 16 * (16 / es) : 16;   // piece: at most one vector per thread for kRB rows
   ColBlocks dA, dB1, dB2;
   int m = 0, k1 = 0, k2 = 0;
-  QDS_CHECK_ARG(to_device_blocks(A, dA, m) && to_device_blocks(B1, dB1, k1), "qds_gram: bad A or B1 blocks");
-  if (B2) {
-    QDS_CHECK_ARG(to_device_blocks(B2, dB2, k2), "qds_gram: bad B2 blocks");
-  } else {
-    dB2 = dB1;
-    dB2.nb = 1;
-    k2 = 0;
-  }
-  QDS_CHECK_ARG(C && workspace, "qds_gram: null pointer");
+  QDS_CHECK_ARG(to_device_blocks(A, dA, m, es, wcap) && to_device_blocks(B1, dB1, k1, es, wcap),
+                "qds_gram: bad or too wide A / B1 blocks (at most %d pieces of %d columns)", kMaxPieces, wcap);
+  dB2.nb = 0;
+  if (B2) QDS_CHECK_ARG(to_device_blocks(B2, dB2, k2, es, wcap), "qds_gram: bad or too wide B2 blocks");
   QDS_CHECK_ARG(!sym || (m == k1 && (k2 == 0 || k2 == m)), "qds_gram: sym needs square halves");
-  QDS_CHECK_ARG(dtype == qds::F32 || dtype == qds::F64, "qds_gram: dtype must be F32 or F64");
   GramShape g = make_shape(m, k1, k2, sym);
   QDS_CHECK_ARG(g.tm <= 255 && g.tk1 + g.tk2 <= 255, "qds_gram: block too wide");
-  const int es = dtype == qds::F32 ? 4 : 8;
-  const bool vec = blocks_vec(dA, A, es) && blocks_vec(dB1, B1, es) && (!B2 || blocks_vec(dB2, B2, es));
-  const int vw_used = vec ? 16 / es : 1;
+  GramStage stg;
+  stg.n = 0;
+  const ColBlocks* ops[3] = {&dA, &dB1, &dB2};
+  for (int o = 0; o < 3; ++o)
+    for (int j = 0; j < ops[o]->nb; ++j) {
+      stg.src[stg.n] = ops[o]->p[j];
+      stg.ld[stg.n] = ops[o]->ld[j];
+      stg.w[stg.n] = ops[o]->start[j + 1] - ops[o]->start[j];
+      stg.op[stg.n] = o;
+      stg.col0[stg.n] = ops[o]->start[j];
+      ++stg.n;
+    }
   const int tk = g.tk1 + g.tk2;
-  // column groups: every wave holds at most 24 accumulators, every thread at most
-  // kGramSlots staged vectors per operand, and the staged rows fit 64 KiB of LDS
+  const int k1pad = 16 * g.tk1;
+  // column groups: at most 16 accumulator tiles per wave, the staged rows in 64 KiB
   auto slots = [&](int tp) {
     int mx = 0;
     for (int j0 = 0; j0 < tk; j0 += tp) {
@@ -634,34 +669,37 @@ QDS_API int qds_gram_blocks(int32_t dtype, int64_t n, const qds_colblocks* A, co
     }
     return mx;
   };
-  auto fits = [&](int tp) {
-    if (slots(tp) > 24) return false;
-    if ((kRB * g.mpad / vw_used + kBlock - 1) / kBlock > kGramSlots) return false;
-    if ((kRB * 16 * tp / vw_used + kBlock - 1) / kBlock > kGramSlots) return false;
-    return (size_t)kRB * (g.mpad + 16 * tp) * sizeof(double) <= 64 * 1024;
-  };
-  int tpg = tk;
-  while (tpg > 1 && !fits(tpg)) --tpg;
-  QDS_CHECK_ARG(fits(tpg), "qds_gram: A block of %d columns too wide", m);
+  int tpg = tk, max_slots = 0;
+  for (; tpg >= 1; --tpg) {
+    max_slots = slots(tpg);
+    if (max_slots <= 16 && (size_t)kRB * (g.mpad + 16 * tpg) * sizeof(double) <= 64 * 1024) break;
+  }
+  QDS_CHECK_ARG(tpg >= 1, "qds_gram: operands of %d columns too wide", m);
+  (void)k1pad;
   const int ngroups = (tk + tpg - 1) / tpg;
-  const int max_slots = slots(tpg);
   const int nblk = gram_blocks(n);
   QDS_CHECK_ARG(ws_bytes >= qds_gram_workspace(n, m, k1, k2), "qds_gram: workspace too small");
   const int64_t rows_per_blk = (((n + nblk - 1) / nblk) + kRB - 1) / kRB * kRB;
   const size_t shm = (size_t)kRB * (g.mpad + 16 * tpg) * sizeof(double);
+  for (int q = 0; q < stg.n; ++q)
+    QDS_CHECK_ARG(stg.ld[q] * kRB < (int64_t(1) << 31), "qds_gram: leading dimension too large");
   hipStream_t st = qds_stream(stream);
   double* partial = static_cast<double*>(workspace);
-  auto launch = [&](auto tag, auto vecc, auto maxt) {
+  auto launch = [&](auto tag, auto vecc, auto maxp, auto maxt) {
     using T = decltype(tag);
-    constexpr bool V = decltype(vecc)::value;
+    constexpr bool VV = decltype(vecc)::value;
+    constexpr int MP = decltype(maxp)::value;
     constexpr int MT_ = decltype(maxt)::value;
-    hipLaunchKernelGGL((gram_partial<T, V, MT_>), dim3(nblk, ngroups), dim3(kBlock), shm, st, n,
-                       rows_per_blk, g, tpg, dA, dB1, dB2, partial);
+    hipLaunchKernelGGL((gram_partial<T, VV, MP, MT_>), dim3(nblk, ngroups), dim3(kBlock), shm, st, n,
+                       rows_per_blk, g, tpg, stg, partial);
+  };
+  auto by_pieces = [&](auto tag, auto vecc, auto maxt) {
+    if (stg.n <= 9) launch(tag, vecc, std::integral_constant<int, 9>(), maxt);
+    else launch(tag, vecc, std::integral_constant<int, kGramPieces>(), maxt);
   };
   auto by_tiles = [&](auto tag, auto vecc) {
-    if (max_slots <= 8) launch(tag, vecc, std::integral_constant<int, 8>());
-    else if (max_slots <= 12) launch(tag, vecc, std::integral_constant<int, 12>());
-    else launch(tag, vecc, std::integral_constant<int, 24>());
+    if (max_slots <= 8) by_pieces(tag, vecc, std::integral_constant<int, 8>());
+    else by_pieces(tag, vecc, std::integral_constant<int, 16>());
   };
   if (dtype == qds::F32 && vec) by_tiles(float(0), std::true_type());
   else if (dtype == qds::F32) by_tiles(float(0), std::false_type());
@@ -686,29 +724,33 @@ QDS_API int qds_gram(int32_t dtype, int64_t n, int32_t m, const void* A, int64_t
 // Y1 = S C1 (n x k), and Y2 = S C2 when C2 != NULL; S given as column blocks.
 // C1, C2: m x k row-major in the storage type.  k <= 64.  Every block's width and
 // leading dimension must be multiples of 4 (F32) or 2 (F64), blocks 16-byte aligned.
+// c2_first: the caller guarantees rows [0, c2_first) of C2 are zero (0: none known).
 QDS_API int qds_block_update_blocks(int32_t dtype, int64_t n, const qds_colblocks* S, int32_t k,
                                     const void* C1, void* Y1, int64_t ldy1, const void* C2, void* Y2,
-                                    int64_t ldy2, void* stream) {
-  ColBlocks dS;
-  int m = 0;
-  QDS_CHECK_ARG(to_device_blocks(S, dS, m), "qds_block_update: bad S blocks");
-  QDS_CHECK_ARG(C1 && Y1 && (!C2 || Y2), "qds_block_update: null pointer");
-  QDS_CHECK_ARG(k >= 1 && k <= 16 * kMaxOutTiles, "qds_block_update: bad shape");
-  if (n == 0) return QDS_OK;
-  hipStream_t st = qds_stream(stream);
+                                    int64_t ldy2, int32_t c2_first, void* stream) {
   QDS_CHECK_ARG(dtype == qds::F32 || dtype == qds::F64, "qds_block_update: dtype must be F32 or F64");
   const int es = dtype == qds::F32 ? 4 : 8;
-  QDS_CHECK_ARG(blocks_vec(dS, S, es), "qds_block_update: S blocks need 16-byte alignment and widths, "
+  QDS_CHECK_ARG(S && blocks_vec(S, es), "qds_block_update: S blocks need 16-byte alignment and widths, "
                 "leading dimensions multiple of %d", 16 / es);
-  if (dtype == qds::F32)
-    return C2 ? launch_update<float, 2>(n, m, k, dS, (const float*)C1, (const float*)C2, (float*)Y1, ldy1,
-                                        (float*)Y2, ldy2, st)
-              : launch_update<float, 1>(n, m, k, dS, (const float*)C1, nullptr, (float*)Y1, ldy1, nullptr,
-                                        ldy2, st);
-  return C2 ? launch_update<double, 2>(n, m, k, dS, (const double*)C1, (const double*)C2, (double*)Y1, ldy1,
-                                       (double*)Y2, ldy2, st)
-            : launch_update<double, 1>(n, m, k, dS, (const double*)C1, nullptr, (double*)Y1, ldy1, nullptr,
-                                       ldy2, st);
+  ColBlocks dS;
+  int m = 0;
+  QDS_CHECK_ARG(to_device_blocks(S, dS, m, es, 16 * (16 / es)), "qds_block_update: bad or too wide S blocks");
+  QDS_CHECK_ARG(C1 && Y1 && (!C2 || Y2), "qds_block_update: null pointer");
+  QDS_CHECK_ARG(k >= 1 && k <= 16 * kMaxOutTiles, "qds_block_update: bad shape");
+  QDS_CHECK_ARG(c2_first >= 0 && c2_first <= m, "qds_block_update: bad c2_first");
+  if (n == 0) return QDS_OK;
+  hipStream_t st = qds_stream(stream);
+  if (!C2) c2_first = 0;
+  if (dtype == qds::F32) {
+    const float *c1 = (const float*)C1, *c2 = (const float*)C2;
+    float *y1 = (float*)Y1, *y2 = (float*)Y2;
+    return C2 ? launch_update<float, 2>(n, m, k, dS, c2_first, c1, c2, y1, ldy1, y2, ldy2, st)
+              : launch_update<float, 1>(n, m, k, dS, 0, c1, nullptr, y1, ldy1, nullptr, ldy2, st);
+  }
+  const double *c1 = (const double*)C1, *c2 = (const double*)C2;
+  double *y1 = (double*)Y1, *y2 = (double*)Y2;
+  return C2 ? launch_update<double, 2>(n, m, k, dS, c2_first, c1, c2, y1, ldy1, y2, ldy2, st)
+            : launch_update<double, 1>(n, m, k, dS, 0, c1, nullptr, y1, ldy1, nullptr, ldy2, st);
 }
 
 // Single-block form: S n x m with leading dimension lds.
@@ -717,7 +759,7 @@ QDS_API int qds_block_update(int32_t dtype, int64_t n, int32_t m, int32_t k, con
                              int64_t ldy2, void* stream) {
   QDS_CHECK_ARG(S && m >= 1, "qds_block_update: bad S");
   qds_colblocks s = single(S, lds, m);
-  return qds_block_update_blocks(dtype, n, &s, k, C1, Y1, ldy1, C2, Y2, ldy2, stream);
+  return qds_block_update_blocks(dtype, n, &s, k, C1, Y1, ldy1, C2, Y2, ldy2, 0, stream);
 }
 
 // R = HX - MX diag(lambda).  Real dtypes: lambda has b doubles.  Complex dtypes:

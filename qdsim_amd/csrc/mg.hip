@@ -97,14 +97,18 @@ __device__ __forceinline__ Vec<T, VEC> apply_at(const Level<T>& L, uint32_t ld, 
   return s;
 }
 
+// Work-item loops of the three level operations.  Item t covers VEC columns of one
+// lattice point; a launch walks items t0, t0 + step, ... (grid-stride in the
+// multi-block kernels, block-stride in the single-block coarse cycle).
+
 // out = u + omega D^-1 (f - A u); zero_init: u = 0 so out = omega D^-1 f
 template <typename T, int VEC>
-__global__ void __launch_bounds__(kBlock)
-jacobi_kernel(Level<T> L, int b, const T* __restrict__ u, int64_t ldu_in, T* __restrict__ out,
-              int64_t ldo, T omega, int zero_init) {
+__device__ __forceinline__ void jacobi_items(const Level<T>& L, int b, const T* __restrict__ u, int64_t ldu_in,
+                                             T* __restrict__ out, int64_t ldo, T omega, int zero_init,
+                                             uint32_t t0, uint32_t step) {
   const uint32_t nv = (uint32_t)(b / VEC);
   const uint32_t total = (uint32_t)(L.nx * L.ny) * nv;
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+  for (uint32_t t = t0; t < total; t += step) {
     const uint32_t p = t / nv;
     const uint32_t c0 = (t - p * nv) * VEC;
     const T sc = omega / L.d[p];
@@ -128,11 +132,11 @@ jacobi_kernel(Level<T> L, int b, const T* __restrict__ u, int64_t ldu_in, T* __r
 
 // coarse f = R (f - A u), R = P^T of bilinear interpolation
 template <typename T, int VEC>
-__global__ void __launch_bounds__(kBlock)
-restrict_residual_kernel(Level<T> F, Level<T> C, int b) {
+__device__ __forceinline__ void restrict_items(const Level<T>& F, const Level<T>& C, int b, uint32_t t0,
+                                               uint32_t step) {
   const uint32_t nv = (uint32_t)(b / VEC);
   const uint32_t total = (uint32_t)(C.nx * C.ny) * nv;
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+  for (uint32_t t = t0; t < total; t += step) {
     const uint32_t pc = t / nv;
     const uint32_t c0 = (t - pc * nv) * VEC;
     const uint32_t jy = pc / (uint32_t)C.nx, jx = pc - jy * (uint32_t)C.nx;
@@ -160,11 +164,11 @@ restrict_residual_kernel(Level<T> F, Level<T> C, int b) {
 
 // fine u += P e (bilinear); coarse point (jx, jy) sits at fine (2jx+1, 2jy+1)
 template <typename T, int VEC>
-__global__ void __launch_bounds__(kBlock)
-prolong_add_kernel(Level<T> F, Level<T> C, int b) {
+__device__ __forceinline__ void prolong_items(const Level<T>& F, const Level<T>& C, int b, uint32_t t0,
+                                              uint32_t step) {
   const uint32_t nv = (uint32_t)(b / VEC);
   const uint32_t total = (uint32_t)(F.nx * F.ny) * nv;
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+  for (uint32_t t = t0; t < total; t += step) {
     const uint32_t p = t / nv;
     const uint32_t c0 = (t - p * nv) * VEC;
     const uint32_t iy = p / (uint32_t)F.nx, ix = p - iy * (uint32_t)F.nx;
@@ -194,6 +198,90 @@ prolong_add_kernel(Level<T> F, Level<T> C, int b) {
 #pragma unroll
     for (int i = 0; i < VEC; ++i) uv.v[i] += acc.v[i];
     uv.store(F.u + p * (uint32_t)F.ldu + c0);
+  }
+}
+
+// dense copy of u into t (first sweep of an odd count on a live u)
+template <typename T>
+__device__ __forceinline__ void copy_items(int64_t np, int b, const T* __restrict__ src, int64_t lds,
+                                           T* __restrict__ dst, uint32_t t0, uint32_t step) {
+  const uint32_t total = (uint32_t)np * (uint32_t)b;
+  for (uint32_t t = t0; t < total; t += step) {
+    const uint32_t p = t / (uint32_t)b;
+    const uint32_t c = t - p * (uint32_t)b;
+    dst[t] = src[p * (uint32_t)lds + c];
+  }
+}
+
+#define QDS_GRID_T0 (blockIdx.x * blockDim.x + threadIdx.x)
+#define QDS_GRID_STEP (gridDim.x * blockDim.x)
+
+template <typename T, int VEC>
+__global__ void __launch_bounds__(kBlock)
+jacobi_kernel(Level<T> L, int b, const T* __restrict__ u, int64_t ldu_in, T* __restrict__ out,
+              int64_t ldo, T omega, int zero_init) {
+  jacobi_items<T, VEC>(L, b, u, ldu_in, out, ldo, omega, zero_init, QDS_GRID_T0, QDS_GRID_STEP);
+}
+
+template <typename T, int VEC>
+__global__ void __launch_bounds__(kBlock)
+restrict_residual_kernel(Level<T> F, Level<T> C, int b) {
+  restrict_items<T, VEC>(F, C, b, QDS_GRID_T0, QDS_GRID_STEP);
+}
+
+template <typename T, int VEC>
+__global__ void __launch_bounds__(kBlock)
+prolong_add_kernel(Level<T> F, Level<T> C, int b) {
+  prolong_items<T, VEC>(F, C, b, QDS_GRID_T0, QDS_GRID_STEP);
+}
+
+// Levels handled by the single-workgroup coarse cycle (level l0 and below).
+constexpr int kMaxCoarseLevels = 12;
+constexpr int kCoarseBlock = 1024;
+constexpr int64_t kCoarseItems = 24576;  // work items of the finest level it takes
+
+template <typename T>
+struct LevelPack {
+  Level<T> L[kMaxCoarseLevels];
+};
+
+// Jacobi sweeps on one level within one workgroup, same ping-pong schedule as the
+// multi-launch path (the result ends in u).
+template <typename T, int VEC>
+__device__ void block_smooth(const Level<T>& L, int b, int sweeps, T omega, bool zero_first) {
+  T* src = L.u;
+  T* dst = (sweeps % 2 == 0) ? L.t : L.u;
+  for (int s = 0; s < sweeps; ++s) {
+    const bool zi = zero_first && s == 0;
+    if (!zi && src == dst) {
+      copy_items<T>(L.nx * L.ny, b, L.u, L.ldu, L.t, threadIdx.x, blockDim.x);
+      __syncthreads();
+      src = L.t;
+    }
+    jacobi_items<T, VEC>(L, b, src, src == L.u ? L.ldu : (int64_t)b, dst, dst == L.u ? L.ldu : (int64_t)b,
+                         omega, zi ? 1 : 0, threadIdx.x, blockDim.x);
+    __syncthreads();
+    src = dst;
+    dst = (dst == L.u) ? L.t : L.u;
+  }
+}
+
+// The V-cycle below (and including) pack level 0, whose f is already set: one
+// workgroup, phases separated by workgroup barriers (global-memory results of a
+// phase are visible to the whole workgroup after the barrier).
+template <typename T, int VEC>
+__global__ void __launch_bounds__(kCoarseBlock)
+coarse_cycle_kernel(LevelPack<T> P, int nl, int b, int nu1, int nu2, int ncoarse, T omega) {
+  for (int l = 0; l < nl - 1; ++l) {
+    block_smooth<T, VEC>(P.L[l], b, nu1, omega, true);
+    restrict_items<T, VEC>(P.L[l], P.L[l + 1], b, threadIdx.x, blockDim.x);
+    __syncthreads();
+  }
+  block_smooth<T, VEC>(P.L[nl - 1], b, ncoarse, omega, true);
+  for (int l = nl - 2; l >= 0; --l) {
+    prolong_items<T, VEC>(P.L[l], P.L[l + 1], b, threadIdx.x, blockDim.x);
+    __syncthreads();
+    block_smooth<T, VEC>(P.L[l], b, nu2, omega, false);
   }
 }
 
@@ -248,7 +336,11 @@ int smooth(const Level<T>& L, int b, int sweeps, T omega, bool zero_first, hipSt
 
 template <typename T, int VEC>
 int vcycle(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, hipStream_t st) {
-  for (int l = 0; l < nl - 1; ++l) {
+  // first level small enough for the single-workgroup coarse cycle
+  int lc = nl;
+  for (int l = 0; l < nl; ++l)
+    if (L[l].nx * L[l].ny * (b / VEC) <= kCoarseItems && nl - l <= kMaxCoarseLevels) { lc = l; break; }
+  for (int l = 0; l < lc && l < nl - 1; ++l) {
     int rc = smooth<T, VEC>(L[l], b, nu1, omega, true, st);
     if (rc) return rc;
     const int64_t work = L[l + 1].nx * L[l + 1].ny * (b / VEC);
@@ -256,14 +348,23 @@ int vcycle(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, h
                        dim3(kBlock), 0, st, L[l], L[l + 1], b);
     QDS_LAUNCH_CHECK();
   }
-  int rc = smooth<T, VEC>(L[nl - 1], b, ncoarse, omega, true, st);
-  if (rc) return rc;
-  for (int l = nl - 2; l >= 0; --l) {
+  if (lc < nl) {
+    LevelPack<T> P;
+    for (int l = lc; l < nl; ++l) P.L[l - lc] = L[l];
+    hipLaunchKernelGGL((coarse_cycle_kernel<T, VEC>), dim3(1), dim3(kCoarseBlock), 0, st, P, nl - lc, b, nu1,
+                       nu2, ncoarse, omega);
+    QDS_LAUNCH_CHECK();
+  } else {
+    int rc = smooth<T, VEC>(L[nl - 1], b, ncoarse, omega, true, st);
+    if (rc) return rc;
+  }
+  const int top = lc < nl ? lc - 1 : nl - 2;
+  for (int l = top; l >= 0; --l) {
     const int64_t work = L[l].nx * L[l].ny * (b / VEC);
     hipLaunchKernelGGL((prolong_add_kernel<T, VEC>), dim3(qds_blocks(work, kBlock, 65536)), dim3(kBlock),
                        0, st, L[l], L[l + 1], b);
     QDS_LAUNCH_CHECK();
-    rc = smooth<T, VEC>(L[l], b, nu2, omega, false, st);
+    int rc = smooth<T, VEC>(L[l], b, nu2, omega, false, st);
     if (rc) return rc;
   }
   return QDS_OK;

@@ -49,58 +49,70 @@ __device__ __forceinline__ void store_vec(T* p, const T (&v)[VEC]) {
   *reinterpret_cast<V*>(p) = t;
 }
 
-// Real block product.  b columns, ncols_units = b.
+// Rows are spread over the 8 XCDs in contiguous ranges: the hardware hands
+// workgroups to the XCDs round-robin, so block id g runs on XCD g % 8; logical
+// block (g % 8) * (G / 8) + g / 8 makes each XCD sweep its own eighth of the rows
+// in order, and the X rows that a window of concurrently running blocks touches
+// (the band of the matrix) stay in that XCD's L2 instead of being fetched by all
+// eight.  G is a multiple of 8.
+__device__ __forceinline__ int64_t xcd_block(int64_t g, int64_t G) {
+  return (g % 8) * (G / 8) + g / 8;
+}
+
+// Real block product.  lpr lanes own one row (lpr = ceil(b / VEC), any value up to
+// 64); a wave holds rpw = 64 / lpr rows, the remaining lanes idle.
 template <typename T, int VEC, bool FUSED>
 __global__ void __launch_bounds__(kBlock)
-spmm_real(int64_t n, int b, int lpr, const int32_t* __restrict__ indptr,
+spmm_real(int64_t n, int b, int lpr, int rpw, const int32_t* __restrict__ indptr,
           const int32_t* __restrict__ indices, const T* __restrict__ va, const T* __restrict__ vb,
           const T* __restrict__ X, int64_t ldx, T* __restrict__ Ya, int64_t ldya,
           T* __restrict__ Yb, int64_t ldyb) {
-  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int li = (int)(gtid % lpr);
-  const int64_t groups = (int64_t)gridDim.x * blockDim.x / lpr;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int rg = lane / lpr;
+  const int li = lane - rg * lpr;
   const int c0 = li * VEC;
-  const bool active = c0 < b;
-  for (int64_t r = gtid / lpr; r < n; r += groups) {
-    T acc_a[VEC], acc_b[VEC];
+  if (rg >= rpw || c0 >= b) return;
+  const int64_t rows_per_block = (int64_t)(kBlock / 64) * rpw;
+  const int64_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t r = lb * rows_per_block + (int64_t)wave * rpw + rg;
+  if (r >= n) return;
+  T acc_a[VEC], acc_b[VEC];
 #pragma unroll
-    for (int v = 0; v < VEC; ++v) { acc_a[v] = T(0); acc_b[v] = T(0); }
-    const int32_t kb = indptr[r], ke = indptr[r + 1];
-    if (active) {
-      int32_t k = kb;
-      for (; k + 2 <= ke; k += 2) {
-        const int32_t c_0 = indices[k], c_1 = indices[k + 1];
-        const T a0 = va[k], a1 = va[k + 1];
-        T b0 = T(0), b1 = T(0);
-        if (FUSED) { b0 = vb[k]; b1 = vb[k + 1]; }
-        T x0[VEC], x1[VEC];
-        load_vec<T, VEC>(X + (int64_t)c_0 * ldx + c0, x0);
-        load_vec<T, VEC>(X + (int64_t)c_1 * ldx + c0, x1);
+  for (int v = 0; v < VEC; ++v) { acc_a[v] = T(0); acc_b[v] = T(0); }
+  const int32_t kb = indptr[r], ke = indptr[r + 1];
+  int32_t k = kb;
+  for (; k + 2 <= ke; k += 2) {
+    const int32_t c_0 = indices[k], c_1 = indices[k + 1];
+    const T a0 = va[k], a1 = va[k + 1];
+    T b0 = T(0), b1 = T(0);
+    if (FUSED) { b0 = vb[k]; b1 = vb[k + 1]; }
+    T x0[VEC], x1[VEC];
+    load_vec<T, VEC>(X + (int64_t)c_0 * ldx + c0, x0);
+    load_vec<T, VEC>(X + (int64_t)c_1 * ldx + c0, x1);
 #pragma unroll
-        for (int v = 0; v < VEC; ++v) {
-          acc_a[v] = fma(a0, x0[v], acc_a[v]);
-          acc_a[v] = fma(a1, x1[v], acc_a[v]);
-          if (FUSED) {
-            acc_b[v] = fma(b0, x0[v], acc_b[v]);
-            acc_b[v] = fma(b1, x1[v], acc_b[v]);
-          }
-        }
+    for (int v = 0; v < VEC; ++v) {
+      acc_a[v] = fma(a0, x0[v], acc_a[v]);
+      acc_a[v] = fma(a1, x1[v], acc_a[v]);
+      if (FUSED) {
+        acc_b[v] = fma(b0, x0[v], acc_b[v]);
+        acc_b[v] = fma(b1, x1[v], acc_b[v]);
       }
-      for (; k < ke; ++k) {
-        const int32_t c_0 = indices[k];
-        const T a0 = va[k];
-        T x0[VEC];
-        load_vec<T, VEC>(X + (int64_t)c_0 * ldx + c0, x0);
-#pragma unroll
-        for (int v = 0; v < VEC; ++v) {
-          acc_a[v] = fma(a0, x0[v], acc_a[v]);
-          if (FUSED) acc_b[v] = fma(vb[k], x0[v], acc_b[v]);
-        }
-      }
-      store_vec<T, VEC>(Ya + r * ldya + c0, acc_a);
-      if (FUSED) store_vec<T, VEC>(Yb + r * ldyb + c0, acc_b);
     }
   }
+  for (; k < ke; ++k) {
+    const int32_t c_0 = indices[k];
+    const T a0 = va[k];
+    T x0[VEC];
+    load_vec<T, VEC>(X + (int64_t)c_0 * ldx + c0, x0);
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) {
+      acc_a[v] = fma(a0, x0[v], acc_a[v]);
+      if (FUSED) acc_b[v] = fma(vb[k], x0[v], acc_b[v]);
+    }
+  }
+  store_vec<T, VEC>(Ya + r * ldya + c0, acc_a);
+  if (FUSED) store_vec<T, VEC>(Yb + r * ldyb + c0, acc_b);
 }
 
 // Complex block product; T is the real component type, VEC counts complex numbers
@@ -188,13 +200,15 @@ template <typename T, int VEC, bool FUSED>
 int launch_real(int64_t n, int b, const int32_t* ip, const int32_t* ix, const void* va,
                 const void* vb, const void* X, int64_t ldx, void* Ya, int64_t ldya, void* Yb,
                 int64_t ldyb, hipStream_t st) {
-  int lpr = 1;
-  while (lpr * VEC < b) lpr <<= 1;
+  const int lpr = (b + VEC - 1) / VEC;
   QDS_CHECK_ARG(lpr <= 64, "qds_spmm: block width %d too large", b);
-  const int64_t rows_per_block = kBlock / lpr;
-  hipLaunchKernelGGL((spmm_real<T, VEC, FUSED>), dim3(qds_blocks(n, (int)rows_per_block, 262144)),
-                     dim3(kBlock), 0, st, n, b, lpr, ip, ix, (const T*)va, (const T*)vb,
-                     (const T*)X, ldx, (T*)Ya, ldya, (T*)Yb, ldyb);
+  const int rpw = 64 / lpr;
+  const int64_t rows_per_block = (int64_t)(kBlock / 64) * rpw;
+  int64_t blocks = (n + rows_per_block - 1) / rows_per_block;
+  blocks = (blocks + 7) / 8 * 8;
+  QDS_CHECK_ARG(blocks < (int64_t(1) << 31), "qds_spmm: too many rows");
+  hipLaunchKernelGGL((spmm_real<T, VEC, FUSED>), dim3((unsigned)blocks), dim3(kBlock), 0, st, n, b, lpr, rpw,
+                     ip, ix, (const T*)va, (const T*)vb, (const T*)X, ldx, (T*)Ya, ldya, (T*)Yb, ldyb);
   QDS_LAUNCH_CHECK();
   return QDS_OK;
 }

@@ -192,8 +192,9 @@ def gram_blocks(A, B1, B2=None, sym=False, out=None):
     return out
 
 
-def block_update_blocks(S, C1, Y1, C2=None, Y2=None):
-    """Y1 = [S_0 | S_1 | ...] C1 (and Y2 = ... C2); C in the storage dtype of S."""
+def block_update_blocks(S, C1, Y1, C2=None, Y2=None, c2_first=0):
+    """Y1 = [S_0 | S_1 | ...] C1 (and Y2 = ... C2); C in the storage dtype of S.
+    c2_first: rows of C2 before it are zero (the kernel skips them)."""
     n = S[0].shape[0]
     k = C1.shape[1]
     dt = S[0].dtype
@@ -201,7 +202,8 @@ def block_update_blocks(S, C1, Y1, C2=None, Y2=None):
     C2 = C2.to(dt).contiguous() if C2 is not None else None
     cs = _colblocks(S)
     check(lib.qds_block_update_blocks(code(S[0]), n, ctypes.byref(cs), k, ptr(C1), ptr(Y1), _ld(Y1),
-                                      ptr(C2), ptr(Y2), _ld(Y2) if Y2 is not None else 1, stream_ptr()),
+                                      ptr(C2), ptr(Y2), _ld(Y2) if Y2 is not None else 1,
+                                      int(c2_first) if C2 is not None else 0, stream_ptr()),
           "qds_block_update_blocks")
 
 

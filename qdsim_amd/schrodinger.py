@@ -106,7 +106,8 @@ class SchrodingerProblem:
         return LatticeMG(self.lat, c_e, v_e, sigma, b, dtype=dtype)
 
     # -- solve ------------------------------------------------------------------
-    def solve(self, k, b=None, mode="mixed", tol=1e-6, tol32=3e-4, maxiter=400, seed=0, verbose=False):
+    def solve(self, k, b=None, mode="mixed", tol=1e-6, tol32=3e-5, maxiter=400, seed=0, verbose=False,
+              **mixed_opts):
         """Lowest k eigenpairs.  Returns (eigenvalues eV (np), full-numbering
         eigenvectors (n, k) float64 device tensor, M_nm-normalised, info)."""
         if self.H is None:
@@ -127,7 +128,7 @@ class SchrodingerProblem:
                 W.copy_(_buf["w"])
 
             res = solve_mixed(op64, k, b, pre32, pre64, tol=tol, tol32=tol32, maxiter=maxiter, seed=seed,
-                              verbose=verbose)
+                              verbose=verbose, **mixed_opts)
         elif mode == "fp64":
             pre = self.preconditioner(b, torch.float64)
             res = lobpcg(op64, k, b, pre, tol=tol, maxiter=maxiter, seed=seed, verbose=verbose)

@@ -143,6 +143,13 @@ def test_gram_and_update_column_blocks(dtype, b, nblk):
         Cd = C.to(dtype).double().cpu().numpy()
         r = Sd @ Cd
         assert np.abs(Y.double().cpu().numpy() - r).max() <= tol * np.abs(Sd).max() * np.abs(Cd).sum(0).max()
+    # leading zero rows of C2 declared to the kernel (the P update of the eigensolver)
+    C2z = C2.clone()
+    C2z[:b] = 0.0
+    kernels.block_update_blocks(S, C1, Y1, C2z, Y2, c2_first=b)
+    Cd = C2z.to(dtype).double().cpu().numpy()
+    r = Sd @ Cd
+    assert np.abs(Y2.double().cpu().numpy() - r).max() <= tol * np.abs(Sd).max() * np.abs(Cd).sum(0).max()
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64, torch.complex64, torch.complex128])

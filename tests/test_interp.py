@@ -1,0 +1,87 @@
+"""FE point interpolation (FEInterpolator, fe_interpolator.cpp:54-613): oracle
+pinning on the CPU, device kernel against the oracle on the GPU."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import interp_ref
+
+
+def _mesh(order, nx=7, ny=5, Lx=50.0, Ly=30.0):
+    from qdsim_amd.mesh import SimpleMesh
+    m = SimpleMesh(nx, ny, Lx, Ly, order)
+    x, y = m.get_nodes()
+    return m, x, y, m.get_elements()
+
+
+def _poly(order, rng):
+    """Random polynomial of total degree `order` with its gradient."""
+    c = rng.standard_normal((order + 1, order + 1))
+    terms = [(a, b) for a in range(order + 1) for b in range(order + 1 - a)]
+
+    def f(x, y):
+        return sum(c[a, b] * (x / 50) ** a * (y / 30) ** b for a, b in terms)
+
+    def g(x, y):
+        gx = sum(c[a, b] * a * (x / 50) ** max(a - 1, 0) / 50 * (y / 30) ** b for a, b in terms if a > 0)
+        gy = sum(c[a, b] * b * (x / 50) ** a * (y / 30) ** max(b - 1, 0) / 30 for a, b in terms if b > 0)
+        return gx, gy
+    return f, g
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_oracle_reproduces_polynomials(order):
+    """A degree-p field is reproduced exactly by P_p interpolation, with its gradient."""
+    rng = np.random.default_rng(order)
+    m, x, y, el = _mesh(order)
+    f, g = _poly(order, rng)
+    field = f(x, y)
+    for px, py in rng.uniform([0, 0], [50, 30], (25, 2)):
+        v, gx, gy = interp_ref.interpolate(px, py, x, y, el, order, field, gradient=True)
+        ex, ey = g(px, py)
+        assert abs(v - f(px, py)) < 1e-11
+        assert abs(gx - ex) < 1e-10 and abs(gy - ey) < 1e-10
+
+
+def test_oracle_outside_and_element_choice():
+    m, x, y, el = _mesh(2)
+    field = np.arange(len(x), dtype=float)
+    # outside: nearest node by brute force over node coordinates
+    for px, py in [(-3.0, 7.1), (55.0, 31.0), (20.2, -4.0)]:
+        nd = int(np.argmin((x - px) ** 2 + (y - py) ** 2))
+        assert interp_ref.interpolate(px, py, x, y, el, 2, field) == field[nd]
+    # a point on a shared edge belongs to the lower-index element
+    e = interp_ref.find_element(x[el[0, 1]], y[el[0, 1]], x, y, el)
+    assert e == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_device_interpolation_matches_oracle(order):
+    from qdsim_amd.interpolator import FEInterpolator
+    rng = np.random.default_rng(10 + order)
+    m, x, y, el = _mesh(order)
+    field = rng.standard_normal(len(x))
+    pts = [tuple(p) for p in rng.uniform([0, 0], [50, 30], (200, 2))]
+    # vertices, edge midpoints and boundary points, and points outside the mesh
+    pts += [(x[i], y[i]) for i in range(0, len(x), 3)]
+    pts += [(-3.0, 7.1), (55.0, 31.0), (20.2, -4.0), (-1.0, -1.0), (50.0, 12.0), (25.0, 30.0 + 1e-12)]
+    px = np.array([p[0] for p in pts])
+    py = np.array([p[1] for p in pts])
+    fi = FEInterpolator(m)
+    v, gx, gy, e = fi.interpolate_points(px, py, field, gradient=True, return_element=True)
+    v, gx, gy, e = (t.cpu().numpy() for t in (v, gx, gy, e))
+    scale = np.abs(field).max()
+    for i, (a, b) in enumerate(pts):
+        rv, rgx, rgy = interp_ref.interpolate(a, b, x, y, el, order, field, gradient=True)
+        assert e[i] == interp_ref.find_element(a, b, x, y, el), (a, b)
+        assert abs(v[i] - rv) <= 1e-12 * scale, (a, b, v[i], rv)
+        gs = max(1.0, abs(rgx), abs(rgy))
+        assert abs(gx[i] - rgx) <= 1e-11 * gs * scale and abs(gy[i] - rgy) <= 1e-11 * gs * scale, (a, b)
+    # per-point reference API
+    assert abs(fi.interpolate(px[0], py[0], field) - v[0]) < 1e-14 * scale
+    assert fi.find_element(px[1], py[1]) == e[1]
+    inside, lam = fi.compute_barycentric_coordinates(px[0], py[0], [(x[j], y[j]) for j in el[e[0], :3]])
+    assert inside and abs(sum(lam) - 1.0) < 1e-12
+    N = fi.evaluate_shape_functions(lam)
+    assert abs(np.dot(N, field[el[e[0]]]) - v[0]) < 1e-12 * scale
