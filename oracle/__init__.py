@@ -20,6 +20,10 @@ Parity status per function:
   fem_ref.assemble ............... pinned: scipy COO->CSR canonical form, nnz counts
   eig_ref ........................ pinned: tridiag(-1,2,-1) spectrum, particle in a box
   poisson_ref .................... pinned: linear solution phi = V x / Lx (P1, rho = 0)
+  cap_ref ........................ pinned: LAPACK zggev of the oracle pencil; W = 0 reduces to eig_ref
+  fixed_ref ...................... pinned: restated matrices vs the restricted symmetric pencil
+  interp_ref ..................... pinned: exact reproduction of degree-p polynomials (values and
+                                   gradients), brute-force nearest node outside the mesh
 """
 
 PINNING = __doc__

@@ -28,6 +28,7 @@ complex64 storage for the bulk of the iterations, complex128 refinement.
 """
 import numpy as np
 import torch
+from threadpoolctl import threadpool_limits
 
 from . import kernels
 from .units import EV_TO_J
@@ -108,6 +109,13 @@ def _rr(G, m, nb, drop_tol, target):
 
 def cap_lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, seed=0, target=None,
                drop_tol=None, stall_iters=None, verbose=False):
+    # small host eigenproblems: single-threaded BLAS avoids thread wake-up costs
+    with threadpool_limits(1, "blas"):
+        return _cap_lobpcg(op, k, b, precond, X0, tol, maxiter, seed, target, drop_tol, stall_iters, verbose)
+
+
+def _cap_lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, seed=0, target=None,
+                drop_tol=None, stall_iters=None, verbose=False):
     dev = op.H.device
     dt = op.dtype
     n = op.n
@@ -218,13 +226,16 @@ def solve_cap(prob, k, b=None, target=None, tol=1e-6, tol32=3e-4, maxiter=600, s
         sel = sel[torch.argsort(lam[sel].real)]
     else:
         sel = torch.arange(k, device=lam.device)
-    lam = lam[sel]
-    Xk = X[:, sel].to(torch.complex128)
-    # Hermitian M-normalisation with the phase making psi^T M psi real positive
+    Xk = X[:, sel].to(torch.complex128).contiguous()
     MX = torch.empty_like(Xk)
     HX = torch.empty_like(Xk)
-    op64.apply(Xk.contiguous(), HX, MX)
+    op64.apply(Xk, HX, MX)
     bil = (Xk * MX).sum(0)
+    # bilinear Rayleigh quotient x^T A x / x^T M x: second-order accurate for a
+    # complex-symmetric pencil (the Hermitian Ritz value of the orthogonal
+    # projection is only first-order accurate for non-Hermitian A)
+    lam = (Xk * HX).sum(0) / bil
+    # Hermitian M-normalisation with the phase making psi^T M psi real positive
     Xk = Xk * torch.exp(-0.5j * torch.angle(bil))[None, :]
     herm = (Xk.conj() * MX * torch.exp(-0.5j * torch.angle(bil))[None, :]).sum(0).real
     Xk = Xk / herm.clamp_min(1e-300).sqrt()[None, :]

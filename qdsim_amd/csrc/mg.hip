@@ -235,6 +235,175 @@ prolong_add_kernel(Level<T> F, Level<T> C, int b) {
   prolong_items<T, VEC>(F, C, b, QDS_GRID_T0, QDS_GRID_STEP);
 }
 
+// ---- fused V(1,1) passes ---------------------------------------------------
+// With a zero initial guess and one damped-Jacobi sweep before and after the
+// coarse correction, a level needs only two passes over its lattice:
+//   down:  u1 = omega D^-1 f,  r = f - A u1 = (1 - omega) f + omega sum_q w_pq f_q / d_q,
+//          coarse f = R r                        (reads f; u1 is never stored)
+//   up:    u2 = u1 + P e,  u = u2 + omega D^-1 (f - A u2)
+//          (reads f and the coarse correction e; writes u)
+// This is the same arithmetic as the sweep/restrict/prolong/sweep sequence.
+
+// sum over the lattice neighbours q of p of w_pq * src_q / d_q
+template <typename T, int VEC>
+__device__ __forceinline__ Vec<T, VEC> nbr_scaled(const Level<T>& L, uint32_t ld, uint32_t p, uint32_t ix,
+                                                  uint32_t iy, uint32_t c0, const T* __restrict__ src) {
+  Vec<T, VEC> s, x;
+#pragma unroll
+  for (int i = 0; i < VEC; ++i) s.v[i] = T(0);
+  const uint32_t nx = (uint32_t)L.nx;
+  if (ix + 1 < nx) {
+    const T w = L.we[p] / L.d[p + 1];
+    x.load(src + (p + 1) * ld + c0);
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) s.v[i] += w * x.v[i];
+  }
+  if (ix > 0) {
+    const T w = L.we[p - 1] / L.d[p - 1];
+    x.load(src + (p - 1) * ld + c0);
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) s.v[i] += w * x.v[i];
+  }
+  if (iy + 1 < (uint32_t)L.ny) {
+    const T w = L.wn[p] / L.d[p + nx];
+    x.load(src + (p + nx) * ld + c0);
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) s.v[i] += w * x.v[i];
+  }
+  if (iy > 0) {
+    const T w = L.wn[p - nx] / L.d[p - nx];
+    x.load(src + (p - nx) * ld + c0);
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) s.v[i] += w * x.v[i];
+  }
+  return s;
+}
+
+// (P e)[p] for fine point (ix, iy): bilinear weights of the coarse points around it
+template <typename T, int VEC>
+__device__ __forceinline__ Vec<T, VEC> prolong_at(const Level<T>& C, uint32_t ix, uint32_t iy, uint32_t c0) {
+  int xs[2], ys[2];
+  T wx[2], wy[2];
+  int nxp, nyp;
+  if (ix & 1) { xs[0] = (int)(ix - 1) / 2; wx[0] = T(1); nxp = 1; }
+  else { xs[0] = (int)ix / 2 - 1; xs[1] = (int)ix / 2; wx[0] = wx[1] = T(0.5); nxp = 2; }
+  if (iy & 1) { ys[0] = (int)(iy - 1) / 2; wy[0] = T(1); nyp = 1; }
+  else { ys[0] = (int)iy / 2 - 1; ys[1] = (int)iy / 2; wy[0] = wy[1] = T(0.5); nyp = 2; }
+  Vec<T, VEC> acc, e;
+#pragma unroll
+  for (int i = 0; i < VEC; ++i) acc.v[i] = T(0);
+  for (int a = 0; a < nyp; ++a) {
+    if (ys[a] < 0 || ys[a] >= (int)C.ny) continue;
+    for (int q = 0; q < nxp; ++q) {
+      if (xs[q] < 0 || xs[q] >= (int)C.nx) continue;
+      e.load(C.u + ((uint32_t)ys[a] * (uint32_t)C.nx + (uint32_t)xs[q]) * (uint32_t)C.ldu + c0);
+      const T w = wx[q] * wy[a];
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) acc.v[i] += w * e.v[i];
+    }
+  }
+  return acc;
+}
+
+template <typename T, int VEC>
+__device__ __forceinline__ void down_items(const Level<T>& F, const Level<T>& C, int b, T omega, uint32_t t0,
+                                           uint32_t step) {
+  const uint32_t nv = (uint32_t)(b / VEC);
+  const uint32_t total = (uint32_t)(C.nx * C.ny) * nv;
+  const uint32_t ldf = (uint32_t)F.ldf;
+  for (uint32_t t = t0; t < total; t += step) {
+    const uint32_t pc = t / nv;
+    const uint32_t c0 = (t - pc * nv) * VEC;
+    const uint32_t jy = pc / (uint32_t)C.nx, jx = pc - jy * (uint32_t)C.nx;
+    Vec<T, VEC> acc;
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) acc.v[i] = T(0);
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) {
+        const uint32_t x = 2 * jx + 1 + dx, y = 2 * jy + 1 + dy;
+        const T w = (dx == 0 ? T(1) : T(0.5)) * (dy == 0 ? T(1) : T(0.5));
+        const uint32_t p = y * (uint32_t)F.nx + x;
+        Vec<T, VEC> fv;
+        fv.load(F.f + p * ldf + c0);
+        const Vec<T, VEC> ns = nbr_scaled<T, VEC>(F, ldf, p, x, y, c0, F.f);
+#pragma unroll
+        for (int i = 0; i < VEC; ++i) acc.v[i] += w * ((T(1) - omega) * fv.v[i] + omega * ns.v[i]);
+      }
+    }
+    acc.store(C.f + pc * (uint32_t)C.ldf + c0);
+  }
+}
+
+template <typename T, int VEC>
+__device__ __forceinline__ void up_items(const Level<T>& F, const Level<T>& C, int b, T omega, uint32_t t0,
+                                         uint32_t step) {
+  const uint32_t nv = (uint32_t)(b / VEC);
+  const uint32_t total = (uint32_t)(F.nx * F.ny) * nv;
+  const uint32_t ldf = (uint32_t)F.ldf;
+  const uint32_t nx = (uint32_t)F.nx;
+  for (uint32_t t = t0; t < total; t += step) {
+    const uint32_t p = t / nv;
+    const uint32_t c0 = (t - p * nv) * VEC;
+    const uint32_t iy = p / nx, ix = p - iy * nx;
+    // u2 at p and its neighbours, then one Jacobi step at p
+    auto u2 = [&](uint32_t q, uint32_t qx, uint32_t qy) {
+      Vec<T, VEC> fv, e = prolong_at<T, VEC>(C, qx, qy, c0);
+      fv.load(F.f + q * ldf + c0);
+      const T sc = omega / F.d[q];
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) e.v[i] += sc * fv.v[i];
+      return e;
+    };
+    Vec<T, VEC> up = u2(p, ix, iy), fp, s;
+    fp.load(F.f + p * ldf + c0);
+    const T dp = F.d[p];
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) s.v[i] = fp.v[i] - dp * up.v[i];
+    if (ix + 1 < nx) {
+      const T w = F.we[p];
+      const Vec<T, VEC> uq = u2(p + 1, ix + 1, iy);
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) s.v[i] += w * uq.v[i];
+    }
+    if (ix > 0) {
+      const T w = F.we[p - 1];
+      const Vec<T, VEC> uq = u2(p - 1, ix - 1, iy);
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) s.v[i] += w * uq.v[i];
+    }
+    if (iy + 1 < (uint32_t)F.ny) {
+      const T w = F.wn[p];
+      const Vec<T, VEC> uq = u2(p + nx, ix, iy + 1);
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) s.v[i] += w * uq.v[i];
+    }
+    if (iy > 0) {
+      const T w = F.wn[p - nx];
+      const Vec<T, VEC> uq = u2(p - nx, ix, iy - 1);
+#pragma unroll
+      for (int i = 0; i < VEC; ++i) s.v[i] += w * uq.v[i];
+    }
+    const T sc = omega / dp;
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) up.v[i] += sc * s.v[i];
+    up.store(F.u + p * (uint32_t)F.ldu + c0);
+  }
+}
+
+template <typename T, int VEC>
+__global__ void __launch_bounds__(kBlock)
+mg_down_kernel(Level<T> F, Level<T> C, int b, T omega) {
+  down_items<T, VEC>(F, C, b, omega, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+template <typename T, int VEC>
+__global__ void __launch_bounds__(kBlock)
+mg_up_kernel(Level<T> F, Level<T> C, int b, T omega) {
+  up_items<T, VEC>(F, C, b, omega, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
 // Levels handled by the single-workgroup coarse cycle (level l0 and below).
 constexpr int kMaxCoarseLevels = 12;
 constexpr int kCoarseBlock = 1024;
@@ -272,6 +441,18 @@ __device__ void block_smooth(const Level<T>& L, int b, int sweeps, T omega, bool
 template <typename T, int VEC>
 __global__ void __launch_bounds__(kCoarseBlock)
 coarse_cycle_kernel(LevelPack<T> P, int nl, int b, int nu1, int nu2, int ncoarse, T omega) {
+  if (nu1 == 1 && nu2 == 1) {
+    for (int l = 0; l < nl - 1; ++l) {
+      down_items<T, VEC>(P.L[l], P.L[l + 1], b, omega, threadIdx.x, blockDim.x);
+      __syncthreads();
+    }
+    block_smooth<T, VEC>(P.L[nl - 1], b, ncoarse, omega, true);
+    for (int l = nl - 2; l >= 0; --l) {
+      up_items<T, VEC>(P.L[l], P.L[l + 1], b, omega, threadIdx.x, blockDim.x);
+      __syncthreads();
+    }
+    return;
+  }
   for (int l = 0; l < nl - 1; ++l) {
     block_smooth<T, VEC>(P.L[l], b, nu1, omega, true);
     restrict_items<T, VEC>(P.L[l], P.L[l + 1], b, threadIdx.x, blockDim.x);
@@ -340,10 +521,17 @@ int vcycle(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, h
   int lc = nl;
   for (int l = 0; l < nl; ++l)
     if (L[l].nx * L[l].ny * (b / VEC) <= kCoarseItems && nl - l <= kMaxCoarseLevels) { lc = l; break; }
+  const bool fused = nu1 == 1 && nu2 == 1;
   for (int l = 0; l < lc && l < nl - 1; ++l) {
+    const int64_t work = L[l + 1].nx * L[l + 1].ny * (b / VEC);
+    if (fused) {
+      hipLaunchKernelGGL((mg_down_kernel<T, VEC>), dim3(qds_blocks(work, kBlock, 65536)), dim3(kBlock), 0, st,
+                         L[l], L[l + 1], b, omega);
+      QDS_LAUNCH_CHECK();
+      continue;
+    }
     int rc = smooth<T, VEC>(L[l], b, nu1, omega, true, st);
     if (rc) return rc;
-    const int64_t work = L[l + 1].nx * L[l + 1].ny * (b / VEC);
     hipLaunchKernelGGL((restrict_residual_kernel<T, VEC>), dim3(qds_blocks(work, kBlock, 65536)),
                        dim3(kBlock), 0, st, L[l], L[l + 1], b);
     QDS_LAUNCH_CHECK();
@@ -361,6 +549,12 @@ int vcycle(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, h
   const int top = lc < nl ? lc - 1 : nl - 2;
   for (int l = top; l >= 0; --l) {
     const int64_t work = L[l].nx * L[l].ny * (b / VEC);
+    if (fused) {
+      hipLaunchKernelGGL((mg_up_kernel<T, VEC>), dim3(qds_blocks(work, kBlock, 65536)), dim3(kBlock), 0, st,
+                         L[l], L[l + 1], b, omega);
+      QDS_LAUNCH_CHECK();
+      continue;
+    }
     hipLaunchKernelGGL((prolong_add_kernel<T, VEC>), dim3(qds_blocks(work, kBlock, 65536)), dim3(kBlock),
                        0, st, L[l], L[l + 1], b);
     QDS_LAUNCH_CHECK();

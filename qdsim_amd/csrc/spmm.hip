@@ -82,6 +82,29 @@ spmm_real(int64_t n, int b, int lpr, int rpw, const int32_t* __restrict__ indptr
   for (int v = 0; v < VEC; ++v) { acc_a[v] = T(0); acc_b[v] = T(0); }
   const int32_t kb = indptr[r], ke = indptr[r + 1];
   int32_t k = kb;
+  // groups of 8 nonzeros: all 8 index/value loads, then all 8 X loads, then the
+  // FMAs -- two dependent memory round trips per 8 nonzeros
+  for (; k + 8 <= ke; k += 8) {
+    int32_t cc[8];
+    T a[8], bb[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      cc[u] = indices[k + u];
+      a[u] = va[k + u];
+      bb[u] = FUSED ? vb[k + u] : T(0);
+    }
+    T x[8][VEC];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) load_vec<T, VEC>(X + (int64_t)cc[u] * ldx + c0, x[u]);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        acc_a[v] = fma(a[u], x[u][v], acc_a[v]);
+        if (FUSED) acc_b[v] = fma(bb[u], x[u][v], acc_b[v]);
+      }
+    }
+  }
   for (; k + 2 <= ke; k += 2) {
     const int32_t c_0 = indices[k], c_1 = indices[k + 1];
     const T a0 = va[k], a1 = va[k + 1];

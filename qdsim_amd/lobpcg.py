@@ -25,7 +25,10 @@ import math
 import time
 
 import numpy as np
+import scipy.linalg as sla
+import scipy.linalg.lapack as sla_lapack
 import torch
+from threadpoolctl import threadpool_limits
 
 from . import kernels
 from .device import device
@@ -91,6 +94,46 @@ def _svqb_rr(G, m, nb, drop_tol):
     return (torch.from_numpy(theta[:nb].copy()).to(dev), torch.from_numpy(C).to(dev), dropped)
 
 
+def _chol_rr(G, m, nb, drop_tol):
+    """Rayleigh-Ritz through a Cholesky factor of the scaled M-Gram when it is well
+    conditioned (reciprocal condition estimate above drop_tol): one Cholesky, two
+    triangular solves and one symmetric eigensolve, half the host cost of the two
+    eigensolves of SVQB (the eigensolves dominate at these sizes).  Returns None
+    when the basis is too close to dependent; the caller then uses SVQB.
+    Columns that vanished (locked residuals) are left out and get zero weight."""
+    dev = G.device
+    g = G.detach().cpu().numpy()
+    GH = 0.5 * (g[:, :m] + g[:, :m].T)
+    GM = 0.5 * (g[:, m:2 * m] + g[:, m:2 * m].T)
+    dg = np.diag(GM).copy()
+    idx = np.nonzero(dg > 1e-30 * dg.max())[0]
+    if len(idx) < nb:
+        return None
+    d = 1.0 / np.sqrt(dg[idx])
+    GMs = GM[np.ix_(idx, idx)] * d[:, None] * d[None, :]
+    GHs = GH[np.ix_(idx, idx)] * d[:, None] * d[None, :]
+    Lc, info = sla_lapack.dpotrf(GMs, lower=1, clean=1)
+    if info != 0:
+        return None
+    rcond, info = sla_lapack.dpocon(Lc, float(np.abs(GMs).sum(0).max()), uplo="L")
+    if info != 0 or rcond < drop_tol:
+        return None
+    T = sla.solve_triangular(Lc, GHs, lower=True, check_finite=False)
+    T = sla.solve_triangular(Lc, T.T, lower=True, check_finite=False)
+    T = 0.5 * (T + T.T)
+    theta, Y = np.linalg.eigh(T)   # full divide-and-conquer: faster than a subset solve at this size
+    theta, Y = theta[:nb], Y[:, :nb]
+    Cs = sla.solve_triangular(Lc, Y, lower=True, trans="T", check_finite=False) * d[:, None]
+    C = np.zeros((m, nb))
+    C[idx] = Cs
+    return torch.from_numpy(theta.copy()).to(dev), torch.from_numpy(C).to(dev), 0
+
+
+def _rayleigh_ritz(G, m, nb, drop_tol):
+    out = _chol_rr(G, m, nb, drop_tol)
+    return out if out is not None else _svqb_rr(G, m, nb, drop_tol)
+
+
 class LOBPCGResult:
     def __init__(self, lam, X, res, iters, history, converged, timings):
         self.eigenvalues = lam
@@ -104,6 +147,14 @@ class LOBPCGResult:
 
 def lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=None, seed=0,
            verbose=False, stall_iters=None, refresh=None):
+    # the host Rayleigh-Ritz works on matrices of at most 96 x 192: multithreaded
+    # BLAS only adds thread wake-ups there (measured 2-70x slower triangular solves)
+    with threadpool_limits(1, "blas"):
+        return _lobpcg(op, k, b, precond, X0, tol, maxiter, drop_tol, seed, verbose, stall_iters, refresh)
+
+
+def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=None, seed=0,
+            verbose=False, stall_iters=None, refresh=None):
     """Lowest k eigenpairs of (op.H, op.M) with block size b >= k.
 
     precond(R, W) writes W = T R (R, W are (n, b) blocks of op.dtype).
@@ -202,7 +253,7 @@ def lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=None
         MS = [MX[c], MW] + ([MP[c]] if have_p else [])
         m = len(S) * b
         G = kernels.gram_blocks(S, HS, MS, sym=True)
-        theta, C, dropped = _svqb_rr(G, m, b, drop_tol)
+        theta, C, dropped = _rayleigh_ritz(G, m, b, drop_tol)
         drops.append(dropped)
         CP = C.clone()
         CP[:b] = 0.0
@@ -222,9 +273,14 @@ def lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=None
     return LOBPCGResult(lam, X, res, it, history, converged, timings)
 
 
-def solve_mixed(op64, k, b, precond32=None, precond64=None, tol=1e-6, tol32=3e-5, maxiter=400,
-                seed=0, verbose=False, drop32=None, stall32=8, refresh32=None):
-    """FP32 LOBPCG to tol32 (or stagnation), then FP64 LOBPCG to tol."""
+def solve_mixed(op64, k, b, precond32=None, precond64=None, tol=1e-6, tol32=1e-3, maxiter=400,
+                seed=0, verbose=False, drop32=None, stall32=8, refresh32=1):
+    """FP32 LOBPCG to tol32 (or stagnation), then FP64 LOBPCG to tol.
+
+    The FP32 phase recomputes the images HX, MX by SpMM every refresh32 iterations
+    (1: every iteration): FP32 rounding of the updated images otherwise makes the
+    reported residual up to 30x smaller than the true one.  tol32 = 1e-3 sits just
+    above the FP32 floor of the quantum-dot problems (‖H‖/|lambda| ~ 1e3)."""
     op32 = op64.astype(torch.float32)
     t0 = time.perf_counter()
     r32 = lobpcg(op32, k, b, precond32, tol=tol32, maxiter=maxiter, seed=seed, verbose=verbose,
