@@ -100,9 +100,10 @@ int qds_interpolate_q(int32_t npe, int32_t nq, int64_t n_elem, const int32_t* co
  * neither) and elem (-1 outside) are optional.                                    */
 int qds_interpolate_points(int32_t order, int64_t cx, int64_t cy, double dx, double dy,
                            const int32_t* conn, const double* x, const double* y,
-                           const int32_t* lattice_node, const double* field, int64_t n_points,
-                           const double* px, const double* py, double* val, double* grad_x,
-                           double* grad_y, int32_t* elem, void* stream);
+                           const int32_t* lattice_node,
+                           const double* field /* [node][nfield] */, int32_t nfield, int64_t n_points,
+                           const double* px, const double* py, double* val /* [point][nfield] */,
+                           double* grad_x, double* grad_y, int32_t* elem, void* stream);
 /* Hellmann-Feynman weights out[e][q][s] = w_q |J_e|/2 psi_s(x_q)^2 (no reference
  * counterpart: adjoint of the eigenvalues w.r.t. the potential at x_q).           */
 int qds_state_density_q(int32_t npe, int32_t nq, int64_t n_elem, const int32_t* conn,
@@ -137,15 +138,17 @@ int64_t qds_gram_workspace(int64_t n, int32_t m, int32_t k1, int32_t k2);
 int qds_gram(int32_t dtype, int64_t n, int32_t m, const void* A, int64_t lda, int32_t k1,
              const void* B1, int64_t ldb1, int32_t k2, const void* B2, int64_t ldb2, int32_t sym,
              double* C, void* workspace, int64_t ws_bytes, void* stream);
+/* Y1 = S C1 (and Y2 = S C2 when C2 != NULL) for one row-major S (n x m, lds);
+ * C1, C2: m x k row-major FP64 (staged in the storage type).                     */
 int qds_block_update(int32_t dtype, int64_t n, int32_t m, int32_t k, const void* S, int64_t lds,
-                     const void* C1, void* Y1, int64_t ldy1, const void* C2, void* Y2, int64_t ldy2,
+                     const double* C1, void* Y1, int64_t ldy1, const double* C2, void* Y2, int64_t ldy2,
                      void* stream);
 /* Column-block forms: the logical n x m operand is up to four separately stored
  * row-major blocks side by side (block j: p[j], leading dimension ld[j], cols[j]
  * columns), so iterate blocks X, W, P can live in their own contiguous buffers.
  * Gram: C = A^T [B1 | B2] (B2 may be NULL).  Update: Y1 = S C1 (Y2 = S C2), C row-major
- * m x k, k <= 64, each block 16-byte aligned with widths and leading dimensions a
- * multiple of 4 (FP32) / 2 (FP64).                                                */
+ * m x k FP64 (staged in the storage type, MFMA in the storage type), k <= 64, each block
+ * 16-byte aligned with widths and leading dimensions a multiple of 4 (FP32) / 2 (FP64). */
 typedef struct {
   const void* p[4];
   int64_t ld[4];
@@ -156,7 +159,7 @@ int qds_gram_blocks(int32_t dtype, int64_t n, const qds_colblocks* A /* host */,
                     const qds_colblocks* B1 /* host */, const qds_colblocks* B2 /* host */,
                     int32_t sym, double* C, void* workspace, int64_t ws_bytes, void* stream);
 int qds_block_update_blocks(int32_t dtype, int64_t n, const qds_colblocks* S /* host */, int32_t k,
-                            const void* C1, void* Y1, int64_t ldy1, const void* C2, void* Y2,
+                            const double* C1, void* Y1, int64_t ldy1, const double* C2, void* Y2,
                             int64_t ldy2, int32_t c2_first /* rows of C2 before it are zero */,
                             void* stream);
 int qds_block_residual(int32_t dtype, int64_t n, int32_t b, const void* HX, int64_t ldh,

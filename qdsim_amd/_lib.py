@@ -52,8 +52,8 @@ _SIGNATURES = {
     "qds_interpolate_q": ([_i32, _i32, _i64, _p, _p, _p, _p, _p], _c.c_int),
     "qds_state_density_q": ([_i32, _i32, _i64, _p, _p, _p, _p, _p, _i64, _i32, _p, _p], _c.c_int),
     "qds_density": ([_i64, _i32, _p, _i64, _p, _p, _p], _c.c_int),
-    "qds_interpolate_points": ([_i32, _i64, _i64, _d, _d, _p, _p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p,
-                                _p], _c.c_int),
+    "qds_interpolate_points": ([_i32, _i64, _i64, _d, _d, _p, _p, _p, _p, _p, _i32, _i64, _p, _p, _p, _p, _p,
+                                _p, _p], _c.c_int),
     "qds_cg_workspace": ([_i64], _i64),
     "qds_cg": ([_i64, _i64, _p, _p, _p, _p, _p, _d, _i32, _i32, _p, _i64, _c.POINTER(_i32),
                 _c.POINTER(_d), _p], _c.c_int),

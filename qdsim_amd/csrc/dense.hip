@@ -292,18 +292,22 @@ constexpr int kUpdLds = 64 * 1024;
 // fits (resident), otherwise the rows of the current column block, staged with
 // barriers.  OT = ceil(k / 16) output tiles exactly.  Column blocks that end before
 // c2_first contribute nothing to Y2 (those rows of C2 are zero) and skip its MFMAs.
+// C arrives in FP64 and is staged in the storage type; the MFMA runs in the storage
+// type.  (FP64 accumulation for FP32 storage was measured: it lowered the FP32
+// residual floor of the eigensolver from ~1e-3 to ~3e-4 but made the update
+// MFMA-bound at twice the time, a net loss.)
 template <typename T, int NOUT, int OT, int MAXJ>
 __global__ void __launch_bounds__(kBlock)
-block_update_kernel(int64_t n, int m, int k, ColBlocks S, const T* __restrict__ C1,
-                    const T* __restrict__ C2, T* __restrict__ Y1, int64_t ldy1, T* __restrict__ Y2,
+block_update_kernel(int64_t n, int m, int k, ColBlocks S, const double* __restrict__ C1,
+                    const double* __restrict__ C2, T* __restrict__ Y1, int64_t ldy1, T* __restrict__ Y2,
                     int64_t ldy2, int resident, int crows, int c2_first) {
   using M = MT<T>;
-  constexpr int VEC = M::VEC;
+  constexpr int VEC = M::VEC;                  // elements of T per 16-byte load
   constexpr int KB = 4 * VEC;  // contraction indices covered by one vector load per lane
   constexpr int KP = OT * 16;
-  using Vt = typename M::vec_t;
+  using Vt = typename MT<T>::vec_t;
   extern __shared__ unsigned char upd_smem[];
-  T* sC = reinterpret_cast<T*>(upd_smem);   // [NOUT][crows][KP], zero padded rows and columns
+  T* sC = reinterpret_cast<T*>(upd_smem);   // [NOUT][crows][KP], zero padded
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int q = lane >> 4;      // 0..3
@@ -314,8 +318,8 @@ block_update_kernel(int64_t n, int m, int k, ColBlocks S, const T* __restrict__ 
       const int o = t / (crows * KP);
       const int rem = t - o * crows * KP;
       const int i = rem / KP, j = rem - i * KP;
-      const T* C = o == 0 ? C1 : C2;
-      sC[t] = (i < rows && j < k) ? C[(r0 + i) * k + j] : T(0);
+      const double* C = o == 0 ? C1 : C2;
+      sC[t] = (i < rows && j < k) ? (T)C[(r0 + i) * k + j] : T(0);
     }
   };
   if (resident) {
@@ -403,8 +407,8 @@ block_update_kernel(int64_t n, int m, int k, ColBlocks S, const T* __restrict__ 
       for (int e = 0; e < 4; ++e) {
         const int64_t orow = r0 + M::drow(q, e);
         if (orow < n) {
-          Y1[orow * ldy1 + col] = acc[0][t][e];
-          if (NOUT == 2) Y2[orow * ldy2 + col] = acc[NOUT - 1][t][e];
+          Y1[orow * ldy1 + col] = (T)acc[0][t][e];
+          if (NOUT == 2) Y2[orow * ldy2 + col] = (T)acc[NOUT - 1][t][e];
         }
       }
     }
@@ -412,8 +416,8 @@ block_update_kernel(int64_t n, int m, int k, ColBlocks S, const T* __restrict__ 
 }
 
 template <typename T, int NOUT>
-int launch_update(int64_t n, int m, int k, const ColBlocks& S, int c2_first, const T* C1, const T* C2, T* Y1,
-                  int64_t ldy1, T* Y2, int64_t ldy2, hipStream_t st) {
+int launch_update(int64_t n, int m, int k, const ColBlocks& S, int c2_first, const double* C1, const double* C2,
+                  T* Y1, int64_t ldy1, T* Y2, int64_t ldy2, hipStream_t st) {
   constexpr int KB = 4 * MT<T>::VEC;
   const int ot = (k + 15) / 16;
   const int kp = ot * 16;
@@ -722,11 +726,11 @@ QDS_API int qds_gram(int32_t dtype, int64_t n, int32_t m, const void* A, int64_t
 }
 
 // Y1 = S C1 (n x k), and Y2 = S C2 when C2 != NULL; S given as column blocks.
-// C1, C2: m x k row-major in the storage type.  k <= 64.  Every block's width and
+// C1, C2: m x k row-major FP64 (rounded to the storage type on the device).  k <= 64.  Every block's width and
 // leading dimension must be multiples of 4 (F32) or 2 (F64), blocks 16-byte aligned.
 // c2_first: the caller guarantees rows [0, c2_first) of C2 are zero (0: none known).
 QDS_API int qds_block_update_blocks(int32_t dtype, int64_t n, const qds_colblocks* S, int32_t k,
-                                    const void* C1, void* Y1, int64_t ldy1, const void* C2, void* Y2,
+                                    const double* C1, void* Y1, int64_t ldy1, const double* C2, void* Y2,
                                     int64_t ldy2, int32_t c2_first, void* stream) {
   QDS_CHECK_ARG(dtype == qds::F32 || dtype == qds::F64, "qds_block_update: dtype must be F32 or F64");
   const int es = dtype == qds::F32 ? 4 : 8;
@@ -741,13 +745,12 @@ QDS_API int qds_block_update_blocks(int32_t dtype, int64_t n, const qds_colblock
   if (n == 0) return QDS_OK;
   hipStream_t st = qds_stream(stream);
   if (!C2) c2_first = 0;
+  const double *c1 = C1, *c2 = C2;
   if (dtype == qds::F32) {
-    const float *c1 = (const float*)C1, *c2 = (const float*)C2;
     float *y1 = (float*)Y1, *y2 = (float*)Y2;
     return C2 ? launch_update<float, 2>(n, m, k, dS, c2_first, c1, c2, y1, ldy1, y2, ldy2, st)
               : launch_update<float, 1>(n, m, k, dS, 0, c1, nullptr, y1, ldy1, nullptr, ldy2, st);
   }
-  const double *c1 = (const double*)C1, *c2 = (const double*)C2;
   double *y1 = (double*)Y1, *y2 = (double*)Y2;
   return C2 ? launch_update<double, 2>(n, m, k, dS, c2_first, c1, c2, y1, ldy1, y2, ldy2, st)
             : launch_update<double, 1>(n, m, k, dS, 0, c1, nullptr, y1, ldy1, nullptr, ldy2, st);
@@ -755,7 +758,7 @@ QDS_API int qds_block_update_blocks(int32_t dtype, int64_t n, const qds_colblock
 
 // Single-block form: S n x m with leading dimension lds.
 QDS_API int qds_block_update(int32_t dtype, int64_t n, int32_t m, int32_t k, const void* S, int64_t lds,
-                             const void* C1, void* Y1, int64_t ldy1, const void* C2, void* Y2,
+                             const double* C1, void* Y1, int64_t ldy1, const double* C2, void* Y2,
                              int64_t ldy2, void* stream) {
   QDS_CHECK_ARG(S && m >= 1, "qds_block_update: bad S");
   qds_colblocks s = single(S, lds, m);

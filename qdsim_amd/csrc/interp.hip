@@ -28,7 +28,8 @@ struct Mesh {
   const double* x;
   const double* y;
   const int32_t* lnode;  // [(p cy + 1) (p cx + 1)] lowest node id at each lattice position
-  const double* f;       // nodal field
+  const double* f;       // nodal fields, row-major [node][nf]
+  int nf;
 };
 
 // barycentric coordinates with the reference's formula and inside test
@@ -44,6 +45,17 @@ __device__ __forceinline__ bool bary(double px, double py, double x0, double y0,
 
 __device__ int64_t locate(const Mesh& M, double px, double py, double l[3]) {
   const int64_t ci = (int64_t)floor(px / M.dx), cj = (int64_t)floor(py / M.dy);
+  // fast path: a point well inside one of the two triangles of its own cell lies in
+  // no other element, so the ordered scan below would return the same element
+  if (ci >= 0 && ci < M.cx && cj >= 0 && cj < M.cy) {
+    for (int t = 0; t < 2; ++t) {
+      const int64_t e = 2 * (cj * M.cx + ci) + t;
+      const int32_t* c = M.conn + e * M.npe;
+      if (bary(px, py, M.x[c[0]], M.y[c[0]], M.x[c[1]], M.y[c[1]], M.x[c[2]], M.y[c[2]], l) &&
+          l[0] > 1e-8 && l[1] > 1e-8 && l[2] > 1e-8)
+        return e;
+    }
+  }
   for (int64_t j = cj - 1; j <= cj + 1; ++j) {
     if (j < 0 || j >= M.cy) continue;
     for (int64_t i = ci - 1; i <= ci + 1; ++i) {
@@ -120,10 +132,12 @@ __device__ int32_t nearest_node(const Mesh& M, double px, double py) {
   return best;
 }
 
+// val, gx, gy: row-major [point][nf]
 __global__ void __launch_bounds__(kBlock)
 interp_points_kernel(Mesh M, int64_t n, const double* __restrict__ px, const double* __restrict__ py,
                      double* __restrict__ val, double* __restrict__ gx, double* __restrict__ gy,
                      int32_t* __restrict__ elem) {
+  const int nf = M.nf;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
     const double X = px[t], Y = py[t];
     double l[3];
@@ -133,29 +147,32 @@ interp_points_kernel(Mesh M, int64_t n, const double* __restrict__ px, const dou
       const int32_t* c = M.conn + e * M.npe;
       double N[10], dN[10][3];
       shape(M.order, l, N, dN);
-      double v = 0.0;
-      for (int a = 0; a < M.npe; ++a) v += M.f[c[a]] * N[a];
-      val[t] = v;
+      double ax[10], ay[10];
       if (gx) {
         const double x0 = M.x[c[0]], y0 = M.y[c[0]], x1 = M.x[c[1]], y1 = M.y[c[1]];
         const double x2 = M.x[c[2]], y2 = M.y[c[2]];
         const double det = (x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0);
         const double lx[3] = {(y1 - y2) / det, (y2 - y0) / det, (y0 - y1) / det};
         const double ly[3] = {(x2 - x1) / det, (x0 - x2) / det, (x1 - x0) / det};
-        double sx = 0.0, sy = 0.0;
         for (int a = 0; a < M.npe; ++a) {
-          const double ax = dN[a][0] * lx[0] + dN[a][1] * lx[1] + dN[a][2] * lx[2];
-          const double ay = dN[a][0] * ly[0] + dN[a][1] * ly[1] + dN[a][2] * ly[2];
-          sx += M.f[c[a]] * ax;
-          sy += M.f[c[a]] * ay;
+          ax[a] = dN[a][0] * lx[0] + dN[a][1] * lx[1] + dN[a][2] * lx[2];
+          ay[a] = dN[a][0] * ly[0] + dN[a][1] * ly[1] + dN[a][2] * ly[2];
         }
-        gx[t] = sx;
-        gy[t] = sy;
+      }
+      for (int f = 0; f < nf; ++f) {
+        double v = 0.0, sx = 0.0, sy = 0.0;
+        for (int a = 0; a < M.npe; ++a) {
+          const double fa = M.f[(int64_t)c[a] * nf + f];
+          v += fa * N[a];
+          if (gx) { sx += fa * ax[a]; sy += fa * ay[a]; }
+        }
+        val[t * nf + f] = v;
+        if (gx) { gx[t * nf + f] = sx; gy[t * nf + f] = sy; }
       }
       continue;
     }
     const int32_t nd = nearest_node(M, X, Y);
-    val[t] = M.f[nd];
+    for (int f = 0; f < nf; ++f) val[t * nf + f] = M.f[(int64_t)nd * nf + f];
     if (!gx) continue;
     // nodes sharing an element with nd, first appearance in (element, local) order
     int32_t seen[64];
@@ -175,26 +192,28 @@ interp_points_kernel(Mesh M, int64_t n, const double* __restrict__ px, const dou
           for (int a = 0; a < M.npe; ++a) {
             if (c[a] == nd) continue;
             bool dup = false;
-            for (int s = 0; s < ns; ++s) dup = dup || seen[s] == c[a];
+            for (int q = 0; q < ns; ++q) dup = dup || seen[q] == c[a];
             if (!dup && ns < 64) seen[ns++] = c[a];
           }
         }
       }
     }
-    double sx = 0.0, sy = 0.0, tw = 0.0;
-    for (int s = 0; s < ns; ++s) {
-      const double ddx = M.x[seen[s]] - M.x[nd], ddy = M.y[seen[s]] - M.y[nd];
-      const double dist = sqrt(ddx * ddx + ddy * ddy);
-      if (dist > 1e-10) {
-        const double w = 1.0 / dist;
-        const double df = M.f[seen[s]] - M.f[nd];
-        sx += w * (df * ddx / dist);
-        sy += w * (df * ddy / dist);
-        tw += w;
+    for (int f = 0; f < nf; ++f) {
+      double sx = 0.0, sy = 0.0, tw = 0.0;
+      for (int q = 0; q < ns; ++q) {
+        const double ddx = M.x[seen[q]] - M.x[nd], ddy = M.y[seen[q]] - M.y[nd];
+        const double dist = sqrt(ddx * ddx + ddy * ddy);
+        if (dist > 1e-10) {
+          const double w = 1.0 / dist;
+          const double df = M.f[(int64_t)seen[q] * nf + f] - M.f[(int64_t)nd * nf + f];
+          sx += w * (df * ddx / dist);
+          sy += w * (df * ddy / dist);
+          tw += w;
+        }
       }
+      gx[t * nf + f] = tw > 0.0 ? sx / tw : 0.0;
+      gy[t * nf + f] = tw > 0.0 ? sy / tw : 0.0;
     }
-    gx[t] = tw > 0.0 ? sx / tw : 0.0;
-    gy[t] = tw > 0.0 ? sy / tw : 0.0;
   }
 }
 
@@ -202,16 +221,16 @@ interp_points_kernel(Mesh M, int64_t n, const double* __restrict__ px, const dou
 
 QDS_API int qds_interpolate_points(int32_t order, int64_t cx, int64_t cy, double dx, double dy,
                                    const int32_t* conn, const double* x, const double* y,
-                                   const int32_t* lattice_node, const double* field, int64_t n_points,
-                                   const double* px, const double* py, double* val, double* grad_x,
-                                   double* grad_y, int32_t* elem, void* stream) {
-  QDS_CHECK_ARG(order >= 1 && order <= 3 && cx >= 1 && cy >= 1 && dx > 0 && dy > 0,
+                                   const int32_t* lattice_node, const double* field, int32_t nfield,
+                                   int64_t n_points, const double* px, const double* py, double* val,
+                                   double* grad_x, double* grad_y, int32_t* elem, void* stream) {
+  QDS_CHECK_ARG(order >= 1 && order <= 3 && cx >= 1 && cy >= 1 && dx > 0 && dy > 0 && nfield >= 1,
                 "qds_interpolate_points: bad mesh description");
   QDS_CHECK_ARG(conn && x && y && lattice_node && field && (n_points == 0 || (px && py && val)),
                 "qds_interpolate_points: null pointer");
   QDS_CHECK_ARG((grad_x == nullptr) == (grad_y == nullptr), "qds_interpolate_points: give both gradients or none");
   if (n_points == 0) return QDS_OK;
-  Mesh M{order, order == 1 ? 3 : (order == 2 ? 6 : 10), cx, cy, dx, dy, conn, x, y, lattice_node, field};
+  Mesh M{order, order == 1 ? 3 : (order == 2 ? 6 : 10), cx, cy, dx, dy, conn, x, y, lattice_node, field, nfield};
   hipLaunchKernelGGL(interp_points_kernel, dim3(qds_blocks(n_points, kBlock, 65536)), dim3(kBlock), 0,
                      qds_stream(stream), M, n_points, px, py, val, grad_x, grad_y, elem);
   QDS_LAUNCH_CHECK();

@@ -52,22 +52,27 @@ class FEInterpolator:
 
     # -- batched device evaluation -------------------------------------------------
     def interpolate_points(self, px, py, field, gradient=False, return_element=False):
-        """Values (and gradients) of the nodal field at the points (px, py).
+        """Values (and gradients) of the nodal field(s) at the points (px, py).
 
-        Inputs may be numpy arrays or device tensors; outputs are FP64 device tensors."""
+        field: (num_nodes,) or (num_nodes, nf).  Inputs may be numpy arrays or device
+        tensors; outputs are FP64 device tensors of shape (n,) or (n, nf)."""
         dev = self._x.device
         px = torch.as_tensor(px, dtype=torch.float64).to(dev).reshape(-1).contiguous()
         py = torch.as_tensor(py, dtype=torch.float64).to(dev).reshape(-1).contiguous()
-        f = torch.as_tensor(field, dtype=torch.float64).to(dev).reshape(-1).contiguous()
-        if f.numel() < self.mesh.num_nodes:
+        f = torch.as_tensor(field, dtype=torch.float64).to(dev)
+        multi = f.dim() == 2
+        f = f.reshape(f.shape[0], -1).contiguous()
+        nf = f.shape[1]
+        if f.shape[0] < self.mesh.num_nodes:
             raise ValueError("field must have one value per mesh node")
         n = px.numel()
-        val = torch.empty(n, dtype=torch.float64, device=dev)
-        gx = torch.empty(n, dtype=torch.float64, device=dev) if gradient else None
-        gy = torch.empty(n, dtype=torch.float64, device=dev) if gradient else None
+        shape = (n, nf) if multi else (n,)
+        val = torch.empty(shape, dtype=torch.float64, device=dev)
+        gx = torch.empty(shape, dtype=torch.float64, device=dev) if gradient else None
+        gy = torch.empty(shape, dtype=torch.float64, device=dev) if gradient else None
         el = torch.empty(n, dtype=torch.int32, device=dev) if return_element else None
         _lib.check(_lib.lib.qds_interpolate_points(self.order, self.cx, self.cy, self.dx, self.dy, ptr(self._conn),
-                                                   ptr(self._x), ptr(self._y), ptr(self._lnode), ptr(f), n,
+                                                   ptr(self._x), ptr(self._y), ptr(self._lnode), ptr(f), nf, n,
                                                    ptr(px), ptr(py), ptr(val), ptr(gx), ptr(gy), ptr(el),
                                                    stream_ptr()), "qds_interpolate_points")
         out = (val, gx, gy) if gradient else (val,)

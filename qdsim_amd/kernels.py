@@ -151,11 +151,11 @@ def gram(A, B1, B2=None, sym=False, out=None):
 
 
 def block_update(S, C1, Y1, C2=None, Y2=None):
-    """Y1 = S C1 (and Y2 = S C2); C in the storage dtype of S."""
+    """Y1 = S C1 (and Y2 = S C2); C passed in FP64."""
     n, m = S.shape
     k = C1.shape[1]
-    C1 = C1.to(S.dtype).contiguous()
-    C2 = C2.to(S.dtype).contiguous() if C2 is not None else None
+    C1 = C1.to(torch.float64).contiguous()
+    C2 = C2.to(torch.float64).contiguous() if C2 is not None else None
     check(lib.qds_block_update(code(S), n, m, k, ptr(S), _ld(S), ptr(C1), ptr(Y1), _ld(Y1),
                                ptr(C2), ptr(Y2), _ld(Y2) if Y2 is not None else 1, stream_ptr()),
           "qds_block_update")
@@ -197,9 +197,9 @@ def block_update_blocks(S, C1, Y1, C2=None, Y2=None, c2_first=0):
     c2_first: rows of C2 before it are zero (the kernel skips them)."""
     n = S[0].shape[0]
     k = C1.shape[1]
-    dt = S[0].dtype
-    C1 = C1.to(dt).contiguous()
-    C2 = C2.to(dt).contiguous() if C2 is not None else None
+    # coefficients in FP64 (the kernel stages them in the storage type)
+    C1 = C1.to(torch.float64).contiguous()
+    C2 = C2.to(torch.float64).contiguous() if C2 is not None else None
     cs = _colblocks(S)
     check(lib.qds_block_update_blocks(code(S[0]), n, ctypes.byref(cs), k, ptr(C1), ptr(Y1), _ld(Y1),
                                       ptr(C2), ptr(Y2), _ld(Y2) if Y2 is not None else 1,

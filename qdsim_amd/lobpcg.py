@@ -257,12 +257,20 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
         drops.append(dropped)
         CP = C.clone()
         CP[:b] = 0.0
-        for src, img, pimg in ((S, X, P), (HS, HX, HP), (MS, MX, MP)):
-            kernels.block_update_blocks(src, C, img[1 - c], CP, pimg[1 - c], c2_first=b)
+        fresh = bool(refresh) and it % refresh == 0
+        kernels.block_update_blocks(S, C, X[1 - c], CP, P[1 - c], c2_first=b)
+        if fresh:
+            # the images of X are recomputed below; the images of P only need the W
+            # and P blocks (the X rows of the P coefficients are zero)
+            for src, pimg in ((HS, HP), (MS, MP)):
+                kernels.block_update_blocks(src[1:], CP[b:], pimg[1 - c])
+        else:
+            for src, img, pimg in ((HS, HX, HP), (MS, MX, MP)):
+                kernels.block_update_blocks(src, C, img[1 - c], CP, pimg[1 - c], c2_first=b)
         c = 1 - c
         lam = theta
         have_p = dropped == 0
-        if refresh and it % refresh == 0:
+        if fresh:
             # recompute the images of X: the updated ones carry the rounding of every
             # update since the last product (FP32 storage)
             op.apply(X[c], HX[c], MX[c])
@@ -274,8 +282,9 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
 
 
 def solve_mixed(op64, k, b, precond32=None, precond64=None, tol=1e-6, tol32=1e-3, maxiter=400,
-                seed=0, verbose=False, drop32=None, stall32=8, refresh32=1):
-    """FP32 LOBPCG to tol32 (or stagnation), then FP64 LOBPCG to tol.
+                seed=0, verbose=False, drop32=None, stall32=8, refresh32=1, X0=None):
+    """FP32 LOBPCG to tol32 (or stagnation), then FP64 LOBPCG to tol.  X0: optional
+    initial block for the FP32 phase (default: seeded random vectors).
 
     The FP32 phase recomputes the images HX, MX by SpMM every refresh32 iterations
     (1: every iteration): FP32 rounding of the updated images otherwise makes the
@@ -283,7 +292,7 @@ def solve_mixed(op64, k, b, precond32=None, precond64=None, tol=1e-6, tol32=1e-3
     above the FP32 floor of the quantum-dot problems (‖H‖/|lambda| ~ 1e3)."""
     op32 = op64.astype(torch.float32)
     t0 = time.perf_counter()
-    r32 = lobpcg(op32, k, b, precond32, tol=tol32, maxiter=maxiter, seed=seed, verbose=verbose,
+    r32 = lobpcg(op32, k, b, precond32, X0=X0, tol=tol32, maxiter=maxiter, seed=seed, verbose=verbose,
                  stall_iters=stall32, drop_tol=drop32, refresh=refresh32)
     t1 = time.perf_counter()
     X0 = r32.X.to(torch.float64)

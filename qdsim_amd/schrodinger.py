@@ -24,6 +24,7 @@ import torch
 
 from . import kernels
 from .fem import FEMSpace
+from .mesh import SimpleMesh
 from .lattice import Lattice, LatticeMG
 from .lobpcg import CSROperator, lobpcg, solve_mixed
 from .units import EV_TO_J, HBAR, NM
@@ -59,6 +60,9 @@ class SchrodingerProblem:
         self.c_q = self.v_q = self.w_q = None
         self.H = self.M = self.W = None
         self.mg_options = {}          # extra LatticeMG arguments (cycle shape)
+        self._m_func = self._v_func = None
+        self._coarse = None           # coarse problem of the nested start (built on first use)
+        self.nested = True
         self.setup_time = time.perf_counter() - t0
         if m_star_func is not None:
             self.set_mass(m_star_func)
@@ -72,15 +76,21 @@ class SchrodingerProblem:
     def set_mass(self, m_star_func):
         m = self.space.sample(m_star_func, self._where())
         self.c_q = (KIN_SCALE / m).contiguous()
+        self._m_func = m_star_func
+        self._coarse = None
 
     def set_potential(self, potential_func, extra_ev=None):
         v = self.space.sample(potential_func, self._where(), 1.0 / EV_TO_J)
         if extra_ev is not None:
             v = v + extra_ev
         self.v_q = v.contiguous()
+        self._v_func = potential_func if extra_ev is None else None
+        self._coarse = None
 
     def set_potential_ev(self, v_q):
         self.v_q = v_q.contiguous()
+        self._v_func = None
+        self._coarse = None
 
     def set_cap_ev(self, w_q):
         self.w_q = None if w_q is None else w_q.contiguous()
@@ -113,9 +123,48 @@ class SchrodingerProblem:
         opts.update(self.mg_options)
         return LatticeMG(self.lat, c_e, v_e, sigma, b, dtype=dtype, **opts)
 
+    # -- nested start -----------------------------------------------------------
+    def _nested_start(self, k, b, factor=4, tol=1e-2):
+        """Initial block from the same problem on a mesh with factor-times fewer cells
+        per direction: its FP32 block eigenvectors (tol relative residual), evaluated
+        at the free DOFs of this mesh with the FE interpolation kernel.  Starting the
+        fine solve there instead of from random vectors saves the iterations that
+        resolve the smooth part of the eigenvectors (nested iteration).  Needs the
+        mass and potential as functions and a cell count divisible by factor with
+        at least 32 coarse cells; returns None otherwise.  The coarse problem (mesh,
+        pattern, quadrature data) is built once and reused; every solve re-assembles
+        and re-solves it."""
+        cx, cy = self.mesh.nx - 1, self.mesh.ny - 1
+        if (not self.nested or self._m_func is None or self._v_func is None or cx % factor or cy % factor
+                or min(cx, cy) // factor < 32):
+            return None
+        if self._coarse is None:
+            self._coarse = {}
+        if factor not in self._coarse:
+            from .interpolator import FEInterpolator
+            cm = SimpleMesh(cx // factor + 1, cy // factor + 1, self.mesh.Lx, self.mesh.Ly,
+                            self.mesh.element_order)
+            cp = SchrodingerProblem(cm, self._m_func, self._v_func, sampling=self.sampling)
+            cp.nested = False
+            L = self.lat
+            # free DOF coordinates of this mesh in lattice order (SI, the mesh's units)
+            iy, ix = np.divmod(np.arange(L.n_free), L.nx_i)
+            px = torch.from_numpy((ix + 1) * (self.mesh.Lx / (L.NX - 1))).to(self.space.dev)
+            py = torch.from_numpy((iy + 1) * (self.mesh.Ly / (L.NY - 1))).to(self.space.dev)
+            self._coarse[factor] = (cp, FEInterpolator(cm), px, py)
+        cp, interp, px, py = self._coarse[factor]
+        cp.assemble()
+        op32 = cp.operator(torch.float32)
+        pre = cp.preconditioner(b, torch.float32)
+        r = lobpcg(op32, k, b, pre, tol=tol, maxiter=200, stall_iters=8, refresh=1)
+        full = torch.zeros((cp.space.n, b), dtype=torch.float64, device=r.X.device)
+        full[cp.old_of_new] = r.X.double()
+        self._nested_iters = r.iterations
+        return interp.interpolate_points(px, py, full)
+
     # -- solve ------------------------------------------------------------------
-    def solve(self, k, b=None, mode="mixed", tol=1e-6, tol32=1e-3, maxiter=400, seed=0, verbose=False,
-              **mixed_opts):
+    def solve(self, k, b=None, mode="mixed", tol=1e-6, tol32=3e-3, maxiter=400, seed=0, verbose=False,
+              nested_opts=None, **mixed_opts):
         """Lowest k eigenpairs.  Returns (eigenvalues eV (np), full-numbering
         eigenvectors (n, k) float64 device tensor, M_nm-normalised, info)."""
         if self.H is None:
@@ -137,12 +186,19 @@ class SchrodingerProblem:
                 _pre(_buf["r"], _buf["w"])
                 W.copy_(_buf["w"])
 
+            t_n = time.perf_counter()
+            self._nested_iters = 0
+            X0 = self._nested_start(k, b, **(nested_opts or {}))
+            torch.cuda.synchronize()
+            t_nested = time.perf_counter() - t_n
             res = solve_mixed(op64, k, b, pre32, pre64, tol=tol, tol32=tol32, maxiter=maxiter, seed=seed,
-                              verbose=verbose, **mixed_opts)
+                              verbose=verbose, X0=X0, **mixed_opts)
         elif mode == "fp64":
             pre = self.preconditioner(b, torch.float64, cycle=dict(nu_pre=2, nu_post=2, coarse_sweeps=40))
             torch.cuda.synchronize()
             t_prep = time.perf_counter() - t0
+            t_nested = 0.0
+            self._nested_iters = 0
             res = lobpcg(op64, k, b, pre, tol=tol, maxiter=maxiter, seed=seed, verbose=verbose)
         else:
             raise ValueError(f"unknown mode {mode}")
@@ -153,6 +209,8 @@ class SchrodingerProblem:
         torch.cuda.synchronize()
         info = {"iterations": res.iterations, "converged": res.converged,
                 "residuals": res.residuals[:k].cpu().numpy() if res.residuals is not None else None,
-                "timings": dict(res.timings, prep_s=t_prep), "history": res.history, "block": b,
+                "timings": dict(res.timings, prep_s=t_prep, nested_s=t_nested,
+                                nested_iters=self._nested_iters),
+                "history": res.history, "block": b,
                 "solve_s": time.perf_counter() - t0}
         return lam.cpu().numpy(), full, info

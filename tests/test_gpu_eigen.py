@@ -63,3 +63,28 @@ def test_dot_p2_p3(order, cells, mode):
     prob = _problem(cells + 1, cells + 1, L, L, order, mf, vf)
     ref = eig_ref.solve_closed(cells + 1, cells + 1, L, L, order, mf, vf, 8)
     _check(prob, ref, 8, mode)
+
+
+def test_nested_start_matches_random_start():
+    """The coarse-mesh initial block (nested iteration) converges to the same pairs as
+    the seeded random start; P2 256 x 256 cells (263,169 DOF), coarse mesh 64 x 64."""
+    from qdsim_amd.mesh import SimpleMesh
+    from qdsim_amd.schrodinger import SchrodingerProblem
+    L = 100e-9
+
+    def m_star(x, y):
+        return 0.067 * 9.1093837015e-31 + 0 * x
+
+    def pot(x, y):
+        r2 = (x - L / 2) ** 2 + (y - L / 2) ** 2
+        e = np.exp(-r2 / (2 * (8e-9) ** 2)) if isinstance(r2, np.ndarray) else torch.exp(-r2 / (2 * (8e-9) ** 2))
+        return -0.25 * 1.602176634e-19 * e
+    mesh = SimpleMesh(257, 257, L, L, 2)
+    prob = SchrodingerProblem(mesh, m_star, pot)
+    lam_n, X_n, info_n = prob.solve(8)
+    assert prob._coarse is not None, "nested start not used"
+    prob.nested = False
+    lam_r, X_r, info_r = prob.solve(8)
+    print("nested", info_n["timings"], "random", info_r["timings"])
+    assert info_n["converged"] and info_r["converged"]
+    assert np.allclose(lam_n, lam_r, rtol=1e-10, atol=0)
