@@ -66,7 +66,7 @@ def _svqb_rr(G, m, nb, drop_tol):
     eigenproblems run in host LAPACK (dsyevd), which measured 3.5x faster than
     the device eigh for these sizes (0.72 vs 2.5 ms including the transfers) and is
     robust for the nearly rank-deficient pairs met close to convergence.
-    Returns (theta[:nb], C (m x nb) as device FP64 tensors, number of dropped
+    Returns (theta[:nb], C (m x nb) as host FP64 arrays, number of dropped
     non-zero directions)."""
     dev = G.device
     g = G.detach().cpu().numpy()
@@ -91,7 +91,7 @@ def _svqb_rr(G, m, nb, drop_tol):
     C = d[:, None] * (Qk @ Y[:, :nb])
     # directions dropped beyond the zero columns: the basis was near-dependent
     dropped = (m - n_zero) - n_ok
-    return (torch.from_numpy(theta[:nb].copy()).to(dev), torch.from_numpy(C).to(dev), dropped)
+    return theta[:nb], C, dropped
 
 
 def _chol_rr(G, m, nb, drop_tol):
@@ -126,12 +126,34 @@ def _chol_rr(G, m, nb, drop_tol):
     Cs = sla.solve_triangular(Lc, Y, lower=True, trans="T", check_finite=False) * d[:, None]
     C = np.zeros((m, nb))
     C[idx] = Cs
-    return torch.from_numpy(theta.copy()).to(dev), torch.from_numpy(C).to(dev), 0
+    return theta, C, 0
 
 
 def _rayleigh_ritz(G, m, nb, drop_tol):
     out = _chol_rr(G, m, nb, drop_tol)
     return out if out is not None else _svqb_rr(G, m, nb, drop_tol)
+
+
+class _Staging:
+    """Page-locked host buffer and its device twin for the small per-iteration
+    transfers (Rayleigh-Ritz coefficients and eigenvalues, locking mask): one
+    asynchronous copy each, ordered on the stream before the kernels that read it."""
+
+    def __init__(self, size, dev):
+        self.host = torch.empty(size, dtype=torch.float64, pin_memory=dev.type == "cuda")
+        self.dev = torch.empty(size, dtype=torch.float64, device=dev)
+
+    def put(self, arrays):
+        """Copy host arrays back to back; returns the device views in order."""
+        h = self.host.numpy()
+        off, views = 0, []
+        for a in arrays:
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            h[off:off + a.size] = a.ravel()
+            views.append((off, a.shape))
+            off += a.size
+        self.dev[:off].copy_(self.host[:off], non_blocking=True)
+        return [self.dev[o:o + int(np.prod(sh))].view(*sh) for o, sh in views]
 
 
 class LOBPCGResult:
@@ -201,11 +223,14 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
     # initial Rayleigh-Ritz on X alone
     op.apply(X[c], HX[c], MX[c])
     G = kernels.gram_blocks([X[c]], [HX[c]], [MX[c]], sym=True)
+    stage_rr = _Staging(2 * 3 * b * b + b, dev)
+    stage_cs = _Staging(b, dev)
     theta, C, _ = _svqb_rr(G, b, b, drop_tol)
+    C_d, lam = stage_rr.put([C, theta])
     for src in (X, HX, MX):
-        kernels.block_update_blocks([src[c]], C, src[1 - c])
+        kernels.block_update_blocks([src[c]], C_d, src[1 - c])
     c = 1 - c
-    lam = theta
+    lam_h = theta
     have_p = False
     history = []
     drops = []
@@ -215,17 +240,21 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
     best_it = min_it = 0
     res = None
     t_iter = time.perf_counter()
-    colscale = torch.ones(b, dtype=torch.float64, device=dev)
+    (colscale,) = stage_cs.put([np.ones(b)])
     for it in range(1, maxiter + 1):
         rn, mn = kernels.residual_norms(HX[c], MX[c], lam, R, colscale)
+        norms = torch.cat([rn, mn]).cpu().numpy()
+        rn_h, mn_h = norms[:b], norms[b:]
         # residual relative to the energy scale of the wanted window, so states with
         # eigenvalues close to zero are held to the same absolute accuracy as the rest
-        scale = lam[:k].abs().max().clamp_min(1e-300)
-        res = (rn.clamp_min(0).sqrt() / (torch.maximum(lam.abs(), scale) * mn.clamp_min(1e-300).sqrt()))
-        worst = float(res[:k].max().item())
+        scale = max(np.abs(lam_h[:k]).max(), 1e-300)
+        res_h = np.sqrt(np.maximum(rn_h, 0.0)) / (np.maximum(np.abs(lam_h), scale) *
+                                                  np.sqrt(np.maximum(mn_h, 1e-300)))
+        res = torch.from_numpy(res_h)
+        worst = float(res_h[:k].max())
         history.append(worst)
-        # lock converged columns from the next residual on (no host sync needed)
-        colscale = (res >= tol).to(torch.float64)
+        # lock converged columns from the next residual on
+        (colscale,) = stage_cs.put([(res_h >= tol).astype(np.float64)])
         if verbose:
             print(f"  lobpcg[{str(dt)[6:]}] it {it:4d} worst rel. residual {worst:.3e}")
         if worst < tol:
@@ -253,10 +282,11 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
         MS = [MX[c], MW] + ([MP[c]] if have_p else [])
         m = len(S) * b
         G = kernels.gram_blocks(S, HS, MS, sym=True)
-        theta, C, dropped = _rayleigh_ritz(G, m, b, drop_tol)
+        theta, C_h, dropped = _rayleigh_ritz(G, m, b, drop_tol)
         drops.append(dropped)
-        CP = C.clone()
-        CP[:b] = 0.0
+        CP_h = C_h.copy()
+        CP_h[:b] = 0.0
+        C, CP, lam_next = stage_rr.put([C_h, CP_h, theta])
         fresh = bool(refresh) and it % refresh == 0
         kernels.block_update_blocks(S, C, X[1 - c], CP, P[1 - c], c2_first=b)
         if fresh:
@@ -268,7 +298,7 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
             for src, img, pimg in ((HS, HX, HP), (MS, MX, MP)):
                 kernels.block_update_blocks(src, C, img[1 - c], CP, pimg[1 - c], c2_first=b)
         c = 1 - c
-        lam = theta
+        lam, lam_h = lam_next, theta
         have_p = dropped == 0
         if fresh:
             # recompute the images of X: the updated ones carry the rounding of every
@@ -278,7 +308,7 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
     t_end = time.perf_counter()
     X = X[c]
     timings = {"setup_s": t_iter - t_start, "iterate_s": t_end - t_iter, "restarts": sum(d > 0 for d in drops)}
-    return LOBPCGResult(lam, X, res, it, history, converged, timings)
+    return LOBPCGResult(lam.clone(), X, res, it, history, converged, timings)
 
 
 def solve_mixed(op64, k, b, precond32=None, precond64=None, tol=1e-6, tol32=1e-3, maxiter=400,
