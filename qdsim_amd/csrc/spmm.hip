@@ -24,12 +24,9 @@ constexpr int kBlock = 256;
 
 template <typename T, int VEC>
 struct VecT;
-struct alignas(32) Float8 { float v[8]; };
-template <> struct VecT<float, 8> { using type = Float8; };
 template <> struct VecT<float, 4> { using type = float4; };
 template <> struct VecT<float, 2> { using type = float2; };
 template <> struct VecT<float, 1> { using type = float; };
-template <> struct VecT<double, 4> { using type = double4; };
 template <> struct VecT<double, 2> { using type = double2; };
 template <> struct VecT<double, 1> { using type = double; };
 
@@ -304,11 +301,6 @@ QDS_API int qds_spmm(int32_t dtype, int64_t n_rows, int64_t nnz, int32_t b, cons
   };
   switch (dtype) {
     case qds::F32:
-      // eight floats per lane (two 16-byte loads) when the block allows it: twice the
-      // rows per wave for the per-nonzero index and value loads
-      if (ok(32, 8))
-        return fused ? launch_real<float, 8, true>(n_rows, b, indptr, indices, vals_a, vals_b, X, ldx, Y_a, ldy_a, Y_b, ldy_b, st)
-                     : launch_real<float, 8, false>(n_rows, b, indptr, indices, vals_a, vals_b, X, ldx, Y_a, ldy_a, Y_b, ldy_b, st);
       if (ok(16, 4))
         return fused ? launch_real<float, 4, true>(n_rows, b, indptr, indices, vals_a, vals_b, X, ldx, Y_a, ldy_a, Y_b, ldy_b, st)
                      : launch_real<float, 4, false>(n_rows, b, indptr, indices, vals_a, vals_b, X, ldx, Y_a, ldy_a, Y_b, ldy_b, st);
@@ -318,11 +310,6 @@ QDS_API int qds_spmm(int32_t dtype, int64_t n_rows, int64_t nnz, int32_t b, cons
       return fused ? launch_real<float, 1, true>(n_rows, b, indptr, indices, vals_a, vals_b, X, ldx, Y_a, ldy_a, Y_b, ldy_b, st)
                    : launch_real<float, 1, false>(n_rows, b, indptr, indices, vals_a, vals_b, X, ldx, Y_a, ldy_a, Y_b, ldy_b, st);
     case qds::F64:
-      // four doubles per lane (two 16-byte loads): twice the rows per wave, so the
-      // per-nonzero index and value loads serve twice as many rows
-      if (ok(32, 4))
-        return fused ? launch_real<double, 4, true>(n_rows, b, indptr, indices, vals_a, vals_b, X, ldx, Y_a, ldy_a, Y_b, ldy_b, st)
-                     : launch_real<double, 4, false>(n_rows, b, indptr, indices, vals_a, vals_b, X, ldx, Y_a, ldy_a, Y_b, ldy_b, st);
       if (ok(16, 2))
         return fused ? launch_real<double, 2, true>(n_rows, b, indptr, indices, vals_a, vals_b, X, ldx, Y_a, ldy_a, Y_b, ldy_b, st)
                      : launch_real<double, 2, false>(n_rows, b, indptr, indices, vals_a, vals_b, X, ldx, Y_a, ldy_a, Y_b, ldy_b, st);
