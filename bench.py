@@ -56,9 +56,8 @@ def _funcs():
 
 
 def _spmm_roofline(prob, b, iters=20):
-    """Time the fused FP32 H+M SpMM (the eigensolver's operator apply) with HIP events."""
+    """Time the fused FP32 H+M product (the eigensolver's operator apply) with HIP events."""
     import torch
-    from qdsim_amd import kernels
     op = prob.operator(torch.float32)
     n, nnz = op.n, op.nnz
     # contiguous (n, b) blocks, the eigensolver's layout
@@ -77,11 +76,20 @@ def _spmm_roofline(prob, b, iters=20):
     e1.record(st)
     torch.cuda.synchronize()
     t = e0.elapsed_time(e1) / iters * 1e-3
-    # algorithmic bytes: values of H and M (FP32) + column index + row pointer +
+    csr_nnz = int(prob.r_indices.numel())
+    # CSR-equivalent bytes: values of H and M (FP32) + column index + row pointer +
     # X read once + HX, MX written
-    nbytes = nnz * (4 + 4 + 4) + 4 * (n + 1) + b * n * (4 + 2 * 4)
-    return {"kernel": "spmm_real<float,4,fused> (H and M, b=%d)" % b, "avg_us": t * 1e6,
-            "bytes": nbytes, "achieved_GBs": nbytes / t / 1e9, "nnz": nnz, "n": n}
+    csr_bytes = csr_nnz * (4 + 4 + 4) + 4 * (n + 1) + b * n * (4 + 2 * 4)
+    if type(op).__name__ == "LatticeOperator":
+        # lattice form: FP32 H and M value slots (no indices) + X once + HX, MX
+        nbytes = nnz * (4 + 4) + b * n * (4 + 2 * 4)
+        kernel = "lattice_spmm_kernel<float,4,3,fused> (H and M, b=%d)" % b
+    else:
+        nbytes = csr_bytes
+        kernel = "spmm_real<float,4,fused> (H and M, b=%d)" % b
+    return {"kernel": kernel, "avg_us": t * 1e6, "bytes": nbytes, "achieved_GBs": nbytes / t / 1e9,
+            "value_slots": nnz, "nnz": csr_nnz, "n": n, "csr_equivalent_bytes": csr_bytes,
+            "csr_equivalent_GBs": csr_bytes / t / 1e9}
 
 
 def _measured_traffic(cells):

@@ -81,7 +81,8 @@ int qds_assemble_gather(int64_t nnz, const int32_t* seg_ptr, const int32_t* seg_
  * fem.cpp:926-943).  bary: [nq][3].  xq, yq: [n_elem][nq].                        */
 int qds_quad_points(int32_t npe, int32_t nq, int64_t n_elem, const int32_t* conn, const double* x,
                     const double* y, const double* bary, double* xq, double* yq, void* stream);
-/* out[k] = re[src[k]] * scale (real dtypes) or re + i*scale*im (complex dtypes).  */
+/* out[k] = re[src[k]] * scale (real dtypes) or re + i*scale*im (complex dtypes);
+ * src[k] < 0 gives 0.                                                              */
 int qds_csr_take(int32_t dtype, int64_t n, const int32_t* src, const double* re, const double* im,
                  double scale, void* out, void* stream);
 
@@ -130,6 +131,48 @@ int qds_spmm_cap(int32_t dtype, int64_t n_rows, int32_t b, const int32_t* indptr
                  const int32_t* indices, const void* vals_h, const void* vals_w, const void* vals_m,
                  const void* X, int64_t ldx, void* Y_h, int64_t ldy_h, void* Y_m, int64_t ldy_m,
                  void* stream);
+
+/* ---- lattice (stencil) form of the free-DOF operator -----------------------------
+ * The eigensolver's operator lives on the interior points of the p-times refined
+ * lattice of a structured order-p mesh (nx x ny points, row-major).  Every row of
+ * class (I mod p, J mod p) (I = i + 1, J = j + 1: full-lattice coordinates) couples
+ * to the same set of lattice offsets (dI, dJ), |dI|, |dJ| <= p, so the operator is
+ * stored as per-class value arrays without column indices:
+ *   value of entry k of row (I, J), class c:
+ *     vals[base[c] + ((J / p) * cells_x + I / p) * len[c] + k]
+ * entries ordered by ascending column (dJ, then dI); missing entries at the
+ * boundary and the padding of len[c] to a multiple of 4 hold 0.  Same matrix as the
+ * CSR form of qds_pattern_restrict_fill (scipy csr of the reference,
+ * eigen_solver.pyx:225-235), products bitwise equal to qds_spmm's.               */
+#define QDS_LATTICE_MAX_CLASS 9
+#define QDS_LATTICE_MAX_ENTRY 52
+typedef struct {
+  int32_t order;                 /* p (1..3) */
+  int32_t n_class;               /* p * p */
+  int64_t nx, ny;                /* interior lattice points per row / column */
+  int64_t cells_x, cells_y;      /* nx / p + 1, ny / p + 1 */
+  int64_t n_slots;               /* length of each value array */
+  int32_t len[QDS_LATTICE_MAX_CLASS];
+  int64_t base[QDS_LATTICE_MAX_CLASS];
+  int8_t di[QDS_LATTICE_MAX_CLASS][QDS_LATTICE_MAX_ENTRY];
+  int8_t dj[QDS_LATTICE_MAX_CLASS][QDS_LATTICE_MAX_ENTRY];
+} qds_lattice_stencil;
+
+/* Host: stencil of a lattice-ordered CSR pattern (rows = nx * ny interior points)
+ * and, when slot_src != NULL, slot_src[slot] = CSR position of each value slot or
+ * -1 (a qds_csr_take map after composing with the CSR value map).  Fails when a
+ * column is not within p lattice steps of its row.                              */
+int qds_lattice_stencil_build(int32_t order, int64_t nx, int64_t ny, const int32_t* indptr,
+                              const int32_t* indices, qds_lattice_stencil* st /* host */,
+                              int64_t* slot_src /* host, optional [st->n_slots] */);
+
+/* Y_a = A_a X (and Y_b = A_b X) for operators in the lattice form: the product of
+ * qds_spmm without index loads; X rows of a lattice tile and its p-point halo are
+ * staged in LDS once per workgroup.  dtype F32 or F64.  Replaces scipy csr @ X in
+ * the block eigensolvers (eigen_solver.pyx:225-235, 254-266).                     */
+int qds_lattice_spmm(int32_t dtype, const qds_lattice_stencil* st /* host */, int32_t b,
+                     const void* vals_a, const void* vals_b, const void* X, int64_t ldx,
+                     void* Y_a, int64_t ldy_a, void* Y_b, int64_t ldy_b, void* stream);
 
 /* ---- dense block kernels for the block eigensolver (device) --------------------
  * Replace the dense Gram / projection steps inside scipy's lobpcg

@@ -54,6 +54,8 @@ _SIGNATURES = {
     "qds_density": ([_i64, _i32, _p, _i64, _p, _p, _p], _c.c_int),
     "qds_interpolate_points": ([_i32, _i64, _i64, _d, _d, _p, _p, _p, _p, _p, _i32, _i64, _p, _p, _p, _p, _p,
                                 _p, _p], _c.c_int),
+    "qds_lattice_stencil_build": ([_i32, _i64, _i64, _p, _p, _p, _p], _c.c_int),
+    "qds_lattice_spmm": ([_i32, _p, _i32, _p, _p, _p, _i64, _p, _i64, _p, _i64, _p], _c.c_int),
     "qds_cg_workspace": ([_i64], _i64),
     "qds_cg": ([_i64, _i64, _p, _p, _p, _p, _p, _d, _i32, _i32, _p, _i64, _c.POINTER(_i32),
                 _c.POINTER(_d), _p], _c.c_int),
@@ -63,6 +65,16 @@ _SIGNATURES = {
 class ColBlocks(ctypes.Structure):
     """Mirror of qds_colblocks."""
     _fields_ = [("p", _p * 4), ("ld", _i64 * 4), ("cols", _i32 * 4), ("nb", _i32)]
+
+
+MAX_CLASS, MAX_ENTRY = 9, 52
+
+
+class LatticeStencil(ctypes.Structure):
+    """Mirror of qds_lattice_stencil."""
+    _fields_ = [("order", _i32), ("n_class", _i32), ("nx", _i64), ("ny", _i64), ("cells_x", _i64),
+                ("cells_y", _i64), ("n_slots", _i64), ("len", _i32 * MAX_CLASS), ("base", _i64 * MAX_CLASS),
+                ("di", (_c.c_int8 * MAX_ENTRY) * MAX_CLASS), ("dj", (_c.c_int8 * MAX_ENTRY) * MAX_CLASS)]
 
 
 class MGLevel(ctypes.Structure):

@@ -139,13 +139,16 @@ __global__ void quad_points_kernel(int npe, int nq, int64_t n_elem, const int32_
 }
 
 // reduced[k] = full[src[k]] * scale (+ optional imaginary part), converted to the
-// storage type of the eigensolver
+// storage type of the eigensolver; src[k] < 0 marks a structural zero (padding of
+// the lattice stencil form)
 template <typename T>
 __global__ void take_real_kernel(int64_t n, const int32_t* __restrict__ src,
                                  const double* __restrict__ vals, double scale, T* __restrict__ out) {
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (int64_t)gridDim.x * blockDim.x)
-    out[t] = static_cast<T>(vals[src[t]] * scale);
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t s = src[t];
+    out[t] = s < 0 ? T(0) : static_cast<T>(vals[s] * scale);
+  }
 }
 
 template <typename T>
@@ -155,8 +158,8 @@ __global__ void take_complex_kernel(int64_t n, const int32_t* __restrict__ src,
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int32_t s = src[t];
-    out[2 * t] = static_cast<T>(re[s]);
-    out[2 * t + 1] = static_cast<T>(im ? im[s] * im_scale : 0.0);
+    out[2 * t] = s < 0 ? T(0) : static_cast<T>(re[s]);
+    out[2 * t + 1] = (s < 0 || !im) ? T(0) : static_cast<T>(im[s] * im_scale);
   }
 }
 

@@ -1,0 +1,382 @@
+// Block product Y_a = A_a X, Y_b = A_b X for operators in the lattice (stencil)
+// form: the free-DOF Hamiltonian and mass matrix of a structured order-p mesh,
+// whose rows (interior lattice points) couple only to points at most p lattice
+// steps away, with one offset list per class (I mod p, J mod p).
+//
+// Reference operation: the sparse products scipy performs inside ARPACK / lobpcg
+// (qdsim_cython/solvers/eigen_solver.pyx:225-235, 254-266) on the CSR matrices of
+// fem_solver.pyx:142-145.  The CSR kernel (qds_spmm) issues, per nonzero and row
+// group, an index load, two value loads and a dependent 16-byte X gather; on
+// gfx950 its address unit is busy ~95% of the kernel time (rocprofv3 TA_TA_BUSY)
+// at ~45 vector memory instructions per wave.  Here:
+//   * a workgroup owns one tile: one cell row (p lattice rows) times `tic` cells;
+//     the X rows of the tile and a p-point halo are staged in LDS with full-width
+//     16-byte loads (each X row is fetched ~3.6 times per product instead of once
+//     per nonzero, and the re-fetches hit L2);
+//   * there are no column indices: entry k of a class-c row reads the LDS point at
+//     a per-class offset, a wave-uniform scalar;
+//   * each wave task holds rows of one class only, so the trip count is uniform;
+//   * values of one class are stored row after row (len[c] entries each, padded
+//     with zeros to a multiple of 4) and read with 16-byte loads.
+// Entries are summed in ascending column order with fused multiply-adds, the order
+// of the CSR kernel; the padding and boundary entries add exact zeros, so both
+// kernels give bitwise equal products.
+#include "qds_common.h"
+#include "../../include/qdsim_hip.h"
+
+#include <algorithm>
+#include <cstdlib>
+#include <vector>
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr int kMaxTasks = 24;            // wave tasks per wave and tile
+constexpr int kBatch = 12;               // 16-byte loads in flight per thread while staging
+constexpr size_t kLdsBudget = 52 * 1024; // LDS per workgroup: 3 workgroups per CU
+constexpr size_t kLdsMax = 64 * 1024;
+
+struct LatticeArgs {
+  int64_t nx, ny, cells_x;
+  int64_t n_tiles;
+  int32_t tiles_x;  // tiles per cell row
+  int32_t tic;      // cells per tile along x
+  int32_t b, lpr, rpw, nch;  // block width, lanes per row, rows per wave, chunks per class row
+  int32_t lw;       // LDS tile width in points (p * tic + 2p)
+  int32_t ps;       // LDS point stride in elements (= b)
+  int32_t row_chunks, xchunks;  // 16-byte chunks per LDS tile row / of the X tile
+  int32_t vlds;     // first 16-byte chunk of the value tiles in LDS
+  int32_t vchunks;  // 16-byte chunks of one value tile (tic cells of every class)
+  int32_t len[QDS_LATTICE_MAX_CLASS];
+  int32_t vpre[QDS_LATTICE_MAX_CLASS];  // first chunk of each class run in a value tile
+  int64_t base[QDS_LATTICE_MAX_CLASS];
+  int16_t off[QDS_LATTICE_MAX_CLASS][QDS_LATTICE_MAX_ENTRY];  // dj * lw + di (LDS points)
+  int8_t tasks[kWaves][kMaxTasks];  // task = class * nch + chunk, -1 terminated
+};
+
+template <typename T, int N>
+struct Vec;
+template <> struct Vec<float, 4> { using type = float4; };
+template <> struct Vec<float, 2> { using type = float2; };
+template <> struct Vec<float, 1> { using type = float; };
+template <> struct Vec<double, 2> { using type = double2; };
+template <> struct Vec<double, 1> { using type = double; };
+
+template <typename T, int N>
+__device__ __forceinline__ void ld(const T* p, T (&v)[N]) {
+  using V = typename Vec<T, N>::type;
+  const V t = *reinterpret_cast<const V*>(p);
+  const T* tp = reinterpret_cast<const T*>(&t);
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = tp[i];
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void st(T* p, const T (&v)[N]) {
+  using V = typename Vec<T, N>::type;
+  V t;
+  T* tp = reinterpret_cast<T*>(&t);
+#pragma unroll
+  for (int i = 0; i < N; ++i) tp[i] = v[i];
+  *reinterpret_cast<V*>(p) = t;
+}
+
+// four consecutive values (16-byte aligned for float, two 16-byte loads for double)
+template <typename T>
+__device__ __forceinline__ void ld4(const T* p, T (&v)[4]) {
+  if constexpr (sizeof(T) == 4) {
+    ld<T, 4>(p, v);
+  } else {
+    T a[2], b[2];
+    ld<T, 2>(p, a);
+    ld<T, 2>(p + 2, b);
+    v[0] = a[0]; v[1] = a[1]; v[2] = b[0]; v[3] = b[1];
+  }
+}
+
+// Workgroups go round-robin to the 8 XCDs; tile g % 8 * (G / 8) + g / 8 makes every
+// XCD sweep a contiguous range of tiles in row-major order, so the halo rows a tile
+// shares with the cell row above were fetched into the same L2 shortly before.
+__device__ __forceinline__ int64_t xcd_tile(int64_t g, int64_t G) { return (g % 8) * (G / 8) + g / 8; }
+
+template <typename T, int VEC, int P, bool FUSED>
+__global__ void __launch_bounds__(kThreads)
+lattice_spmm_kernel(const LatticeArgs a, const T* __restrict__ va, const T* __restrict__ vb,
+                    const T* __restrict__ X, int64_t ldx, int chunked, T* __restrict__ Ya,
+                    int64_t ldya, T* __restrict__ Yb, int64_t ldyb) {
+  constexpr int CE = 16 / sizeof(T);  // elements per 16-byte chunk
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T* lds = reinterpret_cast<T*>(lds_raw);
+  float4* lds4 = reinterpret_cast<float4*>(lds_raw);
+  const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (tile >= a.n_tiles) return;  // uniform over the workgroup
+  const int64_t cr = tile / a.tiles_x;
+  const int64_t cc0 = (tile - cr * a.tiles_x) * a.tic;
+  const int64_t J0 = P * cr - P, I0 = P * cc0 - P;  // full-lattice origin of the LDS tile
+  const int lw = a.lw, ps = a.ps;
+  const int ncell = (int)min((int64_t)a.tic, a.cells_x - cc0);  // cells of this tile with values
+
+  // ---- stage the tile in LDS: X rows of the tile and its p-point halo (zeros
+  // outside the interior), then the tile's values of A_a and A_b (one contiguous
+  // run per class).  Every thread issues all of its 16-byte loads before the
+  // first LDS write, so one memory latency covers the whole tile.
+  const int xchunks = chunked ? a.xchunks : 0;
+  const int total = xchunks + (FUSED ? 2 : 1) * a.vchunks;
+  for (int e0 = 0; e0 < total; e0 += kThreads * kBatch) {
+    float4 v[kBatch];
+    int dst[kBatch];
+#pragma unroll
+    for (int i = 0; i < kBatch; ++i) {
+      const int e = e0 + i * kThreads + (int)threadIdx.x;
+      v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      dst[i] = -1;
+      if (e >= total) continue;
+      const float4* src = nullptr;
+      if (e < xchunks) {
+        const int cpp = ps / CE;
+        const int lr = e / a.row_chunks;
+        const int rem = e - lr * a.row_chunks;
+        const int pt = rem / cpp;
+        const int part = rem - pt * cpp;
+        const int64_t J = J0 + lr, I = I0 + pt;
+        if (J >= 1 && J <= a.ny && I >= 1 && I <= a.nx)
+          src = reinterpret_cast<const float4*>(X + ((J - 1) * a.nx + (I - 1)) * ldx + part * CE);
+        dst[i] = e;
+      } else {
+        int f = e - xchunks;
+        const bool second = f >= a.vchunks;
+        if (second) f -= a.vchunks;
+        int c = 0;
+        while (c + 1 < P * P && f >= a.vpre[c + 1]) ++c;
+        const int off = (f - a.vpre[c]) * CE;  // element within the class run
+        if (off < ncell * a.len[c]) {
+          const T* base = (second ? vb : va) + a.base[c] + (cr * a.cells_x + cc0) * a.len[c];
+          src = reinterpret_cast<const float4*>(base + off);
+        }
+        dst[i] = a.vlds + (second ? a.vchunks : 0) + f;
+      }
+      if (src) v[i] = *src;
+    }
+#pragma unroll
+    for (int i = 0; i < kBatch; ++i)
+      if (dst[i] >= 0) lds4[dst[i]] = v[i];
+  }
+  if (!chunked) {
+    const int row_elems = lw * ps;
+    for (int lr = 0; lr < 3 * P; ++lr) {
+      const int64_t J = J0 + lr;
+      const bool jin = J >= 1 && J <= a.ny;
+      T* dstx = lds + (int64_t)lr * lw * ps;
+      for (int e = threadIdx.x; e < row_elems; e += kThreads) {
+        const int pt = e / ps;
+        const int col = e - pt * ps;
+        const int64_t I = I0 + pt;
+        T x = T(0);
+        if (jin && I >= 1 && I <= a.nx) x = X[((J - 1) * a.nx + (I - 1)) * ldx + col];
+        dstx[e] = x;
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- wave tasks: rpw rows of one class, lpr lanes (VEC columns each) per row,
+  // operands from LDS only
+  const T* lva = lds + (int64_t)a.vlds * CE;
+  const T* lvb = lva + (int64_t)a.vchunks * CE;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int g = lane / a.lpr;
+  const int li = lane - g * a.lpr;
+  const bool lane_ok = g < a.rpw && li * VEC < a.b;
+  const int c0 = lane_ok ? li * VEC : 0;
+  const int gg = g < a.rpw ? g : 0;
+  for (int s = 0; s < kMaxTasks; ++s) {
+    const int task = a.tasks[wave][s];
+    if (task < 0) break;
+    const int c = task / a.nch;
+    const int h = task - c * a.nch;
+    const int ci = c % P, cj = c / P;
+    const int u = h * a.rpw + gg;
+    const int64_t I = P * (cc0 + u) + ci;
+    const int64_t J = P * cr + cj;
+    const bool ok = lane_ok && u < ncell && I >= 1 && I <= a.nx && J >= 1 && J <= a.ny;
+    // lanes without a row read a valid row of the tile and store nothing
+    const int uu = u < ncell ? u : ncell - 1;
+    const int len = a.len[c];
+    const int vofs = a.vpre[c] * CE + uu * len;
+    const T* xl = lds + (int64_t)((P + cj) * lw + P + ci + P * uu) * ps + c0;
+    T acc_a[VEC], acc_b[VEC];
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) { acc_a[v] = T(0); acc_b[v] = T(0); }
+    for (int k = 0; k < len; k += 4) {
+      T av[4], bv[4];
+      ld4<T>(lva + vofs + k, av);
+      if (FUSED) ld4<T>(lvb + vofs + k, bv);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        T x[VEC];
+        ld<T, VEC>(xl + (int)a.off[c][k + q] * ps, x);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+          acc_a[v] = fma(av[q], x[v], acc_a[v]);
+          if (FUSED) acc_b[v] = fma(bv[q], x[v], acc_b[v]);
+        }
+      }
+    }
+    if (ok) {
+      const int64_t row = (J - 1) * a.nx + (I - 1);
+      st<T, VEC>(Ya + row * ldya + c0, acc_a);
+      if (FUSED) st<T, VEC>(Yb + row * ldyb + c0, acc_b);
+    }
+  }
+}
+
+bool aligned(const void* p, int bytes) { return (reinterpret_cast<uintptr_t>(p) % bytes) == 0; }
+
+template <typename T, int VEC, int P, bool FUSED>
+int launch(const LatticeArgs& a, size_t lds, const void* va, const void* vb, const void* X,
+           int64_t ldx, int chunked, void* Ya, int64_t ldya, void* Yb, int64_t ldyb, hipStream_t st) {
+  const int64_t grid = (a.n_tiles + 7) / 8 * 8;
+  hipLaunchKernelGGL((lattice_spmm_kernel<T, VEC, P, FUSED>), dim3((unsigned)grid), dim3(kThreads), lds,
+                     st, a, (const T*)va, (const T*)vb, (const T*)X, ldx, chunked, (T*)Ya, ldya,
+                     (T*)Yb, ldyb);
+  QDS_LAUNCH_CHECK();
+  return QDS_OK;
+}
+
+template <typename T, int VEC>
+int dispatch(int p, bool fused, const LatticeArgs& a, size_t lds, const void* va, const void* vb,
+             const void* X, int64_t ldx, int chunked, void* Ya, int64_t ldya, void* Yb, int64_t ldyb,
+             hipStream_t st) {
+#define QDS_LAT_CASE(PP)                                                                          \
+  case PP:                                                                                      \
+    return fused ? launch<T, VEC, PP, true>(a, lds, va, vb, X, ldx, chunked, Ya, ldya, Yb, ldyb, st) \
+                 : launch<T, VEC, PP, false>(a, lds, va, vb, X, ldx, chunked, Ya, ldya, Yb, ldyb, st);
+  switch (p) {
+    QDS_LAT_CASE(1)
+    QDS_LAT_CASE(2)
+    QDS_LAT_CASE(3)
+    default:
+      QDS_CHECK_ARG(false, "qds_lattice_spmm: order %d not in 1..3", p);
+  }
+#undef QDS_LAT_CASE
+  return QDS_EINVAL;
+}
+
+}  // namespace
+
+QDS_API int qds_lattice_spmm(int32_t dtype, const qds_lattice_stencil* s, int32_t b, const void* vals_a,
+                             const void* vals_b, const void* X, int64_t ldx, void* Y_a, int64_t ldy_a,
+                             void* Y_b, int64_t ldy_b, void* stream) {
+  QDS_CHECK_ARG(s && vals_a && X && Y_a, "qds_lattice_spmm: null pointer");
+  QDS_CHECK_ARG(!vals_b || Y_b, "qds_lattice_spmm: vals_b without Y_b");
+  QDS_CHECK_ARG(dtype == qds::F32 || dtype == qds::F64, "qds_lattice_spmm: dtype must be F32 or F64");
+  QDS_CHECK_ARG(b >= 1 && ldx >= b && ldy_a >= b && (!vals_b || ldy_b >= b), "qds_lattice_spmm: bad block shape");
+  const int p = s->order;
+  QDS_CHECK_ARG(p >= 1 && p <= 3 && s->n_class == p * p && s->nx >= 1 && s->ny >= 1 &&
+                    s->cells_x == s->nx / p + 1 && s->cells_y == s->ny / p + 1,
+                "qds_lattice_spmm: inconsistent stencil description");
+  const bool fused = vals_b != nullptr;
+  const int es = qds::dtype_size(dtype);
+  int sum_len = 0;
+  for (int c = 0; c < s->n_class; ++c) sum_len += s->len[c];
+  const size_t nval = fused ? 2 : 1;
+  // blocks wider than one LDS tile of a single cell allows: column slices
+  const size_t v1 = nval * sum_len * es;
+  QDS_CHECK_ARG(v1 + (size_t)9 * p * p * 4 * es <= kLdsMax, "qds_lattice_spmm: stencil too large for LDS");
+  const int bmax = (int)((kLdsMax - v1) / ((size_t)9 * p * p * es)) / 4 * 4;
+  if (b > bmax) {
+    for (int j = 0; j < b; j += bmax) {
+      const int w = std::min(bmax, b - j);
+      const int rc = qds_lattice_spmm(dtype, s, w, vals_a, vals_b, (const char*)X + (size_t)j * es, ldx,
+                                      (char*)Y_a + (size_t)j * es, ldy_a,
+                                      Y_b ? (char*)Y_b + (size_t)j * es : nullptr, ldy_b, stream);
+      if (rc != QDS_OK) return rc;
+    }
+    return QDS_OK;
+  }
+  // lanes per row: VEC columns each, the widest VEC dividing b whose loads and
+  // stores stay aligned
+  auto vec_ok = [&](int vec) {
+    const int bytes = vec * es;
+    if (b % vec) return false;
+    bool ok = aligned(Y_a, bytes) && (ldy_a * es) % bytes == 0;
+    if (fused) ok = ok && aligned(Y_b, bytes) && (ldy_b * es) % bytes == 0;
+    return ok;
+  };
+  int vec = 1;
+  if (dtype == qds::F32) vec = vec_ok(4) ? 4 : vec_ok(2) ? 2 : 1;
+  else vec = vec_ok(2) ? 2 : 1;
+  QDS_CHECK_ARG(aligned(vals_a, 16) && (!fused || aligned(vals_b, 16)),
+                "qds_lattice_spmm: value arrays must be 16-byte aligned");
+  LatticeArgs a{};
+  a.nx = s->nx;
+  a.ny = s->ny;
+  a.cells_x = s->cells_x;
+  a.b = b;
+  a.lpr = (b + vec - 1) / vec;
+  QDS_CHECK_ARG(a.lpr <= 64, "qds_lattice_spmm: block width %d too large", b);
+  a.rpw = 64 / a.lpr;
+  a.ps = b;
+  const size_t point_bytes = (size_t)b * es;
+  // cells per tile: the most that fit the LDS budget, a whole number of wave
+  // chunks when more than one chunk fits
+  auto x_bytes = [&](int tic) { return ((size_t)3 * p * (size_t)(p * tic + 2 * p) * point_bytes + 15) / 16 * 16; };
+  auto tile_bytes = [&](int tic) { return x_bytes(tic) + nval * (size_t)tic * sum_len * es; };
+  int tic = 1;
+  while (tic < 4 * a.rpw && tile_bytes(tic + 1) <= kLdsBudget) ++tic;
+  if (tic > a.rpw) tic = tic / a.rpw * a.rpw;
+  QDS_CHECK_ARG(tile_bytes(tic) <= kLdsMax, "qds_lattice_spmm: block width %d too large for the LDS tile", b);
+  a.tic = tic;
+  a.nch = (tic + a.rpw - 1) / a.rpw;
+  a.lw = p * tic + 2 * p;
+  const int chunked = (point_bytes % 16 == 0) && aligned(X, 16) && ((ldx * es) % 16 == 0);
+  a.row_chunks = chunked ? (int)(a.lw * point_bytes / 16) : 0;
+  a.xchunks = 3 * p * a.row_chunks;
+  a.vlds = (int)(x_bytes(tic) / 16);
+  a.vchunks = (int)((size_t)tic * sum_len * es / 16);
+  for (int c = 0, acc = 0; c < s->n_class; ++c) {
+    a.vpre[c] = acc;
+    acc += (int)((size_t)tic * s->len[c] * es / 16);
+  }
+  a.tiles_x = (int32_t)((s->cells_x + tic - 1) / tic);
+  a.n_tiles = (int64_t)a.tiles_x * s->cells_y;
+  for (int c = 0; c < s->n_class; ++c) {
+    QDS_CHECK_ARG(s->len[c] % 4 == 0 && s->len[c] <= QDS_LATTICE_MAX_ENTRY && s->base[c] % 4 == 0,
+                  "qds_lattice_spmm: inconsistent stencil description");
+    a.len[c] = s->len[c];
+    a.base[c] = s->base[c];
+    for (int k = 0; k < s->len[c]; ++k) {
+      QDS_CHECK_ARG(std::abs((int)s->di[c][k]) <= p && std::abs((int)s->dj[c][k]) <= p,
+                    "qds_lattice_spmm: stencil offset beyond the halo");
+      a.off[c][k] = (int16_t)(s->dj[c][k] * a.lw + s->di[c][k]);
+    }
+  }
+  // longest-task-first assignment of the (class, chunk) tasks to the waves
+  const int n_tasks = s->n_class * a.nch;
+  std::vector<int> order(n_tasks);
+  for (int t = 0; t < n_tasks; ++t) order[t] = t;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return a.len[x / a.nch] > a.len[y / a.nch]; });
+  int load[kWaves] = {0}, cnt[kWaves] = {0};
+  for (int w = 0; w < kWaves; ++w)
+    for (int t = 0; t < kMaxTasks; ++t) a.tasks[w][t] = -1;
+  for (int t : order) {
+    int w = 0;
+    for (int v = 1; v < kWaves; ++v)
+      if (load[v] < load[w]) w = v;
+    QDS_CHECK_ARG(cnt[w] < kMaxTasks, "qds_lattice_spmm: too many tasks per tile");
+    a.tasks[w][cnt[w]++] = (int8_t)t;
+    load[w] += a.len[t / a.nch];
+  }
+  const size_t lds = tile_bytes(tic);
+  hipStream_t st = qds_stream(stream);
+  if (dtype == qds::F32) {
+    if (vec == 4) return dispatch<float, 4>(p, fused, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
+    if (vec == 2) return dispatch<float, 2>(p, fused, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
+    return dispatch<float, 1>(p, fused, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
+  }
+  if (vec == 2) return dispatch<double, 2>(p, fused, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
+  return dispatch<double, 1>(p, fused, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
+}

@@ -12,6 +12,7 @@
 // segment in that fixed order, so assembly needs no atomics and is bitwise
 // reproducible.
 #include "qds_common.h"
+#include "../../include/qdsim_hip.h"
 
 #include <algorithm>
 #include <thread>
@@ -238,6 +239,104 @@ QDS_API int qds_pattern_restrict_fill(const int32_t* indptr, const int32_t* indi
         r_indices[o] = p.first;
         r_src[o] = p.second;
         ++o;
+      }
+    }
+  });
+  return QDS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Lattice (stencil) form of a lattice-ordered operator: per-class offset lists
+// and the map from value slots to CSR positions.  Rows are the nx * ny interior
+// lattice points in row-major order; row r has full-lattice coordinates
+// I = r % nx + 1, J = r / nx + 1 and class (J % p) * p + I % p.
+QDS_API int qds_lattice_stencil_build(int32_t order, int64_t nx, int64_t ny, const int32_t* indptr,
+                                      const int32_t* indices, qds_lattice_stencil* st,
+                                      int64_t* slot_src) {
+  QDS_CHECK_ARG(indptr && indices && st, "qds_lattice_stencil_build: null pointer");
+  QDS_CHECK_ARG(order >= 1 && order <= 3, "qds_lattice_stencil_build: order %d not in 1..3", order);
+  QDS_CHECK_ARG(nx >= 1 && ny >= 1, "qds_lattice_stencil_build: empty lattice");
+  const int p = order, w = 2 * p + 1, nc = p * p;
+  const int64_t n = nx * ny;
+  // union of the offsets of every row of a class, as a (2p+1)^2 bitmap per class
+  const size_t nbits = static_cast<size_t>(nc) * w * w;
+  const int nt = hw_threads();
+  std::vector<uint8_t> used_t(nbits * nt, 0);
+  std::vector<int> bad_t(nt, 0);
+  {
+    std::vector<std::thread> th;
+    const int64_t chunk = (n + nt - 1) / nt;
+    for (int t = 0; t < nt; ++t) {
+      th.emplace_back([&, t]() {
+        uint8_t* used = used_t.data() + nbits * t;
+        const int64_t lo = t * chunk, hi = std::min(n, lo + chunk);
+        for (int64_t r = lo; r < hi; ++r) {
+          const int64_t i = r % nx, j = r / nx;
+          const int c = static_cast<int>(((j + 1) % p) * p + (i + 1) % p);
+          for (int32_t k = indptr[r]; k < indptr[r + 1]; ++k) {
+            const int64_t col = indices[k];
+            const int64_t di = col % nx - i, dj = col / nx - j;
+            if (col < 0 || col >= n || di < -p || di > p || dj < -p || dj > p) {
+              bad_t[t] = 1;
+              return;
+            }
+            used[(static_cast<size_t>(c) * w + (dj + p)) * w + (di + p)] = 1;
+          }
+        }
+      });
+    }
+    for (auto& x : th) x.join();
+  }
+  std::vector<uint8_t> used(nbits, 0);
+  int bad = 0;
+  for (int t = 0; t < nt; ++t) {
+    bad |= bad_t[t];
+    for (size_t q = 0; q < nbits; ++q) used[q] |= used_t[nbits * t + q];
+  }
+  QDS_CHECK_ARG(!bad, "qds_lattice_stencil_build: a column lies more than %d lattice steps from its row", p);
+  *st = qds_lattice_stencil{};
+  st->order = p;
+  st->n_class = nc;
+  st->nx = nx;
+  st->ny = ny;
+  st->cells_x = nx / p + 1;
+  st->cells_y = ny / p + 1;
+  const int64_t cells = st->cells_x * st->cells_y;
+  // slot index of each bitmap position within its class (ascending column order:
+  // dJ major, dI minor, since |dI| <= p < nx)
+  std::vector<int> kidx(used.size(), -1);
+  int64_t base = 0;
+  for (int c = 0; c < nc; ++c) {
+    int len = 0;
+    for (int q = 0; q < w * w; ++q) {
+      const size_t pos = static_cast<size_t>(c) * w * w + q;
+      if (!used[pos]) continue;
+      QDS_CHECK_ARG(len < QDS_LATTICE_MAX_ENTRY, "qds_lattice_stencil_build: stencil too long");
+      kidx[pos] = len;
+      st->di[c][len] = static_cast<int8_t>(q % w - p);
+      st->dj[c][len] = static_cast<int8_t>(q / w - p);
+      ++len;
+    }
+    const int padded = (len + 3) & ~3;   // whole 4-entry groups; padding entries (0, 0) with value 0
+    QDS_CHECK_ARG(padded <= QDS_LATTICE_MAX_ENTRY, "qds_lattice_stencil_build: stencil too long");
+    st->len[c] = padded;
+    st->base[c] = base;
+    base += cells * padded;
+  }
+  st->n_slots = base;
+  if (!slot_src) return QDS_OK;
+  parallel_rows(base, [&](int64_t lo, int64_t hi) {
+    for (int64_t t = lo; t < hi; ++t) slot_src[t] = -1;
+  });
+  parallel_rows(n, [&](int64_t lo, int64_t hi) {
+    for (int64_t r = lo; r < hi; ++r) {
+      const int64_t i = r % nx, j = r / nx, I = i + 1, J = j + 1;
+      const int c = static_cast<int>((J % p) * p + I % p);
+      const int64_t row0 = st->base[c] + ((J / p) * st->cells_x + I / p) * st->len[c];
+      for (int32_t k = indptr[r]; k < indptr[r + 1]; ++k) {
+        const int64_t col = indices[k];
+        const int64_t di = col % nx - i, dj = col / nx - j;
+        slot_src[row0 + kidx[(static_cast<size_t>(c) * w + (dj + p)) * w + (di + p)]] = k;
       }
     }
   });

@@ -128,6 +128,33 @@ def spmm(indptr, indices, n_rows, nnz, vals_a, X, Ya, vals_b=None, Yb=None):
                        stream_ptr()), "qds_spmm")
 
 
+def lattice_stencil(order, nx, ny, indptr, indices, with_map=True):
+    """Stencil form of a lattice-ordered CSR pattern (host arrays).
+
+    Returns (qds_lattice_stencil, slot_src int64 numpy array or None): slot_src[s]
+    is the CSR position feeding value slot s, -1 for structural zeros."""
+    st = _lib.LatticeStencil()
+    ip = i32(indptr)
+    ix = i32(indices)
+    check(lib.qds_lattice_stencil_build(int(order), int(nx), int(ny), host_ptr(ip), host_ptr(ix),
+                                        ctypes.byref(st), None), "qds_lattice_stencil_build")
+    if not with_map:
+        return st, None
+    slot_src = np.empty(st.n_slots, dtype=np.int64)
+    check(lib.qds_lattice_stencil_build(int(order), int(nx), int(ny), host_ptr(ip), host_ptr(ix),
+                                        ctypes.byref(st), host_ptr(slot_src)), "qds_lattice_stencil_build")
+    return st, slot_src
+
+
+def lattice_spmm(st, vals_a, X, Ya, vals_b=None, Yb=None):
+    """Ya = A_a X (and Yb = A_b X) with A in the lattice form described by st."""
+    assert vals_a.dtype == X.dtype == Ya.dtype
+    b = _width(X)
+    check(lib.qds_lattice_spmm(code(X), ctypes.byref(st), b, ptr(vals_a), ptr(vals_b), ptr(X), _ld(X),
+                               ptr(Ya), _ld(Ya), ptr(Yb), _ld(Yb) if Yb is not None else 1, stream_ptr()),
+          "qds_lattice_spmm")
+
+
 def spmm_cap(indptr, indices, n_rows, h, w, m, X, YH, YM):
     """YH = (H - iW) X, YM = M X for complex blocks, real value arrays."""
     b = _width(X)

@@ -59,6 +59,32 @@ class CSROperator:
         return CSROperator(self.indptr, self.indices, self.n, self.H, self.M, dtype)
 
 
+class LatticeOperator:
+    """H and M of a structured-mesh problem in the lattice (stencil) form: per-class
+    value arrays without column indices, products by qds_lattice_spmm (bitwise
+    equal to the CSR product of the same matrices)."""
+
+    def __init__(self, stencil, n, H, M, dtype):
+        self.stencil = stencil
+        self.n = int(n)
+        self.nnz = int(stencil.n_slots)
+        self.H = H.to(dtype).contiguous()
+        self.M = M.to(dtype).contiguous()
+        self.dtype = dtype
+
+    def apply(self, X, HX, MX):
+        kernels.lattice_spmm(self.stencil, self.H, X, HX, self.M, MX)
+
+    def apply_h(self, X, HX):
+        kernels.lattice_spmm(self.stencil, self.H, X, HX)
+
+    def apply_m(self, X, MX):
+        kernels.lattice_spmm(self.stencil, self.M, X, MX)
+
+    def astype(self, dtype):
+        return LatticeOperator(self.stencil, self.n, self.H, self.M, dtype)
+
+
 def _svqb_rr(G, m, nb, drop_tol):
     """Rayleigh-Ritz on S with Gram G = S^T [HS | MS] (FP64, from the MFMA kernel).
 

@@ -26,7 +26,7 @@ from . import kernels
 from .fem import FEMSpace
 from .mesh import SimpleMesh
 from .lattice import Lattice, LatticeMG
-from .lobpcg import CSROperator, lobpcg, solve_mixed
+from .lobpcg import CSROperator, LatticeOperator, lobpcg, solve_mixed
 from .units import EV_TO_J, HBAR, NM
 
 KIN_SCALE = HBAR * HBAR / 2.0 / EV_TO_J / (NM * NM)   # hbar^2/2 in eV nm^2 kg
@@ -54,6 +54,12 @@ class SchrodingerProblem:
         self.r_indptr = torch.from_numpy(r_ip).to(dev)
         self.r_indices = torch.from_numpy(r_ix).to(dev)
         self.r_src = torch.from_numpy(r_src).to(dev)
+        # lattice (stencil) form of the free-DOF operator: value slot -> full CSR position
+        self.stencil, slot_src = kernels.lattice_stencil(self.lat.p, self.lat.nx_i, self.lat.ny_i, r_ip, r_ix)
+        s_src = np.where(slot_src >= 0, r_src[np.maximum(slot_src, 0)], -1).astype(np.int32)
+        self.s_src = torch.from_numpy(s_src).to(dev)
+        del slot_src, s_src
+        self.use_lattice = True   # products in the lattice form (False: CSR)
         self.old_of_new = torch.from_numpy(self.lat.old_of_new).to(dev)
         self.n_free = self.lat.n_free
         self.sampling = sampling
@@ -102,7 +108,12 @@ class SchrodingerProblem:
         self.H, self.M, self.W = out
         return out
 
-    def operator(self, dtype=torch.float64):
+    def operator(self, dtype=torch.float64, lattice=None):
+        """Free-DOF (H, M) in HBM: the lattice (stencil) form by default, CSR on request."""
+        if self.use_lattice if lattice is None else lattice:
+            H = kernels.csr_take(self.s_src, self.H, dtype=dtype)
+            M = kernels.csr_take(self.s_src, self.M, dtype=dtype)
+            return LatticeOperator(self.stencil, self.n_free, H, M, dtype)
         H = kernels.csr_take(self.r_src, self.H, dtype=dtype)
         M = kernels.csr_take(self.r_src, self.M, dtype=dtype)
         return CSROperator(self.r_indptr, self.r_indices, self.n_free, H, M, dtype)

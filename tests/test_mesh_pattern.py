@@ -113,3 +113,64 @@ def test_product_quadrature_tables():
         l = t["bary"]
         phi = np.stack(fem_ref.shape_values(order, l[:, 0], l[:, 1], l[:, 2]), -1)
         assert np.allclose(t["phi"], phi, atol=1e-13)
+
+
+def _lattice_pattern(p, ncx, ncy):
+    from qdsim_amd import kernels
+    from qdsim_amd.lattice import Lattice
+    m = SimpleMesh(ncx + 1, ncy + 1, 1e-7, 1e-7, p)
+    conn = np.ascontiguousarray(m.get_elements(), dtype=np.int32)
+    pat = kernels.pattern_build(conn, int(m.num_nodes))
+    lat = Lattice(m)
+    r_ip, r_ix, _ = kernels.pattern_restrict(pat["indptr"], pat["indices"], lat.new_of_old, lat.n_free)
+    return lat, r_ip, r_ix
+
+
+@pytest.mark.parametrize("p,ncx,ncy", [(1, 9, 7), (2, 8, 5), (3, 13, 10), (3, 1, 2)])
+def test_lattice_stencil_reproduces_csr(p, ncx, ncy):
+    """Host stencil builder: every CSR entry lands in exactly one value slot, the
+    class stencils are the FE couplings of the structured triangulation, and the
+    product evaluated from the stencil form (offsets, zero slots) equals the CSR
+    product exactly (same summation order)."""
+    from qdsim_amd import kernels
+    lat, r_ip, r_ix = _lattice_pattern(p, ncx, ncy)
+    st, slot = kernels.lattice_stencil(p, lat.nx_i, lat.ny_i, r_ip, r_ix)
+    assert st.order == p and st.n_class == p * p
+    assert np.array_equal(np.sort(slot[slot >= 0]), np.arange(len(r_ix)))
+    # stencil sizes of the interior classes: P1 7-point, P2 19/9, P3 37/16/10
+    want = {1: [7], 2: [19, 9, 9, 9], 3: [37, 16, 16, 16, 10, 16, 16, 16, 10]}[p]
+    if ncx >= 4 and ncy >= 4:
+        lens = []
+        for c in range(p * p):
+            offs = {(st.di[c][k], st.dj[c][k]) for k in range(st.len[c])}
+            lens.append(len(offs))
+        assert lens == want
+    rng = np.random.default_rng(p)
+    vals = rng.standard_normal(len(r_ix))
+    A = sp.csr_matrix((vals, r_ix, r_ip), shape=(lat.n_free, lat.n_free))
+    sv = np.where(slot >= 0, vals[np.maximum(slot, 0)], 0.0)
+    X = rng.standard_normal((lat.n_free, 3))
+    Y = np.zeros_like(X)
+    nx, ny = lat.nx_i, lat.ny_i
+    for r in range(lat.n_free):
+        j, i = divmod(r, nx)
+        I, J = i + 1, j + 1
+        c = (J % p) * p + I % p
+        L = st.len[c]
+        base = st.base[c] + ((J // p) * st.cells_x + I // p) * L
+        for k in range(L):
+            I2, J2 = I + st.di[c][k], J + st.dj[c][k]
+            if 1 <= I2 <= nx and 1 <= J2 <= ny:
+                Y[r] += sv[base + k] * X[(J2 - 1) * nx + I2 - 1]
+            else:
+                assert sv[base + k] == 0.0
+    assert np.array_equal(Y, A @ X)
+
+
+def test_lattice_stencil_rejects_non_lattice_pattern():
+    from qdsim_amd import kernels
+    from qdsim_amd._lib import QDSError
+    ip = np.array([0, 2, 3, 4, 5], dtype=np.int32)
+    ix = np.array([0, 3, 1, 2, 3], dtype=np.int32)   # row 0 couples to a point 3 steps away
+    with pytest.raises(QDSError):
+        kernels.lattice_stencil(1, 4, 1, ip, ix)
