@@ -137,13 +137,18 @@ int qds_spmm_cap(int32_t dtype, int64_t n_rows, int32_t b, const int32_t* indptr
  * lattice of a structured order-p mesh (nx x ny points, row-major).  Every row of
  * class (I mod p, J mod p) (I = i + 1, J = j + 1: full-lattice coordinates) couples
  * to the same set of lattice offsets (dI, dJ), |dI|, |dJ| <= p, so the operator is
- * stored as per-class value arrays without column indices:
- *   value of entry k of row (I, J), class c:
- *     vals[base[c] + ((J / p) * cells_x + I / p) * len[c] + k]
+ * stored as value arrays without column indices.  Cells (I / p, J / p) are grouped
+ * in tiles of `tile` cells along x; the values of one tile are contiguous
+ * (tile_slots entries), class after class, and within class c in groups of four
+ * entries, cell after cell:
+ *   value of entry k of row (I, J), class c, cell (X, Y) = (I / p, J / p):
+ *     vals[(Y * tiles_x + X / tile) * tile_slots + base[c]
+ *          + ((k / 4) * tile + X % tile) * 4 + k % 4]
  * entries ordered by ascending column (dJ, then dI); missing entries at the
- * boundary and the padding of len[c] to a multiple of 4 hold 0.  Same matrix as the
- * CSR form of qds_pattern_restrict_fill (scipy csr of the reference,
- * eigen_solver.pyx:225-235), products bitwise equal to qds_spmm's.               */
+ * boundary, the padding of len[c] to a multiple of 4 and the cells past the last
+ * column hold 0.  Same matrix as the CSR form of qds_pattern_restrict_fill (scipy
+ * csr of the reference, eigen_solver.pyx:225-235), products bitwise equal to
+ * qds_spmm's.                                                                   */
 #define QDS_LATTICE_MAX_CLASS 9
 #define QDS_LATTICE_MAX_ENTRY 52
 typedef struct {
@@ -151,9 +156,13 @@ typedef struct {
   int32_t n_class;               /* p * p */
   int64_t nx, ny;                /* interior lattice points per row / column */
   int64_t cells_x, cells_y;      /* nx / p + 1, ny / p + 1 */
-  int64_t n_slots;               /* length of each value array */
-  int32_t len[QDS_LATTICE_MAX_CLASS];
-  int64_t base[QDS_LATTICE_MAX_CLASS];
+  int32_t tile;                  /* cells per tile along x */
+  int32_t pad0;
+  int64_t tiles_x;               /* ceil(cells_x / tile) */
+  int64_t tile_slots;            /* tile * sum(len) */
+  int64_t n_slots;               /* length of each value array: cells_y * tiles_x * tile_slots */
+  int32_t len[QDS_LATTICE_MAX_CLASS];   /* entries per row of class c, multiple of 4 */
+  int64_t base[QDS_LATTICE_MAX_CLASS];  /* offset of class c within a tile: tile * sum(len[<c]) */
   int8_t di[QDS_LATTICE_MAX_CLASS][QDS_LATTICE_MAX_ENTRY];
   int8_t dj[QDS_LATTICE_MAX_CLASS][QDS_LATTICE_MAX_ENTRY];
 } qds_lattice_stencil;

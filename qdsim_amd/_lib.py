@@ -73,7 +73,8 @@ MAX_CLASS, MAX_ENTRY = 9, 52
 class LatticeStencil(ctypes.Structure):
     """Mirror of qds_lattice_stencil."""
     _fields_ = [("order", _i32), ("n_class", _i32), ("nx", _i64), ("ny", _i64), ("cells_x", _i64),
-                ("cells_y", _i64), ("n_slots", _i64), ("len", _i32 * MAX_CLASS), ("base", _i64 * MAX_CLASS),
+                ("cells_y", _i64), ("tile", _i32), ("pad0", _i32), ("tiles_x", _i64), ("tile_slots", _i64),
+                ("n_slots", _i64), ("len", _i32 * MAX_CLASS), ("base", _i64 * MAX_CLASS),
                 ("di", (_c.c_int8 * MAX_ENTRY) * MAX_CLASS), ("dj", (_c.c_int8 * MAX_ENTRY) * MAX_CLASS)]
 
 

@@ -9,15 +9,19 @@
 // group, an index load, two value loads and a dependent 16-byte X gather; on
 // gfx950 its address unit is busy ~95% of the kernel time (rocprofv3 TA_TA_BUSY)
 // at ~45 vector memory instructions per wave.  Here:
-//   * a workgroup owns one tile: one cell row (p lattice rows) times `tic` cells;
-//     the X rows of the tile and a p-point halo are staged in LDS with full-width
-//     16-byte loads (each X row is fetched ~3.6 times per product instead of once
-//     per nonzero, and the re-fetches hit L2);
+//   * a workgroup owns one block of the lattice: one cell row (p lattice rows)
+//     times `tic` cells (m value tiles of the stencil layout).  The X rows of the
+//     block and a p-point halo, and the block's values of both matrices (one
+//     contiguous run per matrix in HBM), are staged in LDS with 16-byte loads
+//     that are all issued before the first LDS write; each X row is fetched ~3.6
+//     times per product instead of once per nonzero, and the re-fetches hit L2;
 //   * there are no column indices: entry k of a class-c row reads the LDS point at
 //     a per-class offset, a wave-uniform scalar;
 //   * each wave task holds rows of one class only, so the trip count is uniform;
-//   * values of one class are stored row after row (len[c] entries each, padded
-//     with zeros to a multiple of 4) and read with 16-byte loads.
+//   * LDS layouts make every wave read contiguous: an LDS row stores the points
+//     of each residue I mod p together, so the rows of a task (stride p on the
+//     lattice) are adjacent, and the value layout puts the rows of a class next to
+//     each other in groups of four entries.
 // Entries are summed in ascending column order with fused multiply-adds, the order
 // of the CSR kernel; the padding and boundary entries add exact zeros, so both
 // kernels give bitwise equal products.
@@ -32,26 +36,31 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
-constexpr int kMaxTasks = 24;            // wave tasks per wave and tile
+constexpr int kMaxTasks = 24;            // wave tasks per wave and block
 constexpr int kBatch = 12;               // 16-byte loads in flight per thread while staging
+constexpr int kVB = 4;                   // value chunks per thread in the common staging path
 constexpr size_t kLdsBudget = 52 * 1024; // LDS per workgroup: 3 workgroups per CU
 constexpr size_t kLdsMax = 64 * 1024;
 
 struct LatticeArgs {
   int64_t nx, ny, cells_x;
-  int64_t n_tiles;
-  int32_t tiles_x;  // tiles per cell row
-  int32_t tic;      // cells per tile along x
+  int64_t n_blocks;     // cells_y * bx_per_row
+  int64_t tiles_x;      // value tiles per cell row
+  int32_t bx_per_row;   // lattice blocks per cell row
+  int32_t tile;         // cells per value tile
+  int32_t m;            // value tiles per lattice block
+  int32_t tic;          // cells per lattice block (m * tile)
   int32_t b, lpr, rpw, nch;  // block width, lanes per row, rows per wave, chunks per class row
-  int32_t lw;       // LDS tile width in points (p * tic + 2p)
-  int32_t ps;       // LDS point stride in elements (= b)
-  int32_t row_chunks, xchunks;  // 16-byte chunks per LDS tile row / of the X tile
-  int32_t vlds;     // first 16-byte chunk of the value tiles in LDS
-  int32_t vchunks;  // 16-byte chunks of one value tile (tic cells of every class)
+  int32_t lw;           // LDS row width in points (p * tic + 2p)
+  int32_t ps;           // LDS point stride in elements (= b)
+  int32_t row_chunks, xchunks;  // 16-byte chunks per LDS row / of the X block
+  int32_t vlds;         // first 16-byte chunk of the value runs in LDS
+  int32_t tile_chunks;  // 16-byte chunks of one value tile
+  int32_t vchunks;      // 16-byte chunks of one matrix's value run (m tiles)
+  int64_t tile_slots;   // values per tile
   int32_t len[QDS_LATTICE_MAX_CLASS];
-  int32_t vpre[QDS_LATTICE_MAX_CLASS];  // first chunk of each class run in a value tile
-  int64_t base[QDS_LATTICE_MAX_CLASS];
-  int16_t off[QDS_LATTICE_MAX_CLASS][QDS_LATTICE_MAX_ENTRY];  // dj * lw + di (LDS points)
+  int32_t base[QDS_LATTICE_MAX_CLASS];  // class offset within a value tile (elements)
+  int16_t off[QDS_LATTICE_MAX_CLASS][QDS_LATTICE_MAX_ENTRY];  // LDS point offset of each entry
   int8_t tasks[kWaves][kMaxTasks];  // task = class * nch + chunk, -1 terminated
 };
 
@@ -82,7 +91,7 @@ __device__ __forceinline__ void st(T* p, const T (&v)[N]) {
   *reinterpret_cast<V*>(p) = t;
 }
 
-// four consecutive values (16-byte aligned for float, two 16-byte loads for double)
+// four consecutive values (one 16-byte LDS read for float, two for double)
 template <typename T>
 __device__ __forceinline__ void ld4(const T* p, T (&v)[4]) {
   if constexpr (sizeof(T) == 4) {
@@ -95,10 +104,11 @@ __device__ __forceinline__ void ld4(const T* p, T (&v)[4]) {
   }
 }
 
-// Workgroups go round-robin to the 8 XCDs; tile g % 8 * (G / 8) + g / 8 makes every
-// XCD sweep a contiguous range of tiles in row-major order, so the halo rows a tile
-// shares with the cell row above were fetched into the same L2 shortly before.
-__device__ __forceinline__ int64_t xcd_tile(int64_t g, int64_t G) { return (g % 8) * (G / 8) + g / 8; }
+// Workgroups go round-robin to the 8 XCDs; block g % 8 * (G / 8) + g / 8 makes every
+// XCD sweep a contiguous range of lattice blocks in row-major order, so the halo
+// rows a block shares with the cell row above were fetched into the same L2
+// shortly before.
+__device__ __forceinline__ int64_t xcd_block(int64_t g, int64_t G) { return (g % 8) * (G / 8) + g / 8; }
 
 template <typename T, int VEC, int P, bool FUSED>
 __global__ void __launch_bounds__(kThreads)
@@ -106,75 +116,114 @@ lattice_spmm_kernel(const LatticeArgs a, const T* __restrict__ va, const T* __re
                     const T* __restrict__ X, int64_t ldx, int chunked, T* __restrict__ Ya,
                     int64_t ldya, T* __restrict__ Yb, int64_t ldyb) {
   constexpr int CE = 16 / sizeof(T);  // elements per 16-byte chunk
+  constexpr int NV = FUSED ? 2 : 1;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* lds = reinterpret_cast<T*>(lds_raw);
   float4* lds4 = reinterpret_cast<float4*>(lds_raw);
-  const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
-  if (tile >= a.n_tiles) return;  // uniform over the workgroup
-  const int64_t cr = tile / a.tiles_x;
-  const int64_t cc0 = (tile - cr * a.tiles_x) * a.tic;
-  const int64_t J0 = P * cr - P, I0 = P * cc0 - P;  // full-lattice origin of the LDS tile
+  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
+  if (blk >= a.n_blocks) return;  // uniform over the workgroup
+  const int64_t cr = blk / a.bx_per_row;
+  const int64_t t0 = (blk - cr * a.bx_per_row) * a.m;  // first value tile of the block
+  const int64_t cc0 = t0 * a.tile;                    // first cell of the block
+  const int64_t J0 = P * cr - P, I0 = P * cc0 - P;    // full-lattice origin of the LDS block
   const int lw = a.lw, ps = a.ps;
-  const int ncell = (int)min((int64_t)a.tic, a.cells_x - cc0);  // cells of this tile with values
+  const int wr = lw / P;  // points per residue class I mod P in an LDS row
+  const int ncell = (int)min((int64_t)a.tic, a.cells_x - cc0);  // cells of this block
+  // value runs: m tiles of each matrix, contiguous in HBM; tiles past the row end
+  // are not loaded (their cells have no rows)
+  const int vvalid = (int)min((int64_t)a.m, a.tiles_x - t0) * a.tile_chunks;
+  const float4* vsrc_a = reinterpret_cast<const float4*>(va + (cr * a.tiles_x + t0) * a.tile_slots);
+  const float4* vsrc_b = reinterpret_cast<const float4*>((FUSED ? vb : va) + (cr * a.tiles_x + t0) * a.tile_slots);
 
-  // ---- stage the tile in LDS: X rows of the tile and its p-point halo (zeros
-  // outside the interior), then the tile's values of A_a and A_b (one contiguous
-  // run per class).  Every thread issues all of its 16-byte loads before the
-  // first LDS write, so one memory latency covers the whole tile.
-  const int xchunks = chunked ? a.xchunks : 0;
-  const int total = xchunks + (FUSED ? 2 : 1) * a.vchunks;
-  for (int e0 = 0; e0 < total; e0 += kThreads * kBatch) {
-    float4 v[kBatch];
-    int dst[kBatch];
+  // ---- stage the block in LDS: X rows of the block and its p-point halo (zeros
+  // outside the interior), then the value runs.  Every thread issues all of its
+  // 16-byte loads before the first LDS write, so one memory latency covers the
+  // whole block.
+  if (chunked && a.row_chunks <= kThreads && NV * a.vchunks <= kThreads * kVB) {
+    // common case: each thread owns one 16-byte column slot of the X rows
+    // (decoded once, then one load per row) and at most kVB value chunks
+    const int cpp = ps / CE;
+    const int e = threadIdx.x;
+    const bool xact = e < a.row_chunks;
+    const int pt = xact ? e / cpp : 0;
+    const int part = e - pt * cpp;
+    const int64_t I = I0 + pt;
+    const bool iin = xact && I >= 1 && I <= a.nx;
+    const T* xsrc = X + (I - 1) * ldx + part * CE;
+    const int xdst = ((pt % P) * wr + pt / P) * cpp + part;
+    float4 xv[3 * P];
 #pragma unroll
-    for (int i = 0; i < kBatch; ++i) {
-      const int e = e0 + i * kThreads + (int)threadIdx.x;
-      v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-      dst[i] = -1;
-      if (e >= total) continue;
-      const float4* src = nullptr;
-      if (e < xchunks) {
-        const int cpp = ps / CE;
-        const int lr = e / a.row_chunks;
-        const int rem = e - lr * a.row_chunks;
-        const int pt = rem / cpp;
-        const int part = rem - pt * cpp;
-        const int64_t J = J0 + lr, I = I0 + pt;
-        if (J >= 1 && J <= a.ny && I >= 1 && I <= a.nx)
-          src = reinterpret_cast<const float4*>(X + ((J - 1) * a.nx + (I - 1)) * ldx + part * CE);
-        dst[i] = e;
-      } else {
-        int f = e - xchunks;
-        const bool second = f >= a.vchunks;
-        if (second) f -= a.vchunks;
-        int c = 0;
-        while (c + 1 < P * P && f >= a.vpre[c + 1]) ++c;
-        const int off = (f - a.vpre[c]) * CE;  // element within the class run
-        if (off < ncell * a.len[c]) {
-          const T* base = (second ? vb : va) + a.base[c] + (cr * a.cells_x + cc0) * a.len[c];
-          src = reinterpret_cast<const float4*>(base + off);
-        }
-        dst[i] = a.vlds + (second ? a.vchunks : 0) + f;
-      }
-      if (src) v[i] = *src;
-    }
-#pragma unroll
-    for (int i = 0; i < kBatch; ++i)
-      if (dst[i] >= 0) lds4[dst[i]] = v[i];
-  }
-  if (!chunked) {
-    const int row_elems = lw * ps;
     for (int lr = 0; lr < 3 * P; ++lr) {
       const int64_t J = J0 + lr;
-      const bool jin = J >= 1 && J <= a.ny;
-      T* dstx = lds + (int64_t)lr * lw * ps;
-      for (int e = threadIdx.x; e < row_elems; e += kThreads) {
-        const int pt = e / ps;
-        const int col = e - pt * ps;
-        const int64_t I = I0 + pt;
-        T x = T(0);
-        if (jin && I >= 1 && I <= a.nx) x = X[((J - 1) * a.nx + (I - 1)) * ldx + col];
-        dstx[e] = x;
+      xv[lr] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (iin && J >= 1 && J <= a.ny) xv[lr] = *reinterpret_cast<const float4*>(xsrc + (J - 1) * a.nx * ldx);
+    }
+    float4 vv[kVB];
+#pragma unroll
+    for (int i = 0; i < kVB; ++i) {
+      const int f2 = i * kThreads + (int)threadIdx.x;
+      vv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      const bool second = f2 >= a.vchunks;
+      const int f = second ? f2 - a.vchunks : f2;
+      if (f2 < NV * a.vchunks && f < vvalid) vv[i] = second ? vsrc_b[f] : vsrc_a[f];
+    }
+    if (xact) {
+#pragma unroll
+      for (int lr = 0; lr < 3 * P; ++lr) lds4[lr * a.row_chunks + xdst] = xv[lr];
+    }
+#pragma unroll
+    for (int i = 0; i < kVB; ++i) {
+      const int f2 = i * kThreads + (int)threadIdx.x;
+      if (f2 < NV * a.vchunks) lds4[a.vlds + f2] = vv[i];
+    }
+  } else {
+    const int xchunks = chunked ? a.xchunks : 0;
+    const int total = xchunks + NV * a.vchunks;
+    for (int e0 = 0; e0 < total; e0 += kThreads * kBatch) {
+      float4 v[kBatch];
+      int dst[kBatch];
+#pragma unroll
+      for (int i = 0; i < kBatch; ++i) {
+        const int e = e0 + i * kThreads + (int)threadIdx.x;
+        v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        dst[i] = -1;
+        if (e >= total) continue;
+        if (e < xchunks) {
+          const int cpp = ps / CE;
+          const int lr = e / a.row_chunks;
+          const int rem = e - lr * a.row_chunks;
+          const int pt = rem / cpp;
+          const int part = rem - pt * cpp;
+          const int64_t J = J0 + lr, I = I0 + pt;
+          if (J >= 1 && J <= a.ny && I >= 1 && I <= a.nx)
+            v[i] = *reinterpret_cast<const float4*>(X + ((J - 1) * a.nx + (I - 1)) * ldx + part * CE);
+          dst[i] = (lr * lw + (pt % P) * wr + pt / P) * cpp + part;
+        } else {
+          const int f2 = e - xchunks;
+          const bool second = f2 >= a.vchunks;
+          const int f = second ? f2 - a.vchunks : f2;
+          if (f < vvalid) v[i] = second ? vsrc_b[f] : vsrc_a[f];
+          dst[i] = a.vlds + f2;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < kBatch; ++i)
+        if (dst[i] >= 0) lds4[dst[i]] = v[i];
+    }
+    if (!chunked) {
+      const int row_elems = lw * ps;
+      for (int lr = 0; lr < 3 * P; ++lr) {
+        const int64_t J = J0 + lr;
+        const bool jin = J >= 1 && J <= a.ny;
+        T* dstx = lds + (int64_t)lr * lw * ps;
+        for (int e = threadIdx.x; e < row_elems; e += kThreads) {
+          const int pt = e / ps;
+          const int col = e - pt * ps;
+          const int64_t I = I0 + pt;
+          T x = T(0);
+          if (jin && I >= 1 && I <= a.nx) x = X[((J - 1) * a.nx + (I - 1)) * ldx + col];
+          dstx[((pt % P) * wr + pt / P) * ps + col] = x;
+        }
       }
     }
   }
@@ -201,18 +250,21 @@ lattice_spmm_kernel(const LatticeArgs a, const T* __restrict__ va, const T* __re
     const int64_t I = P * (cc0 + u) + ci;
     const int64_t J = P * cr + cj;
     const bool ok = lane_ok && u < ncell && I >= 1 && I <= a.nx && J >= 1 && J <= a.ny;
-    // lanes without a row read a valid row of the tile and store nothing
+    // lanes without a row read a valid row of the block and store nothing
     const int uu = u < ncell ? u : ncell - 1;
     const int len = a.len[c];
-    const int vofs = a.vpre[c] * CE + uu * len;
-    const T* xl = lds + (int64_t)((P + cj) * lw + P + ci + P * uu) * ps + c0;
+    const int seg = uu / a.tile;
+    const int64_t vrow = seg * a.tile_slots + a.base[c] + (uu - seg * a.tile) * 4;
+    const int vstep = a.tile * 4;  // next group of four entries
+    const T* xl = lds + (int64_t)((P + cj) * lw + ci * wr + 1 + uu) * ps + c0;
     T acc_a[VEC], acc_b[VEC];
 #pragma unroll
     for (int v = 0; v < VEC; ++v) { acc_a[v] = T(0); acc_b[v] = T(0); }
     for (int k = 0; k < len; k += 4) {
       T av[4], bv[4];
-      ld4<T>(lva + vofs + k, av);
-      if (FUSED) ld4<T>(lvb + vofs + k, bv);
+      const int64_t vk = vrow + (int64_t)(k >> 2) * vstep;
+      ld4<T>(lva + vk, av);
+      if (FUSED) ld4<T>(lvb + vk, bv);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         T x[VEC];
@@ -237,7 +289,7 @@ bool aligned(const void* p, int bytes) { return (reinterpret_cast<uintptr_t>(p) 
 template <typename T, int VEC, int P, bool FUSED>
 int launch(const LatticeArgs& a, size_t lds, const void* va, const void* vb, const void* X,
            int64_t ldx, int chunked, void* Ya, int64_t ldya, void* Yb, int64_t ldyb, hipStream_t st) {
-  const int64_t grid = (a.n_tiles + 7) / 8 * 8;
+  const int64_t grid = (a.n_blocks + 7) / 8 * 8;
   hipLaunchKernelGGL((lattice_spmm_kernel<T, VEC, P, FUSED>), dim3((unsigned)grid), dim3(kThreads), lds,
                      st, a, (const T*)va, (const T*)vb, (const T*)X, ldx, chunked, (T*)Ya, ldya,
                      (T*)Yb, ldyb);
@@ -275,17 +327,25 @@ QDS_API int qds_lattice_spmm(int32_t dtype, const qds_lattice_stencil* s, int32_
   QDS_CHECK_ARG(b >= 1 && ldx >= b && ldy_a >= b && (!vals_b || ldy_b >= b), "qds_lattice_spmm: bad block shape");
   const int p = s->order;
   QDS_CHECK_ARG(p >= 1 && p <= 3 && s->n_class == p * p && s->nx >= 1 && s->ny >= 1 &&
-                    s->cells_x == s->nx / p + 1 && s->cells_y == s->ny / p + 1,
+                    s->cells_x == s->nx / p + 1 && s->cells_y == s->ny / p + 1 && s->tile >= 1 &&
+                    s->tiles_x == (s->cells_x + s->tile - 1) / s->tile,
                 "qds_lattice_spmm: inconsistent stencil description");
   const bool fused = vals_b != nullptr;
   const int es = qds::dtype_size(dtype);
-  int sum_len = 0;
-  for (int c = 0; c < s->n_class; ++c) sum_len += s->len[c];
+  int64_t sum_len = 0;
+  for (int c = 0; c < s->n_class; ++c) {
+    QDS_CHECK_ARG(s->len[c] % 4 == 0 && s->len[c] <= QDS_LATTICE_MAX_ENTRY && s->base[c] == s->tile * sum_len,
+                  "qds_lattice_spmm: inconsistent stencil description");
+    sum_len += s->len[c];
+  }
+  QDS_CHECK_ARG(s->tile_slots == s->tile * sum_len && s->n_slots == s->cells_y * s->tiles_x * s->tile_slots,
+                "qds_lattice_spmm: inconsistent stencil description");
   const size_t nval = fused ? 2 : 1;
-  // blocks wider than one LDS tile of a single cell allows: column slices
-  const size_t v1 = nval * sum_len * es;
-  QDS_CHECK_ARG(v1 + (size_t)9 * p * p * 4 * es <= kLdsMax, "qds_lattice_spmm: stencil too large for LDS");
-  const int bmax = (int)((kLdsMax - v1) / ((size_t)9 * p * p * es)) / 4 * 4;
+  const size_t tile_vbytes = (size_t)s->tile_slots * es;
+  // blocks wider than one LDS block of a single value tile allows: column slices
+  const size_t xrow1 = (size_t)3 * p * (size_t)(p * s->tile + 2 * p) * es;  // X bytes per column, one tile
+  QDS_CHECK_ARG(nval * tile_vbytes + 4 * xrow1 <= kLdsMax, "qds_lattice_spmm: stencil too large for LDS");
+  const int bmax = (int)((kLdsMax - nval * tile_vbytes) / xrow1) / 4 * 4;
   if (b > bmax) {
     for (int j = 0; j < b; j += bmax) {
       const int w = std::min(bmax, b - j);
@@ -296,8 +356,8 @@ QDS_API int qds_lattice_spmm(int32_t dtype, const qds_lattice_stencil* s, int32_
     }
     return QDS_OK;
   }
-  // lanes per row: VEC columns each, the widest VEC dividing b whose loads and
-  // stores stay aligned
+  // lanes per row: VEC columns each, the widest VEC dividing b whose stores stay
+  // aligned
   auto vec_ok = [&](int vec) {
     const int bytes = vec * es;
     if (b % vec) return false;
@@ -314,43 +374,49 @@ QDS_API int qds_lattice_spmm(int32_t dtype, const qds_lattice_stencil* s, int32_
   a.nx = s->nx;
   a.ny = s->ny;
   a.cells_x = s->cells_x;
+  a.tiles_x = s->tiles_x;
+  a.tile = s->tile;
+  a.tile_slots = s->tile_slots;
   a.b = b;
   a.lpr = (b + vec - 1) / vec;
   QDS_CHECK_ARG(a.lpr <= 64, "qds_lattice_spmm: block width %d too large", b);
   a.rpw = 64 / a.lpr;
   a.ps = b;
   const size_t point_bytes = (size_t)b * es;
-  // cells per tile: the most that fit the LDS budget, a whole number of wave
-  // chunks when more than one chunk fits
+  // value tiles per lattice block: enough cells to fill a wave's rows, within the
+  // LDS budget
   auto x_bytes = [&](int tic) { return ((size_t)3 * p * (size_t)(p * tic + 2 * p) * point_bytes + 15) / 16 * 16; };
-  auto tile_bytes = [&](int tic) { return x_bytes(tic) + nval * (size_t)tic * sum_len * es; };
-  int tic = 1;
-  while (tic < 4 * a.rpw && tile_bytes(tic + 1) <= kLdsBudget) ++tic;
-  if (tic > a.rpw) tic = tic / a.rpw * a.rpw;
-  QDS_CHECK_ARG(tile_bytes(tic) <= kLdsMax, "qds_lattice_spmm: block width %d too large for the LDS tile", b);
+  auto block_bytes = [&](int m) { return x_bytes(m * s->tile) + nval * m * tile_vbytes; };
+  // smallest m whose block gives every row slot of a wave a cell and every wave a task
+  int m = (a.rpw + s->tile - 1) / s->tile;
+  while (p * p * ((m * s->tile + a.rpw - 1) / a.rpw) < kWaves) ++m;
+  while (m > 1 && block_bytes(m) > kLdsBudget) --m;
+  QDS_CHECK_ARG(block_bytes(m) <= kLdsMax, "qds_lattice_spmm: block width %d too large for LDS", b);
+  const int tic = m * s->tile;
+  a.m = m;
   a.tic = tic;
   a.nch = (tic + a.rpw - 1) / a.rpw;
   a.lw = p * tic + 2 * p;
+  a.bx_per_row = (int32_t)((s->tiles_x + m - 1) / m);
+  a.n_blocks = (int64_t)a.bx_per_row * s->cells_y;
   const int chunked = (point_bytes % 16 == 0) && aligned(X, 16) && ((ldx * es) % 16 == 0);
   a.row_chunks = chunked ? (int)(a.lw * point_bytes / 16) : 0;
   a.xchunks = 3 * p * a.row_chunks;
   a.vlds = (int)(x_bytes(tic) / 16);
-  a.vchunks = (int)((size_t)tic * sum_len * es / 16);
-  for (int c = 0, acc = 0; c < s->n_class; ++c) {
-    a.vpre[c] = acc;
-    acc += (int)((size_t)tic * s->len[c] * es / 16);
-  }
-  a.tiles_x = (int32_t)((s->cells_x + tic - 1) / tic);
-  a.n_tiles = (int64_t)a.tiles_x * s->cells_y;
+  a.tile_chunks = (int)(tile_vbytes / 16);
+  a.vchunks = m * a.tile_chunks;
   for (int c = 0; c < s->n_class; ++c) {
-    QDS_CHECK_ARG(s->len[c] % 4 == 0 && s->len[c] <= QDS_LATTICE_MAX_ENTRY && s->base[c] % 4 == 0,
-                  "qds_lattice_spmm: inconsistent stencil description");
     a.len[c] = s->len[c];
-    a.base[c] = s->base[c];
+    a.base[c] = (int32_t)s->base[c];
     for (int k = 0; k < s->len[c]; ++k) {
       QDS_CHECK_ARG(std::abs((int)s->di[c][k]) <= p && std::abs((int)s->dj[c][k]) <= p,
                     "qds_lattice_spmm: stencil offset beyond the halo");
-      a.off[c][k] = (int16_t)(s->dj[c][k] * a.lw + s->di[c][k]);
+      // LDS rows hold the points of each residue I mod p together (p runs of
+      // tic + 2 points): a row of class column ci at run position q has its
+      // neighbour (di, dj) in run (ci + di) mod p at position q + floor((ci + di) / p)
+      const int ci = c % p, t = ci + s->di[c][k];
+      const int run = ((t % p) + p) % p, shift = (t - run) / p;
+      a.off[c][k] = (int16_t)(s->dj[c][k] * a.lw + (run - ci) * (tic + 2) + shift);
     }
   }
   // longest-task-first assignment of the (class, chunk) tasks to the waves
@@ -366,11 +432,11 @@ QDS_API int qds_lattice_spmm(int32_t dtype, const qds_lattice_stencil* s, int32_
     int w = 0;
     for (int v = 1; v < kWaves; ++v)
       if (load[v] < load[w]) w = v;
-    QDS_CHECK_ARG(cnt[w] < kMaxTasks, "qds_lattice_spmm: too many tasks per tile");
+    QDS_CHECK_ARG(cnt[w] < kMaxTasks, "qds_lattice_spmm: too many tasks per block");
     a.tasks[w][cnt[w]++] = (int8_t)t;
     load[w] += a.len[t / a.nch];
   }
-  const size_t lds = tile_bytes(tic);
+  const size_t lds = block_bytes(m);
   hipStream_t st = qds_stream(stream);
   if (dtype == qds::F32) {
     if (vec == 4) return dispatch<float, 4>(p, fused, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);

@@ -301,7 +301,9 @@ QDS_API int qds_lattice_stencil_build(int32_t order, int64_t nx, int64_t ny, con
   st->ny = ny;
   st->cells_x = nx / p + 1;
   st->cells_y = ny / p + 1;
-  const int64_t cells = st->cells_x * st->cells_y;
+  // tiles of cells along x: one LDS tile of the product kernel holds whole tiles
+  st->tile = p == 1 ? 20 : p == 2 ? 10 : 5;
+  st->tiles_x = (st->cells_x + st->tile - 1) / st->tile;
   // slot index of each bitmap position within its class (ascending column order:
   // dJ major, dI minor, since |dI| <= p < nx)
   std::vector<int> kidx(used.size(), -1);
@@ -321,22 +323,25 @@ QDS_API int qds_lattice_stencil_build(int32_t order, int64_t nx, int64_t ny, con
     QDS_CHECK_ARG(padded <= QDS_LATTICE_MAX_ENTRY, "qds_lattice_stencil_build: stencil too long");
     st->len[c] = padded;
     st->base[c] = base;
-    base += cells * padded;
+    base += (int64_t)st->tile * padded;
   }
-  st->n_slots = base;
+  st->tile_slots = base;
+  st->n_slots = st->cells_y * st->tiles_x * st->tile_slots;
   if (!slot_src) return QDS_OK;
-  parallel_rows(base, [&](int64_t lo, int64_t hi) {
+  parallel_rows(st->n_slots, [&](int64_t lo, int64_t hi) {
     for (int64_t t = lo; t < hi; ++t) slot_src[t] = -1;
   });
   parallel_rows(n, [&](int64_t lo, int64_t hi) {
     for (int64_t r = lo; r < hi; ++r) {
       const int64_t i = r % nx, j = r / nx, I = i + 1, J = j + 1;
       const int c = static_cast<int>((J % p) * p + I % p);
-      const int64_t row0 = st->base[c] + ((J / p) * st->cells_x + I / p) * st->len[c];
+      const int64_t X = I / p, Y = J / p;
+      const int64_t row0 = (Y * st->tiles_x + X / st->tile) * st->tile_slots + st->base[c] + (X % st->tile) * 4;
       for (int32_t k = indptr[r]; k < indptr[r + 1]; ++k) {
         const int64_t col = indices[k];
         const int64_t di = col % nx - i, dj = col / nx - j;
-        slot_src[row0 + kidx[(static_cast<size_t>(c) * w + (dj + p)) * w + (di + p)]] = k;
+        const int e = kidx[(static_cast<size_t>(c) * w + (dj + p)) * w + (di + p)];
+        slot_src[row0 + (e / 4) * st->tile * 4 + e % 4] = k;
       }
     }
   });

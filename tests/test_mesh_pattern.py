@@ -157,13 +157,15 @@ def test_lattice_stencil_reproduces_csr(p, ncx, ncy):
         I, J = i + 1, j + 1
         c = (J % p) * p + I % p
         L = st.len[c]
-        base = st.base[c] + ((J // p) * st.cells_x + I // p) * L
+        cx, cy = I // p, J // p
+        base = (cy * st.tiles_x + cx // st.tile) * st.tile_slots + st.base[c] + (cx % st.tile) * 4
         for k in range(L):
+            slot = base + (k // 4) * st.tile * 4 + k % 4
             I2, J2 = I + st.di[c][k], J + st.dj[c][k]
             if 1 <= I2 <= nx and 1 <= J2 <= ny:
-                Y[r] += sv[base + k] * X[(J2 - 1) * nx + I2 - 1]
+                Y[r] += sv[slot] * X[(J2 - 1) * nx + I2 - 1]
             else:
-                assert sv[base + k] == 0.0
+                assert sv[slot] == 0.0
     assert np.array_equal(Y, A @ X)
 
 
