@@ -104,6 +104,31 @@ __device__ __forceinline__ void ld4(const T* p, T (&v)[4]) {
   }
 }
 
+// Entries of one task row, LEN known at compile time (fully unrolled; summation
+// order as in the generic loop: entry by entry).
+template <typename T, int VEC, bool FUSED, int LEN>
+__device__ __forceinline__ void task_entries(const int16_t* __restrict__ off, const T* xl, int ps,
+                                             const T* lva, const T* lvb, int64_t vrow, int vstep,
+                                             T (&acc_a)[VEC], T (&acc_b)[VEC]) {
+#pragma unroll
+  for (int k = 0; k < LEN; k += 4) {
+    T av[4], bv[4];
+    const int64_t vk = vrow + (int64_t)(k >> 2) * vstep;
+    ld4<T>(lva + vk, av);
+    if (FUSED) ld4<T>(lvb + vk, bv);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      T x[VEC];
+      ld<T, VEC>(xl + (int)off[k + q] * ps, x);
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+        acc_a[v] = fma(av[q], x[v], acc_a[v]);
+        if (FUSED) acc_b[v] = fma(bv[q], x[v], acc_b[v]);
+      }
+    }
+  }
+}
+
 // Workgroups go round-robin to the 8 XCDs; block g % 8 * (G / 8) + g / 8 makes every
 // XCD sweep a contiguous range of lattice blocks in row-major order, so the halo
 // rows a block shares with the cell row above were fetched into the same L2
@@ -260,21 +285,31 @@ lattice_spmm_kernel(const LatticeArgs a, const T* __restrict__ va, const T* __re
     T acc_a[VEC], acc_b[VEC];
 #pragma unroll
     for (int v = 0; v < VEC; ++v) { acc_a[v] = T(0); acc_b[v] = T(0); }
-    for (int k = 0; k < len; k += 4) {
-      T av[4], bv[4];
-      const int64_t vk = vrow + (int64_t)(k >> 2) * vstep;
-      ld4<T>(lva + vk, av);
-      if (FUSED) ld4<T>(lvb + vk, bv);
+    // the stencil lengths of P1-P3 meshes get fully unrolled bodies: their LDS
+    // offsets are then loaded once per task (scalar loads) instead of inside a loop
+    switch (len) {
+      case 8: task_entries<T, VEC, FUSED, 8>(a.off[c], xl, ps, lva, lvb, vrow, vstep, acc_a, acc_b); break;
+      case 12: task_entries<T, VEC, FUSED, 12>(a.off[c], xl, ps, lva, lvb, vrow, vstep, acc_a, acc_b); break;
+      case 16: task_entries<T, VEC, FUSED, 16>(a.off[c], xl, ps, lva, lvb, vrow, vstep, acc_a, acc_b); break;
+      case 20: task_entries<T, VEC, FUSED, 20>(a.off[c], xl, ps, lva, lvb, vrow, vstep, acc_a, acc_b); break;
+      case 40: task_entries<T, VEC, FUSED, 40>(a.off[c], xl, ps, lva, lvb, vrow, vstep, acc_a, acc_b); break;
+      default:
+        for (int k = 0; k < len; k += 4) {
+          T av[4], bv[4];
+          const int64_t vk = vrow + (int64_t)(k >> 2) * vstep;
+          ld4<T>(lva + vk, av);
+          if (FUSED) ld4<T>(lvb + vk, bv);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        T x[VEC];
-        ld<T, VEC>(xl + (int)a.off[c][k + q] * ps, x);
+          for (int q = 0; q < 4; ++q) {
+            T x[VEC];
+            ld<T, VEC>(xl + (int)a.off[c][k + q] * ps, x);
 #pragma unroll
-        for (int v = 0; v < VEC; ++v) {
-          acc_a[v] = fma(av[q], x[v], acc_a[v]);
-          if (FUSED) acc_b[v] = fma(bv[q], x[v], acc_b[v]);
+            for (int v = 0; v < VEC; ++v) {
+              acc_a[v] = fma(av[q], x[v], acc_a[v]);
+              if (FUSED) acc_b[v] = fma(bv[q], x[v], acc_b[v]);
+            }
+          }
         }
-      }
     }
     if (ok) {
       const int64_t row = (J - 1) * a.nx + (I - 1);
