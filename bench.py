@@ -93,9 +93,10 @@ def _spmm_roofline(prob, b, iters=20):
 
 
 def _measured_traffic(cells):
-    """HBM bytes per SpMM launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    runs (copy-calibrated), for the benchmark size only; None otherwise."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "spmm_p3_512_hbm_traffic.json")
+    """HBM bytes per operator-product launch (lattice form, FP32, b = 24) from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-calibrated), for the
+    benchmark size only; None otherwise."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "lattice_spmm_p3_512_hbm_traffic.json")
     if cells != CELLS or not os.path.exists(path):
         return None
     with open(path) as f:
