@@ -131,3 +131,22 @@ def reference_mesh(nx, ny, Lx, Ly, order, orphan_nodes=False):
     if order == 2:
         return mesh_cpp_p2(nx, ny, Lx, Ly, orphan_nodes)
     return mesh_p3(nx, ny, Lx, Ly)
+
+
+def mesh_file_text(x, y, elements, quadratic=None, cubic=None, order=1):
+    """backend/src/mesh.cpp:258-289 (Mesh::save): header ``nodes elements order``,
+    ``x y`` per node, three vertex ids per element, then the quadratic block (order
+    >= 2) and the cubic block (order 3), one element per line.  Doubles go through
+    a default std::ostream: printf's %g (precision 6)."""
+    out = [f"{len(x)} {len(elements)} {order}"]
+    for a, b in zip(x, y):
+        out.append("%g %g" % (a, b))
+    for e in elements:
+        out.append(" ".join(str(int(v)) for v in e[:3]))
+    if order >= 2:
+        for e in quadratic:
+            out.append(" ".join(str(int(v)) for v in e))
+    if order == 3:
+        for e in cubic:
+            out.append(" ".join(str(int(v)) for v in e))
+    return "\n".join(out) + "\n"

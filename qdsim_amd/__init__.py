@@ -6,10 +6,12 @@ HIP kernels for gfx950 (libqdsim_hip.so, C-ABI in include/qdsim_hip.h).
 """
 from . import _lib  # noqa: F401  (fails loudly when the native library is missing)
 from .mesh import SimpleMesh, create_simple_mesh
+from .mesh_io import FileMesh, load_csr, load_mesh, load_pattern, save_csr, save_mesh, save_pattern
 
 __all__ = ["SimpleMesh", "create_simple_mesh", "FEMSolver", "EigenSolver", "SelfConsistentSolver",
            "FixedOpenSystemSolver", "PoissonSolver", "SchrodingerProblem", "create_fem_solver",
-           "create_eigen_solver", "lifetimes_fs", "FEInterpolator", "create_interpolator"]
+           "create_eigen_solver", "lifetimes_fs", "FEInterpolator", "create_interpolator",
+           "FileMesh", "load_mesh", "save_mesh", "load_csr", "save_csr", "load_pattern", "save_pattern"]
 
 
 def __getattr__(name):

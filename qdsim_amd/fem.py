@@ -29,7 +29,7 @@ NM = 1e-9
 class FEMSpace:
     """Mesh, pattern and quadrature of one structured FE mesh on the GPU."""
 
-    def __init__(self, mesh, degree=None):
+    def __init__(self, mesh, degree=None, pattern=None):
         self.mesh = mesh
         self.order = int(getattr(mesh, "element_order", 1))
         conn = np.ascontiguousarray(mesh.get_elements(), dtype=np.int32)
@@ -40,7 +40,14 @@ class FEMSpace:
         x, y = mesh.get_nodes()
         self.x_host, self.y_host = x, y
         t0 = time.perf_counter()
-        pat = kernels.pattern_build(conn, self.n)
+        # pattern: built on the host, or a cached one (mesh_io.load_pattern)
+        if pattern is None:
+            pat = kernels.pattern_build(conn, self.n)
+        else:
+            pat = pattern
+            if (pat["indptr"].size != self.n + 1 or pat["seg_src"].size != self.n_elem * self.npe * self.npe
+                    or int(pat["indptr"][-1]) != pat["indices"].size):
+                raise ValueError("cached pattern does not belong to this mesh")
         self.pattern_time = time.perf_counter() - t0
         self.nnz = pat["nnz"]
         self.indptr_host = pat["indptr"]

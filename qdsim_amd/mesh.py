@@ -211,6 +211,17 @@ class SimpleMesh:
         """Row-major lattice position of every DOF (rows = y)."""
         return self.lattice_y * self.lattice_shape[1] + self.lattice_x
 
+    def save(self, filename):
+        """Write the mesh in the reference text format (backend/src/mesh.cpp:258-289)."""
+        from .mesh_io import save_mesh
+        save_mesh(self, filename)
+
+    @staticmethod
+    def load(filename):
+        """Read a reference mesh file (backend/src/mesh.cpp:305-339) as a FileMesh."""
+        from .mesh_io import load_mesh
+        return load_mesh(filename)
+
     def get_mesh_info(self):
         return {
             'nx': self.nx,

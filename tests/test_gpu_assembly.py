@@ -82,3 +82,34 @@ def test_assembly_deterministic():
     H1 = H1.clone(); M1 = M1.clone()
     H2, M2, _ = prob.assemble()
     assert torch.equal(H1, H2) and torch.equal(M1, M2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_assembly_from_mesh_file_and_cached_pattern(tmp_path, order):
+    """FEMSolver on a mesh read back from the reference text format assembles the
+    matrices of the original mesh (coordinates carry six significant digits), and a
+    cached pattern gives bitwise the same matrices as a freshly built one."""
+    import numpy as np
+    from qdsim_amd import kernels
+    from qdsim_amd.fem import FEMSolver, FEMSpace
+    from qdsim_amd.mesh import SimpleMesh
+    from qdsim_amd.mesh_io import load_mesh, load_pattern, save_pattern
+    m = SimpleMesh(9, 8, 60e-9, 45e-9, order)
+    fn = tmp_path / "mesh.txt"
+    m.save(fn)
+    fm = load_mesh(fn)
+    K0 = FEMSolver(m).assemble_stiffness_matrix()
+    K1 = FEMSolver(fm).assemble_stiffness_matrix()
+    assert K0.shape == K1.shape and (abs(K0) > 0).nnz == (abs(K1) > 0).nnz
+    assert abs(K1 - K0).max() <= 1e-4 * abs(K0).max()
+    M0 = FEMSolver(m).assemble_mass_matrix()
+    M1 = FEMSolver(fm).assemble_mass_matrix()
+    assert abs(M1 - M0).max() <= 1e-4 * abs(M0).max()
+    pat = kernels.pattern_build(m.get_elements(), m.num_nodes)
+    save_pattern(tmp_path / "pat.npz", pat, m.num_nodes, m.get_elements().shape[1])
+    cached = load_pattern(tmp_path / "pat.npz", n_dof=m.num_nodes)
+    s0, s1 = FEMSpace(m), FEMSpace(m, pattern=cached)
+    v0 = s0.assemble(c=None)[0]
+    v1 = s1.assemble(c=None)[0]
+    assert np.array_equal(v0.cpu().numpy(), v1.cpu().numpy())
