@@ -17,10 +17,11 @@ def _stencil(d, we, wn, u, nx, ny):
     return out.reshape(nx * ny, -1)
 
 
-def _mg(n_cells=64, b=8, dtype=torch.float64, **kw):
+def _mg(n_cells=64, b=8, dtype=torch.float64, ny_cells=None, **kw):
     from qdsim_amd.lattice import Lattice, LatticeMG
     from qdsim_amd.mesh import SimpleMesh
-    m = SimpleMesh(n_cells + 1, n_cells + 1, 50e-9, 50e-9, 1)
+    ny_cells = n_cells if ny_cells is None else ny_cells
+    m = SimpleMesh(n_cells + 1, ny_cells + 1, 50e-9, 50e-9 * ny_cells / n_cells, 1)
     lat = Lattice(m)
     E = m.num_elements
     dev = torch.device("cuda")
@@ -110,11 +111,13 @@ def _np_vcycle(levels, f, nu1, nu2, ncoarse, omega):
     return cyc(0, f)
 
 
-@pytest.mark.parametrize("b,dtype", [(8, torch.float64), (24, torch.float32), (6, torch.float64), (6, torch.float32)])
-def test_vcycle_matches_numpy_schedule(b, dtype):
+@pytest.mark.parametrize("b,dtype,cells", [(8, torch.float64, (128, 128)), (24, torch.float32, (128, 128)),
+                                           (6, torch.float64, (128, 128)), (6, torch.float32, (128, 128)),
+                                           (24, torch.float32, (288, 160)), (24, torch.float64, (160, 288))])
+def test_vcycle_matches_numpy_schedule(b, dtype, cells):
     """One V-cycle (multi-launch fine levels + single-workgroup coarse levels)
-    against the same cycle in numpy FP64."""
-    lat, mg = _mg(128, b=b, dtype=dtype)
+    against the same cycle in numpy FP64, on square and rectangular lattices."""
+    lat, mg = _mg(cells[0], b=b, dtype=dtype, ny_cells=cells[1])
     levels = [{"nx": lv["nx"], "ny": lv["ny"], "d": lv["d"].double().cpu().numpy(),
                "we": lv["we"].double().cpu().numpy(), "wn": lv["wn"].double().cpu().numpy()}
               for lv in mg.levels_data]
