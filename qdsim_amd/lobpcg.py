@@ -338,21 +338,22 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
 
 
 def solve_mixed(op64, k, b, precond32=None, precond64=None, tol=1e-6, tol32=1e-3, maxiter=400,
-                seed=0, verbose=False, drop32=None, stall32=8, refresh32=1, X0=None):
+                seed=0, verbose=False, drop32=None, stall32=8, refresh32=1, X0=None, b64=None):
     """FP32 LOBPCG to tol32 (or stagnation), then FP64 LOBPCG to tol.  X0: optional
     initial block for the FP32 phase (default: seeded random vectors).
 
     The FP32 phase recomputes the images HX, MX by SpMM every refresh32 iterations
     (1: every iteration): FP32 rounding of the updated images otherwise makes the
     reported residual up to 30x smaller than the true one.  tol32 = 1e-3 sits just
-    above the FP32 floor of the quantum-dot problems (‖H‖/|lambda| ~ 1e3)."""
+    above the FP32 floor of the quantum-dot problems (‖H‖/|lambda| ~ 1e3).  b64: block
+    of the FP64 phase (default b), started from the lowest b64 FP32 Ritz vectors."""
     op32 = op64.astype(torch.float32)
     t0 = time.perf_counter()
     r32 = lobpcg(op32, k, b, precond32, X0=X0, tol=tol32, maxiter=maxiter, seed=seed, verbose=verbose,
                  stall_iters=stall32, drop_tol=drop32, refresh=refresh32)
     t1 = time.perf_counter()
     X0 = r32.X.to(torch.float64)
-    r64 = lobpcg(op64, k, b, precond64, X0=X0, tol=tol, maxiter=maxiter, seed=seed, verbose=verbose)
+    r64 = lobpcg(op64, k, b64 or b, precond64, X0=X0, tol=tol, maxiter=maxiter, seed=seed, verbose=verbose)
     t2 = time.perf_counter()
     r64.timings.update({"fp32_s": t1 - t0, "fp64_s": t2 - t1, "fp32_iters": r32.iterations,
                         "fp64_iters": r64.iterations, "fp32_restarts": r32.timings["restarts"],

@@ -175,7 +175,7 @@ class SchrodingerProblem:
 
     # -- solve ------------------------------------------------------------------
     def solve(self, k, b=None, mode="mixed", tol=1e-6, tol32=3e-3, maxiter=400, seed=0, verbose=False,
-              nested_opts=None, **mixed_opts):
+              nested_opts=None, b64=None, **mixed_opts):
         """Lowest k eigenpairs.  Returns (eigenvalues eV (np), full-numbering
         eigenvectors (n, k) float64 device tensor, M_nm-normalised, info)."""
         if self.H is None:
@@ -186,13 +186,15 @@ class SchrodingerProblem:
         op64 = self.operator(torch.float64)
         if mode == "mixed":
             pre32 = self.preconditioner(b, torch.float32)
+            b64 = b if b64 is None else int(b64)
+            pre32_64 = pre32 if b64 == b else self.preconditioner(b64, torch.float32)
             torch.cuda.synchronize()
             t_prep = time.perf_counter() - t0
 
-            def pre64(R, W, _pre=pre32, _buf={}):
+            def pre64(R, W, _pre=pre32_64, _buf={}):
                 if "r" not in _buf:
-                    _buf["r"] = torch.empty((R.shape[0], b), dtype=torch.float32, device=R.device)
-                    _buf["w"] = torch.empty((R.shape[0], b), dtype=torch.float32, device=R.device)
+                    _buf["r"] = torch.empty((R.shape[0], b64), dtype=torch.float32, device=R.device)
+                    _buf["w"] = torch.empty((R.shape[0], b64), dtype=torch.float32, device=R.device)
                 _buf["r"].copy_(R)
                 _pre(_buf["r"], _buf["w"])
                 W.copy_(_buf["w"])
@@ -203,7 +205,7 @@ class SchrodingerProblem:
             torch.cuda.synchronize()
             t_nested = time.perf_counter() - t_n
             res = solve_mixed(op64, k, b, pre32, pre64, tol=tol, tol32=tol32, maxiter=maxiter, seed=seed,
-                              verbose=verbose, X0=X0, **mixed_opts)
+                              verbose=verbose, X0=X0, b64=b64, **mixed_opts)
         elif mode == "fp64":
             pre = self.preconditioner(b, torch.float64, cycle=dict(nu_pre=2, nu_post=2, coarse_sweeps=40))
             torch.cuda.synchronize()
