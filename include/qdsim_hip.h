@@ -247,6 +247,14 @@ typedef struct {
 int qds_mg_vcycle(int32_t dtype, int32_t n_levels, const qds_mg_level* levels, int32_t b,
                   int32_t nu_pre, int32_t nu_post, int32_t coarse_sweeps, double omega,
                   void* stream);
+/* The same V-cycle with an FP32 hierarchy (dtype 1) on FP64 level-0 blocks
+ * (io_dtype 2): levels[0].f / .u are FP64 (n, b) blocks, every other level FP32.
+ * Lets an FP64 eigensolver phase use the FP32 preconditioner without converting
+ * its residual and correction blocks.  Needs nu_pre = nu_post = 1 and a level 0
+ * that is not inside the single-workgroup coarse cycle (else QDS_EINVAL).      */
+int qds_mg_vcycle_io(int32_t dtype, int32_t io_dtype, int32_t n_levels, const qds_mg_level* levels,
+                     int32_t b, int32_t nu_pre, int32_t nu_post, int32_t coarse_sweeps, double omega,
+                     void* stream);
 int qds_mg_apply(int32_t dtype, const qds_mg_level* level, int32_t b, const void* u, void* out,
                  void* stream);
 

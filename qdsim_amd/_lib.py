@@ -48,6 +48,7 @@ _SIGNATURES = {
                            _c.c_int),
     "qds_coldot": ([_i32, _i64, _i32, _p, _i64, _p, _i64, _i32, _p, _p, _i64, _p], _c.c_int),
     "qds_mg_vcycle": ([_i32, _i32, _p, _i32, _i32, _i32, _i32, _d, _p], _c.c_int),
+    "qds_mg_vcycle_io": ([_i32, _i32, _i32, _p, _i32, _i32, _i32, _i32, _d, _p], _c.c_int),
     "qds_mg_apply": ([_i32, _p, _i32, _p, _p, _p], _c.c_int),
     "qds_interpolate_q": ([_i32, _i32, _i64, _p, _p, _p, _p, _p], _c.c_int),
     "qds_state_density_q": ([_i32, _i32, _i64, _p, _p, _p, _p, _p, _i64, _i32, _p, _p], _c.c_int),

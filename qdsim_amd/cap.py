@@ -157,6 +157,18 @@ def _cap_lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, seed=0, 
         kernels.block_residual(HX[c], MX[c], lam.to(torch.complex128).contiguous(), R)
         rn = kernels.coldot(R, R, conj=True).real
         mn = kernels.coldot(MX[c], MX[c], conj=True).real
+        # the next search directions and their Gram do not depend on the
+        # convergence test: queue them before the host reads the residuals
+        if precond is not None:
+            precond(_rv(R), _rv(W))
+        else:
+            W.copy_(R)
+        op.apply(W, HW, MW)
+        S = [X[c], W] + ([P[c]] if have_p else [])
+        HS = [HX[c], HW] + ([HP[c]] if have_p else [])
+        MS = [MX[c], MW] + ([MP[c]] if have_p else [])
+        m = len(S) * b
+        G = kernels.gram_blocks([_rv(x) for x in S], [_rv(x) for x in HS], [_rv(x) for x in MS])
         scale = lam[:k].abs().max().clamp_min(1e-300)
         res = rn.clamp_min(0).sqrt() / (torch.maximum(lam.abs(), scale) * mn.clamp_min(1e-300).sqrt())
         worst = float(res[:k].max().item())
@@ -171,16 +183,6 @@ def _cap_lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, seed=0, 
                 best, best_it = worst, it
             elif it - best_it >= stall_iters:
                 break
-        if precond is not None:
-            precond(_rv(R), _rv(W))
-        else:
-            W.copy_(R)
-        op.apply(W, HW, MW)
-        S = [X[c], W] + ([P[c]] if have_p else [])
-        HS = [HX[c], HW] + ([HP[c]] if have_p else [])
-        MS = [MX[c], MW] + ([MP[c]] if have_p else [])
-        m = len(S) * b
-        G = kernels.gram_blocks([_rv(x) for x in S], [_rv(x) for x in HS], [_rv(x) for x in MS])
         lam, C = _rr(G, m, b, drop_tol, target)
         CP = C.clone()
         CP[:b] = 0

@@ -18,6 +18,8 @@
 // smoothing is launched by qds_mg_vcycle; all kernels run on the caller's stream.
 #include "qds_common.h"
 
+#include <type_traits>
+
 namespace {
 
 constexpr int kBlock = 256;
@@ -60,6 +62,28 @@ struct Vec {
     *reinterpret_cast<typename V<T, VEC>::t*>(p) = x;
   }
 };
+
+// VEC values of a block stored in type S (the level-0 right-hand side and result
+// may be FP64 blocks of an FP64 eigensolver phase while the hierarchy is FP32)
+template <typename T, int VEC, typename S>
+__device__ __forceinline__ void load_as(Vec<T, VEC>& v, const S* p) {
+  if constexpr (std::is_same<T, S>::value) {
+    v.load(p);
+  } else {
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) v.v[i] = static_cast<T>(p[i]);
+  }
+}
+
+template <typename T, int VEC, typename S>
+__device__ __forceinline__ void store_as(const Vec<T, VEC>& v, S* p) {
+  if constexpr (std::is_same<T, S>::value) {
+    v.store(p);
+  } else {
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) p[i] = static_cast<S>(v.v[i]);
+  }
+}
 
 // (A u)[p, c0:c0+VEC]
 template <typename T, int VEC>
@@ -245,34 +269,34 @@ prolong_add_kernel(Level<T> F, Level<T> C, int b) {
 // This is the same arithmetic as the sweep/restrict/prolong/sweep sequence.
 
 // sum over the lattice neighbours q of p of w_pq * src_q / d_q
-template <typename T, int VEC>
+template <typename T, int VEC, typename S>
 __device__ __forceinline__ Vec<T, VEC> nbr_scaled(const Level<T>& L, uint32_t ld, uint32_t p, uint32_t ix,
-                                                  uint32_t iy, uint32_t c0, const T* __restrict__ src) {
+                                                  uint32_t iy, uint32_t c0, const S* __restrict__ src) {
   Vec<T, VEC> s, x;
 #pragma unroll
   for (int i = 0; i < VEC; ++i) s.v[i] = T(0);
   const uint32_t nx = (uint32_t)L.nx;
   if (ix + 1 < nx) {
     const T w = L.we[p] / L.d[p + 1];
-    x.load(src + (p + 1) * ld + c0);
+    load_as<T, VEC, S>(x, src + (p + 1) * ld + c0);
 #pragma unroll
     for (int i = 0; i < VEC; ++i) s.v[i] += w * x.v[i];
   }
   if (ix > 0) {
     const T w = L.we[p - 1] / L.d[p - 1];
-    x.load(src + (p - 1) * ld + c0);
+    load_as<T, VEC, S>(x, src + (p - 1) * ld + c0);
 #pragma unroll
     for (int i = 0; i < VEC; ++i) s.v[i] += w * x.v[i];
   }
   if (iy + 1 < (uint32_t)L.ny) {
     const T w = L.wn[p] / L.d[p + nx];
-    x.load(src + (p + nx) * ld + c0);
+    load_as<T, VEC, S>(x, src + (p + nx) * ld + c0);
 #pragma unroll
     for (int i = 0; i < VEC; ++i) s.v[i] += w * x.v[i];
   }
   if (iy > 0) {
     const T w = L.wn[p - nx] / L.d[p - nx];
-    x.load(src + (p - nx) * ld + c0);
+    load_as<T, VEC, S>(x, src + (p - nx) * ld + c0);
 #pragma unroll
     for (int i = 0; i < VEC; ++i) s.v[i] += w * x.v[i];
   }
@@ -305,9 +329,12 @@ __device__ __forceinline__ Vec<T, VEC> prolong_at(const Level<T>& C, uint32_t ix
   return acc;
 }
 
-template <typename T, int VEC>
+template <typename T, int VEC, typename S = T>
 __device__ __forceinline__ void down_items(const Level<T>& F, const Level<T>& C, int b, T omega, uint32_t t0,
-                                           uint32_t step) {
+                                           uint32_t step, const S* __restrict__ ff = nullptr) {
+  if constexpr (std::is_same<T, S>::value) {
+    if (!ff) ff = F.f;
+  }
   const uint32_t nv = (uint32_t)(b / VEC);
   const uint32_t total = (uint32_t)(C.nx * C.ny) * nv;
   const uint32_t ldf = (uint32_t)F.ldf;
@@ -326,8 +353,8 @@ __device__ __forceinline__ void down_items(const Level<T>& F, const Level<T>& C,
         const T w = (dx == 0 ? T(1) : T(0.5)) * (dy == 0 ? T(1) : T(0.5));
         const uint32_t p = y * (uint32_t)F.nx + x;
         Vec<T, VEC> fv;
-        fv.load(F.f + p * ldf + c0);
-        const Vec<T, VEC> ns = nbr_scaled<T, VEC>(F, ldf, p, x, y, c0, F.f);
+        load_as<T, VEC, S>(fv, ff + p * ldf + c0);
+        const Vec<T, VEC> ns = nbr_scaled<T, VEC, S>(F, ldf, p, x, y, c0, ff);
 #pragma unroll
         for (int i = 0; i < VEC; ++i) acc.v[i] += w * ((T(1) - omega) * fv.v[i] + omega * ns.v[i]);
       }
@@ -336,9 +363,13 @@ __device__ __forceinline__ void down_items(const Level<T>& F, const Level<T>& C,
   }
 }
 
-template <typename T, int VEC>
+template <typename T, int VEC, typename S = T>
 __device__ __forceinline__ void up_items(const Level<T>& F, const Level<T>& C, int b, T omega, uint32_t t0,
-                                         uint32_t step) {
+                                         uint32_t step, const S* __restrict__ ff = nullptr, S* __restrict__ uu = nullptr) {
+  if constexpr (std::is_same<T, S>::value) {
+    if (!ff) ff = F.f;
+    if (!uu) uu = F.u;
+  }
   const uint32_t nv = (uint32_t)(b / VEC);
   const uint32_t total = (uint32_t)(F.nx * F.ny) * nv;
   const uint32_t ldf = (uint32_t)F.ldf;
@@ -350,14 +381,14 @@ __device__ __forceinline__ void up_items(const Level<T>& F, const Level<T>& C, i
     // u2 at p and its neighbours, then one Jacobi step at p
     auto u2 = [&](uint32_t q, uint32_t qx, uint32_t qy) {
       Vec<T, VEC> fv, e = prolong_at<T, VEC>(C, qx, qy, c0);
-      fv.load(F.f + q * ldf + c0);
+      load_as<T, VEC, S>(fv, ff + q * ldf + c0);
       const T sc = omega / F.d[q];
 #pragma unroll
       for (int i = 0; i < VEC; ++i) e.v[i] += sc * fv.v[i];
       return e;
     };
     Vec<T, VEC> up = u2(p, ix, iy), fp, s;
-    fp.load(F.f + p * ldf + c0);
+    load_as<T, VEC, S>(fp, ff + p * ldf + c0);
     const T dp = F.d[p];
 #pragma unroll
     for (int i = 0; i < VEC; ++i) s.v[i] = fp.v[i] - dp * up.v[i];
@@ -388,20 +419,20 @@ __device__ __forceinline__ void up_items(const Level<T>& F, const Level<T>& C, i
     const T sc = omega / dp;
 #pragma unroll
     for (int i = 0; i < VEC; ++i) up.v[i] += sc * s.v[i];
-    up.store(F.u + p * (uint32_t)F.ldu + c0);
+    store_as<T, VEC, S>(up, uu + p * (uint32_t)F.ldu + c0);
   }
 }
 
-template <typename T, int VEC>
+template <typename T, int VEC, typename S>
 __global__ void __launch_bounds__(kBlock)
-mg_down_kernel(Level<T> F, Level<T> C, int b, T omega) {
-  down_items<T, VEC>(F, C, b, omega, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+mg_down_kernel(Level<T> F, Level<T> C, int b, T omega, const S* __restrict__ ff) {
+  down_items<T, VEC, S>(F, C, b, omega, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, ff);
 }
 
-template <typename T, int VEC>
+template <typename T, int VEC, typename S>
 __global__ void __launch_bounds__(kBlock)
-mg_up_kernel(Level<T> F, Level<T> C, int b, T omega) {
-  up_items<T, VEC>(F, C, b, omega, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+mg_up_kernel(Level<T> F, Level<T> C, int b, T omega, const S* __restrict__ ff, S* __restrict__ uu) {
+  up_items<T, VEC, S>(F, C, b, omega, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, ff, uu);
 }
 
 // Levels handled by the single-workgroup coarse cycle (level l0 and below).
@@ -515,18 +546,31 @@ int smooth(const Level<T>& L, int b, int sweeps, T omega, bool zero_first, hipSt
   return QDS_OK;
 }
 
-template <typename T, int VEC>
-int vcycle(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, hipStream_t st) {
+// S: element type of level 0's right-hand side and result (T, or FP64 with an
+// FP32 hierarchy; the mixed form needs the fused V(1,1) with level 0 outside the
+// single-workgroup coarse cycle)
+template <typename T, int VEC, typename S = T>
+int vcycle(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, hipStream_t st,
+           const S* f0 = nullptr, S* u0 = nullptr) {
   // first level small enough for the single-workgroup coarse cycle
   int lc = nl;
   for (int l = 0; l < nl; ++l)
     if (L[l].nx * L[l].ny * (b / VEC) <= kCoarseItems && nl - l <= kMaxCoarseLevels) { lc = l; break; }
   const bool fused = nu1 == 1 && nu2 == 1;
+  constexpr bool mixed = !std::is_same<T, S>::value;
+  if (mixed)
+    QDS_CHECK_ARG(fused && lc > 0 && nl > 1 && f0 && u0,
+                  "qds_mg_vcycle_io: a level-0 block type other than the hierarchy's needs the fused "
+                  "V(1,1) cycle and a level 0 outside the single-workgroup coarse cycle");
   for (int l = 0; l < lc && l < nl - 1; ++l) {
     const int64_t work = L[l + 1].nx * L[l + 1].ny * (b / VEC);
     if (fused) {
-      hipLaunchKernelGGL((mg_down_kernel<T, VEC>), dim3(qds_blocks(work, kBlock, 65536)), dim3(kBlock), 0, st,
-                         L[l], L[l + 1], b, omega);
+      if (l == 0 && mixed)
+        hipLaunchKernelGGL((mg_down_kernel<T, VEC, S>), dim3(qds_blocks(work, kBlock, 65536)), dim3(kBlock), 0,
+                           st, L[l], L[l + 1], b, omega, f0);
+      else
+        hipLaunchKernelGGL((mg_down_kernel<T, VEC, T>), dim3(qds_blocks(work, kBlock, 65536)), dim3(kBlock), 0,
+                           st, L[l], L[l + 1], b, omega, (const T*)L[l].f);
       QDS_LAUNCH_CHECK();
       continue;
     }
@@ -550,8 +594,12 @@ int vcycle(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, h
   for (int l = top; l >= 0; --l) {
     const int64_t work = L[l].nx * L[l].ny * (b / VEC);
     if (fused) {
-      hipLaunchKernelGGL((mg_up_kernel<T, VEC>), dim3(qds_blocks(work, kBlock, 65536)), dim3(kBlock), 0, st,
-                         L[l], L[l + 1], b, omega);
+      if (l == 0 && mixed)
+        hipLaunchKernelGGL((mg_up_kernel<T, VEC, S>), dim3(qds_blocks(work, kBlock, 65536)), dim3(kBlock), 0, st,
+                           L[l], L[l + 1], b, omega, f0, u0);
+      else
+        hipLaunchKernelGGL((mg_up_kernel<T, VEC, T>), dim3(qds_blocks(work, kBlock, 65536)), dim3(kBlock), 0, st,
+                           L[l], L[l + 1], b, omega, (const T*)L[l].f, L[l].u);
       QDS_LAUNCH_CHECK();
       continue;
     }
@@ -564,8 +612,9 @@ int vcycle(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, h
   return QDS_OK;
 }
 
-template <typename T>
-int vcycle_any(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, hipStream_t st) {
+template <typename T, typename S = T>
+int vcycle_any(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, hipStream_t st,
+               const S* f0 = nullptr, S* u0 = nullptr) {
   constexpr int VW = 16 / sizeof(T);
   bool vec = b % VW == 0;
   for (int l = 0; l < nl && vec; ++l) {
@@ -573,8 +622,8 @@ int vcycle_any(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omeg
           reinterpret_cast<uintptr_t>(L[l].u) % 16 == 0 && reinterpret_cast<uintptr_t>(L[l].f) % 16 == 0 &&
           reinterpret_cast<uintptr_t>(L[l].t) % 16 == 0;
   }
-  if (vec) return vcycle<T, VW>(nl, L, b, nu1, nu2, ncoarse, omega, st);
-  return vcycle<T, 1>(nl, L, b, nu1, nu2, ncoarse, omega, st);
+  if (vec) return vcycle<T, VW, S>(nl, L, b, nu1, nu2, ncoarse, omega, st, f0, u0);
+  return vcycle<T, 1, S>(nl, L, b, nu1, nu2, ncoarse, omega, st, f0, u0);
 }
 
 }  // namespace
@@ -632,6 +681,41 @@ QDS_API int qds_mg_vcycle(int32_t dtype, int32_t n_levels, const qds_mg_level* l
   }
   QDS_CHECK_ARG(false, "qds_mg_vcycle: dtype must be F32 or F64");
   return QDS_EINVAL;
+}
+
+// V-cycle with an FP32 hierarchy (dtype F32) on an FP64 right-hand side and result
+// (io_dtype F64): levels[0].f / .u point at FP64 blocks (strides ldf / ldu in
+// elements), every other level is FP32.  Equal dtypes behave as qds_mg_vcycle.
+QDS_API int qds_mg_vcycle_io(int32_t dtype, int32_t io_dtype, int32_t n_levels, const qds_mg_level* levels,
+                             int32_t b, int32_t nu_pre, int32_t nu_post, int32_t coarse_sweeps, double omega,
+                             void* stream) {
+  if (dtype == io_dtype)
+    return qds_mg_vcycle(dtype, n_levels, levels, b, nu_pre, nu_post, coarse_sweeps, omega, stream);
+  QDS_CHECK_ARG(dtype == qds::F32 && io_dtype == qds::F64, "qds_mg_vcycle_io: only an FP32 hierarchy with FP64 level-0 blocks");
+  QDS_CHECK_ARG(levels && n_levels >= 2 && n_levels <= 32 && b >= 1, "qds_mg_vcycle_io: bad arguments");
+  for (int l = 0; l + 1 < n_levels; ++l)
+    QDS_CHECK_ARG(levels[l].nx == 2 * levels[l + 1].nx + 1 && levels[l].ny == 2 * levels[l + 1].ny + 1,
+                  "qds_mg_vcycle_io: level %d is not a vertex-centred coarsening of level %d", l + 1, l);
+  for (int l = 0; l < n_levels; ++l)
+    QDS_CHECK_ARG(levels[l].d && levels[l].we && levels[l].wn && levels[l].u && levels[l].f && levels[l].t,
+                  "qds_mg_vcycle_io: null pointer on level %d", l);
+  for (int l = 0; l < n_levels; ++l)
+    QDS_CHECK_ARG(levels[l].nx * levels[l].ny * (int64_t)(levels[l].ldu ? levels[l].ldu : b) < (int64_t(1) << 31) &&
+                  levels[l].nx * levels[l].ny * (int64_t)(levels[l].ldf ? levels[l].ldf : b) < (int64_t(1) << 31),
+                  "qds_mg_vcycle_io: level %d exceeds 32-bit indexing", l);
+  Level<float> L[32];
+  for (int l = 0; l < n_levels; ++l)
+    L[l] = Level<float>{levels[l].nx, levels[l].ny, (const float*)levels[l].d, (const float*)levels[l].we,
+                        (const float*)levels[l].wn, (float*)levels[l].u, (float*)levels[l].f, (float*)levels[l].t,
+                        levels[l].ldu ? levels[l].ldu : b, levels[l].ldf ? levels[l].ldf : b};
+  // level 0's FP32 pointers are never dereferenced in the mixed cycle; its f / u are
+  // the FP64 blocks; its t is the hierarchy's own scratch
+  const double* f0 = (const double*)levels[0].f;
+  double* u0 = (double*)levels[0].u;
+  L[0].f = L[0].t;
+  L[0].u = L[0].t;
+  return vcycle_any<float, double>(n_levels, L, b, nu_pre, nu_post, coarse_sweeps, (float)omega,
+                                   qds_stream(stream), f0, u0);
 }
 
 // out = A u on one level (tests of the lattice operator).

@@ -270,6 +270,11 @@ def mg_vcycle(dtype_code, levels_struct, n_levels, b, nu_pre, nu_post, coarse, o
                             nu_pre, nu_post, coarse, float(omega), stream_ptr()), "qds_mg_vcycle")
 
 
+def mg_vcycle_io(dtype_code, io_code, levels_struct, n_levels, b, nu_pre, nu_post, coarse, omega):
+    check(lib.qds_mg_vcycle_io(dtype_code, io_code, n_levels, ctypes.cast(levels_struct, ctypes.c_void_p), b,
+                               nu_pre, nu_post, coarse, float(omega), stream_ptr()), "qds_mg_vcycle_io")
+
+
 def mg_apply(dtype_code, level_struct, b, u, out):
     check(lib.qds_mg_apply(dtype_code, ctypes.cast(ctypes.byref(level_struct), ctypes.c_void_p), b,
                            ptr(u), ptr(out), stream_ptr()), "qds_mg_apply")

@@ -127,6 +127,17 @@ class LatticeMG:
             s.u, s.f, s.t = lv["u"].data_ptr(), lv["f"].data_ptr(), lv["t"].data_ptr()
             s.ldu = s.ldf = 0
 
+    @property
+    def takes_fp64(self):
+        """An FP32 hierarchy whose fused V(1,1) keeps level 0 out of the
+        single-workgroup coarse cycle (level 0 above 24576 work items of 16 bytes
+        or 4 bytes per lane) can run on FP64 blocks directly."""
+        if self.dtype != torch.float32 or self.nu_pre != 1 or self.nu_post != 1 or self.n_levels < 2:
+            return False
+        lv = self.levels_data[0]
+        vec = 4 if self.b % 4 == 0 else 1
+        return lv["nx"] * lv["ny"] * (self.b // vec) > 24576
+
     def _interior(self, WE, WN, Mterm, NX, NY, dev):
         d = (WE[1:-1, :-1] + WE[1:-1, 1:] + WN[:-1, 1:-1] + WN[1:, 1:-1] + Mterm[1:-1, 1:-1])
         we = WE[1:-1, 1:].clone()
@@ -158,8 +169,22 @@ class LatticeMG:
         return WEc, WNc, Mc, NXc, NYc
 
     def __call__(self, R, W):
-        """W = V-cycle(R); R dense (n, b), W may be a column slice (row stride >= b)."""
-        assert R.dtype == self.dtype and W.dtype == self.dtype
+        """W = V-cycle(R); R dense (n, b), W may be a column slice (row stride >= b).
+        An FP32 hierarchy also takes FP64 R and W (qds_mg_vcycle_io) when its cycle
+        allows it (see ``takes_fp64``)."""
+        assert R.dtype == W.dtype
+        if R.dtype != self.dtype:
+            assert self.dtype == torch.float32 and R.dtype == torch.float64 and self.takes_fp64
+            s0 = self._structs[0]
+            f_old, u_old, ldf_old, ldu_old = s0.f, s0.u, s0.ldf, s0.ldu
+            s0.f, s0.ldf = R.data_ptr(), R.stride(0)
+            s0.u, s0.ldu = W.data_ptr(), W.stride(0)
+            try:
+                kernels.mg_vcycle_io(_lib.F32, _lib.F64, self._structs, self.n_levels, self.b, self.nu_pre,
+                                     self.nu_post, self.coarse_sweeps, self.omega)
+            finally:
+                s0.f, s0.u, s0.ldf, s0.ldu = f_old, u_old, ldf_old, ldu_old
+            return
         s0 = self._structs[0]
         s0.f = R.data_ptr()
         s0.ldf = R.stride(0)

@@ -269,6 +269,21 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
     (colscale,) = stage_cs.put([np.ones(b)])
     for it in range(1, maxiter + 1):
         rn, mn = kernels.residual_norms(HX[c], MX[c], lam, R, colscale)
+        # the search directions of this iteration do not depend on its convergence
+        # test (locking acts from the next residual on), so the preconditioner, the
+        # product and the Gram are queued before the host reads the norms: one
+        # host synchronisation per iteration instead of two (a converged last
+        # iteration computes them in vain)
+        if precond is not None:
+            precond(R, W)
+        else:
+            W.copy_(R)
+        op.apply(W, HW, MW)
+        S = [X[c], W] + ([P[c]] if have_p else [])
+        HS = [HX[c], HW] + ([HP[c]] if have_p else [])
+        MS = [MX[c], MW] + ([MP[c]] if have_p else [])
+        m = len(S) * b
+        G = kernels.gram_blocks(S, HS, MS, sym=True)
         norms = torch.cat([rn, mn]).cpu().numpy()
         rn_h, mn_h = norms[:b], norms[b:]
         # residual relative to the energy scale of the wanted window, so states with
@@ -298,16 +313,6 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
                     break
             elif it - min_it >= 4 * stall_iters:
                 break
-        if precond is not None:
-            precond(R, W)
-        else:
-            W.copy_(R)
-        op.apply(W, HW, MW)
-        S = [X[c], W] + ([P[c]] if have_p else [])
-        HS = [HX[c], HW] + ([HP[c]] if have_p else [])
-        MS = [MX[c], MW] + ([MP[c]] if have_p else [])
-        m = len(S) * b
-        G = kernels.gram_blocks(S, HS, MS, sym=True)
         theta, C_h, dropped = _rayleigh_ritz(G, m, b, drop_tol)
         drops.append(dropped)
         CP_h = C_h.copy()

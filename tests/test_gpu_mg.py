@@ -131,3 +131,22 @@ def test_vcycle_matches_numpy_schedule(b, dtype, cells):
     tol = 1e-11 if dtype == torch.float64 else 2e-5
     err = np.abs(w.double().cpu().numpy() - ref).max() / np.abs(ref).max()
     assert err < tol, err
+
+
+@pytest.mark.parametrize("cells,b", [((128, 128), 24), ((288, 160), 8)])
+def test_vcycle_fp64_blocks_on_fp32_hierarchy(cells, b):
+    """qds_mg_vcycle_io: an FP32 hierarchy applied to FP64 R, W gives bitwise the
+    result of converting R to FP32, cycling, and converting W back (the loads and
+    stores convert with the same rounding)."""
+    lat, mg = _mg(cells[0], b=b, dtype=torch.float32, ny_cells=cells[1], nu_pre=1, nu_post=1)
+    assert mg.takes_fp64
+    n = mg.levels_data[0]["nx"] * mg.levels_data[0]["ny"]
+    g = torch.Generator(device="cuda").manual_seed(3)
+    R = torch.randn((n, b), dtype=torch.float64, device="cuda", generator=g)
+    W = torch.full_like(R, float("nan"))
+    mg(R, W)
+    R32 = R.float()
+    W32 = torch.empty_like(R32)
+    mg(R32, W32)
+    torch.cuda.synchronize()
+    assert torch.equal(W, W32.double())
