@@ -122,34 +122,49 @@ def _svqb_rr(G, m, nb, drop_tol):
 
 def _chol_rr(G, m, nb, drop_tol):
     """Rayleigh-Ritz through a Cholesky factor of the scaled M-Gram when it is well
-    conditioned (reciprocal condition estimate above drop_tol): one Cholesky, two
-    triangular solves and one symmetric eigensolve, half the host cost of the two
-    eigensolves of SVQB (the eigensolves dominate at these sizes).  Returns None
-    when the basis is too close to dependent; the caller then uses SVQB.
-    Columns that vanished (locked residuals) are left out and get zero weight."""
-    dev = G.device
+    conditioned (reciprocal condition estimate above drop_tol): one Cholesky, its
+    triangular inverse, three small products and one symmetric eigensolve, about
+    half the host cost of the two eigensolves of SVQB.  Returns None when the basis
+    is too close to dependent; the caller then uses SVQB.  Columns that vanished
+    (locked residuals) are left out and get zero weight.  This runs on the host in
+    every iteration while the GPU waits, so it calls LAPACK directly and avoids
+    fancy indexing when no column vanished."""
     g = G.detach().cpu().numpy()
-    GH = 0.5 * (g[:, :m] + g[:, :m].T)
-    GM = 0.5 * (g[:, m:2 * m] + g[:, m:2 * m].T)
+    GH = g[:, :m]
+    GM = g[:, m:2 * m]
     dg = np.diag(GM).copy()
-    idx = np.nonzero(dg > 1e-30 * dg.max())[0]
-    if len(idx) < nb:
-        return None
-    d = 1.0 / np.sqrt(dg[idx])
-    GMs = GM[np.ix_(idx, idx)] * d[:, None] * d[None, :]
-    GHs = GH[np.ix_(idx, idx)] * d[:, None] * d[None, :]
+    keep = dg > 1e-30 * dg.max()
+    if keep.all():
+        idx = None
+        d = 1.0 / np.sqrt(dg)
+    else:
+        idx = np.nonzero(keep)[0]
+        if len(idx) < nb:
+            return None
+        d = 1.0 / np.sqrt(dg[idx])
+        GH = GH.take(idx, 0).take(idx, 1)
+        GM = GM.take(idx, 0).take(idx, 1)
+    dd = d[:, None] * d[None, :]
+    GMs = (GM + GM.T) * (0.5 * dd)
+    GHs = (GH + GH.T) * (0.5 * dd)
     Lc, info = sla_lapack.dpotrf(GMs, lower=1, clean=1)
     if info != 0:
         return None
     rcond, info = sla_lapack.dpocon(Lc, float(np.abs(GMs).sum(0).max()), uplo="L")
     if info != 0 or rcond < drop_tol:
         return None
-    T = sla.solve_triangular(Lc, GHs, lower=True, check_finite=False)
-    T = sla.solve_triangular(Lc, T.T, lower=True, check_finite=False)
+    Li, info = sla_lapack.dtrtri(Lc, lower=1)
+    if info != 0:
+        return None
+    T = Li @ GHs @ Li.T
     T = 0.5 * (T + T.T)
-    theta, Y = np.linalg.eigh(T)   # full divide-and-conquer: faster than a subset solve at this size
+    theta, Y, info = sla_lapack.dsyevd(T, compute_v=1, lower=1)
+    if info != 0:
+        return None
     theta, Y = theta[:nb], Y[:, :nb]
-    Cs = sla.solve_triangular(Lc, Y, lower=True, trans="T", check_finite=False) * d[:, None]
+    Cs = (Li.T @ Y) * d[:, None]
+    if idx is None:
+        return theta, Cs, 0
     C = np.zeros((m, nb))
     C[idx] = Cs
     return theta, C, 0
