@@ -122,9 +122,9 @@ def _svqb_rr(G, m, nb, drop_tol):
 
 def _chol_rr(G, m, nb, drop_tol):
     """Rayleigh-Ritz through a Cholesky factor of the scaled M-Gram when it is well
-    conditioned (reciprocal condition estimate above drop_tol): one Cholesky, its
-    triangular inverse, three small products and one symmetric eigensolve, about
-    half the host cost of the two eigensolves of SVQB.  Returns None when the basis
+    conditioned (reciprocal condition estimate above drop_tol): one Cholesky, three
+    triangular solves and one symmetric eigensolve, about half the host cost of the
+    two eigensolves of SVQB.  Returns None when the basis
     is too close to dependent; the caller then uses SVQB.  Columns that vanished
     (locked residuals) are left out and get zero weight.  This runs on the host in
     every iteration while the GPU waits, so it calls LAPACK directly and avoids
@@ -153,16 +153,21 @@ def _chol_rr(G, m, nb, drop_tol):
     rcond, info = sla_lapack.dpocon(Lc, float(np.abs(GMs).sum(0).max()), uplo="L")
     if info != 0 or rcond < drop_tol:
         return None
-    Li, info = sla_lapack.dtrtri(Lc, lower=1)
-    if info != 0:
+    # triangular solves, not an explicit inverse: the factor may be conditioned
+    # down to drop_tol, where the inverse would cost digits of the Ritz vectors
+    T, info = sla_lapack.dtrtrs(Lc, GHs, lower=1)
+    T, info2 = sla_lapack.dtrtrs(Lc, np.ascontiguousarray(T.T), lower=1)
+    if info != 0 or info2 != 0:
         return None
-    T = Li @ GHs @ Li.T
     T = 0.5 * (T + T.T)
     theta, Y, info = sla_lapack.dsyevd(T, compute_v=1, lower=1)
     if info != 0:
         return None
     theta, Y = theta[:nb], Y[:, :nb]
-    Cs = (Li.T @ Y) * d[:, None]
+    Cs, info = sla_lapack.dtrtrs(Lc, Y, lower=1, trans=1)
+    if info != 0:
+        return None
+    Cs = Cs * d[:, None]
     if idx is None:
         return theta, Cs, 0
     C = np.zeros((m, nb))

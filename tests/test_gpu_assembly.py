@@ -110,6 +110,7 @@ def test_assembly_from_mesh_file_and_cached_pattern(tmp_path, order):
     save_pattern(tmp_path / "pat.npz", pat, m.num_nodes, m.get_elements().shape[1])
     cached = load_pattern(tmp_path / "pat.npz", n_dof=m.num_nodes)
     s0, s1 = FEMSpace(m), FEMSpace(m, pattern=cached)
-    v0 = s0.assemble(c=None)[0]
-    v1 = s1.assemble(c=None)[0]
+    c = torch.ones(s0.n_elem, dtype=torch.float64, device="cuda")
+    v0 = s0.assemble(c=c)[0]
+    v1 = s1.assemble(c=c)[0]
     assert np.array_equal(v0.cpu().numpy(), v1.cpu().numpy())
