@@ -435,6 +435,251 @@ mg_up_kernel(Level<T> F, Level<T> C, int b, T omega, const S* __restrict__ ff, S
   up_items<T, VEC, S>(F, C, b, omega, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, ff, uu);
 }
 
+// ---- LDS-tiled fused passes for large levels ---------------------------------
+// The grid-stride passes above re-read f, d, we, wn of every neighbour from global
+// memory (the down pass ~45 16-byte loads and ~36 scalar loads per coarse item,
+// the up pass ~27 per fine item): on the level-0 lattice of the 2.36M-DOF problem
+// they ran at 1.1-1.7 TB/s, bound by the address unit.  The tiled passes stage a
+// small tile with its halo in LDS -- all b columns of a point, loaded as
+// consecutive 16-byte chunks so every wave reads whole contiguous row segments --
+// and then compute every item from LDS.  Same arithmetic as down_items / up_items.
+constexpr int kDX = 4, kDY = 8;            // coarse tile of the down pass
+constexpr int kDFX = 2 * kDX + 3, kDFY = 2 * kDY + 3;  // its fine f tile (with halo)
+constexpr int kUX = 8, kUY = 8;            // fine tile of the up pass
+constexpr int kUFX = kUX + 2, kUFY = kUY + 2;          // its halo tile
+constexpr int kUCX = kUX / 2 + 2, kUCY = kUY / 2 + 2;  // its coarse e tile
+
+// rows of 16-byte chunks: tile points (i, j) of a (w x h) window starting at
+// (x0, y0) of a (nx x ny) lattice, b columns each; zero outside the lattice
+template <typename T, typename S>
+__device__ __forceinline__ void stage_window(T* dst, const S* __restrict__ src, int64_t ld, int nx, int ny,
+                                             int x0, int y0, int w, int h, int b) {
+  constexpr int CE = 16 / sizeof(T);
+  const int cpp = b / CE;
+  const int total = w * h * cpp;
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    const int pt = e / cpp;
+    const int part = e - pt * cpp;
+    const int i = pt % w, j = pt / w;
+    const int x = x0 + i, y = y0 + j;
+    Vec<T, CE> v;
+#pragma unroll
+    for (int q = 0; q < CE; ++q) v.v[q] = T(0);
+    if (x >= 0 && x < nx && y >= 0 && y < ny) {
+      const S* p = src + ((int64_t)y * nx + x) * ld + part * CE;
+      if constexpr (std::is_same<T, S>::value) {
+        v.load(p);
+      } else {
+        // FP64 block into an FP32 tile: two 16-byte loads per 16-byte chunk
+        Vec<S, 2> a, c;
+        a.load(p);
+        c.load(p + 2);
+        v.v[0] = static_cast<T>(a.v[0]);
+        v.v[1] = static_cast<T>(a.v[1]);
+        v.v[2] = static_cast<T>(c.v[0]);
+        v.v[3] = static_cast<T>(c.v[1]);
+      }
+    }
+    v.store(dst + (int64_t)pt * b + part * CE);
+  }
+}
+
+// down: coarse f on a kDX x kDY coarse tile
+template <typename T, int VEC, typename S>
+__global__ void __launch_bounds__(256)
+mg_down_tiled(Level<T> F, Level<T> C, int b, T omega, const S* __restrict__ ff) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T* fs = reinterpret_cast<T*>(lds_raw);                  // kDFX * kDFY points x b
+  T* cinv = fs + kDFX * kDFY * b;                          // 1 / d
+  T* cwe = cinv + kDFX * kDFY;
+  T* cwn = cwe + kDFX * kDFY;
+  const int cnx = (int)C.nx, cny = (int)C.ny, fnx = (int)F.nx, fny = (int)F.ny;
+  const int ntx = (cnx + kDX - 1) / kDX;
+  const int tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
+  const int cx0 = tx * kDX, cy0 = ty * kDY;
+  const int fx0 = 2 * cx0 - 1, fy0 = 2 * cy0 - 1;   // fine point of tile entry (0, 0)
+  for (int e = threadIdx.x; e < kDFX * kDFY; e += blockDim.x) {
+    const int x = fx0 + e % kDFX, y = fy0 + e / kDFX;
+    T inv = T(0), we = T(0), wn = T(0);
+    if (x >= 0 && x < fnx && y >= 0 && y < fny) {
+      const uint32_t p = (uint32_t)y * (uint32_t)fnx + (uint32_t)x;
+      inv = T(1) / F.d[p];
+      we = F.we[p];
+      wn = F.wn[p];
+    }
+    cinv[e] = inv;
+    cwe[e] = we;
+    cwn[e] = wn;
+  }
+  stage_window<T, S>(fs, ff, F.ldf, fnx, fny, fx0, fy0, kDFX, kDFY, b);
+  __syncthreads();
+  // r = (1 - omega) f + omega sum_q w_pq f_q / d_q on the inner (kDFX-2) x (kDFY-2)
+  // points, kept in registers until every thread has read its neighbours
+  const int nv = b / VEC;
+  constexpr int kIn = (kDFX - 2) * (kDFY - 2);
+  constexpr int kPer = (kIn * (32 / VEC) + 255) / 256;   // items per thread for b <= 32
+  Vec<T, VEC> rr[kPer];
+  int rq[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int it = threadIdx.x + 256 * k;
+    rq[k] = -1;
+    if (it < kIn * nv) {
+      const int pt = it / nv;
+      const int c0 = (it - pt * nv) * VEC;
+      const int i = 1 + pt % (kDFX - 2), j = 1 + pt / (kDFX - 2);
+      const int q = j * kDFX + i;
+      Vec<T, VEC> fc, fe, fw, fn, fsou;
+      fc.load(fs + q * b + c0);
+      fe.load(fs + (q + 1) * b + c0);
+      fw.load(fs + (q - 1) * b + c0);
+      fn.load(fs + (q + kDFX) * b + c0);
+      fsou.load(fs + (q - kDFX) * b + c0);
+      const T ae = cwe[q] * cinv[q + 1], aw = cwe[q - 1] * cinv[q - 1];
+      const T an = cwn[q] * cinv[q + kDFX], as = cwn[q - kDFX] * cinv[q - kDFX];
+#pragma unroll
+      for (int v = 0; v < VEC; ++v)
+        rr[k].v[v] = (T(1) - omega) * fc.v[v] + omega * (ae * fe.v[v] + aw * fw.v[v] + an * fn.v[v] + as * fsou.v[v]);
+      rq[k] = q * b + c0;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kPer; ++k)
+    if (rq[k] >= 0) rr[k].store(fs + rq[k]);
+  __syncthreads();
+  for (int it = threadIdx.x; it < kDX * kDY * nv; it += blockDim.x) {
+    const int pc = it / nv;
+    const int c0 = (it - pc * nv) * VEC;
+    const int ci = pc % kDX, cj = pc / kDX;
+    const int jx = cx0 + ci, jy = cy0 + cj;
+    if (jx >= cnx || jy >= cny) continue;
+    // fine centre (2 jx + 1, 2 jy + 1) is tile entry (2 ci + 2, 2 cj + 2)
+    Vec<T, VEC> acc;
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) acc.v[v] = T(0);
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) {
+        const T w = (dx == 0 ? T(1) : T(0.5)) * (dy == 0 ? T(1) : T(0.5));
+        Vec<T, VEC> r;
+        r.load(fs + ((2 * cj + 2 + dy) * kDFX + 2 * ci + 2 + dx) * b + c0);
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) acc.v[v] += w * r.v[v];
+      }
+    }
+    acc.store(C.f + ((uint32_t)jy * (uint32_t)cnx + (uint32_t)jx) * (uint32_t)C.ldf + c0);
+  }
+}
+
+// up: fine u on a kUX x kUY fine tile
+template <typename T, int VEC, typename S>
+__global__ void __launch_bounds__(256)
+mg_up_tiled(Level<T> F, Level<T> C, int b, T omega, const S* __restrict__ ff, S* __restrict__ uu) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T* fs = reinterpret_cast<T*>(lds_raw);                  // kUFX * kUFY points x b
+  T* us = fs + kUFX * kUFY * b;                            // u2 on the halo tile
+  T* es = us + kUFX * kUFY * b;                            // coarse e, kUCX * kUCY points x b
+  T* cd = es + kUCX * kUCY * b;
+  T* cinv = cd + kUFX * kUFY;
+  T* cwe = cinv + kUFX * kUFY;
+  T* cwn = cwe + kUFX * kUFY;
+  const int fnx = (int)F.nx, fny = (int)F.ny, cnx = (int)C.nx, cny = (int)C.ny;
+  const int ntx = (fnx + kUX - 1) / kUX;
+  const int tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
+  const int fx0 = tx * kUX, fy0 = ty * kUY;              // tile origin (even)
+  const int hx0 = fx0 - 1, hy0 = fy0 - 1;                // halo origin
+  const int ex0 = fx0 / 2 - 1, ey0 = fy0 / 2 - 1;        // coarse window origin
+  for (int e = threadIdx.x; e < kUFX * kUFY; e += blockDim.x) {
+    const int x = hx0 + e % kUFX, y = hy0 + e / kUFX;
+    T d = T(0), inv = T(0), we = T(0), wn = T(0);
+    if (x >= 0 && x < fnx && y >= 0 && y < fny) {
+      const uint32_t p = (uint32_t)y * (uint32_t)fnx + (uint32_t)x;
+      d = F.d[p];
+      inv = T(1) / d;
+      we = F.we[p];
+      wn = F.wn[p];
+    }
+    cd[e] = d;
+    cinv[e] = inv;
+    cwe[e] = we;
+    cwn[e] = wn;
+  }
+  stage_window<T, S>(fs, ff, F.ldf, fnx, fny, hx0, hy0, kUFX, kUFY, b);
+  stage_window<T, T>(es, C.u, C.ldu, cnx, cny, ex0, ey0, kUCX, kUCY, b);
+  __syncthreads();
+  const int nv = b / VEC;
+  // u2 = P e + omega f / d on the halo tile (0 outside the lattice)
+  for (int it = threadIdx.x; it < kUFX * kUFY * nv; it += blockDim.x) {
+    const int e = it / nv;
+    const int c0 = (it - e * nv) * VEC;
+    const int x = hx0 + e % kUFX, y = hy0 + e / kUFX;
+    Vec<T, VEC> acc;
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) acc.v[v] = T(0);
+    if (x >= 0 && x < fnx && y >= 0 && y < fny) {
+      int xs0, ys0, nxp, nyp;
+      T wx, wy;
+      if (x & 1) { xs0 = (x - 1) / 2 - ex0; nxp = 1; wx = T(1); }
+      else { xs0 = x / 2 - 1 - ex0; nxp = 2; wx = T(0.5); }
+      if (y & 1) { ys0 = (y - 1) / 2 - ey0; nyp = 1; wy = T(1); }
+      else { ys0 = y / 2 - 1 - ey0; nyp = 2; wy = T(0.5); }
+      for (int a = 0; a < nyp; ++a)
+        for (int q = 0; q < nxp; ++q) {
+          Vec<T, VEC> ev;
+          ev.load(es + ((ys0 + a) * kUCX + xs0 + q) * b + c0);
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) acc.v[v] += (wx * wy) * ev.v[v];
+        }
+      Vec<T, VEC> fv;
+      fv.load(fs + e * b + c0);
+      const T sc = omega * cinv[e];
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) acc.v[v] += sc * fv.v[v];
+    }
+    acc.store(us + e * b + c0);
+  }
+  __syncthreads();
+  for (int it = threadIdx.x; it < kUX * kUY * nv; it += blockDim.x) {
+    const int pt = it / nv;
+    const int c0 = (it - pt * nv) * VEC;
+    const int li = pt % kUX, lj = pt / kUX;
+    const int px = fx0 + li, py = fy0 + lj;
+    if (px >= fnx || py >= fny) continue;
+    const int q = (lj + 1) * kUFX + li + 1;
+    Vec<T, VEC> up, fp, ue, uw, un, uso;
+    up.load(us + q * b + c0);
+    fp.load(fs + q * b + c0);
+    ue.load(us + (q + 1) * b + c0);
+    uw.load(us + (q - 1) * b + c0);
+    un.load(us + (q + kUFX) * b + c0);
+    uso.load(us + (q - kUFX) * b + c0);
+    const T dp = cd[q], sc = omega * cinv[q];
+    const T we = cwe[q], ww = cwe[q - 1], wn = cwn[q], ws = cwn[q - kUFX];
+    Vec<T, VEC> out;
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) {
+      const T s = fp.v[v] - dp * up.v[v] + we * ue.v[v] + ww * uw.v[v] + wn * un.v[v] + ws * uso.v[v];
+      out.v[v] = up.v[v] + sc * s;
+    }
+    store_as<T, VEC, S>(out, uu + ((uint32_t)py * (uint32_t)fnx + (uint32_t)px) * (uint32_t)F.ldu + c0);
+  }
+}
+
+template <typename T>
+size_t down_tiled_lds(int b) { return sizeof(T) * ((size_t)kDFX * kDFY * b + 3 * kDFX * kDFY); }
+template <typename T>
+size_t up_tiled_lds(int b) {
+  return sizeof(T) * ((size_t)2 * kUFX * kUFY * b + (size_t)kUCX * kUCY * b + 4 * kUFX * kUFY);
+}
+// the tiled passes need whole 16-byte chunks per point, b <= 32 and a level of at
+// least a few tiles; smaller or odd-width levels keep the grid-stride passes
+template <typename T>
+bool use_tiled(const Level<T>& L, int b, int vec) {
+  return vec * (int)sizeof(T) == 16 && b % vec == 0 && b <= 32 && L.nx >= 32 && L.ny >= 32;
+}
+
 // Levels handled by the single-workgroup coarse cycle (level l0 and below).
 constexpr int kMaxCoarseLevels = 12;
 constexpr int kCoarseBlock = 1024;
@@ -562,15 +807,26 @@ int vcycle(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, h
     QDS_CHECK_ARG(fused && lc > 0 && nl > 1 && f0 && u0,
                   "qds_mg_vcycle_io: a level-0 block type other than the hierarchy's needs the fused "
                   "V(1,1) cycle and a level 0 outside the single-workgroup coarse cycle");
+  // the tiled passes read level 0's FP64 blocks with 16-byte loads
+  const bool io16 = !mixed || (reinterpret_cast<uintptr_t>(f0) % 16 == 0 && L[0].ldf % 2 == 0);
   for (int l = 0; l < lc && l < nl - 1; ++l) {
     const int64_t work = L[l + 1].nx * L[l + 1].ny * (b / VEC);
     if (fused) {
-      if (l == 0 && mixed)
+      if (use_tiled(L[l], b, VEC) && (l > 0 || io16)) {
+        const unsigned tiles = (unsigned)(((L[l + 1].nx + kDX - 1) / kDX) * ((L[l + 1].ny + kDY - 1) / kDY));
+        if (l == 0 && mixed)
+          hipLaunchKernelGGL((mg_down_tiled<T, VEC, S>), dim3(tiles), dim3(256), down_tiled_lds<T>(b), st, L[l],
+                             L[l + 1], b, omega, f0);
+        else
+          hipLaunchKernelGGL((mg_down_tiled<T, VEC, T>), dim3(tiles), dim3(256), down_tiled_lds<T>(b), st, L[l],
+                             L[l + 1], b, omega, (const T*)L[l].f);
+      } else if (l == 0 && mixed) {
         hipLaunchKernelGGL((mg_down_kernel<T, VEC, S>), dim3(qds_blocks(work, kBlock, 65536)), dim3(kBlock), 0,
                            st, L[l], L[l + 1], b, omega, f0);
-      else
+      } else {
         hipLaunchKernelGGL((mg_down_kernel<T, VEC, T>), dim3(qds_blocks(work, kBlock, 65536)), dim3(kBlock), 0,
                            st, L[l], L[l + 1], b, omega, (const T*)L[l].f);
+      }
       QDS_LAUNCH_CHECK();
       continue;
     }
@@ -594,12 +850,21 @@ int vcycle(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, h
   for (int l = top; l >= 0; --l) {
     const int64_t work = L[l].nx * L[l].ny * (b / VEC);
     if (fused) {
-      if (l == 0 && mixed)
+      if (use_tiled(L[l], b, VEC) && (l > 0 || io16)) {
+        const unsigned tiles = (unsigned)(((L[l].nx + kUX - 1) / kUX) * ((L[l].ny + kUY - 1) / kUY));
+        if (l == 0 && mixed)
+          hipLaunchKernelGGL((mg_up_tiled<T, VEC, S>), dim3(tiles), dim3(256), up_tiled_lds<T>(b), st, L[l],
+                             L[l + 1], b, omega, f0, u0);
+        else
+          hipLaunchKernelGGL((mg_up_tiled<T, VEC, T>), dim3(tiles), dim3(256), up_tiled_lds<T>(b), st, L[l],
+                             L[l + 1], b, omega, (const T*)L[l].f, L[l].u);
+      } else if (l == 0 && mixed) {
         hipLaunchKernelGGL((mg_up_kernel<T, VEC, S>), dim3(qds_blocks(work, kBlock, 65536)), dim3(kBlock), 0, st,
                            L[l], L[l + 1], b, omega, f0, u0);
-      else
+      } else {
         hipLaunchKernelGGL((mg_up_kernel<T, VEC, T>), dim3(qds_blocks(work, kBlock, 65536)), dim3(kBlock), 0, st,
                            L[l], L[l + 1], b, omega, (const T*)L[l].f, L[l].u);
+      }
       QDS_LAUNCH_CHECK();
       continue;
     }
