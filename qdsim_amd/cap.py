@@ -76,6 +76,31 @@ class CAPOperator:
         return CAPOperator(self.indptr, self.indices, self.n, self.H, self.W, self.M, cdtype)
 
 
+class DeflatedCAPOperator:
+    """A' = A + mu (M Q)(M Q)^T with Q^T M Q = I (bilinear, no conjugation): the
+    found eigenpairs (lambda_i, q_i) of the complex-symmetric pencil move to
+    lambda_i + mu while every other eigenpair is unchanged (its eigenvector is
+    bilinearly M-orthogonal to Q).  The rank-k_q term runs as two tall-skinny
+    library GEMMs after the CAP product."""
+
+    def __init__(self, base, MQ, mu):
+        self.base = base
+        self.n = base.n
+        self.dtype = base.dtype
+        self.H = base.H
+        self.MQ = MQ.to(base.dtype).contiguous()
+        self.MQt = self.MQ.transpose(0, 1).contiguous()
+        self.mu = float(mu)
+
+    def apply(self, X, HX, MX):
+        self.base.apply(X, HX, MX)
+        C = torch.matmul(self.MQt, X)
+        HX.add_(torch.matmul(self.MQ, C), alpha=self.mu)
+
+    def astype(self, cdtype):
+        return DeflatedCAPOperator(self.base.astype(cdtype), self.MQ, self.mu)
+
+
 def _rr(G, m, nb, drop_tol, target):
     """Orthogonal Rayleigh-Ritz: S^H A S y = theta S^H M S y.  Its Ritz values lie
     in the field of values of the pencil (Re >= lambda_min(H, M), Im <= 0 for W >= 0),
@@ -196,10 +221,39 @@ def _cap_lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, seed=0, 
     return lam, X[c], res, it, converged, history
 
 
+def _cap_pair(op64, k, b, pre32, pre64, tol, tol32, maxiter, seed, verbose):
+    """FP32 then FP64 block iteration for the k lowest-real-part eigenpairs."""
+    op32 = op64.astype(torch.complex64)
+    lam32, X32, _, it32, _, h32 = cap_lobpcg(op32, k, b, pre32, tol=tol32, maxiter=maxiter, seed=seed,
+                                            stall_iters=10, verbose=verbose)
+    lam, X, res, it64, conv, h64 = cap_lobpcg(op64, k, b, pre64, X0=X32.to(torch.complex128), tol=tol,
+                                             maxiter=maxiter, seed=seed, verbose=verbose)
+    return lam, X, res, it32, it64, conv, h32, h64
+
+
+def _bilinear_pairs(op64, X):
+    """Bilinear Rayleigh quotients and images of the columns of X (original pencil)."""
+    Xk = X.to(torch.complex128).contiguous()
+    MX = torch.empty_like(Xk)
+    HX = torch.empty_like(Xk)
+    op64.apply(Xk, HX, MX)
+    bil = (Xk * MX).sum(0)
+    return (Xk * HX).sum(0) / bil, Xk, MX, bil
+
+
 def solve_cap(prob, k, b=None, target=None, tol=1e-6, tol32=3e-4, maxiter=600, seed=0, verbose=False):
     """Lowest-real-part (or nearest-target, eV) eigenpairs of the CAP pencil of a
     SchrodingerProblem whose CAP coefficient is set.  Returns (E complex eV (np),
-    eigenvectors (n_full, k) complex128 np, Hermitian M_nm-normalised)."""
+    eigenvectors (n_full, k) complex128 np, Hermitian M_nm-normalised).
+
+    Nearest-target mode (the reference's eigs(H, k, M, sigma) of config 3) is
+    computed from the bottom of the spectrum: batches of the lowest-real-part
+    eigenpairs, each batch found on the pencil with the previous ones deflated
+    (DeflatedCAPOperator), until the computed set brackets the k eigenvalues
+    nearest the target (every eigenvalue not yet computed has a real part above
+    the largest computed one, hence is farther from the target than the k-th
+    nearest).  Selecting Ritz pairs by distance to an interior target inside the
+    block iteration converged to spurious Ritz values on the full-size device."""
     if prob.H is None or prob.W is None:
         prob.assemble()
     if b is None:
@@ -208,10 +262,12 @@ def solve_cap(prob, k, b=None, target=None, tol=1e-6, tol32=3e-4, maxiter=600, s
     W = kernels.csr_take(prob.r_src, prob.W)
     M = kernels.csr_take(prob.r_src, prob.M)
     op64 = CAPOperator(prob.r_indptr, prob.r_indices, prob.n_free, H, W, M, torch.complex128)
-    op32 = op64.astype(torch.complex64)
     pre32 = prob.preconditioner(2 * b, torch.float32)
 
     def pre64(Rr, Wr, _buf={}):
+        if pre32.takes_fp64 and Rr.stride(0) == Rr.shape[1] and Wr.stride(0) == Wr.shape[1]:
+            pre32(Rr, Wr)
+            return
         if "r" not in _buf:
             _buf["r"] = torch.empty(Rr.shape, dtype=torch.float32, device=Rr.device)
             _buf["w"] = torch.empty(Rr.shape, dtype=torch.float32, device=Rr.device)
@@ -219,24 +275,54 @@ def solve_cap(prob, k, b=None, target=None, tol=1e-6, tol32=3e-4, maxiter=600, s
         pre32(_buf["r"], _buf["w"])
         Wr.copy_(_buf["w"])
 
-    lam32, X32, _, it32, _, h32 = cap_lobpcg(op32, k, b, pre32, tol=tol32, maxiter=maxiter, seed=seed,
-                                            target=target, stall_iters=10, verbose=verbose)
-    lam, X, res, it64, conv, h64 = cap_lobpcg(op64, k, b, pre64, X0=X32.to(torch.complex128), tol=tol,
-                                             maxiter=maxiter, seed=seed, target=target, verbose=verbose)
-    if target is not None:
+    if target is None:
+        lam, X, res, it32, it64, conv, h32, h64 = _cap_pair(op64, k, b, pre32, pre64, tol, tol32, maxiter,
+                                                             seed, verbose)
+        sel = torch.arange(k, device=lam.device)
+        X = X[:, sel]
+        res = res[sel] if res is not None else None
+    else:
+        kb = max(1, b - 8)                      # eigenpairs per batch
+        Qs, lams, ress = [], [], []
+        it32 = it64 = 0
+        conv = True
+        h32, h64 = [], []
+        op = op64
+        for batch in range(16):
+            lb, Xb, rb, i32, i64, cb, g32, g64 = _cap_pair(op, kb, b, pre32, pre64, tol, tol32, maxiter,
+                                                            seed + batch, verbose)
+            it32, it64, conv = it32 + i32, it64 + i64, conv and cb
+            h32, h64 = h32 + g32, h64 + g64
+            lb_o, Xb_o, MXb, bil = _bilinear_pairs(op64, Xb[:, :kb])
+            Qs.append(Xb_o / bil.sqrt()[None, :])          # Q^T M Q = I (bilinear)
+            lams.append(lb_o)
+            ress.append(rb[:kb] if rb is not None else torch.zeros(kb, dtype=torch.float64, device=lb.device))
+            lam_all = torch.cat(lams)
+            dist = (lam_all - target).abs()
+            if len(lam_all) >= k:
+                dk = torch.sort(dist).values[k - 1]
+                if float(lam_all.real.max() - target) >= float(dk):
+                    break
+            Q = torch.cat(Qs, 1)
+            MQ = torch.empty_like(Q)
+            HQ = torch.empty_like(Q)
+            op64.apply(Q, HQ, MQ)
+            # shift the found eigenvalues well above the window searched next
+            mu = 4.0 * float(lam_all.abs().max()) + 4.0 * abs(float(target)) + 1.0
+            op = DeflatedCAPOperator(op64, MQ, mu)
+        else:
+            raise RuntimeError("solve_cap: the target window was not bracketed after 16 batches")
+        lam = torch.cat(lams)
+        X = torch.cat(Qs, 1)
+        res = torch.cat(ress)
         sel = torch.argsort((lam - target).abs())[:k]
         sel = sel[torch.argsort(lam[sel].real)]
-    else:
-        sel = torch.arange(k, device=lam.device)
-    Xk = X[:, sel].to(torch.complex128).contiguous()
-    MX = torch.empty_like(Xk)
-    HX = torch.empty_like(Xk)
-    op64.apply(Xk, HX, MX)
-    bil = (Xk * MX).sum(0)
+        X = X[:, sel]
+        res = res[sel]
+    lam, Xk, MX, bil = _bilinear_pairs(op64, X)
     # bilinear Rayleigh quotient x^T A x / x^T M x: second-order accurate for a
     # complex-symmetric pencil (the Hermitian Ritz value of the orthogonal
     # projection is only first-order accurate for non-Hermitian A)
-    lam = (Xk * HX).sum(0) / bil
     # Hermitian M-normalisation with the phase making psi^T M psi real positive
     Xk = Xk * torch.exp(-0.5j * torch.angle(bil))[None, :]
     herm = (Xk.conj() * MX * torch.exp(-0.5j * torch.angle(bil))[None, :]).sum(0).real
@@ -244,6 +330,6 @@ def solve_cap(prob, k, b=None, target=None, tol=1e-6, tol32=3e-4, maxiter=600, s
     full = torch.zeros((prob.space.n, k), dtype=torch.complex128, device=Xk.device)
     full[prob.old_of_new] = Xk
     prob.last_cap_info = {"fp32_iters": it32, "fp64_iters": it64, "converged": conv,
-                          "residuals": res[sel].cpu().numpy() if res is not None else None,
+                          "residuals": res.cpu().numpy() if res is not None else None,
                           "history": {"c64": h32, "c128": h64}}
     return lam.cpu().numpy(), full.cpu().numpy()

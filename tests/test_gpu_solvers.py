@@ -220,3 +220,30 @@ def test_fem_solver_reference_workflow():
     assert np.allclose(sol, ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
     q = fs.compute_element_quality()
     assert q.shape == (mesh.num_elements,) and np.all((q > 0) & (q <= 1))
+
+
+def test_cap_target_above_several_batches_matches_dense_oracle():
+    """Nearest-target eigenvalues deep inside the spectrum: found batch by batch
+    from the bottom with the earlier batches deflated (3+ batches of 4), against
+    dense zggev."""
+    from qdsim_amd.cap import solve_cap
+    from qdsim_amd.mesh import SimpleMesh
+    from qdsim_amd.schrodinger import SchrodingerProblem
+    Lx, Ly, cx, cy, k = 40.0, 30.0, 16, 12, 4
+    vfun = lambda x, y: np.where((np.abs(x - 15.0) < 1.5) | (np.abs(x - 25.0) < 1.5), 0.2, 0.0) + 0 * y
+    mfun = lambda x, y: 0.067 + 0 * x
+    cfun = _cap_nm(Lx, Ly, 0.05, 5.0)
+    ref = cap_ref.cap_dense(cx + 1, cy + 1, Lx, Ly, 2, mfun, vfun, cfun)
+    vals = ref["vals"][np.argsort(ref["vals"].real)]
+    target = float(vals[11].real)          # between the 12th and 13th state
+    mesh = SimpleMesh(cx + 1, cy + 1, Lx * 1e-9, Ly * 1e-9, 2)
+    prob = SchrodingerProblem(mesh, lambda x, y: mfun(x * 1e9, y * 1e9) * M_E,
+                              lambda x, y: vfun(x * 1e9, y * 1e9) * EV)
+    prob.set_cap_ev(prob.space.sample(lambda x, y: cfun(x * 1e9, y * 1e9), "quadrature"))
+    prob.assemble()
+    lam, X = solve_cap(prob, k, b=12, target=target)
+    near = ref["vals"][np.argsort(np.abs(ref["vals"] - target))[:k]]
+    near = near[np.argsort(near.real)]
+    err = np.abs(np.sort_complex(lam) - np.sort_complex(near)) / np.abs(near)
+    print("deflated target eigenvalues", lam, "ref", near, "max rel err", err.max())
+    assert err.max() < 1e-8
