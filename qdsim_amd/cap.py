@@ -80,25 +80,56 @@ class DeflatedCAPOperator:
     """A' = A + mu (M Q)(M Q)^T with Q^T M Q = I (bilinear, no conjugation): the
     found eigenpairs (lambda_i, q_i) of the complex-symmetric pencil move to
     lambda_i + mu while every other eigenpair is unchanged (its eigenvector is
-    bilinearly M-orthogonal to Q).  The rank-k_q term runs as two tall-skinny
-    library GEMMs after the CAP product."""
+    bilinearly M-orthogonal to Q).  ``project`` removes the Q components of a
+    block, Y <- Y - Q (M Q)^T Y: the complement {x : Q^T M x = 0} is invariant
+    under M^-1 A (A^T = A, A Q = M Q Lambda), so the iteration's search directions
+    stay there.  The rank-k_q terms run on the interleaved real views: the
+    split-row FP64 Gram for (M Q)^T X and the MFMA block update for Q C, in chunks
+    of at most 64 columns of Q."""
 
-    def __init__(self, base, MQ, mu):
+    _CHUNK = 64
+
+    def __init__(self, base, Q, MQ, mu):
         self.base = base
         self.n = base.n
         self.dtype = base.dtype
         self.H = base.H
-        self.MQ = MQ.to(base.dtype).contiguous()
-        self.MQt = self.MQ.transpose(0, 1).contiguous()
         self.mu = float(mu)
+        self.Q, self.MQ = [], []
+        for j in range(0, Q.shape[1], self._CHUNK):
+            q, mq = Q[:, j:j + self._CHUNK], MQ[:, j:j + self._CHUNK]
+            if q.shape[1] % 2:      # even widths keep the FP32 real views 16-byte vectors
+                z = torch.zeros((q.shape[0], 1), dtype=q.dtype, device=q.device)
+                q, mq = torch.cat([q, z], 1), torch.cat([mq, z], 1)
+            self.Q.append(q.to(base.dtype).contiguous())
+            self.MQ.append(mq.to(base.dtype).contiguous())
+        self._tmp = None
+
+    def _low_rank(self, L, R, X, scale, Y):
+        """Y += scale * L (R^T X) for chunked (n, k_q) blocks L, R."""
+        if self._tmp is None or self._tmp.shape != X.shape:
+            self._tmp = torch.empty(X.shape, dtype=self.dtype, device=X.device)
+        Xr = _rv(X)
+        for l, r in zip(L, R):
+            C = _bilinear(kernels.gram(_rv(r), Xr)) * scale
+            kernels.block_update(_rv(l), _real_form(C), _rv(self._tmp))
+            Y.add_(self._tmp)
 
     def apply(self, X, HX, MX):
         self.base.apply(X, HX, MX)
-        C = torch.matmul(self.MQt, X)
-        HX.add_(torch.matmul(self.MQ, C), alpha=self.mu)
+        self._low_rank(self.MQ, self.MQ, X, self.mu, HX)
+
+    def project(self, Y):
+        self._low_rank(self.Q, self.MQ, Y, -1.0, Y)
 
     def astype(self, cdtype):
-        return DeflatedCAPOperator(self.base.astype(cdtype), self.MQ, self.mu)
+        out = DeflatedCAPOperator.__new__(DeflatedCAPOperator)
+        out.base = self.base.astype(cdtype)
+        out.n, out.dtype, out.H, out.mu = self.n, cdtype, out.base.H, self.mu
+        out.Q = [q.to(cdtype) for q in self.Q]
+        out.MQ = [q.to(cdtype) for q in self.MQ]
+        out._tmp = None
+        return out
 
 
 def _rr(G, m, nb, drop_tol, target):
@@ -133,14 +164,17 @@ def _rr(G, m, nb, drop_tol, target):
 
 
 def cap_lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, seed=0, target=None,
-               drop_tol=None, stall_iters=None, verbose=False):
+               drop_tol=None, stall_iters=None, verbose=False, constraint=None):
+    """constraint: optional in-place projection of a complex (n, b) block applied to
+    the initial block and to every preconditioned residual (deflation)."""
     # small host eigenproblems: single-threaded BLAS avoids thread wake-up costs
     with threadpool_limits(1, "blas"):
-        return _cap_lobpcg(op, k, b, precond, X0, tol, maxiter, seed, target, drop_tol, stall_iters, verbose)
+        return _cap_lobpcg(op, k, b, precond, X0, tol, maxiter, seed, target, drop_tol, stall_iters, verbose,
+                           constraint)
 
 
 def _cap_lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, seed=0, target=None,
-                drop_tol=None, stall_iters=None, verbose=False):
+                drop_tol=None, stall_iters=None, verbose=False, constraint=None):
     dev = op.H.device
     dt = op.dtype
     n = op.n
@@ -165,6 +199,8 @@ def _cap_lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, seed=0, 
             precond(_rv(R), _rv(X[c]))
     else:
         X[c].copy_(X0[:, :b].to(dt))
+    if constraint is not None:
+        constraint(X[c])
     op.apply(X[c], HX[c], MX[c])
     G = kernels.gram_blocks([_rv(X[c])], [_rv(HX[c])], [_rv(MX[c])])
     lam, C = _rr(G, b, b, drop_tol, target)
@@ -188,6 +224,8 @@ def _cap_lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, seed=0, 
             precond(_rv(R), _rv(W))
         else:
             W.copy_(R)
+        if constraint is not None:
+            constraint(W)
         op.apply(W, HW, MW)
         S = [X[c], W] + ([P[c]] if have_p else [])
         HS = [HX[c], HW] + ([HP[c]] if have_p else [])
@@ -225,10 +263,41 @@ def _cap_pair(op64, k, b, pre32, pre64, tol, tol32, maxiter, seed, verbose):
     """FP32 then FP64 block iteration for the k lowest-real-part eigenpairs."""
     op32 = op64.astype(torch.complex64)
     lam32, X32, _, it32, _, h32 = cap_lobpcg(op32, k, b, pre32, tol=tol32, maxiter=maxiter, seed=seed,
-                                            stall_iters=10, verbose=verbose)
+                                            stall_iters=10, verbose=verbose,
+                                            constraint=getattr(op32, "project", None))
     lam, X, res, it64, conv, h64 = cap_lobpcg(op64, k, b, pre64, X0=X32.to(torch.complex128), tol=tol,
-                                             maxiter=maxiter, seed=seed, verbose=verbose)
+                                             maxiter=maxiter, seed=seed, verbose=verbose,
+                                             constraint=getattr(op64, "project", None))
     return lam, X, res, it32, it64, conv, h32, h64
+
+
+def _csym_chol(G):
+    """Lower-triangular L with G = L L^T (transpose, no conjugation) for a complex
+    symmetric G near the identity (unpivoted elimination)."""
+    n = G.shape[0]
+    L = np.zeros_like(G)
+    for j in range(n):
+        L[j, j] = np.sqrt(G[j, j] - np.sum(L[j, :j] ** 2))
+        L[j + 1:, j] = (G[j + 1:, j] - L[j + 1:, :j] @ L[j, :j]) / L[j, j]
+    return L
+
+
+def _deflation_block(op64, op, X):
+    """Bilinearly M-orthonormal basis of span(X) in the complement of the current
+    deflation basis (op.project when op deflates), and its M image."""
+    Q = X.clone()
+    if hasattr(op, "project"):
+        op.project(Q)
+    HQ, MQ = torch.empty_like(Q), torch.empty_like(Q)
+    op64.apply(Q, HQ, MQ)
+    G = _bilinear(kernels.gram(_rv(Q), _rv(MQ))).cpu().numpy()
+    G = 0.5 * (G + G.T)
+    Linv_t = np.linalg.inv(_csym_chol(G)).T
+    Cr = _real_form(torch.from_numpy(Linv_t).to(Q.device))
+    Qo, MQo = torch.empty_like(Q), torch.empty_like(Q)
+    kernels.block_update(_rv(Q), Cr, _rv(Qo))
+    kernels.block_update(_rv(MQ), Cr, _rv(MQo))
+    return Qo, MQo
 
 
 def _bilinear_pairs(op64, X):
@@ -288,13 +357,14 @@ def solve_cap(prob, k, b=None, target=None, tol=1e-6, tol32=3e-4, maxiter=600, s
         conv = True
         h32, h64 = [], []
         op = op64
+        Qd = MQd = None             # deflation basis, Q^T M Q = I (bilinear)
         for batch in range(16):
             lb, Xb, rb, i32, i64, cb, g32, g64 = _cap_pair(op, kb, b, pre32, pre64, tol, tol32, maxiter,
                                                             seed + batch, verbose)
             it32, it64, conv = it32 + i32, it64 + i64, conv and cb
             h32, h64 = h32 + g32, h64 + g64
             lb_o, Xb_o, MXb, bil = _bilinear_pairs(op64, Xb[:, :kb])
-            Qs.append(Xb_o / bil.sqrt()[None, :])          # Q^T M Q = I (bilinear)
+            Qs.append(Xb_o / bil.sqrt()[None, :])
             lams.append(lb_o)
             ress.append(rb[:kb] if rb is not None else torch.zeros(kb, dtype=torch.float64, device=lb.device))
             lam_all = torch.cat(lams)
@@ -303,13 +373,13 @@ def solve_cap(prob, k, b=None, target=None, tol=1e-6, tol32=3e-4, maxiter=600, s
                 dk = torch.sort(dist).values[k - 1]
                 if float(lam_all.real.max() - target) >= float(dk):
                     break
-            Q = torch.cat(Qs, 1)
-            MQ = torch.empty_like(Q)
-            HQ = torch.empty_like(Q)
-            op64.apply(Q, HQ, MQ)
-            # shift the found eigenvalues well above the window searched next
+            Qn, MQn = _deflation_block(op64, op, Qs[-1])
+            Qd = Qn if Qd is None else torch.cat([Qd, Qn], 1)
+            MQd = MQn if MQd is None else torch.cat([MQd, MQn], 1)
+            # shift the found eigenvalues well above the window searched next and
+            # keep the search directions bilinearly M-orthogonal to them
             mu = 4.0 * float(lam_all.abs().max()) + 4.0 * abs(float(target)) + 1.0
-            op = DeflatedCAPOperator(op64, MQ, mu)
+            op = DeflatedCAPOperator(op64, Qd, MQd, mu)
         else:
             raise RuntimeError("solve_cap: the target window was not bracketed after 16 batches")
         lam = torch.cat(lams)

@@ -87,14 +87,14 @@ struct GramStage {
 
 template <typename T, bool VEC, int MAXP, int MAXT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
-gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, GramStage stg,
+gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, int g0, GramStage stg,
              double* __restrict__ partial) {
   using L = GLoad<T, VEC>;
   using V = typename L::v;
   constexpr int VW = L::W;
   extern __shared__ double gsm[];
   const int tk = g.tk1 + g.tk2;
-  const int tj0 = blockIdx.y * tpg;
+  const int tj0 = (g0 + blockIdx.y) * tpg;
   const int tj1 = min(tk, tj0 + tpg);
   const int k1pad = 16 * g.tk1;
   const int c_lo = tj0 * 16, c_hi = tj1 * 16;
@@ -673,13 +673,13 @@ QDS_API int qds_gram_blocks(int32_t dtype, int64_t n, const qds_colblocks* A, co
     }
     return mx;
   };
+  // at most 8 accumulator tiles per wave: 16 spill registers to scratch
   int tpg = tk, max_slots = 0;
   for (; tpg >= 1; --tpg) {
     max_slots = slots(tpg);
-    if (max_slots <= 16 && (size_t)kRB * (g.mpad + 16 * tpg) * sizeof(double) <= 64 * 1024) break;
+    if (max_slots <= 8 && (size_t)kRB * (g.mpad + 16 * tpg) * sizeof(double) <= 64 * 1024) break;
   }
   QDS_CHECK_ARG(tpg >= 1, "qds_gram: operands of %d columns too wide", m);
-  (void)k1pad;
   const int ngroups = (tk + tpg - 1) / tpg;
   const int nblk = gram_blocks(n);
   QDS_CHECK_ARG(ws_bytes >= qds_gram_workspace(n, m, k1, k2), "qds_gram: workspace too small");
@@ -689,26 +689,47 @@ QDS_API int qds_gram_blocks(int32_t dtype, int64_t n, const qds_colblocks* A, co
     QDS_CHECK_ARG(stg.ld[q] * kRB < (int64_t(1) << 31), "qds_gram: leading dimension too large");
   hipStream_t st = qds_stream(stream);
   double* partial = static_cast<double*>(workspace);
-  auto launch = [&](auto tag, auto vecc, auto maxp, auto maxt) {
+  // a thread keeps offsets and one staged vector per piece: more than 9 pieces
+  // spill, so wide operands run one launch per column group with only the pieces
+  // that group reads (all of A, the B pieces inside the group's columns)
+  auto launch = [&](auto tag, auto vecc, const GramStage& sg, int g0, int ny) {
     using T = decltype(tag);
     constexpr bool VV = decltype(vecc)::value;
-    constexpr int MP = decltype(maxp)::value;
-    constexpr int MT_ = decltype(maxt)::value;
-    hipLaunchKernelGGL((gram_partial<T, VV, MP, MT_>), dim3(nblk, ngroups), dim3(kBlock), shm, st, n,
-                       rows_per_blk, g, tpg, stg, partial);
+    if (sg.n <= 9)
+      hipLaunchKernelGGL((gram_partial<T, VV, 9, 8>), dim3(nblk, ny), dim3(kBlock), shm, st, n, rows_per_blk,
+                         g, tpg, g0, sg, partial);
+    else
+      hipLaunchKernelGGL((gram_partial<T, VV, kGramPieces, 8>), dim3(nblk, ny), dim3(kBlock), shm, st, n,
+                         rows_per_blk, g, tpg, g0, sg, partial);
   };
-  auto by_pieces = [&](auto tag, auto vecc, auto maxt) {
-    if (stg.n <= 9) launch(tag, vecc, std::integral_constant<int, 9>(), maxt);
-    else launch(tag, vecc, std::integral_constant<int, kGramPieces>(), maxt);
+  auto launch_all = [&](auto tag, auto vecc) {
+    if (stg.n <= 9 || ngroups == 1) {
+      launch(tag, vecc, stg, 0, ngroups);
+      return;
+    }
+    for (int gi = 0; gi < ngroups; ++gi) {
+      const int c_lo = gi * tpg * 16, c_hi = (gi + 1 < ngroups ? (gi + 1) * tpg : tk) * 16;
+      GramStage sg;
+      sg.n = 0;
+      for (int q = 0; q < stg.n; ++q) {
+        if (stg.op[q] != 0) {
+          const int t0 = (stg.op[q] == 1 ? 0 : k1pad) + stg.col0[q];
+          if (t0 + stg.w[q] <= c_lo || t0 >= c_hi) continue;
+        }
+        sg.src[sg.n] = stg.src[q];
+        sg.ld[sg.n] = stg.ld[q];
+        sg.w[sg.n] = stg.w[q];
+        sg.op[sg.n] = stg.op[q];
+        sg.col0[sg.n] = stg.col0[q];
+        ++sg.n;
+      }
+      launch(tag, vecc, sg, gi, 1);
+    }
   };
-  auto by_tiles = [&](auto tag, auto vecc) {
-    if (max_slots <= 8) by_pieces(tag, vecc, std::integral_constant<int, 8>());
-    else by_pieces(tag, vecc, std::integral_constant<int, 16>());
-  };
-  if (dtype == qds::F32 && vec) by_tiles(float(0), std::true_type());
-  else if (dtype == qds::F32) by_tiles(float(0), std::false_type());
-  else if (vec) by_tiles(double(0), std::true_type());
-  else by_tiles(double(0), std::false_type());
+  if (dtype == qds::F32 && vec) launch_all(float(0), std::true_type());
+  else if (dtype == qds::F32) launch_all(float(0), std::false_type());
+  else if (vec) launch_all(double(0), std::true_type());
+  else launch_all(double(0), std::false_type());
   QDS_LAUNCH_CHECK();
   const int tot = m * (k1 + k2);
   hipLaunchKernelGGL(gram_reduce, dim3((tot + kRedOut - 1) / kRedOut), dim3(kBlock), 0, st, nblk, g, partial, C);

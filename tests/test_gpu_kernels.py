@@ -105,8 +105,29 @@ def test_block_update(dtype, b, two):
         assert torch.all(out[:, 2 * b:] == 7.0)
 
 
+@pytest.mark.parametrize("dtype,b", [(torch.float32, 72), (torch.float64, 56), (torch.float64, 40),
+                                     (torch.float64, 64)])
+@pytest.mark.parametrize("sym", [False, True])
+def test_gram_wide_column_blocks(dtype, b, sym):
+    """Operands of more than 9 staging pieces (the complex solver's real views):
+    one launch per output column group, each with only the pieces it reads."""
+    from qdsim_amd import kernels
+    dev = torch.device("cuda")
+    n = 3001
+    rng = np.random.default_rng(5 + b)
+    blk = lambda: [torch.from_numpy(rng.standard_normal((n, b))).to(dev, dtype) for _ in range(3)]
+    S = blk()
+    H = S if sym else blk()
+    M = [2.0 * x for x in S] if sym else blk()       # exact scaling keeps S^T M symmetric
+    print(f"wide gram {dtype} 3 x {b} columns sym={sym}", flush=True)
+    G = kernels.gram_blocks(S, H, M, sym=sym).cpu().numpy()
+    cat = lambda L: torch.cat([x.double() for x in L], 1).cpu().numpy()
+    ref = cat(S).T @ np.concatenate([cat(H), cat(M)], 1)
+    assert np.allclose(G, ref, rtol=1e-12, atol=1e-10 * np.abs(ref).max())
+
+
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
-@pytest.mark.parametrize("b", [8, 24])
+@pytest.mark.parametrize("b", [8, 24, 56])
 @pytest.mark.parametrize("nblk", [1, 2, 3])
 def test_gram_and_update_column_blocks(dtype, b, nblk):
     """Column-block forms against FP64 products of the concatenated matrices; the
