@@ -105,14 +105,20 @@ class LatticeMG:
         vs = sum(v_elem[L.element_at(xp + sx * L.hx, yp + sy * L.hy)]
                  for sx in (-0.25, 0.25) for sy in (-0.25, 0.25)) * 0.25
         Mterm = (vs - sigma) * (L.hx * L.hy)
+        # lumped mass of each level (the coefficient of -sigma in its diagonal)
+        Mass = torch.full_like(Mterm, L.hx * L.hy)
+        self.sigma = float(sigma)
         self.levels_data = []
         NX, NY = L.NX, L.NY
         while True:
-            self.levels_data.append(self._interior(WE, WN, Mterm, NX, NY, dev))
+            lv = self._interior(WE, WN, Mterm, NX, NY, dev)
+            lv["mass"] = Mass[1:-1, 1:-1].contiguous()
+            self.levels_data.append(lv)
             nxi, nyi = NX - 2, NY - 2
             if not (nxi % 2 == 1 and nyi % 2 == 1 and (nxi - 1) // 2 >= min_coarse
                     and (nyi - 1) // 2 >= min_coarse):
                 break
+            Mass = self._coarsen(WE, WN, Mass, NX, NY)[2]
             WE, WN, Mterm, NX, NY = self._coarsen(WE, WN, Mterm, NX, NY)
         self.n_levels = len(self.levels_data)
         self._structs = (_lib.MGLevel * self.n_levels)()
@@ -126,6 +132,17 @@ class LatticeMG:
             s.d, s.we, s.wn = lv["d"].data_ptr(), lv["we"].data_ptr(), lv["wn"].data_ptr()
             s.u, s.f, s.t = lv["u"].data_ptr(), lv["f"].data_ptr(), lv["t"].data_ptr()
             s.ldu = s.ldf = 0
+
+    def set_sigma(self, sigma):
+        """Move the shift of every level to sigma (diagonals updated in place:
+        d = d(sigma_0) - (sigma - sigma_0) * lumped mass)."""
+        sigma = float(sigma)
+        if sigma == self.sigma:
+            return
+        for lv in self.levels_data:
+            lv["d64"] = lv["d64"] - (sigma - self.sigma) * lv["mass"]
+            lv["d"].copy_(lv["d64"].reshape(-1).to(self.dtype))
+        self.sigma = sigma
 
     @property
     def takes_fp64(self):

@@ -363,7 +363,7 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
 
 
 def solve_mixed(op64, k, b, precond32=None, precond64=None, tol=1e-6, tol32=1e-3, maxiter=400,
-                seed=0, verbose=False, drop32=None, stall32=8, refresh32=1, X0=None, b64=None):
+                seed=0, verbose=False, drop32=None, stall32=8, refresh32=1, X0=None, b64=None, between=None):
     """FP32 LOBPCG to tol32 (or stagnation), then FP64 LOBPCG to tol.  X0: optional
     initial block for the FP32 phase (default: seeded random vectors).
 
@@ -371,11 +371,15 @@ def solve_mixed(op64, k, b, precond32=None, precond64=None, tol=1e-6, tol32=1e-3
     (1: every iteration): FP32 rounding of the updated images otherwise makes the
     reported residual up to 30x smaller than the true one.  tol32 = 1e-3 sits just
     above the FP32 floor of the quantum-dot problems (‖H‖/|lambda| ~ 1e3).  b64: block
-    of the FP64 phase (default b), started from the lowest b64 FP32 Ritz vectors."""
+    of the FP64 phase (default b), started from the lowest b64 FP32 Ritz vectors.
+    between: optional callable taking the FP32 result, run before the FP64 phase
+    (e.g. to move the preconditioner's shift)."""
     op32 = op64.astype(torch.float32)
     t0 = time.perf_counter()
     r32 = lobpcg(op32, k, b, precond32, X0=X0, tol=tol32, maxiter=maxiter, seed=seed, verbose=verbose,
                  stall_iters=stall32, drop_tol=drop32, refresh=refresh32)
+    if between is not None:
+        between(r32)
     t1 = time.perf_counter()
     X0 = r32.X.to(torch.float64)
     r64 = lobpcg(op64, k, b64 or b, precond64, X0=X0, tol=tol, maxiter=maxiter, seed=seed, verbose=verbose)

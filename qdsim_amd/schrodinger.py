@@ -40,6 +40,20 @@ def default_block(k, n):
     return b
 
 
+def shift_below(theta, floor, frac):
+    """Multigrid shift just below the lowest eigenvalue: theta_1 - frac (theta_b -
+    theta_1) from a block's Ritz values theta (upper bounds of the eigenvalues),
+    never below floor (the shift below min V, always safe).  The block iteration
+    with the preconditioner (H - sigma M)^-1 converges at a rate set by the
+    relative gaps (lambda_b+1 - lambda_i) / (lambda_b+1 - sigma): on a quantum dot
+    sigma below min V makes them ~0.015, a shift at theta_1 minus a tenth of the
+    block's spread nearly doubles them."""
+    th = np.sort(np.asarray(theta.detach().cpu().numpy() if torch.is_tensor(theta) else theta, dtype=np.float64))
+    if th.size < 2 or not np.all(np.isfinite(th)):
+        return floor
+    return max(float(floor), float(th[0] - frac * (th[-1] - th[0])))
+
+
 class SchrodingerProblem:
     """H = K(hbar^2/2m*) + M(V) [- i M(W)], M on a structured SimpleMesh."""
 
@@ -171,6 +185,7 @@ class SchrodingerProblem:
         full = torch.zeros((cp.space.n, b), dtype=torch.float64, device=r.X.device)
         full[cp.old_of_new] = r.X.double()
         self._nested_iters = r.iterations
+        self._nested_theta = r.eigenvalues
         return interp.interpolate_points(px, py, full)
 
     # -- solve ------------------------------------------------------------------
@@ -205,11 +220,21 @@ class SchrodingerProblem:
 
             t_n = time.perf_counter()
             self._nested_iters = 0
+            self._nested_theta = None
             X0 = self._nested_start(k, b, **(nested_opts or {}))
+            sigma0 = pre32.sigma
+            if self._nested_theta is not None:
+                # coarse Ritz values: a conservative shift for the FP32 phase
+                pre32.set_sigma(shift_below(self._nested_theta, sigma0, 0.25))
             torch.cuda.synchronize()
             t_nested = time.perf_counter() - t_n
+
+            def retune(r32):
+                # fine Ritz values of the FP32 phase: the shift of the FP64 phase
+                pre32_64.set_sigma(shift_below(r32.eigenvalues, sigma0, 0.1))
+
             res = solve_mixed(op64, k, b, pre32, pre64, tol=tol, tol32=tol32, maxiter=maxiter, seed=seed,
-                              verbose=verbose, X0=X0, b64=b64, **mixed_opts)
+                              verbose=verbose, X0=X0, b64=b64, between=retune, **mixed_opts)
         elif mode == "fp64":
             pre = self.preconditioner(b, torch.float64, cycle=dict(nu_pre=2, nu_post=2, coarse_sweeps=40))
             torch.cuda.synchronize()
