@@ -31,6 +31,7 @@ import torch
 from threadpoolctl import threadpool_limits
 
 from . import kernels
+from .device import sync
 from .units import EV_TO_J
 
 
@@ -255,7 +256,7 @@ def _cap_lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, seed=0, 
                                         c2_first=2 * b)
         c = 1 - c
         have_p = True
-    torch.cuda.synchronize()
+    sync()
     return lam, X[c], res, it, converged, history
 
 

@@ -4,6 +4,7 @@ PyTorch-ROCm provides allocation and the current HIP stream; every computation
 on those buffers is a qdsim_hip kernel.  A GPU is required.
 """
 import ctypes
+import threading
 
 import torch
 
@@ -21,6 +22,12 @@ def device():
 
 def stream_ptr():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def sync():
+    """Wait for the current stream (every kernel of the calling thread runs there;
+    other threads' streams keep running)."""
+    torch.cuda.current_stream().synchronize()
 
 
 def ptr(t):
@@ -54,10 +61,15 @@ class Workspace:
         return ctypes.c_void_p(b.data_ptr()), b.numel()
 
 
-_WS = {}
+_WS = threading.local()
 
 
 def workspace(name="default"):
-    if name not in _WS:
-        _WS[name] = Workspace()
-    return _WS[name]
+    """Named scratch buffer of the calling thread: threads that run independent
+    problems on their own streams never share scratch memory."""
+    d = getattr(_WS, "d", None)
+    if d is None:
+        d = _WS.d = {}
+    if name not in d:
+        d[name] = Workspace()
+    return d[name]

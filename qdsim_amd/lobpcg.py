@@ -31,7 +31,7 @@ import torch
 from threadpoolctl import threadpool_limits
 
 from . import kernels
-from .device import device
+from .device import device, sync
 
 
 class CSROperator:
@@ -355,7 +355,7 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
             # recompute the images of X: the updated ones carry the rounding of every
             # update since the last product (FP32 storage)
             op.apply(X[c], HX[c], MX[c])
-    torch.cuda.synchronize()
+    sync()
     t_end = time.perf_counter()
     X = X[c]
     timings = {"setup_s": t_iter - t_start, "iterate_s": t_end - t_iter, "restarts": sum(d > 0 for d in drops)}

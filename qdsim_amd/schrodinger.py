@@ -23,6 +23,7 @@ import numpy as np
 import torch
 
 from . import kernels
+from .device import sync
 from .fem import FEMSpace
 from .mesh import SimpleMesh
 from .lattice import Lattice, LatticeMG
@@ -38,6 +39,14 @@ def default_block(k, n):
     while 4 * b > n and b > 4:
         b -= 4
     return b
+
+
+def _warm(start, n_free, b):
+    """True when ``start`` (an earlier solve's info) holds a block for this problem."""
+    if not start or start.get("X_block") is None:
+        return False
+    X = start["X_block"]
+    return X.dim() == 2 and X.shape[0] == n_free and X.shape[1] >= b
 
 
 def shift_below(theta, floor, frac):
@@ -190,9 +199,14 @@ class SchrodingerProblem:
 
     # -- solve ------------------------------------------------------------------
     def solve(self, k, b=None, mode="mixed", tol=1e-6, tol32=3e-3, maxiter=400, seed=0, verbose=False,
-              nested_opts=None, b64=None, **mixed_opts):
+              nested_opts=None, b64=None, start=None, **mixed_opts):
         """Lowest k eigenpairs.  Returns (eigenvalues eV (np), full-numbering
-        eigenvectors (n, k) float64 device tensor, M_nm-normalised, info)."""
+        eigenvectors (n, k) float64 device tensor, M_nm-normalised, info).
+
+        start: the info dict of an earlier solve of this problem with the same k
+        and block (its final block ``X_block`` and Ritz values ``ritz``): a warm
+        start for a slightly changed potential (the self-consistent loop), which
+        replaces the nested or random start."""
         if self.H is None:
             self.assemble()
         if b is None:
@@ -203,7 +217,7 @@ class SchrodingerProblem:
             pre32 = self.preconditioner(b, torch.float32)
             b64 = b if b64 is None else int(b64)
             pre32_64 = pre32 if b64 == b else self.preconditioner(b64, torch.float32)
-            torch.cuda.synchronize()
+            sync()
             t_prep = time.perf_counter() - t0
 
             def pre64(R, W, _pre=pre32_64, _buf={}):
@@ -221,12 +235,16 @@ class SchrodingerProblem:
             t_n = time.perf_counter()
             self._nested_iters = 0
             self._nested_theta = None
-            X0 = self._nested_start(k, b, **(nested_opts or {}))
             sigma0 = pre32.sigma
+            if _warm(start, self.n_free, b):
+                X0 = start["X_block"]
+                self._nested_theta = start["ritz"]
+            else:
+                X0 = self._nested_start(k, b, **(nested_opts or {}))
             if self._nested_theta is not None:
-                # coarse Ritz values: a conservative shift for the FP32 phase
+                # coarse (or previous) Ritz values: a conservative shift for the FP32 phase
                 pre32.set_sigma(shift_below(self._nested_theta, sigma0, 0.25))
-            torch.cuda.synchronize()
+            sync()
             t_nested = time.perf_counter() - t_n
 
             def retune(r32):
@@ -237,22 +255,23 @@ class SchrodingerProblem:
                               verbose=verbose, X0=X0, b64=b64, between=retune, **mixed_opts)
         elif mode == "fp64":
             pre = self.preconditioner(b, torch.float64, cycle=dict(nu_pre=2, nu_post=2, coarse_sweeps=40))
-            torch.cuda.synchronize()
+            sync()
             t_prep = time.perf_counter() - t0
             t_nested = 0.0
             self._nested_iters = 0
-            res = lobpcg(op64, k, b, pre, tol=tol, maxiter=maxiter, seed=seed, verbose=verbose)
+            X0 = start["X_block"] if _warm(start, self.n_free, b) else None
+            res = lobpcg(op64, k, b, pre, X0=X0, tol=tol, maxiter=maxiter, seed=seed, verbose=verbose)
         else:
             raise ValueError(f"unknown mode {mode}")
         lam = res.eigenvalues[:k].double()
         X = res.X[:, :k].double()
         full = torch.zeros((self.space.n, k), dtype=torch.float64, device=X.device)
         full[self.old_of_new] = X
-        torch.cuda.synchronize()
+        sync()
         info = {"iterations": res.iterations, "converged": res.converged,
                 "residuals": res.residuals[:k].cpu().numpy() if res.residuals is not None else None,
                 "timings": dict(res.timings, prep_s=t_prep, nested_s=t_nested,
                                 nested_iters=self._nested_iters),
-                "history": res.history, "block": b,
+                "history": res.history, "block": b, "X_block": res.X, "ritz": res.eigenvalues,
                 "solve_s": time.perf_counter() - t0}
         return lam.cpu().numpy(), full, info

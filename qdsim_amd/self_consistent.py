@@ -31,6 +31,10 @@ from .schrodinger import SchrodingerProblem
 from .units import EV_TO_J, K_B, Q_E
 
 
+def _mean(a):
+    return float(a.mean().item()) if torch.is_tensor(a) else float(a)
+
+
 class SelfConsistentSolver:
     def __init__(self, mesh, epsilon_r_func, m_star_func, potential_func, temperature=300.0,
                  mixing=0.3):
@@ -84,13 +88,23 @@ class SelfConsistentSolver:
                                         ptr(n), stream_ptr()), "qds_density")
         s = float(n.mean().item())
         if s > 0:
-            n = n * (N_D / s)
-        p = torch.where(phi > 0, N_A * torch.exp(-Q_E * phi / kT), torch.full_like(phi, N_A))
+            n = n * (_mean(N_D) / s)
+        NA = N_A if torch.is_tensor(N_A) else torch.full_like(phi, N_A)
+        p = torch.where(phi > 0, NA * torch.exp(-Q_E * phi / kT), NA)
         return n, p
 
     def solve_self_consistent(self, V_p, V_n, num_quantum_states=5, N_A=1e24, N_D=1e24):
+        """N_A, N_D: scalars as in the reference, or per-node arrays (a doping
+        profile such as an abrupt junction); the electron density is rescaled to
+        the mean donor density.  Each Schroedinger solve after the first starts
+        from the previous iteration's block (the potential changes by the mixing
+        step only)."""
         t0 = time.time()
         dev = self.schrodinger.space.dev
+        if not np.isscalar(N_A):
+            N_A = torch.as_tensor(np.asarray(N_A, dtype=np.float64), device=dev)
+        if not np.isscalar(N_D):
+            N_D = torch.as_tensor(np.asarray(N_D, dtype=np.float64), device=dev)
         x = torch.from_numpy(self._x).to(dev)
         Lx = self.mesh.Lx
         phi = V_p + (V_n - V_p) * x / Lx
@@ -102,12 +116,13 @@ class SelfConsistentSolver:
         psi = None
         n = torch.zeros_like(phi)
         p = torch.zeros_like(phi)
+        info = None
         while it < self.max_iterations and residual > self.tolerance:
             it += 1
             # Schroedinger with V_total = V - q phi  (eV: V - phi)
             self.schrodinger.set_potential_ev((self.v0_q - self._phi_at_quadrature(phi)).contiguous())
             self.schrodinger.assemble()
-            lam, psi, info = self.schrodinger.solve(num_quantum_states)
+            lam, psi, info = self.schrodinger.solve(num_quantum_states, start=info)
             # psi: M_nm-normalised -> SI normalisation (psi^T M_SI psi = 1)
             psi_si = psi * 1e9
             n, p = self._densities(lam, psi_si, phi, N_A, N_D)
