@@ -36,6 +36,18 @@ __device__ __forceinline__ double coef_at(const Coef& c, int64_t e, int q) {
   return c.p[e * c.se + (int64_t)q * c.sq];
 }
 
+// Coefficient samples of one wave's elements, staged in LDS: up to four fields
+// (c, v, r, w) x elements per wave x quadrature points.
+constexpr int kMaxCoefSlots = 4 * 10 * kMaxQ;
+
+// wave-level barrier with LDS visibility (the waves of a block run different
+// numbers of elements, so a block barrier cannot sit inside the element loop)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __global__ void __launch_bounds__(kBlock)
 element_kernel(int npe, int nq, int64_t n_elem, const int32_t* __restrict__ conn,
                const double* __restrict__ x, const double* __restrict__ y,
@@ -45,6 +57,7 @@ element_kernel(int npe, int nq, int64_t n_elem, const int32_t* __restrict__ conn
   __shared__ double s_phi[kMaxQ * kMaxNpe];
   __shared__ double s_dx[kMaxQ * kMaxNpe];
   __shared__ double s_dy[kMaxQ * kMaxNpe];
+  __shared__ double s_cf[kBlock / kWave][kMaxCoefSlots];
   const int tq = nq * npe;
   for (int t = threadIdx.x; t < nq; t += blockDim.x) s_w[t] = tables[t];
   for (int t = threadIdx.x; t < tq; t += blockDim.x) {
@@ -60,45 +73,70 @@ element_kernel(int npe, int nq, int64_t n_elem, const int32_t* __restrict__ conn
   const int wave = threadIdx.x / kWave;
   const int sub = lane / nsym;
   const int s = lane - sub * nsym;
-  if (sub >= per_wave) return;
   // (i, j) of packed entry s: row-major upper triangle
   int i = 0, rem = s;
   while (rem >= npe - i) { rem -= npe - i; ++i; }
   const int j = i + rem;
+  // staged fields: c, v, r, w in that order, each per_wave * nq samples
+  const bool use[4] = {c.p != nullptr, v.p != nullptr, out1 != nullptr && r.p != nullptr,
+                       out2 != nullptr && w.p != nullptr};
+  const int per_field = per_wave * nq;
+  int foff[4], nf = 0;
+#pragma unroll
+  for (int f = 0; f < 4; ++f) foff[f] = use[f] ? (nf++) * per_field : -1;
+  double* cf = s_cf[wave];
 
   const int64_t waves_total = (int64_t)gridDim.x * (kBlock / kWave);
   for (int64_t wb = (int64_t)blockIdx.x * (kBlock / kWave) + wave; wb * per_wave < n_elem;
        wb += waves_total) {
-    const int64_t e = wb * per_wave + sub;
-    if (e >= n_elem) continue;
-    const int32_t* ce = conn + e * npe;
-    const double x0 = x[ce[0]], y0 = y[ce[0]];
-    const double a = x[ce[1]] - x0, b = x[ce[2]] - x0;
-    const double cc = y[ce[1]] - y0, d = y[ce[2]] - y0;
-    const double det = a * d - b * cc;
-    const double area = 0.5 * fabs(det);
-    const double inv2 = 1.0 / (det * det);
-    const double g11 = (b * b + d * d) * inv2;
-    const double g12 = -(a * b + cc * d) * inv2;
-    const double g22 = (a * a + cc * cc) * inv2;
-    double k0 = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
-    for (int q = 0; q < nq; ++q) {
-      const double wq = s_w[q] * area;
-      const double pp = s_phi[q * npe + i] * s_phi[q * npe + j];
-      if (c.p) {
-        const double xi = s_dx[q * npe + i], eti = s_dy[q * npe + i];
-        const double xj = s_dx[q * npe + j], etj = s_dy[q * npe + j];
-        const double gg = xi * xj * g11 + (xi * etj + eti * xj) * g12 + eti * etj * g22;
-        k0 += wq * coef_at(c, e, q) * gg;
+    const int64_t e_first = wb * per_wave;
+    // one batched load of every coefficient sample of this wave's elements: the
+    // quadrature loop below then reads LDS instead of issuing a dependent global
+    // load per point
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      if (!use[f]) continue;
+      const Coef& cw = f == 0 ? c : f == 1 ? v : f == 2 ? r : w;
+      for (int t = lane; t < per_field; t += kWave) {
+        const int se = t / nq, q = t - se * nq;
+        const int64_t e = e_first + se;
+        cf[foff[f] + t] = e < n_elem ? coef_at(cw, e, q) : 0.0;
       }
-      if (v.p) m0 += wq * coef_at(v, e, q) * pp;
-      if (out1) m1 += wq * (r.p ? coef_at(r, e, q) : 1.0) * pp;
-      if (out2) m2 += wq * coef_at(w, e, q) * pp;
     }
-    const int64_t o = e * nsym + s;
-    if (out0) out0[o] = k0 + m0;
-    if (out1) out1[o] = m1;
-    if (out2) out2[o] = m2;
+    wave_sync();
+    const int64_t e = e_first + sub;
+    if (sub < per_wave && e < n_elem) {
+      const int32_t* ce = conn + e * npe;
+      const double x0 = x[ce[0]], y0 = y[ce[0]];
+      const double a = x[ce[1]] - x0, b = x[ce[2]] - x0;
+      const double cc = y[ce[1]] - y0, d = y[ce[2]] - y0;
+      const double det = a * d - b * cc;
+      const double area = 0.5 * fabs(det);
+      const double inv2 = 1.0 / (det * det);
+      const double g11 = (b * b + d * d) * inv2;
+      const double g12 = -(a * b + cc * d) * inv2;
+      const double g22 = (a * a + cc * cc) * inv2;
+      const double* sc = cf + sub * nq;
+      double k0 = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
+      for (int q = 0; q < nq; ++q) {
+        const double wq = s_w[q] * area;
+        const double pp = s_phi[q * npe + i] * s_phi[q * npe + j];
+        if (use[0]) {
+          const double xi = s_dx[q * npe + i], eti = s_dy[q * npe + i];
+          const double xj = s_dx[q * npe + j], etj = s_dy[q * npe + j];
+          const double gg = xi * xj * g11 + (xi * etj + eti * xj) * g12 + eti * etj * g22;
+          k0 += wq * sc[foff[0] + q] * gg;
+        }
+        if (use[1]) m0 += wq * sc[foff[1] + q] * pp;
+        if (out1) m1 += wq * (use[2] ? sc[foff[2] + q] : 1.0) * pp;
+        if (use[3]) m2 += wq * sc[foff[3] + q] * pp;
+      }
+      const int64_t o = e * nsym + s;
+      if (out0) out0[o] = k0 + m0;
+      if (out1) out1[o] = m1;
+      if (out2) out2[o] = m2;
+    }
+    wave_sync();   // every lane has read this element group's samples
   }
 }
 
