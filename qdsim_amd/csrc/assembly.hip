@@ -25,6 +25,7 @@ constexpr int kMaxQ = 16;
 constexpr int kMaxNpe = 10;
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
+typedef double v4d __attribute__((ext_vector_type(4)));
 
 struct Coef {
   const double* p;
@@ -140,6 +141,143 @@ element_kernel(int npe, int nq, int64_t n_elem, const int32_t* __restrict__ conn
   }
 }
 
+// ---------------------------------------------------------------------------
+// Element kernel on FP64 MFMA.  Per element e and packed entry s = (i, j):
+//   out0[e][s] = sum_q w_q |J|/2 [ c_q (g11 A_q + g12 B_q + g22 C_q) + v_q P_q ](s)
+// with A = dxi_i dxi_j, B = dxi_i deta_j + deta_i dxi_j, C = deta_i deta_j,
+// P = phi_i phi_j (reference-triangle tables, element independent) and
+// g = J^-1 J^-T / det^2 the element metric.  That is a product
+//   out0 (E x nsym) = Acoef (E x 4nq) . T (4nq x nsym),
+//   Acoef[e][part nq + q] = w_q |J|/2 (g11 c, g12 c, g22 c, v)[part],
+// and likewise out1 / out2 with the P rows of T and r / w.  A wave computes a
+// tile of 16 elements with v_mfma_f64_16x16x4f64 (rows: elements, columns: 16
+// packed entries, k: 4 of the 4nq coefficients); T lives in LDS, built once per
+// workgroup; each lane forms its A operands from its element's geometry and
+// coefficient samples.  Compared with one lane per entry looping over the
+// quadrature points (9 LDS reads per point and lane), LDS traffic drops ~30x.
+constexpr int kTile = 16;
+
+template <int NPE, int NQ>
+__global__ void __launch_bounds__(kBlock)
+element_mfma_kernel(int64_t n_elem, const int32_t* __restrict__ conn, const double* __restrict__ x,
+                    const double* __restrict__ y, const double* __restrict__ tables, Coef c, Coef v, Coef r,
+                    Coef w, double* __restrict__ out0, double* __restrict__ out1, double* __restrict__ out2) {
+  constexpr int NSYM = NPE * (NPE + 1) / 2;
+  constexpr int NCT = (NSYM + 15) / 16;          // 16-column tiles of packed entries
+  constexpr int NSP = 16 * NCT;
+  constexpr int K0 = 4 * NQ;                     // rows of T used by out0
+  constexpr int KS0 = K0 / 4;                    // k-steps of out0 (4 NQ is a multiple of 4)
+  constexpr int KS1 = (NQ + 3) / 4;              // k-steps of out1 / out2
+  constexpr int TROWS = K0 + 4;                  // 4 zero rows pad the last out1 k-step
+  __shared__ double s_w[NQ];
+  __shared__ double s_t[TROWS * NSP];
+  for (int t = threadIdx.x; t < NQ; t += blockDim.x) s_w[t] = tables[t];
+  for (int t = threadIdx.x; t < TROWS * NSP; t += blockDim.x) {
+    const int k = t / NSP, sc = t - k * NSP;
+    double val = 0.0;
+    if (k < K0 && sc < NSYM) {
+      int i = 0, rem = sc;
+      while (rem >= NPE - i) { rem -= NPE - i; ++i; }
+      const int j = i + rem;
+      const int part = k / NQ, q = k - part * NQ;
+      const double* phi = tables + NQ;
+      const double* dx = phi + NQ * NPE;
+      const double* dy = dx + NQ * NPE;
+      const double xi = dx[q * NPE + i], xj = dx[q * NPE + j];
+      const double ei = dy[q * NPE + i], ej = dy[q * NPE + j];
+      val = part == 0 ? xi * xj : part == 1 ? xi * ej + ei * xj : part == 2 ? ei * ej
+                                                                             : phi[q * NPE + i] * phi[q * NPE + j];
+    }
+    s_t[t] = val;
+  }
+  __syncthreads();
+
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int li = lane & 15, kg = lane >> 4;
+  const bool has_c = c.p != nullptr, has_v = v.p != nullptr;
+  const int64_t n_tiles = (n_elem + kTile - 1) / kTile;
+  for (int64_t tile = (int64_t)blockIdx.x * (kBlock / kWave) + wave; tile < n_tiles;
+       tile += (int64_t)gridDim.x * (kBlock / kWave)) {
+    const int64_t e = tile * kTile + li;
+    const bool live = e < n_elem;
+    double g11 = 0.0, g12 = 0.0, g22 = 0.0, area = 0.0;
+    if (live) {
+      const int32_t* ce = conn + e * NPE;
+      const double x0 = x[ce[0]], y0 = y[ce[0]];
+      const double a = x[ce[1]] - x0, b = x[ce[2]] - x0;
+      const double cc = y[ce[1]] - y0, d = y[ce[2]] - y0;
+      const double det = a * d - b * cc;
+      area = 0.5 * fabs(det);
+      const double inv2 = 1.0 / (det * det);
+      g11 = (b * b + d * d) * inv2;
+      g12 = -(a * b + cc * d) * inv2;
+      g22 = (a * a + cc * cc) * inv2;
+    }
+    const v4d zero = {0.0, 0.0, 0.0, 0.0};
+    if (out0) {
+      v4d acc[NCT];
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) acc[ct] = zero;
+#pragma unroll
+      for (int kk = 0; kk < KS0; ++kk) {
+        const int k = 4 * kk + kg;
+        const int part = k / NQ, q = k - part * NQ;
+        double a = 0.0;
+        if (live) {
+          const double wa = s_w[q] * area;
+          if (part < 3) {
+            if (has_c) a = (part == 0 ? g11 : part == 1 ? g12 : g22) * wa * coef_at(c, e, q);
+          } else if (has_v) {
+            a = wa * coef_at(v, e, q);
+          }
+        }
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct)
+          acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, s_t[k * NSP + 16 * ct + li], acc[ct], 0, 0, 0);
+      }
+      // result: lane l, register rr holds D[l/16 + 4 rr][l%16]
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) {
+        const int sc = 16 * ct + li;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int64_t er = tile * kTile + kg + 4 * rr;
+          if (sc < NSYM && er < n_elem) out0[er * NSYM + sc] = acc[ct][rr];
+        }
+      }
+    }
+    // mass-type outputs: the P rows of T with coefficient r (1 when absent) or w
+#pragma unroll
+    for (int o = 1; o <= 2; ++o) {
+      double* out = o == 1 ? out1 : out2;
+      if (!out) continue;
+      const Coef& cw = o == 1 ? r : w;
+      v4d acc[NCT];
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) acc[ct] = zero;
+#pragma unroll
+      for (int kk = 0; kk < KS1; ++kk) {
+        const int q = 4 * kk + kg;
+        double a = 0.0;
+        if (live && q < NQ) a = s_w[q] * area * (cw.p ? coef_at(cw, e, q) : 1.0);
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct)
+          acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, s_t[(3 * NQ + q) * NSP + 16 * ct + li], acc[ct], 0, 0, 0);
+      }
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) {
+        const int sc = 16 * ct + li;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int64_t er = tile * kTile + kg + 4 * rr;
+          if (sc < NSYM && er < n_elem) out[er * NSYM + sc] = acc[ct][rr];
+        }
+      }
+    }
+  }
+}
+
 __global__ void __launch_bounds__(kBlock)
 gather_kernel(int64_t nnz, const int32_t* __restrict__ seg_ptr, const int32_t* __restrict__ seg_src,
               const double* __restrict__ e0, const double* __restrict__ e1,
@@ -220,11 +358,26 @@ QDS_API int qds_assemble_elements(int32_t npe, int32_t nq, int64_t n_elem, const
   QDS_CHECK_ARG(!out0 || c_q || v_q, "qds_assemble_elements: out0 needs c_q or v_q");
   if (n_elem == 0) return QDS_OK;
   const int nsym = npe * (npe + 1) / 2;
+  Coef C{c_q, c_se, c_sq}, V{v_q, v_se, v_sq}, R{r_q, r_se, r_sq}, W{w_q, w_se, w_sq};
+  hipStream_t st = qds_stream(stream);
+  // the default rules of P1-P3 (degree 2p + 2: 6, 12, 16 points) run on MFMA
+  const int64_t tiles = (n_elem + kTile - 1) / kTile;
+  const unsigned mblocks = qds_blocks(tiles, kBlock / kWave, 4096);
+#define QDS_ELEM_MFMA(P, Q)                                                                          \
+  if (npe == P && nq == Q) {                                                                         \
+    hipLaunchKernelGGL((element_mfma_kernel<P, Q>), dim3(mblocks), dim3(kBlock), 0, st, n_elem, conn, x, y, \
+                       tables, C, V, R, W, out0, out1, out2);                                        \
+    QDS_LAUNCH_CHECK();                                                                              \
+    return QDS_OK;                                                                                   \
+  }
+  QDS_ELEM_MFMA(3, 6)
+  QDS_ELEM_MFMA(6, 12)
+  QDS_ELEM_MFMA(10, 16)
+#undef QDS_ELEM_MFMA
   const int per_wave = kWave / nsym;
   const int64_t waves = (n_elem + per_wave - 1) / per_wave;
   const unsigned blocks = qds_blocks(waves, kBlock / kWave, 65536);
-  Coef C{c_q, c_se, c_sq}, V{v_q, v_se, v_sq}, R{r_q, r_se, r_sq}, W{w_q, w_se, w_sq};
-  hipLaunchKernelGGL(element_kernel, dim3(blocks), dim3(kBlock), 0, qds_stream(stream), npe, nq,
+  hipLaunchKernelGGL(element_kernel, dim3(blocks), dim3(kBlock), 0, st, npe, nq,
                      n_elem, conn, x, y, tables, C, V, R, W, out0, out1, out2);
   QDS_LAUNCH_CHECK();
   return QDS_OK;
