@@ -92,6 +92,46 @@ def _spmm_roofline(prob, b, iters=20):
             "csr_equivalent_GBs": csr_bytes / t / 1e9}
 
 
+def _assembly_roofline(prob, iters=10):
+    """HIP-event times of the two assembly kernels on the benchmark mesh (H = K + M(V)
+    and M in one element pass, then the segmented gather into CSR) and their
+    algorithmic bytes: element pass = connectivity + 3 vertex coordinates + the c and
+    v samples per quadrature point + two packed-symmetric FP64 outputs; gather =
+    row pointer + per contribution an index and two FP64 reads + two FP64 outputs."""
+    import torch
+    from qdsim_amd import kernels
+    sp = prob.space
+    c, v = prob.c_q, prob.v_q
+
+    def elem():
+        return kernels.assemble_elements(sp.npe, sp.nq, sp.conn, sp.x, sp.y, sp.tables, c=c, v=v,
+                                         out0=True, out1=True)
+    e = elem()
+    kernels.assemble_gather(sp.nnz, sp.seg_ptr, sp.seg_src, e[0], e[1])
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record(st)
+    for _ in range(iters):
+        e = elem()
+    ev[1].record(st)
+    for _ in range(iters):
+        kernels.assemble_gather(sp.nnz, sp.seg_ptr, sp.seg_src, e[0], e[1])
+    ev[2].record(st)
+    torch.cuda.synchronize()
+    te = ev[0].elapsed_time(ev[1]) / iters * 1e-3
+    tg = ev[1].elapsed_time(ev[2]) / iters * 1e-3
+    E = int(sp.conn.shape[0])
+    nsym = sp.npe * (sp.npe + 1) // 2
+    be = E * (sp.npe * 4 + 6 * 8 + 2 * sp.nq * 8 + 2 * nsym * 8)
+    ncontrib = int(sp.seg_src.numel())
+    bg = 4 * (sp.nnz + 1) + ncontrib * (4 + 2 * 8) + sp.nnz * 2 * 8
+    return {"element_kernel": "element_mfma_kernel<%d,%d> (H and M)" % (sp.npe, sp.nq),
+            "element_us": te * 1e6, "element_bytes": be, "element_GBs": be / te / 1e9,
+            "gather_us": tg * 1e6, "gather_bytes": bg, "gather_GBs": bg / tg / 1e9,
+            "elements": E, "nnz": sp.nnz, "contributions": ncontrib}
+
+
 def _measured_traffic(cells):
     """HBM bytes per operator-product launch (lattice form, FP32, b = 24) from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-calibrated), for the
@@ -142,6 +182,8 @@ def main():
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch the fused SpMM (20x) and a 1 GiB device copy (5x), for rocprofv3")
+    ap.add_argument("--assembly-kernels", action="store_true",
+                    help="only time the assembly kernels (element pass and gather, 10x each) and print them")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -163,6 +205,9 @@ def main():
     print(f"[bench] setup {setup_s:.2f} s: N={mesh.num_nodes} free={prob.n_free} nnz={prob.space.nnz}",
           flush=True)
 
+    if args.assembly_kernels:
+        print(json.dumps({"assembly": _assembly_roofline(prob)}))
+        return
     if args.profile_kernels:
         prob.assemble()
         roof = _spmm_roofline(prob, 24, iters=20)
@@ -216,6 +261,8 @@ def main():
     b = infos[-1]["block"]
     roof = _spmm_roofline(prob, b)
     print(f"[bench] spmm {roof}", flush=True)
+    asm = _assembly_roofline(prob)
+    print(f"[bench] assembly kernels {asm}", flush=True)
 
     # reference eigenvalues for the metric's accuracy test: the all-FP64 solve (not
     # timed); without it, converged pairs are counted by their residuals alone
@@ -269,6 +316,7 @@ def main():
         "rel_err_vs_fp64_gpu": err_vs_fp64,
         "plain_fp64_gpu": fp64,
         "assembly_ms": assemble_ms,
+        "assembly_kernels": asm,
         "setup_s": setup_s,
         "solver_timings": infos[-1]["timings"],
     }
