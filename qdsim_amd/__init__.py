@@ -11,7 +11,8 @@ from .mesh_io import FileMesh, load_csr, load_mesh, load_pattern, save_csr, save
 __all__ = ["SimpleMesh", "create_simple_mesh", "FEMSolver", "EigenSolver", "SelfConsistentSolver",
            "FixedOpenSystemSolver", "PoissonSolver", "SchrodingerProblem", "create_fem_solver",
            "create_eigen_solver", "lifetimes_fs", "FEInterpolator", "create_interpolator",
-           "FileMesh", "load_mesh", "save_mesh", "load_csr", "save_csr", "load_pattern", "save_pattern"]
+           "FileMesh", "load_mesh", "save_mesh", "load_csr", "save_csr", "load_pattern", "save_pattern",
+           "bias_sweep", "bias_sweep_instances", "gate_gradients", "gate_parameter_sets"]
 
 
 def __getattr__(name):
@@ -37,4 +38,7 @@ def __getattr__(name):
     if name == "SchrodingerProblem":
         from .schrodinger import SchrodingerProblem
         return SchrodingerProblem
+    if name in ("bias_sweep", "bias_sweep_instances", "gate_gradients", "gate_parameter_sets"):
+        from . import batch
+        return getattr(batch, name)
     raise AttributeError(name)
