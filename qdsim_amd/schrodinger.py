@@ -204,9 +204,9 @@ class SchrodingerProblem:
         eigenvectors (n, k) float64 device tensor, M_nm-normalised, info).
 
         start: the info dict of an earlier solve of this problem with the same k
-        and block (its final block ``X_block`` and Ritz values ``ritz``): a warm
-        start for a slightly changed potential (the self-consistent loop), which
-        replaces the nested or random start."""
+        and block (its final block ``X_block``): a warm start for a changed
+        potential (the self-consistent loop), which replaces the nested or random
+        start."""
         if self.H is None:
             self.assemble()
         if b is None:
@@ -237,8 +237,13 @@ class SchrodingerProblem:
             self._nested_theta = None
             sigma0 = pre32.sigma
             if _warm(start, self.n_free, b):
-                X0 = start["X_block"]
-                self._nested_theta = start["ritz"]
+                # the start block's Rayleigh quotients under this operator bound its
+                # eigenvalues from above (the previous solve's Ritz values do not:
+                # the operator has changed since)
+                X0 = start["X_block"][:, :b].contiguous()
+                HX0, MX0 = torch.empty_like(X0), torch.empty_like(X0)
+                op64.apply(X0, HX0, MX0)
+                self._nested_theta = kernels.coldot(X0, HX0) / kernels.coldot(X0, MX0)
             else:
                 X0 = self._nested_start(k, b, **(nested_opts or {}))
             if self._nested_theta is not None:
@@ -251,8 +256,20 @@ class SchrodingerProblem:
                 # fine Ritz values of the FP32 phase: the shift of the FP64 phase
                 pre32_64.set_sigma(shift_below(r32.eigenvalues, sigma0, 0.1))
 
-            res = solve_mixed(op64, k, b, pre32, pre64, tol=tol, tol32=tol32, maxiter=maxiter, seed=seed,
-                              verbose=verbose, X0=X0, b64=b64, between=retune, **mixed_opts)
+            try:
+                res = solve_mixed(op64, k, b, pre32, pre64, tol=tol, tol32=tol32, maxiter=maxiter, seed=seed,
+                                  verbose=verbose, X0=X0, b64=b64, between=retune, **mixed_opts)
+                ok = bool(torch.isfinite(res.eigenvalues).all())
+            except np.linalg.LinAlgError:
+                ok = False
+            if not ok:
+                # a shift above the lowest eigenvalue (a block that missed the lowest
+                # state, whose Ritz values then bound the wrong eigenvalues) makes the
+                # multigrid operator indefinite; solve again with the shift below min V
+                pre32.set_sigma(sigma0)
+                pre32_64.set_sigma(sigma0)
+                res = solve_mixed(op64, k, b, pre32, pre64, tol=tol, tol32=tol32, maxiter=maxiter, seed=seed,
+                                  verbose=verbose, X0=X0, b64=b64, **mixed_opts)
         elif mode == "fp64":
             pre = self.preconditioner(b, torch.float64, cycle=dict(nu_pre=2, nu_post=2, coarse_sweeps=40))
             sync()

@@ -43,6 +43,7 @@ class SelfConsistentSolver:
         self.max_iterations = 100
         self.tolerance = 1e-6
         self.mixing = float(mixing)
+        self.warm_start_dphi = 1e-2
         self.fermi_level = 0.0
         n = mesh.num_nodes
         self.potential = np.zeros(n)
@@ -97,8 +98,8 @@ class SelfConsistentSolver:
         """N_A, N_D: scalars as in the reference, or per-node arrays (a doping
         profile such as an abrupt junction); the electron density is rescaled to
         the mean donor density.  Each Schroedinger solve after the first starts
-        from the previous iteration's block (the potential changes by the mixing
-        step only)."""
+        from the previous iteration's block once the potential update max |dphi|
+        is below ``warm_start_dphi`` (10 mV)."""
         t0 = time.time()
         dev = self.schrodinger.space.dev
         if not np.isscalar(N_A):
@@ -122,7 +123,11 @@ class SelfConsistentSolver:
             # Schroedinger with V_total = V - q phi  (eV: V - phi)
             self.schrodinger.set_potential_ev((self.v0_q - self._phi_at_quadrature(phi)).contiguous())
             self.schrodinger.assemble()
-            lam, psi, info = self.schrodinger.solve(num_quantum_states, start=info)
+            # warm start once the potential moves by less than warm_start_dphi (V):
+            # earlier outer iterations can move it by volts, and the previous block
+            # may then miss the new lowest states
+            warm = info if residual < self.warm_start_dphi else None
+            lam, psi, info = self.schrodinger.solve(num_quantum_states, start=warm)
             # psi: M_nm-normalised -> SI normalisation (psi^T M_SI psi = 1)
             psi_si = psi * 1e9
             n, p = self._densities(lam, psi_si, phi, N_A, N_D)
