@@ -243,6 +243,9 @@ typedef struct {
   void* f;
   void* t;
   int64_t ldu, ldf; /* per-point strides of u and f (0 means b) */
+  int32_t neumann_x, neumann_y; /* 0: Dirichlet outside, coarse point j at fine 2j+1 (nx = 2 nx_c + 1);
+                                   1: the boundary rows are unknowns, coarse point j at fine 2j
+                                   (nx = 2 nx_c - 1); describes the coarsening of this level     */
 } qds_mg_level;
 int qds_mg_vcycle(int32_t dtype, int32_t n_levels, const qds_mg_level* levels, int32_t b,
                   int32_t nu_pre, int32_t nu_post, int32_t coarse_sweeps, double omega,
@@ -266,6 +269,16 @@ int qds_cg(int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* indices
            const double* b, double* x, double rtol, int32_t max_iter, int32_t check_every,
            void* workspace, int64_t ws_bytes, int32_t* iters_out /* host */,
            double* relres_out /* host */, void* stream);
+/* The same CG preconditioned by one multigrid V(nu, nu)-cycle per iteration on
+ * an FP64 lattice hierarchy (levels[0] holds the n unknowns in lattice order;
+ * the Poisson problem's insulating top and bottom rows use neumann_y).  Same
+ * workspace (qds_cg_workspace), stopping rule and outputs as qds_cg; replaces
+ * the same spsolve (qdsim_cython/solvers/poisson_solver.pyx:241).               */
+int qds_cg_mg(int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* indices, const double* vals,
+              const double* b, double* x, double rtol, int32_t max_iter, int32_t check_every,
+              int32_t n_levels, const qds_mg_level* levels, int32_t nu, int32_t coarse_sweeps, double omega,
+              void* workspace, int64_t ws_bytes, int32_t* iters_out /* host */,
+              double* relres_out /* host */, void* stream);
 
 #ifdef __cplusplus
 }

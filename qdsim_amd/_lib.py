@@ -60,6 +60,8 @@ _SIGNATURES = {
     "qds_cg_workspace": ([_i64], _i64),
     "qds_cg": ([_i64, _i64, _p, _p, _p, _p, _p, _d, _i32, _i32, _p, _i64, _c.POINTER(_i32),
                 _c.POINTER(_d), _p], _c.c_int),
+    "qds_cg_mg": ([_i64, _i64, _p, _p, _p, _p, _p, _d, _i32, _i32, _i32, _p, _i32, _i32, _d, _p, _i64,
+                   _c.POINTER(_i32), _c.POINTER(_d), _p], _c.c_int),
 }
 
 
@@ -82,7 +84,8 @@ class LatticeStencil(ctypes.Structure):
 class MGLevel(ctypes.Structure):
     """Mirror of qds_mg_level."""
     _fields_ = [("nx", _i64), ("ny", _i64), ("d", _p), ("we", _p), ("wn", _p),
-                ("u", _p), ("f", _p), ("t", _p), ("ldu", _i64), ("ldf", _i64)]
+                ("u", _p), ("f", _p), ("t", _p), ("ldu", _i64), ("ldf", _i64),
+                ("neumann_x", _i32), ("neumann_y", _i32)]
 
 
 class QDSError(RuntimeError):

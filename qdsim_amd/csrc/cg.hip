@@ -8,10 +8,7 @@
 // The whole iteration runs on the device: alpha and beta live in device memory
 // and the host only reads the residual norm every `check_every` iterations.
 #include "qds_common.h"
-
-QDS_API int qds_spmm(int32_t dtype, int64_t n_rows, int64_t nnz, int32_t b, const int32_t* indptr,
-                     const int32_t* indices, const void* vals_a, const void* vals_b, const void* X,
-                     int64_t ldx, void* Y_a, int64_t ldy_a, void* Y_b, int64_t ldy_b, void* stream);
+#include "../../include/qdsim_hip.h"   // qds_spmm, qds_mg_level, qds_mg_vcycle
 
 namespace {
 
@@ -100,6 +97,24 @@ cg_update_p(int64_t n, const double* __restrict__ sc, const double* __restrict__
   const double beta = sc[4];
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     p[i] = z[i] + beta * p[i];
+}
+
+// x += alpha p ; r -= alpha q (the preconditioner runs separately)
+__global__ void __launch_bounds__(kBlock)
+cg_update_x_r(int64_t n, const double* __restrict__ sc, double* __restrict__ x, double* __restrict__ r,
+              const double* __restrict__ p, const double* __restrict__ q) {
+  const double alpha = sc[3];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    r[i] -= alpha * q[i];
+  }
+}
+
+// r = b - r (r holds A x on entry)
+__global__ void __launch_bounds__(kBlock)
+cg_residual(int64_t n, const double* __restrict__ b, double* __restrict__ r) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    r[i] = b[i] - r[i];
 }
 
 // r = b - r (r holds A x on entry); z = r * dinv; p = z
@@ -191,6 +206,90 @@ QDS_API int qds_cg(int64_t n, int64_t nnz, const int32_t* indptr, const int32_t*
       hipLaunchKernelGGL(cg_update_xr, g, blk, 0, st, n, sc, x, r, z, p, q, dinv);
       hipLaunchKernelGGL(dot2_partial, g, blk, 0, st, n, r, z, r, r, partial);
       hipLaunchKernelGGL(dot2_final, dim3(1), blk, 0, st, nblk, partial, sc, 2);
+      hipLaunchKernelGGL(cg_update_p, g, blk, 0, st, n, sc, z, p);
+      QDS_LAUNCH_CHECK();
+    }
+    QDS_HIP(hipMemcpyAsync(&rr, sc + 2, sizeof(double), hipMemcpyDeviceToHost, st));
+    QDS_HIP(hipStreamSynchronize(st));
+  }
+  if (iters_out) *iters_out = it;
+  if (relres_out) *relres_out = sqrt(rr / bb);
+  return QDS_OK;
+}
+
+// Conjugate gradients preconditioned by one multigrid V(nu, nu)-cycle per
+// iteration (qds_mg_vcycle, FP64 hierarchy, one column).  levels[0] describes the
+// unknowns of the system in lattice order (levels[0].nx * levels[0].ny == n); its
+// f and u are replaced by the CG residual and preconditioned residual.  Same
+// stopping rule, workspace and outputs as qds_cg.
+QDS_API int qds_cg_mg(int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* indices, const double* vals,
+                      const double* b, double* x, double rtol, int32_t max_iter, int32_t check_every,
+                      int32_t n_levels, const qds_mg_level* levels, int32_t nu, int32_t coarse_sweeps,
+                      double omega, void* workspace, int64_t ws_bytes, int32_t* iters_out, double* relres_out,
+                      void* stream) {
+  QDS_CHECK_ARG(indptr && indices && vals && b && x && workspace && levels, "qds_cg_mg: null pointer");
+  QDS_CHECK_ARG(n_levels >= 1 && n_levels <= 32 && levels[0].nx * levels[0].ny == n,
+                "qds_cg_mg: level 0 (%lld x %lld) does not hold the %lld unknowns", (long long)levels[0].nx,
+                (long long)levels[0].ny, (long long)n);
+  QDS_CHECK_ARG(ws_bytes >= qds_cg_workspace(n), "qds_cg_mg: workspace too small");
+  if (check_every < 1) check_every = 1;
+  hipStream_t st = qds_stream(stream);
+  double* r = static_cast<double*>(workspace);
+  double* z = r + n;
+  double* p = z + n;
+  double* q = p + n;
+  double* partial = q + 2 * n;   // after the (unused) Jacobi diagonal slot
+  double* sc = partial + 2 * kRedBlocks;
+  const dim3 g(qds_blocks(n, kBlock, kRedBlocks)), blk(kBlock);
+  const int nblk = (int)g.x;
+  qds_mg_level lv[32];
+  for (int l = 0; l < n_levels; ++l) lv[l] = levels[l];
+  lv[0].f = r;
+  lv[0].u = z;
+  lv[0].ldf = lv[0].ldu = 1;
+  auto precondition = [&]() {
+    return qds_mg_vcycle(qds::F64, n_levels, lv, 1, nu, nu, coarse_sweeps, omega, stream);
+  };
+
+  hipLaunchKernelGGL(dot2_partial, g, blk, 0, st, n, b, b, (const double*)nullptr, (const double*)nullptr,
+                     partial);
+  hipLaunchKernelGGL(dot2_final, dim3(1), blk, 0, st, nblk, partial, sc, 1);  // sc[1] = b.b
+  QDS_LAUNCH_CHECK();
+  double bb = 0.0;
+  QDS_HIP(hipMemcpyAsync(&bb, sc + 1, sizeof(double), hipMemcpyDeviceToHost, st));
+  QDS_HIP(hipStreamSynchronize(st));
+  if (bb == 0.0) {
+    QDS_HIP(hipMemsetAsync(x, 0, n * sizeof(double), st));
+    if (iters_out) *iters_out = 0;
+    if (relres_out) *relres_out = 0.0;
+    return QDS_OK;
+  }
+  int rc = qds_spmm(qds::F64, n, nnz, 1, indptr, indices, vals, nullptr, x, 1, r, 1, nullptr, 1, stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(cg_residual, g, blk, 0, st, n, b, r);
+  QDS_LAUNCH_CHECK();
+  if ((rc = precondition())) return rc;
+  QDS_HIP(hipMemcpyAsync(p, z, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(dot2_partial, g, blk, 0, st, n, r, z, r, r, partial);
+  hipLaunchKernelGGL(dot2_final, dim3(1), blk, 0, st, nblk, partial, sc, 0);   // rz, rr
+  QDS_LAUNCH_CHECK();
+  int it = 0;
+  double rr = 0.0;
+  const double tol2 = rtol * rtol * bb;
+  QDS_HIP(hipMemcpyAsync(&rr, sc + 2, sizeof(double), hipMemcpyDeviceToHost, st));
+  QDS_HIP(hipStreamSynchronize(st));
+  while (rr > tol2 && it < max_iter) {
+    for (int k = 0; k < check_every && it < max_iter; ++k, ++it) {
+      rc = qds_spmm(qds::F64, n, nnz, 1, indptr, indices, vals, nullptr, p, 1, q, 1, nullptr, 1, stream);
+      if (rc) return rc;
+      hipLaunchKernelGGL(dot2_partial, g, blk, 0, st, n, p, q, (const double*)nullptr,
+                         (const double*)nullptr, partial);
+      hipLaunchKernelGGL(dot2_final, dim3(1), blk, 0, st, nblk, partial, sc, 1);   // alpha
+      hipLaunchKernelGGL(cg_update_x_r, g, blk, 0, st, n, sc, x, r, p, q);
+      QDS_LAUNCH_CHECK();
+      if ((rc = precondition())) return rc;
+      hipLaunchKernelGGL(dot2_partial, g, blk, 0, st, n, r, z, r, r, partial);
+      hipLaunchKernelGGL(dot2_final, dim3(1), blk, 0, st, nblk, partial, sc, 2);   // beta, rr
       hipLaunchKernelGGL(cg_update_p, g, blk, 0, st, n, sc, z, p);
       QDS_LAUNCH_CHECK();
     }

@@ -10,8 +10,12 @@
 // five-point stencil with edge weights w = c * h_perp / h_par, plus a lumped mass
 // term.  Per level: d[p] (diagonal, including the couplings to Dirichlet
 // neighbours), we[p] (edge p -> east), wn[p] (edge p -> north), zero on edges that
-// leave the interior.  Coarse levels come from vertex-centred coarsening
-// (fine interior 2*nc + 1), bilinear prolongation P and restriction R = P^T.
+// leave the interior.  Coarse levels come from vertex-centred coarsening,
+// bilinear prolongation P and restriction R = P^T.  Per direction a level either
+// has Dirichlet neighbours outside (coarse point j at fine 2j + 1, fine count
+// 2 nc + 1: the eigensolver's interior lattice) or Neumann boundary rows that are
+// unknowns themselves (coarse point j at fine 2j, fine count 2 nc - 1: the
+// insulating top and bottom of the Poisson problem).
 //
 // Vectors hold b columns per lattice point (row-major point x column), matching
 // the eigensolver's block layout.  One V(nu1, nu2)-cycle with damped Jacobi
@@ -34,6 +38,7 @@ struct Level {
   T* f;
   T* t;
   int64_t ldu, ldf;  // per-point strides of u and f (t is dense, stride b)
+  int32_t nbx, nby;  // 1: coarsening of this level keeps its boundary rows (Neumann), 0: Dirichlet
 };
 
 // Column vectors: each thread owns VEC consecutive columns of one lattice point,
@@ -62,6 +67,21 @@ struct Vec {
     *reinterpret_cast<typename V<T, VEC>::t*>(p) = x;
   }
 };
+
+// 1-D bilinear weights of fine index i: coarse point j sits at fine 2j + o
+// (o = 1 Dirichlet, 0 Neumann); a coincident point has weight 1, a point between
+// two coarse points 1/2 each (indices outside the coarse range are skipped by the
+// caller)
+__device__ __forceinline__ int coarse_nbrs(int i, int o, int* js) {
+  const int r = i - o;
+  if ((r & 1) == 0) {
+    js[0] = r / 2;
+    return 1;
+  }
+  js[0] = (r - 1) / 2;   // r >= -1, so (r - 1) / 2 >= -1
+  js[1] = (r + 1) / 2;
+  return 2;
+}
 
 // VEC values of a block stored in type S (the level-0 right-hand side and result
 // may be FP64 blocks of an FP64 eigensolver phase while the hierarchy is FP32)
@@ -164,7 +184,7 @@ __device__ __forceinline__ void restrict_items(const Level<T>& F, const Level<T>
     const uint32_t pc = t / nv;
     const uint32_t c0 = (t - pc * nv) * VEC;
     const uint32_t jy = pc / (uint32_t)C.nx, jx = pc - jy * (uint32_t)C.nx;
-    const uint32_t fx = 2 * jx + 1, fy = 2 * jy + 1;
+    const uint32_t fx = 2 * jx + 1 - F.nbx, fy = 2 * jy + 1 - F.nby;
     Vec<T, VEC> acc;
 #pragma unroll
     for (int i = 0; i < VEC; ++i) acc.v[i] = T(0);
@@ -173,6 +193,7 @@ __device__ __forceinline__ void restrict_items(const Level<T>& F, const Level<T>
 #pragma unroll
       for (int dx = -1; dx <= 1; ++dx) {
         const uint32_t x = fx + dx, y = fy + dy;
+        if (x >= (uint32_t)F.nx || y >= (uint32_t)F.ny) continue;   // outside a Neumann boundary
         const T w = (dx == 0 ? T(1) : T(0.5)) * (dy == 0 ? T(1) : T(0.5));
         const uint32_t p = y * (uint32_t)F.nx + x;
         Vec<T, VEC> fv;
@@ -196,14 +217,10 @@ __device__ __forceinline__ void prolong_items(const Level<T>& F, const Level<T>&
     const uint32_t p = t / nv;
     const uint32_t c0 = (t - p * nv) * VEC;
     const uint32_t iy = p / (uint32_t)F.nx, ix = p - iy * (uint32_t)F.nx;
-    // 1-D weights: odd fine index -> one coarse point, even -> two neighbours
     int xs[2], ys[2];
-    T wx[2], wy[2];
-    int nxp, nyp;
-    if (ix & 1) { xs[0] = (int)(ix - 1) / 2; wx[0] = T(1); nxp = 1; }
-    else { xs[0] = (int)ix / 2 - 1; xs[1] = (int)ix / 2; wx[0] = wx[1] = T(0.5); nxp = 2; }
-    if (iy & 1) { ys[0] = (int)(iy - 1) / 2; wy[0] = T(1); nyp = 1; }
-    else { ys[0] = (int)iy / 2 - 1; ys[1] = (int)iy / 2; wy[0] = wy[1] = T(0.5); nyp = 2; }
+    const int nxp = coarse_nbrs((int)ix, 1 - F.nbx, xs);
+    const int nyp = coarse_nbrs((int)iy, 1 - F.nby, ys);
+    const T wx = nxp == 1 ? T(1) : T(0.5), wy = nyp == 1 ? T(1) : T(0.5);
     Vec<T, VEC> acc, e;
 #pragma unroll
     for (int i = 0; i < VEC; ++i) acc.v[i] = T(0);
@@ -212,7 +229,7 @@ __device__ __forceinline__ void prolong_items(const Level<T>& F, const Level<T>&
       for (int q = 0; q < nxp; ++q) {
         if (xs[q] < 0 || xs[q] >= (int)C.nx) continue;
         e.load(C.u + ((uint32_t)ys[a] * (uint32_t)C.nx + (uint32_t)xs[q]) * (uint32_t)C.ldu + c0);
-        const T w = wx[q] * wy[a];
+        const T w = wx * wy;
 #pragma unroll
         for (int i = 0; i < VEC; ++i) acc.v[i] += w * e.v[i];
       }
@@ -303,16 +320,15 @@ __device__ __forceinline__ Vec<T, VEC> nbr_scaled(const Level<T>& L, uint32_t ld
   return s;
 }
 
-// (P e)[p] for fine point (ix, iy): bilinear weights of the coarse points around it
+// (P e)[p] for fine point (ix, iy) of level F: bilinear weights of the coarse
+// points around it
 template <typename T, int VEC>
-__device__ __forceinline__ Vec<T, VEC> prolong_at(const Level<T>& C, uint32_t ix, uint32_t iy, uint32_t c0) {
+__device__ __forceinline__ Vec<T, VEC> prolong_at(const Level<T>& C, int nbx, int nby, uint32_t ix, uint32_t iy,
+                                                  uint32_t c0) {
   int xs[2], ys[2];
-  T wx[2], wy[2];
-  int nxp, nyp;
-  if (ix & 1) { xs[0] = (int)(ix - 1) / 2; wx[0] = T(1); nxp = 1; }
-  else { xs[0] = (int)ix / 2 - 1; xs[1] = (int)ix / 2; wx[0] = wx[1] = T(0.5); nxp = 2; }
-  if (iy & 1) { ys[0] = (int)(iy - 1) / 2; wy[0] = T(1); nyp = 1; }
-  else { ys[0] = (int)iy / 2 - 1; ys[1] = (int)iy / 2; wy[0] = wy[1] = T(0.5); nyp = 2; }
+  const int nxp = coarse_nbrs((int)ix, 1 - nbx, xs);
+  const int nyp = coarse_nbrs((int)iy, 1 - nby, ys);
+  const T wx = nxp == 1 ? T(1) : T(0.5), wy = nyp == 1 ? T(1) : T(0.5);
   Vec<T, VEC> acc, e;
 #pragma unroll
   for (int i = 0; i < VEC; ++i) acc.v[i] = T(0);
@@ -321,7 +337,7 @@ __device__ __forceinline__ Vec<T, VEC> prolong_at(const Level<T>& C, uint32_t ix
     for (int q = 0; q < nxp; ++q) {
       if (xs[q] < 0 || xs[q] >= (int)C.nx) continue;
       e.load(C.u + ((uint32_t)ys[a] * (uint32_t)C.nx + (uint32_t)xs[q]) * (uint32_t)C.ldu + c0);
-      const T w = wx[q] * wy[a];
+      const T w = wx * wy;
 #pragma unroll
       for (int i = 0; i < VEC; ++i) acc.v[i] += w * e.v[i];
     }
@@ -349,7 +365,10 @@ __device__ __forceinline__ void down_items(const Level<T>& F, const Level<T>& C,
     for (int dy = -1; dy <= 1; ++dy) {
 #pragma unroll
       for (int dx = -1; dx <= 1; ++dx) {
-        const uint32_t x = 2 * jx + 1 + dx, y = 2 * jy + 1 + dy;
+        // fine neighbour of the coarse point; beside a Neumann boundary row the
+        // outer neighbour does not exist (unsigned wrap makes it >= nx / ny)
+        const uint32_t x = 2 * jx + 1 - F.nbx + dx, y = 2 * jy + 1 - F.nby + dy;
+        if (x >= (uint32_t)F.nx || y >= (uint32_t)F.ny) continue;
         const T w = (dx == 0 ? T(1) : T(0.5)) * (dy == 0 ? T(1) : T(0.5));
         const uint32_t p = y * (uint32_t)F.nx + x;
         Vec<T, VEC> fv;
@@ -380,7 +399,7 @@ __device__ __forceinline__ void up_items(const Level<T>& F, const Level<T>& C, i
     const uint32_t iy = p / nx, ix = p - iy * nx;
     // u2 at p and its neighbours, then one Jacobi step at p
     auto u2 = [&](uint32_t q, uint32_t qx, uint32_t qy) {
-      Vec<T, VEC> fv, e = prolong_at<T, VEC>(C, qx, qy, c0);
+      Vec<T, VEC> fv, e = prolong_at<T, VEC>(C, F.nbx, F.nby, qx, qy, c0);
       load_as<T, VEC, S>(fv, ff + q * ldf + c0);
       const T sc = omega / F.d[q];
 #pragma unroll
@@ -677,7 +696,8 @@ size_t up_tiled_lds(int b) {
 // least a few tiles; smaller or odd-width levels keep the grid-stride passes
 template <typename T>
 bool use_tiled(const Level<T>& L, int b, int vec) {
-  return vec * (int)sizeof(T) == 16 && b % vec == 0 && b <= 32 && L.nx >= 32 && L.ny >= 32;
+  // the tiled passes assume Dirichlet coarsening in both directions
+  return vec * (int)sizeof(T) == 16 && b % vec == 0 && b <= 32 && L.nx >= 32 && L.ny >= 32 && !L.nbx && !L.nby;
 }
 
 // Levels handled by the single-workgroup coarse cycle (level l0 and below).
@@ -905,19 +925,31 @@ typedef struct {
   void* f;
   void* t;
   int64_t ldu, ldf;  // per-point strides of u and f (0 means b)
+  int32_t neumann_x, neumann_y;  // 1: boundary rows are unknowns and coarse points (see the top of this file)
 } qds_mg_level;
 }
 
+namespace {
+// level l + 1 is the vertex-centred coarsening of level l in both directions
+bool coarsening_ok(const qds_mg_level* lv, int n_levels) {
+  for (int l = 0; l + 1 < n_levels; ++l) {
+    const int64_t ex = 2 * lv[l + 1].nx + 1 - 2 * (lv[l].neumann_x ? 1 : 0);
+    const int64_t ey = 2 * lv[l + 1].ny + 1 - 2 * (lv[l].neumann_y ? 1 : 0);
+    if (lv[l].nx != ex || lv[l].ny != ey) return false;
+  }
+  return true;
+}
+}  // namespace
+
 // One V-cycle for A z = r on level 0.  levels[0].f must hold r and levels[0].u
 // receives z (callers may point them at their own buffers).  Requires
-// levels[l].nx == 2 * levels[l+1].nx + 1 (same for ny).
+// levels[l].nx == 2 * levels[l+1].nx + 1 (same for ny), or 2 * levels[l+1].nx - 1
+// in a direction where levels[l] sets neumann_x (neumann_y).
 QDS_API int qds_mg_vcycle(int32_t dtype, int32_t n_levels, const qds_mg_level* levels, int32_t b,
                           int32_t nu_pre, int32_t nu_post, int32_t coarse_sweeps, double omega,
                           void* stream) {
   QDS_CHECK_ARG(levels && n_levels >= 1 && n_levels <= 32 && b >= 1, "qds_mg_vcycle: bad arguments");
-  for (int l = 0; l + 1 < n_levels; ++l)
-    QDS_CHECK_ARG(levels[l].nx == 2 * levels[l + 1].nx + 1 && levels[l].ny == 2 * levels[l + 1].ny + 1,
-                  "qds_mg_vcycle: level %d is not a vertex-centred coarsening of level %d", l + 1, l);
+  QDS_CHECK_ARG(coarsening_ok(levels, n_levels), "qds_mg_vcycle: levels are not vertex-centred coarsenings");
   for (int l = 0; l < n_levels; ++l)
     QDS_CHECK_ARG(levels[l].d && levels[l].we && levels[l].wn && levels[l].u && levels[l].f && levels[l].t,
                   "qds_mg_vcycle: null pointer on level %d", l);
@@ -932,7 +964,8 @@ QDS_API int qds_mg_vcycle(int32_t dtype, int32_t n_levels, const qds_mg_level* l
       L[l] = Level<float>{levels[l].nx, levels[l].ny, (const float*)levels[l].d, (const float*)levels[l].we,
                           (const float*)levels[l].wn, (float*)levels[l].u, (float*)levels[l].f,
                           (float*)levels[l].t, levels[l].ldu ? levels[l].ldu : b,
-                          levels[l].ldf ? levels[l].ldf : b};
+                          levels[l].ldf ? levels[l].ldf : b, levels[l].neumann_x ? 1 : 0,
+                          levels[l].neumann_y ? 1 : 0};
     return vcycle_any<float>(n_levels, L, b, nu_pre, nu_post, coarse_sweeps, (float)omega, st);
   }
   if (dtype == qds::F64) {
@@ -941,7 +974,8 @@ QDS_API int qds_mg_vcycle(int32_t dtype, int32_t n_levels, const qds_mg_level* l
       L[l] = Level<double>{levels[l].nx, levels[l].ny, (const double*)levels[l].d,
                            (const double*)levels[l].we, (const double*)levels[l].wn,
                            (double*)levels[l].u, (double*)levels[l].f, (double*)levels[l].t,
-                           levels[l].ldu ? levels[l].ldu : b, levels[l].ldf ? levels[l].ldf : b};
+                           levels[l].ldu ? levels[l].ldu : b, levels[l].ldf ? levels[l].ldf : b,
+                           levels[l].neumann_x ? 1 : 0, levels[l].neumann_y ? 1 : 0};
     return vcycle_any<double>(n_levels, L, b, nu_pre, nu_post, coarse_sweeps, omega, st);
   }
   QDS_CHECK_ARG(false, "qds_mg_vcycle: dtype must be F32 or F64");
@@ -958,9 +992,7 @@ QDS_API int qds_mg_vcycle_io(int32_t dtype, int32_t io_dtype, int32_t n_levels, 
     return qds_mg_vcycle(dtype, n_levels, levels, b, nu_pre, nu_post, coarse_sweeps, omega, stream);
   QDS_CHECK_ARG(dtype == qds::F32 && io_dtype == qds::F64, "qds_mg_vcycle_io: only an FP32 hierarchy with FP64 level-0 blocks");
   QDS_CHECK_ARG(levels && n_levels >= 2 && n_levels <= 32 && b >= 1, "qds_mg_vcycle_io: bad arguments");
-  for (int l = 0; l + 1 < n_levels; ++l)
-    QDS_CHECK_ARG(levels[l].nx == 2 * levels[l + 1].nx + 1 && levels[l].ny == 2 * levels[l + 1].ny + 1,
-                  "qds_mg_vcycle_io: level %d is not a vertex-centred coarsening of level %d", l + 1, l);
+  QDS_CHECK_ARG(coarsening_ok(levels, n_levels), "qds_mg_vcycle_io: levels are not vertex-centred coarsenings");
   for (int l = 0; l < n_levels; ++l)
     QDS_CHECK_ARG(levels[l].d && levels[l].we && levels[l].wn && levels[l].u && levels[l].f && levels[l].t,
                   "qds_mg_vcycle_io: null pointer on level %d", l);
@@ -972,7 +1004,8 @@ QDS_API int qds_mg_vcycle_io(int32_t dtype, int32_t io_dtype, int32_t n_levels, 
   for (int l = 0; l < n_levels; ++l)
     L[l] = Level<float>{levels[l].nx, levels[l].ny, (const float*)levels[l].d, (const float*)levels[l].we,
                         (const float*)levels[l].wn, (float*)levels[l].u, (float*)levels[l].f, (float*)levels[l].t,
-                        levels[l].ldu ? levels[l].ldu : b, levels[l].ldf ? levels[l].ldf : b};
+                        levels[l].ldu ? levels[l].ldu : b, levels[l].ldf ? levels[l].ldf : b,
+                        levels[l].neumann_x ? 1 : 0, levels[l].neumann_y ? 1 : 0};
   // level 0's FP32 pointers are never dereferenced in the mixed cycle; its f / u are
   // the FP64 blocks; its t is the hierarchy's own scratch
   const double* f0 = (const double*)levels[0].f;

@@ -280,6 +280,19 @@ def mg_apply(dtype_code, level_struct, b, u, out):
                            ptr(u), ptr(out), stream_ptr()), "qds_mg_apply")
 
 
+def cg_mg(indptr, indices, vals, n, nnz, b, x, levels_struct, n_levels, nu=1, coarse=10, omega=0.8, rtol=1e-10,
+          max_iter=10000, check_every=5):
+    """CG preconditioned by a multigrid V-cycle (qds_cg_mg); x holds the initial guess."""
+    wsb = lib.qds_cg_workspace(n)
+    wp, wn = workspace("cg").ptr(wsb)
+    it = ctypes.c_int32()
+    rr = ctypes.c_double()
+    check(lib.qds_cg_mg(n, nnz, ptr(indptr), ptr(indices), ptr(vals), ptr(b), ptr(x), float(rtol), int(max_iter),
+                        int(check_every), int(n_levels), levels_struct, int(nu), int(coarse), float(omega), wp, wn,
+                        ctypes.byref(it), ctypes.byref(rr), stream_ptr()), "qds_cg_mg")
+    return it.value, rr.value
+
+
 def cg(indptr, indices, vals, n, nnz, b, x, rtol=1e-10, max_iter=10000, check_every=10):
     wsb = lib.qds_cg_workspace(n)
     wp, wn = workspace("cg").ptr(wsb)

@@ -209,3 +209,97 @@ class LatticeMG:
         s0.ldu = W.stride(0)
         kernels.mg_vcycle(_lib.F32 if self.dtype == torch.float32 else _lib.F64, self._structs,
                           self.n_levels, self.b, self.nu_pre, self.nu_post, self.coarse_sweeps, self.omega)
+
+
+class PoissonLatticeMG:
+    """Multigrid hierarchy (FP64, one column) for the Poisson stiffness on the
+    unknowns of the self-consistent loop's Poisson problem: Dirichlet contacts at
+    x = 0 and x = Lx, insulating (natural) boundary at y = 0 and y = Ly, so the
+    unknowns are lattice columns 1 .. NX-2 of every row 0 .. NY-1, numbered row
+    by row.  Low-order-refined five-point operator with edge weight
+    eps * h_perp / h_par (eps averaged over the FE elements on the edge; a
+    boundary-row edge has one element and half the weight).  Coarsening is
+    vertex-centred: Dirichlet in x (coarse column j at fine 2j + 1), Neumann in y
+    (the boundary rows are coarse points: coarse row j at fine 2j), coarse edge
+    weight = harmonic mean of the two fine edges it spans.  Used by
+    ``qds_cg_mg`` as the CG preconditioner (one V(1,1)-cycle per iteration)."""
+
+    def __init__(self, lattice, eps_elem, min_coarse=3):
+        L = lattice
+        dev = eps_elem.device
+        f64 = dict(dtype=torch.float64, device=dev)
+        eps = eps_elem.to(torch.float64)
+        NX, NY = L.NX, L.NY
+        iy = torch.arange(NY, **f64)[:, None]
+        ix = torch.arange(NX - 1, **f64)[None, :]
+        # horizontal edges between full columns X and X+1 (X = 0 .. NX-2): (NY, NX-1)
+        xm = ((ix + 0.5) * L.hx).expand(NY, NX - 1)
+        yy = (iy * L.hy).expand(NY, NX - 1)
+        above = eps[L.element_at(xm, yy + 0.25 * L.hy)] * (iy < NY - 1).to(torch.float64)
+        below = eps[L.element_at(xm, yy - 0.25 * L.hy)] * (iy > 0).to(torch.float64)
+        WE = 0.5 * (above + below) * (L.hy / L.hx)
+        # vertical edges between rows Y and Y+1 at the unknown columns X = 1 .. NX-2
+        jy = torch.arange(NY - 1, **f64)[:, None]
+        jx = torch.arange(1, NX - 1, **f64)[None, :]
+        ym = ((jy + 0.5) * L.hy).expand(NY - 1, NX - 2)
+        xx = (jx * L.hx).expand(NY - 1, NX - 2)
+        WN = 0.5 * (eps[L.element_at(xx + 0.25 * L.hx, ym)] + eps[L.element_at(xx - 0.25 * L.hx, ym)]) * (L.hx / L.hy)
+        self.levels_data = []
+        while True:
+            self.levels_data.append(self._level(WE, WN, dev))
+            ny, nx = WE.shape[0], WE.shape[1] - 1
+            if not (nx % 2 == 1 and ny % 2 == 1 and (nx - 1) // 2 >= min_coarse and (ny + 1) // 2 >= min_coarse):
+                break
+            WE, WN = self._coarsen(WE, WN)
+        self.n_levels = len(self.levels_data)
+        self._structs = (_lib.MGLevel * self.n_levels)()
+        for l, lv in enumerate(self.levels_data):
+            s = self._structs[l]
+            s.nx, s.ny = lv["nx"], lv["ny"]
+            s.d, s.we, s.wn = lv["d"].data_ptr(), lv["we"].data_ptr(), lv["wn"].data_ptr()
+            s.u, s.f, s.t = lv["u"].data_ptr(), lv["f"].data_ptr(), lv["t"].data_ptr()
+            s.ldu = s.ldf = 0
+            s.neumann_x, s.neumann_y = 0, 1
+
+    nu, coarse_sweeps, omega = 1, 10, 0.8
+
+    def __call__(self, r, z):
+        """z = V(1,1)-cycle(r) for one column (tests; the CG calls the cycle itself)."""
+        assert r.dtype == torch.float64 and r.is_contiguous() and z.is_contiguous()
+        s0 = self._structs[0]
+        f_old, u_old = s0.f, s0.u
+        s0.f, s0.u = r.data_ptr(), z.data_ptr()
+        try:
+            kernels.mg_vcycle(_lib.F64, self._structs, self.n_levels, 1, self.nu, self.nu, self.coarse_sweeps,
+                              self.omega)
+        finally:
+            s0.f, s0.u = f_old, u_old
+
+    @staticmethod
+    def _level(WE, WN, dev):
+        # WE (ny, nx+1) includes the edges to the Dirichlet columns; WN (ny-1, nx)
+        ny, nx = WE.shape[0], WE.shape[1] - 1
+        d = WE[:, :-1] + WE[:, 1:]
+        d = d.clone()
+        d[:-1] += WN
+        d[1:] += WN
+        we = torch.zeros((ny, nx), dtype=torch.float64, device=dev)
+        we[:, :-1] = WE[:, 1:-1]
+        wn = torch.zeros((ny, nx), dtype=torch.float64, device=dev)
+        wn[:-1] = WN
+        n = nx * ny
+        return {"nx": nx, "ny": ny, "d": d.contiguous().reshape(-1), "we": we.reshape(-1), "wn": wn.reshape(-1),
+                "u": torch.zeros(n, dtype=torch.float64, device=dev),
+                "f": torch.zeros(n, dtype=torch.float64, device=dev),
+                "t": torch.zeros(n, dtype=torch.float64, device=dev)}
+
+    @staticmethod
+    def _coarsen(WE, WN):
+        def hmean(a, b):
+            s = a + b
+            return torch.where(s > 0, 2.0 * a * b / torch.where(s > 0, s, torch.ones_like(s)), torch.zeros_like(s))
+        # coarse rows = fine rows 2R; coarse edge E spans fine edges 2E and 2E+1
+        WEc = hmean(WE[0::2, 0::2], WE[0::2, 1::2])
+        # coarse columns = fine columns 2j+1; coarse vertical edge R spans fine rows 2R, 2R+1
+        WNc = hmean(WN[0::2, 1::2], WN[1::2, 1::2])
+        return WEc.contiguous(), WNc.contiguous()

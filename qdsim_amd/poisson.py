@@ -8,8 +8,11 @@ consistent mass, right-hand side M (-rho_nodal) (:254-274; the reference's sign
 convention is kept), Dirichlet V_p on x = 0 and V_n on x = Lx (:276-306), natural
 boundary elsewhere.  The reference factorises the row-replaced matrix with spsolve
 (:241); here the Dirichlet values are lifted into the right-hand side, the
-remaining SPD system is solved with the device CG (qds_cg), and the solution of
-the row-replaced system is identical in exact arithmetic.
+remaining SPD system is solved with the device CG, and the solution of the
+row-replaced system is identical in exact arithmetic.  On a structured mesh the
+unknowns are numbered in lattice order and the CG is preconditioned by one
+multigrid V-cycle per iteration (qds_cg_mg, PoissonLatticeMG); other meshes use
+Jacobi (qds_cg).
 """
 import time
 
@@ -45,6 +48,17 @@ class PoissonProblem:
         fixed[self.left] = True
         fixed[self.right] = True
         free = np.nonzero(~fixed & used)[0]
+        # on a structured mesh the unknowns are lattice columns 1 .. rx-2 of every
+        # row: numbered row by row, the system gets a multigrid preconditioner
+        self.mg = None
+        if hasattr(mesh, "lattice_y") and rx >= 3:
+            ly = np.asarray(mesh.lattice_y)
+            key = ly[free] * (rx - 2) + (lx[free] - 1)
+            if free.size == (rx - 2) * ry and np.array_equal(np.sort(key), np.arange(free.size)):
+                free = free[np.argsort(key)]
+                from .lattice import Lattice, PoissonLatticeMG
+                eps_e = eps.mean(dim=1) if eps.dim() == 2 else eps
+                self.mg = PoissonLatticeMG(Lattice(mesh), eps_e)
         self.free = free
         new = np.full(mesh.num_nodes, -1, dtype=np.int64)
         new[free] = np.arange(free.size)
@@ -74,8 +88,13 @@ class PoissonProblem:
         t = self.spmv(self.K, g)
         b = (rhs - t)[self.free_d].contiguous()
         x = torch.zeros_like(b) if x0 is None else x0[self.free_d].contiguous()
-        it, rr = kernels.cg(self.r_indptr, self.r_indices, self.K_ff, self.free.size,
-                            int(self.r_indices.numel()), b, x, rtol=rtol, max_iter=20000)
+        if self.mg is not None:
+            it, rr = kernels.cg_mg(self.r_indptr, self.r_indices, self.K_ff, self.free.size,
+                                   int(self.r_indices.numel()), b, x, self.mg._structs, self.mg.n_levels,
+                                   self.mg.nu, self.mg.coarse_sweeps, self.mg.omega, rtol=rtol, max_iter=2000)
+        else:
+            it, rr = kernels.cg(self.r_indptr, self.r_indices, self.K_ff, self.free.size,
+                                int(self.r_indices.numel()), b, x, rtol=rtol, max_iter=20000)
         self.last_iterations, self.last_relres = it, rr
         phi = g.clone()
         phi[self.free_d] = x

@@ -150,3 +150,72 @@ def test_vcycle_fp64_blocks_on_fp32_hierarchy(cells, b):
     mg(R32, W32)
     torch.cuda.synchronize()
     assert torch.equal(W, W32.double())
+
+
+def _np_vcycle_neumann_y(levels, f, ncoarse, omega):
+    """V(1,1) on a hierarchy that is Dirichlet in x (coarse j at fine 2j+1) and
+    Neumann in y (the boundary rows are unknowns; coarse row j at fine 2j)."""
+    def A(l, u):
+        return _stencil(l["d"], l["we"], l["wn"], u, l["nx"], l["ny"])
+
+    def smooth(l, u, f, n):
+        for s in range(n):
+            u = omega * f / l["d"][:, None] if u is None else u + omega * (f - A(l, u)) / l["d"][:, None]
+        return u
+
+    def restrict(l, lc, r):
+        R = r.reshape(l["ny"], l["nx"], -1)
+        out = np.zeros((lc["ny"], lc["nx"], R.shape[2]))
+        for cy in range(lc["ny"]):
+            for dy, wy in ((-1, 0.5), (0, 1.0), (1, 0.5)):
+                y = 2 * cy + dy
+                if 0 <= y < l["ny"]:
+                    row = R[y]
+                    out[cy] += wy * (0.5 * row[0:2 * lc["nx"]:2] + row[1:2 * lc["nx"] + 1:2]
+                                     + 0.5 * row[2:2 * lc["nx"] + 2:2])
+        return out.reshape(lc["nx"] * lc["ny"], -1)
+
+    def prolong(l, lc, e):
+        E = e.reshape(lc["ny"], lc["nx"], -1)
+        rows = np.zeros((lc["ny"], l["nx"], E.shape[2]))
+        rows[:, 1::2] = E
+        rows[:, 2:-1:2] = 0.5 * (E[:, :-1] + E[:, 1:])
+        rows[:, 0] = 0.5 * E[:, 0]
+        rows[:, -1] = 0.5 * E[:, -1]
+        out = np.zeros((l["ny"], l["nx"], E.shape[2]))
+        out[0::2] = rows
+        out[1::2] = 0.5 * (rows[:-1] + rows[1:])
+        return out.reshape(l["nx"] * l["ny"], -1)
+
+    def cyc(i, f):
+        l = levels[i]
+        if i == len(levels) - 1:
+            return smooth(l, None, f, ncoarse)
+        u = smooth(l, None, f, 1)
+        e = cyc(i + 1, restrict(l, levels[i + 1], f - A(l, u)))
+        return smooth(l, u + prolong(l, levels[i + 1], e), f, 1)
+    return cyc(0, f)
+
+
+@pytest.mark.parametrize("order,cells", [(1, (64, 32)), (2, (96, 48)), (2, (200, 100))])
+def test_poisson_vcycle_neumann_rows_matches_numpy(order, cells):
+    """The Poisson hierarchy (Dirichlet columns, Neumann top and bottom rows whose
+    boundary rows are coarse points): one device V-cycle against numpy FP64."""
+    from qdsim_amd.lattice import Lattice, PoissonLatticeMG
+    from qdsim_amd.mesh import SimpleMesh
+    m = SimpleMesh(cells[0] + 1, cells[1] + 1, 200e-9, 100e-9, order)
+    rng = np.random.default_rng(1)
+    eps = torch.from_numpy(10.0 + 5.0 * rng.random(m.num_elements)).to("cuda")
+    mg = PoissonLatticeMG(Lattice(m), eps)
+    print("poisson hierarchy", [(lv["nx"], lv["ny"]) for lv in mg.levels_data], flush=True)
+    assert mg.n_levels >= 3
+    levels = [{"nx": lv["nx"], "ny": lv["ny"], "d": lv["d"].cpu().numpy(), "we": lv["we"].cpu().numpy(),
+               "wn": lv["wn"].cpu().numpy()} for lv in mg.levels_data]
+    n = levels[0]["nx"] * levels[0]["ny"]
+    f = rng.standard_normal(n)
+    fd = torch.from_numpy(f).to("cuda")
+    z = torch.empty_like(fd)
+    mg(fd, z)
+    ref = _np_vcycle_neumann_y(levels, f[:, None], mg.coarse_sweeps, mg.omega)[:, 0]
+    err = np.abs(z.cpu().numpy() - ref).max() / np.abs(ref).max()
+    assert err < 1e-11, err
