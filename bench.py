@@ -184,6 +184,10 @@ def main():
                     help="only launch the fused SpMM (20x) and a 1 GiB device copy (5x), for rocprofv3")
     ap.add_argument("--assembly-kernels", action="store_true",
                     help="only time the assembly kernels (element pass and gather, 10x each) and print them")
+    ap.add_argument("--sweep-instances", type=int, default=0,
+                    help="only run the first N instances of the seeded bias sweep (self-consistent loop, "
+                         "30 outer iterations each) and print their timings")
+    ap.add_argument("--sweep-workers", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -196,6 +200,18 @@ def main():
     from qdsim_amd.schrodinger import SchrodingerProblem
 
     torch.cuda.set_device(0)
+    if args.sweep_instances > 0:
+        from qdsim_amd.batch import bias_sweep, bias_sweep_instances
+        inst = bias_sweep_instances(256, seed=4)[:args.sweep_instances]
+        t0 = time.perf_counter()
+        out = bias_sweep(inst, workers=args.sweep_workers)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        print(json.dumps({"sweep": {"instances": len(out), "workers": args.sweep_workers, "seconds": t,
+                                    "instances_per_s": len(out) / t,
+                                    "per_instance": [{k: o[k] for k in ("index", "cells", "order", "dofs",
+                                                                         "solve_time")} for o in out]}}))
+        return
     m_star, potential = _funcs()
     t0 = time.perf_counter()
     mesh = SimpleMesh(args.cells + 1, args.cells + 1, L_NM * 1e-9, L_NM * 1e-9, 3)

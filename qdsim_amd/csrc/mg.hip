@@ -704,6 +704,10 @@ bool use_tiled(const Level<T>& L, int b, int vec) {
 constexpr int kMaxCoarseLevels = 12;
 constexpr int kCoarseBlock = 1024;
 constexpr int64_t kCoarseItems = 24576;  // work items of the finest level it takes
+// and lattice points: with one or two column groups per point (the Poisson
+// preconditioner's single column) the item cap alone would hand a 150 x 150
+// level to one workgroup (measured: 290 us per cycle, most of a Poisson solve)
+constexpr int64_t kCoarsePoints = 6144;
 
 template <typename T>
 struct LevelPack {
@@ -820,7 +824,11 @@ int vcycle(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, h
   // first level small enough for the single-workgroup coarse cycle
   int lc = nl;
   for (int l = 0; l < nl; ++l)
-    if (L[l].nx * L[l].ny * (b / VEC) <= kCoarseItems && nl - l <= kMaxCoarseLevels) { lc = l; break; }
+    if (L[l].nx * L[l].ny * (b / VEC) <= kCoarseItems && L[l].nx * L[l].ny <= kCoarsePoints &&
+        nl - l <= kMaxCoarseLevels) {
+      lc = l;
+      break;
+    }
   const bool fused = nu1 == 1 && nu2 == 1;
   constexpr bool mixed = !std::is_same<T, S>::value;
   if (mixed)
