@@ -704,10 +704,11 @@ bool use_tiled(const Level<T>& L, int b, int vec) {
 constexpr int kMaxCoarseLevels = 12;
 constexpr int kCoarseBlock = 1024;
 constexpr int64_t kCoarseItems = 24576;  // work items of the finest level it takes
-// and lattice points: with one or two column groups per point (the Poisson
-// preconditioner's single column) the item cap alone would hand a 150 x 150
-// level to one workgroup (measured: 290 us per cycle, most of a Poisson solve)
-constexpr int64_t kCoarsePoints = 6144;
+// and lattice points: the item cap alone handed a 150 x 150 level of the Poisson
+// preconditioner (one column) to one workgroup (290 us per cycle, most of a
+// Poisson solve) and 4-6k points of the eigensolver's (b = 16-24) to one
+// workgroup (180-250 us); 2048 points measured best (512: same within 1%)
+constexpr int64_t kCoarsePoints = 2048;
 
 template <typename T>
 struct LevelPack {
