@@ -280,6 +280,14 @@ int qds_cg_mg(int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* indi
               void* workspace, int64_t ws_bytes, int32_t* iters_out /* host */,
               double* relres_out /* host */, void* stream);
 
+/* ---- adaptive refinement indicator (device) --------------------------------
+ * Replaces AdaptiveMesh::computeRefinementFlags
+ * (backend/src/adaptive_mesh.cpp:431-475): out[e] = |gradient of the least-squares
+ * fit psi ~ a + b x + c y over element e's npe nodes|, coordinates scaled by
+ * coord_scale first.  The caller compares out with the threshold.               */
+int qds_gradient_indicator(int32_t npe, int64_t n_elem, const int32_t* conn, const double* x, const double* y,
+                           const double* psi, double coord_scale, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,6 +60,7 @@ _SIGNATURES = {
     "qds_cg_workspace": ([_i64], _i64),
     "qds_cg": ([_i64, _i64, _p, _p, _p, _p, _p, _d, _i32, _i32, _p, _i64, _c.POINTER(_i32),
                 _c.POINTER(_d), _p], _c.c_int),
+    "qds_gradient_indicator": ([_i32, _i64, _p, _p, _p, _p, _d, _p, _p], _c.c_int),
     "qds_cg_mg": ([_i64, _i64, _p, _p, _p, _p, _p, _d, _i32, _i32, _i32, _p, _i32, _i32, _d, _p, _i64,
                    _c.POINTER(_i32), _c.POINTER(_d), _p], _c.c_int),
 }

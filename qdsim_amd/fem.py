@@ -123,6 +123,8 @@ class FEMSolver:
     """Drop-in for CythonFEMSolver (qdsim_cython/solvers/fem_solver.pyx:24)."""
 
     def __init__(self, mesh, element_order=1, coefficient_sampling="centroid"):
+        self.mesh = mesh
+        self._last_funcs = {}
         self.num_nodes = mesh.num_nodes
         self.num_elements = mesh.num_elements
         self.Lx = mesh.Lx
@@ -145,6 +147,7 @@ class FEMSolver:
         t0 = time.time()
         if coefficient_func is None:
             coefficient_func = lambda x, y: 1.0  # noqa: E731  (reference default)
+        self._last_funcs["stiffness"] = coefficient_func
         c = _centroid_or_quadrature(self.space, coefficient_func, self.coefficient_sampling)
         vals = self.space.assemble(c=c)[0]
         self.stiffness_matrix = self.space.to_scipy(vals)
@@ -156,11 +159,45 @@ class FEMSolver:
         t0 = time.time()
         if density_func is None:
             density_func = lambda x, y: 1.0  # noqa: E731
+        self._last_funcs["mass"] = density_func
         r = _centroid_or_quadrature(self.space, density_func, self.coefficient_sampling)
         vals = self.space.assemble(r=r, stiffness_mass=False, mass=True)[1]
         self.mass_matrix = self.space.to_scipy(vals, NM * NM)
         self.last_assembly_time += time.time() - t0
         return self.mass_matrix
+
+    def adapt_mesh(self, eigenvector, threshold, cache_dir=""):
+        """FEMSolver::adapt_mesh (backend/src/fem.cpp:486-542): flag the elements whose
+        least-squares gradient of ``eigenvector`` (nodal values, 1/m with SI
+        coordinates) exceeds ``threshold``, refine them (conforming red-green,
+        ``adapt.refine_mesh``), rebuild the FE space and reassemble the matrices
+        assembled before.  With ``cache_dir`` the refined mesh is saved as
+        ``mesh_<|eigenvector|>.txt`` in the reference text format and loaded from
+        there on a later call with the same vector norm (fem.cpp:488-500).
+        Returns the element flags of the old mesh."""
+        import os
+        from .adapt import gradient_indicator, refine_mesh
+        from .mesh_io import load_mesh, save_mesh
+        psi = np.asarray(eigenvector, dtype=np.float64)
+        if psi.shape != (self.num_nodes,):
+            raise ValueError(f"eigenvector has {psi.size} values, the mesh {self.num_nodes} nodes")
+        sp_ = self.space
+        flags = (gradient_indicator(sp_.conn, sp_.x, sp_.y, psi, sp_.npe) > threshold).cpu().numpy()
+        cache = os.path.join(cache_dir, f"mesh_{np.linalg.norm(psi):.6f}.txt") if cache_dir else ""
+        if cache and os.path.exists(cache):
+            new = load_mesh(cache)
+        else:
+            new = refine_mesh(self.mesh, flags)
+            if cache:
+                os.makedirs(cache_dir, exist_ok=True)
+                save_mesh(new, cache)
+        funcs = dict(self._last_funcs)
+        self.__init__(new, self.element_order, self.coefficient_sampling)
+        if "stiffness" in funcs:
+            self.assemble_stiffness_matrix(funcs["stiffness"])
+        if "mass" in funcs:
+            self.assemble_mass_matrix(funcs["mass"])
+        return flags
 
     def assemble_load_vector(self, source_func):
         """f_i = int s phi_i (consistent load; equals fem_solver.pyx:223-270's
