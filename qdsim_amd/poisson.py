@@ -37,13 +37,21 @@ class PoissonProblem:
         K, M, _ = sp_.assemble(c=eps, stiffness_mass=True, mass=True)
         self.K = K                       # SI (scale free in 2-D)
         self.M = M * (NM * NM)           # m^2
-        ry, rx = mesh.lattice_shape
-        lx = np.asarray(mesh.lattice_x)
         conn = mesh.get_elements()
         used = np.zeros(mesh.num_nodes, dtype=bool)
         used[conn.ravel()] = True
-        self.left = np.nonzero((lx == 0) & used)[0]
-        self.right = np.nonzero((lx == rx - 1) & used)[0]
+        lattice = hasattr(mesh, "lattice_shape") and hasattr(mesh, "lattice_x")
+        if lattice:
+            ry, rx = mesh.lattice_shape
+            lx = np.asarray(mesh.lattice_x)
+            self.left = np.nonzero((lx == 0) & used)[0]
+            self.right = np.nonzero((lx == rx - 1) & used)[0]
+        else:
+            # contacts by coordinate: the nodes on x = min x and x = max x
+            xn = np.asarray(mesh.get_nodes()[0])
+            tol = 1e-9 * (xn[used].max() - xn[used].min())
+            self.left = np.nonzero((np.abs(xn - xn[used].min()) <= tol) & used)[0]
+            self.right = np.nonzero((np.abs(xn - xn[used].max()) <= tol) & used)[0]
         fixed = np.zeros(mesh.num_nodes, dtype=bool)
         fixed[self.left] = True
         fixed[self.right] = True
@@ -51,7 +59,7 @@ class PoissonProblem:
         # on a structured mesh the unknowns are lattice columns 1 .. rx-2 of every
         # row: numbered row by row, the system gets a multigrid preconditioner
         self.mg = None
-        if hasattr(mesh, "lattice_y") and rx >= 3:
+        if lattice and hasattr(mesh, "lattice_y") and rx >= 3:
             ly = np.asarray(mesh.lattice_y)
             key = ly[free] * (rx - 2) + (lx[free] - 1)
             if free.size == (rx - 2) * ry and np.array_equal(np.sort(key), np.arange(free.size)):

@@ -28,6 +28,28 @@ def test_gradient_indicator_matches_oracle(order):
     assert np.allclose(g, ref, rtol=1e-9, atol=1e-9 * ref.max())
 
 
+@pytest.mark.parametrize("order", [1, 3])
+def test_adapt_mesh_other_orders(order):
+    """P1 and P3 meshes: refinement conforming, reassembled mass sums to the area."""
+    from qdsim_amd.fem import FEMSolver
+    from qdsim_amd.mesh import SimpleMesh
+    m = SimpleMesh(13, 9, 60e-9, 40e-9, order)
+    s = FEMSolver(m)
+    s.assemble_mass_matrix()
+    x, y = m.get_nodes()
+    psi = np.tanh((x - 30e-9) / 3e-9)
+    ref = adapt_ref.gradient_norms(x, y, m.get_elements(), psi)
+    thr = float(np.quantile(ref, 0.75)) * (1 + 1e-6)
+    flags = s.adapt_mesh(psi, thr)
+    assert flags.sum() > 0 and s.mesh.num_elements > m.num_elements
+    assert np.isclose(s.mass_matrix.sum(), 60e-9 * 40e-9, rtol=1e-10)
+    X, Y = s.mesh.get_nodes()
+    conn = s.mesh.get_elements().astype(np.int64)
+    _, _, M = fem_ref.element_matrices(order, X[conn[:, :3]], Y[conn[:, :3]], 2 * order + 2)
+    Mref = fem_ref.assemble(conn, s.mesh.num_nodes, M)
+    assert abs(s.mass_matrix - Mref).max() <= 1e-10 * abs(Mref).max()
+
+
 def test_adapt_mesh_refines_reassembles_and_caches(tmp_path):
     from qdsim_amd.adapt import gradient_indicator
     from qdsim_amd.fem import FEMSolver

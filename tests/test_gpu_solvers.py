@@ -249,7 +249,7 @@ def test_cap_target_above_several_batches_matches_dense_oracle():
     assert err.max() < 1e-8
 
 
-@pytest.mark.parametrize("order,cells", [(1, (64, 32)), (2, (96, 48))])
+@pytest.mark.parametrize("order,cells", [(1, (64, 32)), (2, (96, 48)), (3, (40, 20)), (1, (30, 22)), (2, (25, 13))])
 def test_poisson_multigrid_cg_matches_jacobi_cg(order, cells):
     """On a structured mesh the Poisson CG is preconditioned by the lattice
     multigrid: same solution as the Jacobi CG on the same system, in far fewer
@@ -268,6 +268,30 @@ def test_poisson_multigrid_cg_matches_jacobi_cg(order, cells):
     pp_j.mg = None
     phi_j = pp_j.solve(rho, 0.0, 0.3)
     it_j = pp_j.last_iterations
-    print(f"P{order} {cells}: multigrid CG {it_mg} iterations, Jacobi CG {it_j}")
-    assert it_mg < 60 and it_mg * 5 < it_j
+    print(f"P{order} {cells}: {pp.mg.n_levels} levels, multigrid CG {it_mg} iterations, Jacobi CG {it_j}")
+    # lattices whose counts stop the coarsening early keep a short hierarchy: still
+    # a valid (weaker) preconditioner
+    assert it_mg < (60 if pp.mg.n_levels >= 4 else it_j)
+    if pp.mg.n_levels >= 4:
+        assert it_mg * 5 < it_j
     assert np.allclose(phi_mg.cpu().numpy(), phi_j.cpu().numpy(), rtol=0, atol=1e-10 * np.abs(phi_j.cpu().numpy()).max())
+
+
+def test_poisson_on_mesh_file_uses_jacobi_cg(tmp_path):
+    """A mesh without lattice structure (read from a file) keeps the Jacobi CG and
+    gives the solution of the structured mesh it came from."""
+    from qdsim_amd.mesh import SimpleMesh
+    from qdsim_amd.mesh_io import load_mesh
+    from qdsim_amd.poisson import PoissonProblem
+    m = SimpleMesh(33, 17, 200e-9, 100e-9, 1)
+    m.save(tmp_path / "m.txt")
+    fm = load_mesh(tmp_path / "m.txt")
+    eps = lambda x, y: 13.9 + 0 * x
+    x, y = m.get_nodes()
+    rho = torch.from_numpy(1e6 * np.exp(-((x - 100e-9) ** 2) / (2 * (20e-9) ** 2))).to("cuda")
+    a = PoissonProblem(m, eps)
+    b = PoissonProblem(fm, eps)
+    assert a.mg is not None and b.mg is None
+    pa = a.solve(rho, 0.0, 0.2).cpu().numpy()
+    pb = b.solve(rho, 0.0, 0.2).cpu().numpy()
+    assert np.allclose(pa, pb, rtol=0, atol=1e-5 * np.abs(pa).max())   # file coordinates carry 6 digits
