@@ -316,7 +316,7 @@ def solve_cap(prob, k, b=None, target=None, tol=1e-6, tol32=3e-4, maxiter=600, s
     SchrodingerProblem whose CAP coefficient is set.  Returns (E complex eV (np),
     eigenvectors (n_full, k) complex128 np, Hermitian M_nm-normalised).
 
-    Nearest-target mode (the reference's eigs(H, k, M, sigma) of config 3) is
+    Nearest-target mode (the states nearest an energy, as shift-invert eigs(H, k, M, sigma) selects them) is
     computed from the bottom of the spectrum: batches of the lowest-real-part
     eigenpairs, each batch found on the pencil with the previous ones deflated
     (DeflatedCAPOperator), until the computed set brackets the k eigenvalues

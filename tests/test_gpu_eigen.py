@@ -8,7 +8,7 @@ from oracle import eig_ref
 pytestmark = pytest.mark.gpu
 
 
-def _well(x, y):          # config-1 potential, x in nm
+def _well(x, y):          # -60 meV well for 5 < x < 15 nm, x in nm
     return np.where((x > 5.0) & (x < 15.0), -0.06, 0.0) + 0 * y
 
 
@@ -48,8 +48,8 @@ def _check(prob, ref, k, mode, rel=1e-9):
 
 
 @pytest.mark.parametrize("mode", ["mixed", "fp64"])
-def test_config1_p1(mode):
-    # config 1, node-count reading: FixedOpenSystemSolver(nx=64, ny=48), 25 x 20 nm
+def test_p1_quantum_well_64x48(mode):
+    # node-count reading of FixedOpenSystemSolver(nx=64, ny=48), 25 x 20 nm, P1
     prob = _problem(64, 48, 25.0, 20.0, 1, _const_mass, _well)
     ref = eig_ref.solve_closed(64, 48, 25.0, 20.0, 1, _const_mass, _well, 5)
     _check(prob, ref, 5, mode)

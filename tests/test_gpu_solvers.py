@@ -180,7 +180,7 @@ def test_cap_solver_matches_dense_oracle():
 
 
 def test_cap_solver_target_mode_matches_dense_oracle():
-    """Eigenvalues nearest a target energy (config-3 style selection) on a small
+    """Eigenvalues nearest a target energy (selection by distance to an energy) on a small
     double-barrier device, against dense zggev."""
     from qdsim_amd.cap import solve_cap
     from qdsim_amd.mesh import SimpleMesh

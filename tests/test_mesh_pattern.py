@@ -67,11 +67,11 @@ def test_native_pattern_matches_scipy(order):
     assert np.allclose(vals, ref.data, rtol=1e-14, atol=1e-14)
 
 
-def test_native_pattern_config3_nnz():
+def test_native_pattern_p2_384x288_nnz():
     from qdsim_amd import kernels
     m = SimpleMesh(385, 289, 80e-9, 60e-9, 2)
     pat = kernels.pattern_build(m.get_elements(), m.num_nodes)
-    assert pat["nnz"] == 5092609                  # survey Table A, config 3
+    assert pat["nnz"] == 5092609                  # P2 384 x 288 cells: structured-mesh combinatorics
 
 
 def test_restrict_pattern():

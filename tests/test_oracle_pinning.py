@@ -150,8 +150,9 @@ def test_assembly_canonical_and_invariants():
     assert np.abs(Kg @ np.ones(len(x))).max() < 1e-11
 
 
-def test_config1_pattern_nnz():
-    # survey Table A: FixedOpenSystemSolver(nx=64, ny=48) P1 pattern nnz 21,058
+def test_p1_64x48_pattern_nnz():
+    # FixedOpenSystemSolver(nx=64, ny=48), 25 x 20 nm P1 mesh: 21,058 nonzeros from
+    # the structured-mesh combinatorics (3072 nodes, 5922 triangles)
     x, y, el = mesh_ref.simple_mesh_p1(64, 48, 25e-9, 20e-9)
     A = fem_ref.assemble(el, len(x), np.ones((len(el), 3, 3)))
     assert A.nnz == 21058
