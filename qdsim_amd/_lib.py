@@ -7,6 +7,12 @@ import ctypes
 import os
 
 import numpy as np
+# torch first: it loads the HIP runtime it ships with, and libqdsim_hip.so then
+# binds to that same runtime.  Loaded the other way round, the dynamic linker
+# brings in the system libamdhip64 as a second runtime, and kernels launched
+# through it reported "no ROCm-capable device" on the GPU host although torch
+# saw the device.  Device buffers come from torch, so one runtime per process.
+import torch  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libqdsim_hip.so")
