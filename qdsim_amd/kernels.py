@@ -242,10 +242,13 @@ def block_residual(HX, MX, lam, R):
                                  ptr(R), _ld(R), stream_ptr()), "qds_block_residual")
 
 
-def residual_norms(HX, MX, lam, R, colscale=None):
-    """R = (HX - MX diag(lam)) diag(colscale); returns (||HX - MX lam||_j^2, ||MX_j||^2)."""
+def residual_norms(HX, MX, lam, R, colscale=None, out=None):
+    """R = (HX - MX diag(lam)) diag(colscale); returns (||HX - MX lam||_j^2, ||MX_j||^2)
+    (views of ``out``, 2b FP64 device values, allocated when None)."""
     n, b = HX.shape
-    out = torch.empty(2 * b, dtype=torch.float64, device=HX.device)
+    if out is None:
+        out = torch.empty(2 * b, dtype=torch.float64, device=HX.device)
+    assert out.dtype == torch.float64 and out.is_contiguous() and out.numel() >= 2 * b
     wsb = lib.qds_residual_norms_workspace(b)
     wp, wn = workspace("resnorm").ptr(wsb)
     cs = colscale.contiguous() if colscale is not None else None

@@ -94,8 +94,7 @@ def _svqb_rr(G, m, nb, drop_tol):
     robust for the nearly rank-deficient pairs met close to convergence.
     Returns (theta[:nb], C (m x nb) as host FP64 arrays, number of dropped
     non-zero directions)."""
-    dev = G.device
-    g = G.detach().cpu().numpy()
+    g = G if isinstance(G, np.ndarray) else G.detach().cpu().numpy()
     GH = 0.5 * (g[:, :m] + g[:, :m].T)
     GM = 0.5 * (g[:, m:2 * m] + g[:, m:2 * m].T)
     dg = np.diag(GM).copy()
@@ -129,7 +128,7 @@ def _chol_rr(G, m, nb, drop_tol):
     (locked residuals) are left out and get zero weight.  This runs on the host in
     every iteration while the GPU waits, so it calls LAPACK directly and avoids
     fancy indexing when no column vanished."""
-    g = G.detach().cpu().numpy()
+    g = G if isinstance(G, np.ndarray) else G.detach().cpu().numpy()
     GH = g[:, :m]
     GM = g[:, m:2 * m]
     dg = np.diag(GM).copy()
@@ -271,6 +270,10 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
     G = kernels.gram_blocks([X[c]], [HX[c]], [MX[c]], sym=True)
     stage_rr = _Staging(2 * 3 * b * b + b, dev)
     stage_cs = _Staging(b, dev)
+    # residual norms and the Gram of an iteration come back in one copy: device
+    # buffer [norms (2b) | Gram (m x 2m)] and its page-locked host twin
+    xfer = torch.empty(2 * b + 18 * b * b, dtype=torch.float64, device=dev)
+    xfer_h = torch.empty(xfer.numel(), dtype=torch.float64, pin_memory=dev.type == "cuda")
     theta, C, _ = _svqb_rr(G, b, b, drop_tol)
     C_d, lam = stage_rr.put([C, theta])
     for src in (X, HX, MX):
@@ -288,7 +291,7 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
     t_iter = time.perf_counter()
     (colscale,) = stage_cs.put([np.ones(b)])
     for it in range(1, maxiter + 1):
-        rn, mn = kernels.residual_norms(HX[c], MX[c], lam, R, colscale)
+        kernels.residual_norms(HX[c], MX[c], lam, R, colscale, out=xfer[:2 * b])
         # the search directions of this iteration do not depend on its convergence
         # test (locking acts from the next residual on), so the preconditioner, the
         # product and the Gram are queued before the host reads the norms: one
@@ -303,9 +306,12 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
         HS = [HX[c], HW] + ([HP[c]] if have_p else [])
         MS = [MX[c], MW] + ([MP[c]] if have_p else [])
         m = len(S) * b
-        G = kernels.gram_blocks(S, HS, MS, sym=True)
-        norms = torch.cat([rn, mn]).cpu().numpy()
-        rn_h, mn_h = norms[:b], norms[b:]
+        tot = 2 * b + 2 * m * m
+        kernels.gram_blocks(S, HS, MS, sym=True, out=xfer[2 * b:tot].view(m, 2 * m))
+        xfer_h[:tot].copy_(xfer[:tot], non_blocking=True)
+        sync()
+        rn_h, mn_h = xfer_h[:b].numpy(), xfer_h[b:2 * b].numpy()
+        G = xfer_h[2 * b:tot].numpy().reshape(m, 2 * m)
         # residual relative to the energy scale of the wanted window, so states with
         # eigenvalues close to zero are held to the same absolute accuracy as the rest
         scale = max(np.abs(lam_h[:k]).max(), 1e-300)
