@@ -268,7 +268,9 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
     # initial Rayleigh-Ritz on X alone
     op.apply(X[c], HX[c], MX[c])
     G = kernels.gram_blocks([X[c]], [HX[c]], [MX[c]], sym=True)
-    stage_rr = _Staging(2 * 3 * b * b + b, dev)
+    # per iteration one copy to the device: Rayleigh-Ritz coefficients,
+    # eigenvalues and the locking mask of the next residual
+    stage_rr = _Staging(2 * 3 * b * b + 2 * b, dev)
     stage_cs = _Staging(b, dev)
     # residual norms and the Gram of an iteration come back in one copy: device
     # buffer [norms (2b) | Gram (m x 2m)] and its page-locked host twin
@@ -320,8 +322,9 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
         res = torch.from_numpy(res_h)
         worst = float(res_h[:k].max())
         history.append(worst)
-        # lock converged columns from the next residual on
-        (colscale,) = stage_cs.put([(res_h >= tol).astype(np.float64)])
+        # lock converged columns from the next residual on (staged with the
+        # Rayleigh-Ritz coefficients below)
+        mask = (res_h >= tol).astype(np.float64)
         if verbose:
             print(f"  lobpcg[{str(dt)[6:]}] it {it:4d} worst rel. residual {worst:.3e}")
         if worst < tol:
@@ -343,7 +346,7 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
         drops.append(dropped)
         CP_h = C_h.copy()
         CP_h[:b] = 0.0
-        C, CP, lam_next = stage_rr.put([C_h, CP_h, theta])
+        C, CP, lam_next, colscale = stage_rr.put([C_h, CP_h, theta, mask])
         fresh = bool(refresh) and it % refresh == 0
         kernels.block_update_blocks(S, C, X[1 - c], CP, P[1 - c], c2_first=b)
         if fresh:
