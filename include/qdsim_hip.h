@@ -210,6 +210,22 @@ typedef struct {
 int qds_gram_blocks(int32_t dtype, int64_t n, const qds_colblocks* A /* host */,
                     const qds_colblocks* B1 /* host */, const qds_colblocks* B2 /* host */,
                     int32_t sym, double* C, void* workspace, int64_t ws_bytes, void* stream);
+/* qds_gram_blocks without a workspace argument.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The per-slab partial sums live
+ * in a static device array of qds_gram_noalloc_scratch_bytes() (64 MiB) on each
+ * GPU, kept until the process ends; padded m x (k1 + k2) (each padded to a
+ * multiple of 16) may be at most 131072 entries (e.g. m = 96 with k1 + k2 = 192
+ * fits), larger shapes return QDS_EINVAL.  The rows are split into the same slabs
+ * as qds_gram_blocks (so the results are bitwise equal) whenever the array holds
+ * them all: padded m x (k1 + k2) <= 16384 entries (m = 72, k1 + k2 = 144 fits),
+ * or fewer than 16 * (64 MiB / (8 * padded entries)) rows; otherwise it uses fewer,
+ * longer slabs.  Calls that overlap on one GPU share the array and race: serialise
+ * them (one stream, or events).                                                  */
+int64_t qds_gram_noalloc_scratch_bytes(void);
+int qds_gram_blocks_noalloc(int32_t dtype, int64_t n, const qds_colblocks* A /* host */,
+                            const qds_colblocks* B1 /* host */, const qds_colblocks* B2 /* host */,
+                            int32_t sym, double* C, void* stream);
 int qds_block_update_blocks(int32_t dtype, int64_t n, const qds_colblocks* S /* host */, int32_t k,
                             const double* C1, void* Y1, int64_t ldy1, const double* C2, void* Y2,
                             int64_t ldy2, int32_t c2_first /* rows of C2 before it are zero */,

@@ -46,6 +46,8 @@ _SIGNATURES = {
                  _c.c_int),
     "qds_block_update": ([_i32, _i64, _i32, _i32, _p, _i64, _p, _p, _i64, _p, _p, _i64, _p], _c.c_int),
     "qds_gram_blocks": ([_i32, _i64, _p, _p, _p, _i32, _p, _p, _i64, _p], _c.c_int),
+    "qds_gram_noalloc_scratch_bytes": ([], _i64),
+    "qds_gram_blocks_noalloc": ([_i32, _i64, _p, _p, _p, _i32, _p, _p], _c.c_int),
     "qds_block_update_blocks": ([_i32, _i64, _p, _i32, _p, _p, _i64, _p, _p, _i64, _i32, _p], _c.c_int),
     "qds_block_residual": ([_i32, _i64, _i32, _p, _i64, _p, _i64, _p, _p, _i64, _p], _c.c_int),
     "qds_coldot_workspace": ([_i32], _i64),

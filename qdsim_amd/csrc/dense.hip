@@ -14,6 +14,7 @@
 // * qds_block_residual, qds_coldot: R = HX - MX diag(lambda); column dot products.
 #include "qds_common.h"
 
+#include <mutex>
 #include <type_traits>
 
 namespace {
@@ -561,6 +562,8 @@ GramShape make_shape(int m, int k1, int k2, int sym) {
 }
 
 constexpr int kColdotBlocks = 512;
+// fewest row slabs the no-alloc Gram accepts (the workspace form uses up to 512)
+constexpr int kMinStaticSlabs = 64;
 
 }  // namespace
 
@@ -621,14 +624,16 @@ qds_colblocks single(const void* p, int64_t ld, int cols) {
 }
 }  // namespace
 
-// C (m x (k1+k2), row-major FP64) = A^T [B1 | B2] for column-block matrices.
-// B2 may be NULL.  sym = 1 asserts each half is symmetric (m == k1 == k2) and
-// computes only its upper tiles.  dtype F32 or F64 gives the storage type.
-QDS_API int qds_gram_blocks(int32_t dtype, int64_t n, const qds_colblocks* A, const qds_colblocks* B1,
-                            const qds_colblocks* B2, int32_t sym, double* C, void* workspace,
-                            int64_t ws_bytes, void* stream) {
+namespace {
+// C (m x (k1+k2), row-major FP64) = A^T [B1 | B2] for column-block matrices, with
+// the per-slab partial sums in `partial`.  The row slabs are gram_blocks(n), at
+// most max_slabs of them; partial_bytes == 0 takes the caller's qds_gram_workspace
+// size check instead of the slab cap.
+int gram_blocks_impl(int32_t dtype, int64_t n, const qds_colblocks* A, const qds_colblocks* B1,
+                     const qds_colblocks* B2, int32_t sym, double* C, double* partial, int64_t ws_bytes,
+                     int64_t partial_bytes, void* stream) {
   QDS_CHECK_ARG(dtype == qds::F32 || dtype == qds::F64, "qds_gram: dtype must be F32 or F64");
-  QDS_CHECK_ARG(C && workspace && A && B1, "qds_gram: null pointer");
+  QDS_CHECK_ARG(C && partial && A && B1, "qds_gram: null pointer");
   const int es = dtype == qds::F32 ? 4 : 8;
   const bool vec = blocks_vec(A, es) && blocks_vec(B1, es) && (!B2 || blocks_vec(B2, es));
   const int wcap = vec ? 16 * (16 / es) : 16;   // piece: at most one vector per thread for kRB rows
@@ -681,14 +686,25 @@ QDS_API int qds_gram_blocks(int32_t dtype, int64_t n, const qds_colblocks* A, co
   }
   QDS_CHECK_ARG(tpg >= 1, "qds_gram: operands of %d columns too wide", m);
   const int ngroups = (tk + tpg - 1) / tpg;
-  const int nblk = gram_blocks(n);
-  QDS_CHECK_ARG(ws_bytes >= qds_gram_workspace(n, m, k1, k2), "qds_gram: workspace too small");
+  int nblk = gram_blocks(n);
+  if (partial_bytes == 0) {
+    QDS_CHECK_ARG(ws_bytes >= qds_gram_workspace(n, m, k1, k2), "qds_gram: workspace too small");
+  } else {
+    const int64_t per_slab = (int64_t)g.mpad * g.kpad * (int64_t)sizeof(double);
+    const int64_t cap = partial_bytes / per_slab;
+    QDS_CHECK_ARG(cap >= kMinStaticSlabs,
+                  "qds_gram_blocks_noalloc: a %d x %d Gram (padded %d x %d) is too large for the %lld-byte "
+                  "static scratch: it needs %d row slabs of %lld bytes; padded m * (k1 + k2) must be at "
+                  "most %lld",
+                  m, k1 + k2, g.mpad, g.kpad, (long long)partial_bytes, kMinStaticSlabs, (long long)per_slab,
+                  (long long)(partial_bytes / ((int64_t)kMinStaticSlabs * (int64_t)sizeof(double))));
+    if (nblk > cap) nblk = (int)cap;
+  }
   const int64_t rows_per_blk = (((n + nblk - 1) / nblk) + kRB - 1) / kRB * kRB;
   const size_t shm = (size_t)kRB * (g.mpad + 16 * tpg) * sizeof(double);
   for (int q = 0; q < stg.n; ++q)
     QDS_CHECK_ARG(stg.ld[q] * kRB < (int64_t(1) << 31), "qds_gram: leading dimension too large");
   hipStream_t st = qds_stream(stream);
-  double* partial = static_cast<double*>(workspace);
   // a thread keeps offsets and one staged vector per piece: more than 9 pieces
   // spill, so wide operands run one launch per column group with only the pieces
   // that group reads (all of A, the B pieces inside the group's columns)
@@ -735,6 +751,58 @@ QDS_API int qds_gram_blocks(int32_t dtype, int64_t n, const qds_colblocks* A, co
   hipLaunchKernelGGL(gram_reduce, dim3((tot + kRedOut - 1) / kRedOut), dim3(kBlock), 0, st, nblk, g, partial, C);
   QDS_LAUNCH_CHECK();
   return QDS_OK;
+}
+}  // namespace
+
+// C (m x (k1+k2), row-major FP64) = A^T [B1 | B2] for column-block matrices.
+// B2 may be NULL.  sym = 1 asserts each half is symmetric (m == k1 == k2) and
+// computes only its upper tiles.  dtype F32 or F64 gives the storage type.
+QDS_API int qds_gram_blocks(int32_t dtype, int64_t n, const qds_colblocks* A, const qds_colblocks* B1,
+                            const qds_colblocks* B2, int32_t sym, double* C, void* workspace,
+                            int64_t ws_bytes, void* stream) {
+  QDS_CHECK_ARG(workspace, "qds_gram: null workspace");
+  return gram_blocks_impl(dtype, n, A, B1, B2, sym, C, static_cast<double*>(workspace), ws_bytes, 0, stream);
+}
+
+// ---- the same Gram without a caller workspace ---------------------------------
+// For callers that cannot allocate (HIP graph capture, serving loops): the
+// per-slab partial sums live in a file-scope device array, one copy per GPU,
+// held until the process ends.  Its device address is looked up once per device
+// with hipGetSymbolAddress and only that address is handed to kernels.  Calls
+// that overlap on one GPU share the array and race.  Every partial that the
+// reduction reads is written by the same call first, so nothing needs clearing.
+namespace {
+constexpr int64_t kStaticGramBytes = 64ll << 20;
+__device__ __align__(256) double scratch[kStaticGramBytes / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int static_scratch(double** out) {
+  int dev = -1;
+  QDS_HIP(hipGetDevice(&dev));
+  QDS_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "qds_gram_blocks_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    QDS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    QDS_CHECK_ARG(p != nullptr, "qds_gram_blocks_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return QDS_OK;
+}
+}  // namespace
+
+QDS_API int64_t qds_gram_noalloc_scratch_bytes(void) { return kStaticGramBytes; }
+
+QDS_API int qds_gram_blocks_noalloc(int32_t dtype, int64_t n, const qds_colblocks* A, const qds_colblocks* B1,
+                                    const qds_colblocks* B2, int32_t sym, double* C, void* stream) {
+  double* partial = nullptr;
+  const int rc = static_scratch(&partial);
+  if (rc) return rc;
+  return gram_blocks_impl(dtype, n, A, B1, B2, sym, C, partial, 0, kStaticGramBytes, stream);
 }
 
 // Single-block form: A n x m (lda), B1 n x k1, B2 n x k2 (or NULL).
