@@ -12,16 +12,25 @@ the tests that exercise it.
 
 Parity status per function:
   mesh_ref.simple_mesh_p1 ........ pinned: mesh_minimal.pyx:236-261 asserts + counts
-  mesh_ref.mesh_cpp_p2 ........... pinned: brute-force std::map restatement, counts
-  mesh_ref.mesh_p3 ............... pinned: counts (Table of DOF/nnz counts), lattice
+  mesh_ref.mesh_cpp_p2 ........... pinned: hand-derived ids and coordinates of 1x1 and 2x1-cell
+                                   meshes with and without the m03 orphan (mesh.cpp:95-115
+                                   call order), counts
+  mesh_ref.mesh_p3 ............... pinned: hand-derived ids of 1x1 and 2x1-cell meshes (a shared
+                                   edge met in both orientations), counts, lattice
   fem_ref shape functions ........ pinned: partition of unity, Kronecker, gradients sum 0
   fem_ref quadrature ............. pinned: monomial exactness up to the rule degree
   fem_ref element matrices ....... pinned: closed forms (P1 K, P1/P2 M), K*1=0, sum M = area
   fem_ref.assemble ............... pinned: scipy COO->CSR canonical form, nnz counts
   eig_ref ........................ pinned: tridiag(-1,2,-1) spectrum, particle in a box
-  poisson_ref .................... pinned: linear solution phi = V x / Lx (P1, rho = 0)
+  poisson_ref .................... pinned: linear solution phi = V x / Lx (P1, rho = 0); the
+                                   nodally exact quadratic for uniform rho in both sign
+                                   conventions (fixes the sign and the mass weighting)
   cap_ref ........................ pinned: LAPACK zggev of the oracle pencil; W = 0 reduces to eig_ref
-  fixed_ref ...................... pinned: restated matrices vs the restricted symmetric pencil
+  fixed_ref ...................... pinned: hand-computed H and M of 2x2 and 3x2-node meshes from the
+                                   lumped formulas of fixed_open_system_solver.pyx:230-247
+  sc_ref ......................... pinned by composition: eig_ref and poisson_ref (each pinned
+                                   above) joined by the density formulas of
+                                   self_consistent_solver.pyx:257-286
   interp_ref ..................... pinned: exact reproduction of degree-p polynomials (values and
                                    gradients), brute-force nearest node outside the mesh
 """

@@ -10,10 +10,20 @@ any torch expression that produces v_q from gate or doping parameters gets exact
 gradients by the chain rule, and ``nodal_to_quadrature`` provides dE/dV_nodal
 for a potential given at the DOFs.
 
+Degenerate levels: within a cluster of (near-)equal eigenvalues the individual
+E_k are not differentiable and the per-state weights psi_k^2 depend on the
+basis the solver returned; only symmetric functions of the cluster (e.g. its
+sum, or any loss that weighs its members equally) have well-defined gradients,
+sum_k psi_k^2 being basis independent.  ``degenerate_clusters`` finds the
+clusters, and the backward warns when the incoming gradient differs inside one.
+
 The Poisson part of the self-consistent loop is linear: phi = K^-1 (f(theta) -
 K g), so d/dtheta of an objective through phi needs one adjoint solve
 K lambda = dJ/dphi with the same device CG (``poisson_adjoint``).
 """
+import warnings
+
+import numpy as np
 import torch
 
 from . import _lib
@@ -31,6 +41,27 @@ def state_density_q(space, psi):
     return out
 
 
+def degenerate_clusters(lam, rel_gap=1e-6):
+    """Groups of consecutive (sorted) eigenvalues closer than rel_gap times the
+    spread of lam (or of |lam| for a single value): lists of indices, only groups
+    of two or more."""
+    lam = np.asarray(lam, dtype=np.float64)
+    if lam.size < 2:
+        return []
+    scale = max(float(lam.max() - lam.min()), float(np.abs(lam).max()), 1e-300)
+    out, cur = [], [0]
+    for i in range(1, lam.size):
+        if lam[i] - lam[i - 1] <= rel_gap * scale:
+            cur.append(i)
+        else:
+            if len(cur) > 1:
+                out.append(cur)
+            cur = [i]
+    if len(cur) > 1:
+        out.append(cur)
+    return out
+
+
 class _Eigenvalues(torch.autograd.Function):
     @staticmethod
     def forward(ctx, v_q, prob, k, mode, tol):
@@ -39,6 +70,7 @@ class _Eigenvalues(torch.autograd.Function):
         lam, psi, info = prob.solve(k, mode=mode, tol=tol)
         rho = state_density_q(prob.space, psi)
         ctx.save_for_backward(rho)
+        ctx.clusters = degenerate_clusters(lam)
         prob.last_eigenvectors = psi
         prob.last_info = info
         return torch.as_tensor(lam, dtype=torch.float64, device=v_q.device)
@@ -46,6 +78,13 @@ class _Eigenvalues(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gE):
         (rho,) = ctx.saved_tensors
+        if ctx.clusters:
+            g = gE.detach().double().cpu().numpy()
+            for c in ctx.clusters:
+                if np.ptp(g[c]) > 1e-12 * max(np.abs(g).max(), 1e-300):
+                    warnings.warn(f"gradient through degenerate levels {c}: the incoming gradient differs "
+                                  f"inside the cluster, so the result depends on the eigenvector basis; "
+                                  f"use a loss symmetric in these levels", RuntimeWarning, stacklevel=2)
         gV = torch.einsum("eqk,k->eq", rho, gE.to(rho.dtype))
         return gV, None, None, None, None
 

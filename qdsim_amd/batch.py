@@ -99,7 +99,7 @@ def bias_sweep_instances(n=256, seed=4, cells=(64, 384), orders=(1, 2), Lx=200e-
     return out
 
 
-def _sc_instance(spec, num_states, iterations, mixing, keep_fields):
+def _sc_instance(spec, num_states, iterations, mixing, keep_fields, poisson_sign="physical", retry=True):
     from .mesh import SimpleMesh
     from .self_consistent import SelfConsistentSolver
     Lx, Ly, c = spec["Lx"], spec["Ly"], spec["cells"]
@@ -109,7 +109,8 @@ def _sc_instance(spec, num_states, iterations, mixing, keep_fields):
     m = spec["m_star"] * M_E
     sc = SelfConsistentSolver(mesh, lambda x, y: spec["eps_r"] + 0 * x, lambda x, y: m + 0 * x,
                               lambda x, y: -V0 * np.exp(-((x - x0) ** 2 + (y - y0) ** 2) / s2),
-                              mixing=mixing)
+                              mixing=mixing, poisson_sign=poisson_sign)
+    sc.schrodinger.retry = retry
     sc.set_solver_parameters(max_iterations=iterations, tolerance=0.0)
     x, _ = mesh.get_nodes()
     left = x < spec["junction"]
@@ -117,18 +118,26 @@ def _sc_instance(spec, num_states, iterations, mixing, keep_fields):
                                    N_A=np.where(left, spec["N_A"], 0.0), N_D=np.where(left, 0.0, spec["N_D"]))
     out = {"index": spec.get("index"), "cells": c, "order": spec["order"], "dofs": mesh.num_nodes,
            "eigenvalues_eV": res["eigenvalues_eV"], "iterations": res["iterations"],
-           "convergence_history": res["convergence_history"], "solve_time": res["solve_time"]}
+           "convergence_history": res["convergence_history"], "solve_time": res["solve_time"],
+           "breakdown_solves": sc.breakdown_solves, "retried_solves": sc.retried_solves,
+           "unconverged_solves": sc.unconverged_solves}
     if keep_fields:
         out.update({k: res[k] for k in ("potential", "electron_density", "hole_density")})
     return out
 
 
-def bias_sweep(instances, num_quantum_states=8, iterations=30, mixing=0.3, workers=4, keep_fields=False):
+def bias_sweep(instances, num_quantum_states=8, iterations=30, mixing=0.3, workers=4, keep_fields=False,
+               poisson_sign="physical", retry=True):
     """Self-consistent loop of every instance (``bias_sweep_instances``): exactly
     ``iterations`` outer iterations with linear mixing, CG Poisson and the lowest
     ``num_quantum_states`` states.  Returns one summary dict per instance (fields
-    on the host only with keep_fields)."""
-    return run_batch(lambda _, spec: _sc_instance(spec, num_quantum_states, iterations, mixing, keep_fields),
+    on the host only with keep_fields), with the number of eigensolves that broke
+    down or did not converge at first (``breakdown_solves``), were redone with the
+    safe shift (``retried_solves``; with retry=False a breakdown raises instead and
+    a solve that does not converge is kept) and ended unconverged.  poisson_sign: "physical" or "reference" (the
+    reference's K phi = -M rho)."""
+    return run_batch(lambda _, spec: _sc_instance(spec, num_quantum_states, iterations, mixing, keep_fields,
+                                                  poisson_sign, retry),
                      instances, workers=workers)
 
 

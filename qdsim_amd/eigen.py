@@ -12,8 +12,18 @@ message, retries with the default method, and finally returns empty arrays.
 Methods 'arpack' and 'lobpcg' both run the GPU block LOBPCG (the matrices carry no
 geometry here, so the preconditioner is the Jacobi scaling D^-1/2 (A, B) D^-1/2);
 'dense' runs a dense FP64 eigensolve on the GPU (n <= 1000, as :268-271).
-``which`` 'SM'/'SA' return the lowest eigenvalues ('SM' and 'SA' coincide for the
-positive definite pencils of the reference tests); 'LA'/'LM' return the highest.
+
+``which`` and ``sigma`` select the same eigenpairs in every method, with the
+meaning scipy's eigsh gives them (the reference's 'arpack' call,
+eigen_solver.pyx:214-235): 'SA' / 'LA' the algebraically smallest / largest,
+'SM' / 'LM' the smallest / largest magnitude; with ``sigma`` set, 'LM' selects
+the eigenvalues nearest sigma (shift-invert mode, the reference's large-N recipe
+advanced_eigenvalue_solvers.py:106-122) and any other ``which`` is a ValueError.
+Results come back in ascending order of eigenvalue, as eigsh returns them.  The
+reference's dense path orders 'SM' by magnitude and treats 'LM' as 'SA'
+(eigen_solver.pyx:282-296); here it selects like eigsh.  Its lobpcg path
+returns the largest eigenvalues of an unseeded start (scipy's default, :241-266);
+here lobpcg honours ``which`` and is seeded.
 """
 import time
 
@@ -29,14 +39,9 @@ HBAR = 1.054571817e-34
 _PI = 3.14159265359
 
 
-def _to_device_csr(A, dev):
-    A = sp.csr_matrix(A)
-    A.sort_indices()
-    return A
-
-
-def gpu_eigsh(A, B, k, which="SM", tol=1e-8, maxiter=1000, seed=0):
-    """Lowest (or highest) k eigenpairs of A x = lambda B x on the GPU."""
+def _block_extreme(A, B, k, largest, tol, maxiter, seed):
+    """k algebraically lowest (largest=False) or highest eigenpairs of the pencil
+    (A, B) by the GPU block LOBPCG, ascending; converged flag."""
     dev = device()
     n = A.shape[0]
     if B is None:
@@ -45,10 +50,9 @@ def gpu_eigsh(A, B, k, which="SM", tol=1e-8, maxiter=1000, seed=0):
     P = (abs(sp.csr_matrix(A)) + abs(sp.csr_matrix(B))).tocsr()
     P.sort_indices()
     P.data[:] = 1.0
-    Au = (sp.csr_matrix(A) + 0 * P).tocsr(); Au.sort_indices()
-    Bu = (sp.csr_matrix(B) + 0 * P).tocsr(); Bu.sort_indices()
-    Au = _align(Au, P); Bu = _align(Bu, P)
-    sign = -1.0 if which in ("LA", "LM") else 1.0
+    Au = _align(sp.csr_matrix(A), P)
+    Bu = _align(sp.csr_matrix(B), P)
+    sign = -1.0 if largest else 1.0
     dA = np.abs(Au.diagonal())
     dB = np.abs(Bu.diagonal())
     sA = dA.max() if dA.max() > 0 else 1.0
@@ -56,9 +60,8 @@ def gpu_eigsh(A, B, k, which="SM", tol=1e-8, maxiter=1000, seed=0):
     d = dA / sA + dB / sB
     d[d == 0] = 1.0
     Dm = sp.diags(1.0 / np.sqrt(d))
-    As = (Dm @ Au @ Dm).tocsr() * (sign / sA)
-    Bs = (Dm @ Bu @ Dm).tocsr() / sB
-    As = _align(As.tocsr(), P); Bs = _align(Bs.tocsr(), P)
+    As = _align((Dm @ Au @ Dm).tocsr() * (sign / sA), P)
+    Bs = _align((Dm @ Bu @ Dm).tocsr() / sB, P)
     ip = torch.from_numpy(P.indptr.astype(np.int32)).to(dev)
     ix = torch.from_numpy(P.indices.astype(np.int32)).to(dev)
     Hv = torch.from_numpy(As.data.astype(np.float64)).to(dev)
@@ -71,8 +74,82 @@ def gpu_eigsh(A, B, k, which="SM", tol=1e-8, maxiter=1000, seed=0):
     res = lobpcg(op, k, b, None, tol=rtol, maxiter=maxiter, seed=seed)
     lam = res.eigenvalues[:k].cpu().numpy() * sign * (sA / sB)
     X = res.X[:, :k].cpu().numpy() / np.sqrt(d)[:, None] / np.sqrt(sB)
-    order = np.argsort(lam) if sign > 0 else np.argsort(-lam)
+    order = np.argsort(lam)
     return lam[order], X[:, order], res.converged
+
+
+def _select(vals, k, which, sigma):
+    """Indices of the k eigenvalues that eigsh(which, sigma) returns, in ascending
+    order of eigenvalue."""
+    vals = np.asarray(vals)
+    if sigma is not None:
+        if which != "LM":
+            raise ValueError(f"which={which!r} with sigma: only 'LM' (eigenvalues nearest sigma) is supported")
+        key = np.abs(vals - sigma)
+    elif which == "SA":
+        key = vals
+    elif which == "LA":
+        key = -vals
+    elif which == "SM":
+        key = np.abs(vals)
+    elif which == "LM":
+        key = -np.abs(vals)
+    else:
+        raise ValueError(f"which must be one of 'SA', 'LA', 'SM', 'LM', got {which!r}")
+    idx = np.argsort(key, kind="stable")[:k]
+    return idx[np.argsort(vals[idx], kind="stable")]
+
+
+def gpu_eigsh(A, B, k, which="SM", tol=1e-8, maxiter=1000, seed=0, sigma=None):
+    """The k eigenpairs of A x = lambda B x (B symmetric positive definite, or None)
+    that eigsh(A, k, M=B, which=which, sigma=sigma) returns, on the GPU.
+
+    'SA' / 'LA' are one block solve from the bottom / top of the spectrum; 'LM' is
+    the k largest magnitudes among the k lowest and the k highest.  'SM' and
+    'LM' with sigma select around a target t (0 or sigma): blocks of growing size
+    are solved from the bottom of the spectrum until they bracket the k eigenvalues
+    nearest t (the highest computed eigenvalue is at least t plus the k-th
+    distance), falling back to the dense FP64 eigensolve once the block would not
+    fit the problem."""
+    n = A.shape[0]
+    if k >= n:
+        raise ValueError(f"k={k} must be smaller than the matrix size {n}")
+    if which not in ("SA", "LA", "SM", "LM"):
+        raise ValueError(f"which must be one of 'SA', 'LA', 'SM', 'LM', got {which!r}")
+    if sigma is not None and which != "LM":
+        raise ValueError(f"which={which!r} with sigma: only 'LM' (eigenvalues nearest sigma) is supported")
+
+    def fits(kk):
+        return default_block(kk, n) is not None
+
+    def block(kk, largest):
+        if not fits(kk):
+            w, V = gpu_dense_eigh(A, B, n, "SA")
+            sl = slice(n - kk, n) if largest else slice(0, kk)
+            return w[sl], V[:, sl], True
+        return _block_extreme(A, B, kk, largest, tol, maxiter, seed)
+
+    if sigma is None and which in ("SA", "LA"):
+        return block(k, which == "LA")
+    if sigma is None and which == "LM":
+        lo_v, lo_x, c1 = block(k, False)
+        hi_v, hi_x, c2 = block(k, True)
+        vals = np.concatenate([lo_v, hi_v])
+        vecs = np.concatenate([lo_x, hi_x], axis=1)
+        # the same eigenpair may be in both halves when 2k > n; drop repeats
+        _, first = np.unique(np.round(vals, 12), return_index=True)
+        vals, vecs = vals[first], vecs[:, first]
+        idx = _select(vals, k, "LM", None)
+        return vals[idx], vecs[:, idx], c1 and c2
+    target = 0.0 if sigma is None else float(sigma)
+    kk = k
+    while True:
+        vals, vecs, conv = block(kk, False)
+        dist = np.sort(np.abs(vals - target))
+        if kk >= n - 1 or vals.max() >= target + dist[k - 1] or not fits(kk):
+            idx = _select(vals, k, which, sigma)
+            return vals[idx], vecs[:, idx], conv
+        kk = min(2 * kk, n - 1)
 
 
 def _align(A, P):
@@ -87,7 +164,9 @@ def _align(A, P):
     return C
 
 
-def gpu_dense_eigh(A, B, k, which):
+def gpu_dense_eigh(A, B, k, which, sigma=None):
+    """Dense FP64 eigensolve on the GPU; the k eigenpairs eigsh(which, sigma)
+    would return, ascending."""
     dev = device()
     Ad = torch.from_numpy(sp.csr_matrix(A).toarray()).to(dev, torch.float64)
     if B is None:
@@ -99,13 +178,7 @@ def gpu_dense_eigh(A, B, k, which):
         w, Y = torch.linalg.eigh(Li @ Ad @ Li.T)
         V = Li.T @ Y
     w = w.cpu().numpy(); V = V.cpu().numpy()
-    if which == 'SM':
-        idx = np.argsort(np.abs(w))
-    elif which == 'LA':
-        idx = np.argsort(w)[::-1]
-    else:
-        idx = np.argsort(w)
-    idx = idx[:k]
+    idx = _select(w, k, which, sigma)
     return w[idx], V[:, idx]
 
 
@@ -129,15 +202,19 @@ class EigenSolver:
 
     def _run(self, A, B, k, method, options):
         which = options.get('which', 'SM')
+        sigma = options.get('sigma', None)
         if method in ('arpack', 'lobpcg'):
-            vals, vecs, conv = gpu_eigsh(A, B, k, which, options['tolerance'], options['max_iterations'])
+            vals, vecs, conv = gpu_eigsh(A, B, k, which, options['tolerance'], options['max_iterations'],
+                                         sigma=sigma)
             if not conv:
+                # eigsh raises ArpackNoConvergence here; the caller prints and returns
+                # empty arrays (eigen_solver.pyx:117-135)
                 raise RuntimeError("block eigensolver did not converge")
             return vals, vecs
         if method == 'dense':
             if A.shape[0] > 1000:
                 raise ValueError("Dense solver not recommended for large matrices")
-            return gpu_dense_eigh(A, B, k, which)
+            return gpu_dense_eigh(A, B, k, which, sigma)
         raise ValueError(f"Unknown method: {method}")
 
     def _solve(self, A, B, k, method, kwargs, label):

@@ -4,9 +4,11 @@ Call surface of CythonPoissonSolver (qdsim_cython/solvers/poisson_solver.pyx:31-
 ``PoissonSolver(mesh, epsilon_r_func, rho_func)``, ``solve(V_p, V_n, n=None, p=None)``,
 ``get_potential()``, public ``stiffness_matrix``/``mass_matrix``.  Discretisation as
 in the reference: stiffness with epsilon = eps_r(centroid) EPSILON_0 (:130-140),
-consistent mass, right-hand side M (-rho_nodal) (:254-274; the reference's sign
-convention is kept), Dirichlet V_p on x = 0 and V_n on x = Lx (:276-306), natural
-boundary elsewhere.  The reference factorises the row-replaced matrix with spsolve
+consistent mass, Dirichlet V_p on x = 0 and V_n on x = Lx (:276-306), natural
+boundary elsewhere.  Right-hand side: M rho_nodal by default (``sign="physical"``,
+the weak form of -div(eps grad phi) = rho); ``sign="reference"`` reproduces the
+reference's M (-rho_nodal) (:267-274), i.e. K phi = -M rho, under which extra
+electrons raise phi and, through V - q phi, attract more electrons.  The reference factorises the row-replaced matrix with spsolve
 (:241); here the Dirichlet values are lifted into the right-hand side, the
 remaining SPD system is solved with the device CG, and the solution of the
 row-replaced system is identical in exact arithmetic.  On a structured mesh the
@@ -15,6 +17,7 @@ multigrid V-cycle per iteration (qds_cg_mg, PoissonLatticeMG); other meshes use
 Jacobi (qds_cg).
 """
 import time
+import warnings
 
 import numpy as np
 import torch
@@ -28,7 +31,10 @@ from .units import EPSILON_0, Q_E
 class PoissonProblem:
     """Device-resident Poisson operator of one mesh (values in SI)."""
 
-    def __init__(self, mesh, epsilon_r_func, sampling="centroid"):
+    def __init__(self, mesh, epsilon_r_func, sampling="centroid", sign="physical"):
+        if sign not in ("physical", "reference"):
+            raise ValueError(f"sign must be 'physical' or 'reference', got {sign!r}")
+        self.sign = sign
         self.mesh = mesh
         self.space = FEMSpace(mesh)
         sp_ = self.space
@@ -81,6 +87,7 @@ class PoissonProblem:
         self.n = mesh.num_nodes
         self.last_iterations = 0
         self.last_relres = 0.0
+        self.last_converged = True
 
     def spmv(self, vals, x):
         y = torch.empty_like(x)
@@ -89,7 +96,8 @@ class PoissonProblem:
 
     def solve(self, rho_nodal, V_p, V_n, x0=None, rtol=1e-12):
         """phi (device float64, SI volts) for nodal charge density rho (C/m^3)."""
-        rhs = self.spmv(self.M, -rho_nodal.contiguous())
+        rho = rho_nodal.contiguous()
+        rhs = self.spmv(self.M, rho if self.sign == "physical" else -rho)
         g = torch.zeros(self.n, dtype=torch.float64, device=rhs.device)
         g[self.left_d] = V_p
         g[self.right_d] = V_n
@@ -104,6 +112,12 @@ class PoissonProblem:
             it, rr = kernels.cg(self.r_indptr, self.r_indices, self.K_ff, self.free.size,
                                 int(self.r_indices.numel()), b, x, rtol=rtol, max_iter=20000)
         self.last_iterations, self.last_relres = it, rr
+        self.last_converged = rr <= rtol
+        if not self.last_converged:
+            # spsolve (poisson_solver.pyx:241) is direct; an iterative solve that
+            # stops short is reported rather than passed on silently
+            warnings.warn(f"Poisson CG stopped after {it} iterations at relative residual {rr:.3e} "
+                          f"(rtol {rtol:.1e})", RuntimeWarning, stacklevel=2)
         phi = g.clone()
         phi[self.free_d] = x
         return phi
@@ -112,7 +126,7 @@ class PoissonProblem:
 class PoissonSolver:
     """Drop-in for CythonPoissonSolver (poisson_solver.pyx:31)."""
 
-    def __init__(self, mesh, epsilon_r_func, rho_func):
+    def __init__(self, mesh, epsilon_r_func, rho_func, sign="physical"):
         self.num_nodes = mesh.num_nodes
         self.num_elements = mesh.num_elements
         self.Lx, self.Ly = mesh.Lx, mesh.Ly
@@ -121,7 +135,7 @@ class PoissonSolver:
         self.nodes_x, self.nodes_y = mesh.get_nodes()
         self.epsilon_r_func = epsilon_r_func
         self.rho_func = rho_func
-        self.problem = PoissonProblem(mesh, epsilon_r_func)
+        self.problem = PoissonProblem(mesh, epsilon_r_func, sign=sign)
         self.stiffness_matrix = self.problem.space.to_scipy(self.problem.K)
         self.mass_matrix = self.problem.space.to_scipy(self.problem.M)
         self.potential = np.zeros(self.num_nodes)

@@ -34,16 +34,34 @@ KIN_SCALE = HBAR * HBAR / 2.0 / EV_TO_J / (NM * NM)   # hbar^2/2 in eV nm^2 kg
 
 
 def default_block(k, n):
-    b = ((k + 8) + 3) // 4 * 4
-    b = max(b, 8)
-    while 4 * b > n and b > 4:
+    """Block size of the block eigensolver for k wanted pairs of an n-dimensional
+    pencil: k + 8 rounded up to a multiple of 4 (at least 8), reduced while 4b > n
+    but never below k (rounded up to a multiple of 4).  Returns None when no such
+    block has 3b <= n (the [X | W | P] basis would not fit): the caller then needs
+    the dense eigensolve."""
+    kmin = max(4, (k + 3) // 4 * 4)
+    b = max(((k + 8) + 3) // 4 * 4, 8)
+    while 4 * b > n and b - 4 >= kmin:
         b -= 4
-    return b
+    return b if 3 * b <= n else None
+
+
+def breakdown_reason(ritz, converged, error=None):
+    """Why a mixed solve must be redone with the safe shift, or None: a failed
+    Rayleigh-Ritz (error), non-finite Ritz values, or no convergence."""
+    if error is not None:
+        return f"Rayleigh-Ritz failed: {error}"
+    if ritz is None or not np.all(np.isfinite(ritz)):
+        return "non-finite Ritz values"
+    if not converged:
+        return "not converged"
+    return None
 
 
 def _warm(start, n_free, b):
-    """True when ``start`` (an earlier solve's info) holds a block for this problem."""
-    if not start or start.get("X_block") is None:
+    """True when ``start`` (an earlier, converged solve's info) holds a block for
+    this problem."""
+    if not start or start.get("X_block") is None or not start.get("converged", True):
         return False
     X = start["X_block"]
     return X.dim() == 2 and X.shape[0] == n_free and X.shape[1] >= b
@@ -92,6 +110,10 @@ class SchrodingerProblem:
         self._m_func = self._v_func = None
         self._coarse = None           # coarse problem of the nested start (built on first use)
         self.nested = True
+        # a mixed solve that breaks down or does not converge is solved again with
+        # the shift below min V (info["breakdown"] names the reason, info["retried"]
+        # says whether it was solved again)
+        self.retry = True
         self.setup_time = time.perf_counter() - t0
         if m_star_func is not None:
             self.set_mass(m_star_func)
@@ -211,6 +233,9 @@ class SchrodingerProblem:
             self.assemble()
         if b is None:
             b = default_block(k, self.n_free)
+            if b is None:
+                raise ValueError(f"{k} states of a {self.n_free}-DOF problem: the block eigensolver needs "
+                                 f"3 b <= n with b >= k; use a finer mesh or fewer states")
         t0 = time.perf_counter()
         op64 = self.operator(torch.float64)
         if mode == "mixed":
@@ -259,10 +284,11 @@ class SchrodingerProblem:
             try:
                 res = solve_mixed(op64, k, b, pre32, pre64, tol=tol, tol32=tol32, maxiter=maxiter, seed=seed,
                                   verbose=verbose, X0=X0, b64=b64, between=retune, **mixed_opts)
-                ok = bool(torch.isfinite(res.eigenvalues).all())
-            except np.linalg.LinAlgError:
-                ok = False
-            if not ok:
+                breakdown = breakdown_reason(res.eigenvalues.cpu().numpy(), res.converged)
+            except np.linalg.LinAlgError as e:
+                breakdown = breakdown_reason(None, False, e)
+            retried = breakdown is not None and self.retry
+            if retried:
                 # a shift above the lowest eigenvalue (a block that missed the lowest
                 # state, whose Ritz values then bound the wrong eigenvalues) makes the
                 # multigrid operator indefinite; solve again with the shift below min V
@@ -270,7 +296,10 @@ class SchrodingerProblem:
                 pre32_64.set_sigma(sigma0)
                 res = solve_mixed(op64, k, b, pre32, pre64, tol=tol, tol32=tol32, maxiter=maxiter, seed=seed,
                                   verbose=verbose, X0=X0, b64=b64, **mixed_opts)
+            elif breakdown is not None and breakdown != "not converged":
+                raise np.linalg.LinAlgError(f"mixed-precision solve broke down ({breakdown}) with the retry off")
         elif mode == "fp64":
+            breakdown, retried = None, False
             pre = self.preconditioner(b, torch.float64, cycle=dict(nu_pre=2, nu_post=2, coarse_sweeps=40))
             sync()
             t_prep = time.perf_counter() - t0
@@ -285,7 +314,8 @@ class SchrodingerProblem:
         full = torch.zeros((self.space.n, k), dtype=torch.float64, device=X.device)
         full[self.old_of_new] = X
         sync()
-        info = {"iterations": res.iterations, "converged": res.converged,
+        info = {"iterations": res.iterations, "converged": bool(res.converged),
+                "breakdown": breakdown, "retried": retried,
                 "residuals": res.residuals[:k].cpu().numpy() if res.residuals is not None else None,
                 "timings": dict(res.timings, prep_s=t_prep, nested_s=t_nested,
                                 nested_iters=self._nested_iters),

@@ -20,6 +20,7 @@ phi <- phi + alpha (phi_Poisson - phi), alpha = 0.3 (the damping of
 backend/src/self_consistent.cpp:1270-1272).
 """
 import time
+import warnings
 
 import numpy as np
 import torch
@@ -37,13 +38,14 @@ def _mean(a):
 
 class SelfConsistentSolver:
     def __init__(self, mesh, epsilon_r_func, m_star_func, potential_func, temperature=300.0,
-                 mixing=0.3):
+                 mixing=0.3, poisson_sign="physical"):
         self.mesh = mesh
         self.temperature = float(temperature)
         self.max_iterations = 100
         self.tolerance = 1e-6
         self.mixing = float(mixing)
         self.warm_start_dphi = 1e-2
+        self.eig_tol = 1e-6           # relative residual of each eigensolve
         self.fermi_level = 0.0
         n = mesh.num_nodes
         self.potential = np.zeros(n)
@@ -55,7 +57,8 @@ class SelfConsistentSolver:
         self.last_iterations = 0
         self.converged = False
         self.convergence_history = []
-        self.poisson = PoissonProblem(mesh, epsilon_r_func)
+        # "physical": -div(eps grad phi) = rho; "reference": the reference's K phi = -M rho
+        self.poisson = PoissonProblem(mesh, epsilon_r_func, sign=poisson_sign)
         self.schrodinger = SchrodingerProblem(mesh, m_star_func, None)
         sp_ = self.schrodinger.space
         # base potential (eV) at the quadrature points of the Schroedinger space
@@ -111,6 +114,9 @@ class SelfConsistentSolver:
         phi = V_p + (V_n - V_p) * x / Lx
         self.convergence_history = []
         self.converged = False
+        self.unconverged_solves = 0   # eigensolves that ended unconverged
+        self.breakdown_solves = 0     # eigensolves that broke down or did not converge
+        self.retried_solves = 0       # ... and were redone with the safe shift
         it = 0
         residual = float("inf")
         lam = np.array([])
@@ -127,7 +133,15 @@ class SelfConsistentSolver:
             # earlier outer iterations can move it by volts, and the previous block
             # may then miss the new lowest states
             warm = info if residual < self.warm_start_dphi else None
-            lam, psi, info = self.schrodinger.solve(num_quantum_states, start=warm)
+            lam, psi, info = self.schrodinger.solve(num_quantum_states, start=warm, tol=self.eig_tol)
+            if not info["converged"]:
+                self.unconverged_solves += 1
+                warnings.warn(f"self-consistent iteration {it}: eigensolver did not converge "
+                              f"(residuals {info['residuals']})", RuntimeWarning, stacklevel=2)
+            if info.get("breakdown"):
+                self.breakdown_solves += 1
+            if info.get("retried"):
+                self.retried_solves += 1
             # psi: M_nm-normalised -> SI normalisation (psi^T M_SI psi = 1)
             psi_si = psi * 1e9
             n, p = self._densities(lam, psi_si, phi, N_A, N_D)

@@ -21,6 +21,7 @@ Formulations:
   broadening -i 0.1 cap_strength (:321-328).  Solved on the GPU as well.
 """
 import time
+import warnings
 
 import numpy as np
 import torch
@@ -84,6 +85,13 @@ class FixedOpenSystemSolver:
                 prob = self._get_problem()
                 prob.assemble()
                 lam, X, info = prob.solve(k)
+                self.last_info = info
+                if not info["converged"]:
+                    # eigsh raises ArpackNoConvergence here; the reference then prints
+                    # and returns empty arrays (fixed_open_system_solver.pyx:190-195).
+                    # The pairs are returned with a warning instead.
+                    warnings.warn(f"eigensolver did not converge: residuals {info['residuals']}",
+                                  RuntimeWarning, stacklevel=2)
                 vals = lam * EV_TO_J
                 vecs = X.cpu().numpy() * 1e9          # SI M-normalisation
             self.eigenvalues = vals

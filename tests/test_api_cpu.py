@@ -64,3 +64,58 @@ def test_fem_solver_rejects_unknown_method_without_gpu_work():
     import inspect
     src = inspect.getsource(fem.FEMSolver.solve_linear_system)
     assert "Unknown solver method" in src
+
+
+def test_which_sigma_selection_matches_eigsh():
+    """_select picks what scipy's eigsh returns for which / sigma, ascending, on an
+    indefinite pencil (checked against ARPACK on the CPU, the reference's call
+    eigen_solver.pyx:225-235)."""
+    import scipy.linalg as sla
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    from qdsim_amd.eigen import _select
+    n = 60
+    rng = np.random.default_rng(3)
+    A = sp.diags([-np.ones(n - 1), np.linspace(-3, 2, n), -np.ones(n - 1)], [-1, 0, 1]).tocsc()
+    B = sp.diags([0.1 * np.ones(n - 1), 1 + rng.random(n), 0.1 * np.ones(n - 1)], [-1, 0, 1]).tocsc()
+    w = sla.eigh(A.toarray(), B.toarray(), eigvals_only=True)
+    assert w.min() < 0 < w.max()
+    k = 5
+    for which, sigma in (("SA", None), ("LA", None), ("SM", None), ("LM", None), ("LM", 0.3), ("LM", -1.7)):
+        ref = spla.eigsh(A, k, M=B, which=which, sigma=sigma, tol=1e-12, return_eigenvectors=False)
+        got = w[_select(w, k, which, sigma)]
+        assert np.allclose(got, np.sort(ref), rtol=0, atol=1e-9), (which, sigma, got, np.sort(ref))
+        assert np.all(np.diff(got) >= 0)
+    with pytest.raises(ValueError):
+        _select(w, k, "SM", 0.3)
+    with pytest.raises(ValueError):
+        _select(w, k, "XX", None)
+
+
+def test_default_block_never_below_k():
+    from qdsim_amd.schrodinger import default_block
+    for n in range(4, 400, 7):
+        for k in range(1, 40):
+            b = default_block(k, n)
+            if b is None:
+                # no multiple-of-4 block b >= k with 3 b <= n exists
+                assert 3 * max(4, (k + 3) // 4 * 4) > n
+            else:
+                assert b >= k and b % 4 == 0 and 3 * b <= n
+    assert default_block(10, 40) == 12      # was 8 < k
+    assert default_block(16, 2_356_225) == 24
+
+
+def test_breakdown_reason():
+    from qdsim_amd.schrodinger import breakdown_reason
+    assert breakdown_reason(np.array([-0.1, 0.2]), True) is None
+    assert breakdown_reason(np.array([-0.1, 0.2]), False) == "not converged"
+    assert breakdown_reason(np.array([np.nan, 0.2]), True) == "non-finite Ritz values"
+    assert breakdown_reason(None, False, np.linalg.LinAlgError("x")).startswith("Rayleigh-Ritz failed")
+
+
+def test_degenerate_clusters():
+    from qdsim_amd.adjoint import degenerate_clusters
+    lam = np.array([-0.172, -0.0703, -0.0703 + 1e-12, -0.0083, 0.01, 0.01, 0.01])
+    assert degenerate_clusters(lam) == [[1, 2], [4, 5, 6]]
+    assert degenerate_clusters(np.array([-0.172, -0.07028, -0.07002])) == []

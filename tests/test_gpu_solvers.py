@@ -45,6 +45,33 @@ def test_eigensolver_generalized_and_errors():
     assert np.allclose(v2, r["vals"], rtol=1e-9)
 
 
+@pytest.mark.parametrize("method", ["arpack", "lobpcg", "dense"])
+def test_eigensolver_which_and_sigma_on_indefinite_pencil(method):
+    """which / sigma select what scipy's eigsh selects, in every method, on a pencil
+    with eigenvalues of both signs (a well deeper than its confinement energy)."""
+    import scipy.sparse.linalg as spla
+    from qdsim_amd.eigen import EigenSolver
+    r = eig_ref.solve_closed(21, 17, 20.0, 15.0, 1, lambda x, y: 0.067 + 0 * x,
+                             lambda x, y: np.where(np.hypot(x - 10, y - 7.5) < 5, -0.25, -0.1), 4)
+    H = sp.csr_matrix(r["H"][r["free"]][:, r["free"]])
+    M = sp.csr_matrix(r["M"][r["free"]][:, r["free"]])
+    es = EigenSolver()
+    k = 4
+    for which, sigma in (("SA", None), ("SM", None), ("LM", None), ("LM", 0.02), ("LA", None)):
+        ref = np.sort(spla.eigsh(H.tocsc(), k, M=M.tocsc(), which=which, sigma=sigma, tol=1e-12,
+                                 return_eigenvectors=False))
+        print(f"{method} which={which} sigma={sigma} ref={ref}", flush=True)
+        vals, vecs = es.solve_generalized_eigenvalue(H, M, k, method=method, which=which, sigma=sigma,
+                                                     tolerance=1e-12)
+        assert es.get_solve_info()["converged"]
+        assert np.allclose(vals, ref, rtol=1e-8, atol=1e-12 * np.abs(ref).max()), (which, sigma, vals, ref)
+        res = H @ vecs - (M @ vecs) * vals[None]
+        assert np.abs(res).max() <= 1e-6 * np.abs(H).max() * np.abs(vecs).max()
+    # eigsh with sigma and which != 'LM' (farthest from sigma) is not supported
+    v, _ = es.solve_generalized_eigenvalue(H, M, k, method=method, which="SM", sigma=0.02)
+    assert v.size == 0 and not es.get_solve_info()["converged"]
+
+
 # ---- FixedOpenSystemSolver -------------------------------------------------------
 def _well_si(x, y):
     return -0.06 * EV if 5e-9 < x < 15e-9 else 0.0
@@ -87,17 +114,26 @@ def test_fixed_open_system_reference_formulation():
 
 
 # ---- Poisson + CG ----------------------------------------------------------------
-def test_poisson_matches_reference_and_linear_case():
+@pytest.mark.parametrize("sign", ["physical", "reference"])
+def test_poisson_matches_reference_and_linear_case(sign):
     from qdsim_amd.poisson import PoissonSolver
     from qdsim_amd.mesh import SimpleMesh
     nx, ny, Lx, Ly = 21, 11, 100e-9, 40e-9
     mesh = SimpleMesh(nx, ny, Lx, Ly)
     rng = np.random.default_rng(0)
     rho_nodes = rng.uniform(-1, 1, nx * ny) * 1.6e-19 * 1e23
-    ps = PoissonSolver(mesh, lambda x, y: 13.9, lambda x, y, n, p: n)   # rho = n (test hook)
+    ps = PoissonSolver(mesh, lambda x, y: 13.9, lambda x, y, n, p: n, sign=sign)   # rho = n (test hook)
     ps.solve(0.0, 0.5, n=rho_nodes)
-    ref, xr, yr = poisson_ref.solve(nx, ny, Lx, Ly, lambda x, y: 13.9 + 0 * x, rho_nodes, 0.0, 0.5)
+    ref, xr, yr = poisson_ref.solve(nx, ny, Lx, Ly, lambda x, y: 13.9 + 0 * x, rho_nodes, 0.0, 0.5, sign=sign)
     assert np.allclose(ps.get_potential(), ref, rtol=1e-9, atol=1e-10)
+    # uniform charge: the closed form that pins the oracle's sign (test_oracle_pinning)
+    rho0 = 1.602176634e-19 * 1e24
+    ps2 = PoissonSolver(mesh, lambda x, y: 13.9, lambda x, y, n, p: n, sign=sign)
+    ps2.solve(0.2, -0.3, n=np.full(nx * ny, rho0))
+    x, _ = mesh.get_nodes()
+    s = 1.0 if sign == "reference" else -1.0
+    exact = 0.2 - 0.5 * x / Lx + s * rho0 / (2 * 13.9 * poisson_ref.EPSILON_0) * x * (x - Lx)
+    assert np.abs(ps2.get_potential() - exact).max() <= 1e-9 * np.abs(exact).max()
     ps.solve(0.2, 0.7)   # rho_func returns zeros -> linear profile
     x, _ = mesh.get_nodes()
     assert np.allclose(ps.get_potential(), 0.2 + 0.5 * x / Lx, atol=1e-11)

@@ -183,3 +183,133 @@ def test_poisson_linear_solution():
     nx, ny, Lx, Ly = 9, 6, 50e-9, 20e-9
     phi, x, y = poisson_ref.solve(nx, ny, Lx, Ly, lambda x, y: 13.9 + 0 * x, np.zeros(nx * ny), 0.0, 0.7)
     assert np.allclose(phi, 0.7 * x / Lx, atol=1e-12)
+
+
+@pytest.mark.parametrize("sign", ["reference", "physical"])
+def test_poisson_uniform_charge_closed_form(sign):
+    """Uniform rho: the 1-D solution of eps phi'' = +rho (reference sign, K phi = -M rho,
+    poisson_solver.pyx:267-274) or -rho (physical sign) with phi(0) = V_p, phi(L) = V_n
+    and insulating top/bottom.  P1 Galerkin reproduces it at the nodes; a flipped
+    sign or a lumped/halved mass weighting is off by volts."""
+    nx, ny, Lx, Ly = 17, 5, 100e-9, 40e-9
+    eps_r = 13.9
+    rho = 1.602176634e-19 * 1e24          # 1e24 m^-3 of unit charges
+    V_p, V_n = 0.2, -0.3
+    phi, x, y = poisson_ref.solve(nx, ny, Lx, Ly, lambda x, y: eps_r + 0 * x, np.full(nx * ny, rho), V_p, V_n,
+                                  sign=sign)
+    eps = eps_r * poisson_ref.EPSILON_0
+    s = 1.0 if sign == "reference" else -1.0
+    exact = V_p + (V_n - V_p) * x / Lx + s * rho / (2 * eps) * x * (x - Lx)
+    scale = np.abs(exact).max()
+    assert scale > 1.0                      # the charge term dominates the bias
+    assert np.abs(phi - exact).max() <= 1e-12 * scale
+    with pytest.raises(ValueError):
+        poisson_ref.solve(3, 3, 1e-8, 1e-8, lambda x, y: 1 + 0 * x, np.zeros(9), 0, 0, sign="other")
+
+
+# ---- P2 / P3 numbering: hand-derived from backend/src/mesh.cpp:66-154 ------------
+# Vertices are row-major (mesh.cpp:77-81); per cell n0 = (i, j), n1 = n0 + 1,
+# n2 = n0 + row, n3 = n2 + 1; midpoints in the call order m01, m12, m20, m23, m13
+# [, m03] through the std::minmax cache (mesh.cpp:95-115).  P3: two DOFs per edge
+# at creation, the first nearer the smaller vertex id; element edge DOF "a" is the
+# one nearer the first-named vertex; centroids c012, c132 after the cell's edges.
+P2_1x1 = {  # Lx = Ly = 2: integer coordinates
+    "xy": [(0, 0), (2, 0), (0, 2), (2, 2), (1, 0), (1, 1), (0, 1), (1, 2), (2, 1)],
+    "el": [[0, 1, 2, 4, 5, 6], [1, 3, 2, 8, 7, 5]],
+}
+P2_1x1_ORPHAN_XY = P2_1x1["xy"] + [(1, 1)]      # m03 = mid(n0, n3), the same point as m12
+P2_2x1 = {  # Lx = 4, Ly = 2
+    "xy": [(0, 0), (2, 0), (4, 0), (0, 2), (2, 2), (4, 2),
+           (1, 0), (1, 1), (0, 1), (1, 2), (2, 1),          # cell 0: m01 m12 m20 m23 m13
+           (3, 0), (3, 1), (3, 2), (4, 1)],                 # cell 1: m01 m12 (m20 = 10) m23 m13
+    "el": [[0, 1, 3, 6, 7, 8], [1, 4, 3, 10, 9, 7], [1, 2, 4, 11, 12, 10], [2, 5, 4, 14, 13, 12]],
+}
+P2_2x1_ORPHAN = {  # with m03 created after m13 in each cell
+    "xy": P2_2x1["xy"][:11] + [(1, 1), (3, 0), (3, 1), (3, 2), (4, 1), (3, 1)],
+    "el": [[0, 1, 3, 6, 7, 8], [1, 4, 3, 10, 9, 7], [1, 2, 4, 12, 13, 10], [2, 5, 4, 15, 14, 13]],
+}
+P3_1x1 = {  # Lx = Ly = 3
+    "xy": [(0, 0), (3, 0), (0, 3), (3, 3),
+           (1, 0), (2, 0),      # edge (0,1): nearer 0, nearer 1
+           (2, 1), (1, 2),      # edge (1,2)
+           (0, 1), (0, 2),      # edge (0,2)
+           (1, 3), (2, 3),      # edge (2,3)
+           (3, 1), (3, 2),      # edge (1,3)
+           (1, 1), (2, 2)],     # c012, c132
+    "el": [[0, 1, 2, 4, 5, 6, 7, 9, 8, 14], [1, 3, 2, 12, 13, 11, 10, 7, 6, 15]],
+}
+P3_2x1 = {  # Lx = 6, Ly = 3; edge (1,4) is shared with reversed orientation
+    "xy": [(0, 0), (3, 0), (6, 0), (0, 3), (3, 3), (6, 3),
+           (1, 0), (2, 0), (2, 1), (1, 2), (0, 1), (0, 2), (1, 3), (2, 3), (3, 1), (3, 2), (1, 1), (2, 2),
+           (4, 0), (5, 0), (5, 1), (4, 2), (4, 3), (5, 3), (6, 1), (6, 2), (4, 1), (5, 2)],
+    "el": [[0, 1, 3, 6, 7, 8, 9, 11, 10, 16], [1, 4, 3, 14, 15, 13, 12, 9, 8, 17],
+           [1, 2, 4, 18, 19, 20, 21, 15, 14, 26], [2, 5, 4, 24, 25, 23, 22, 21, 20, 27]],
+}
+
+
+@pytest.mark.parametrize("case", ["p2_1x1", "p2_1x1_orphan", "p2_2x1", "p2_2x1_orphan", "p3_1x1", "p3_2x1"])
+def test_numbering_hand_derived(case):
+    from qdsim_amd.mesh import SimpleMesh
+    order = int(case[1])
+    cx = int(case[3])
+    orphan = case.endswith("orphan")
+    fx = {"p2_1x1": P2_1x1, "p2_1x1_orphan": {"xy": P2_1x1_ORPHAN_XY, "el": P2_1x1["el"]},
+          "p2_2x1": P2_2x1, "p2_2x1_orphan": P2_2x1_ORPHAN, "p3_1x1": P3_1x1, "p3_2x1": P3_2x1}[case]
+    L = float(order * 1)                     # one cell = `order` units per side
+    Lx, Ly = cx * L, L
+    if order == 2:
+        x, y, el = mesh_ref.mesh_cpp_p2(cx + 1, 2, Lx, Ly, orphan_nodes=orphan)
+    else:
+        x, y, el = mesh_ref.mesh_p3(cx + 1, 2, Lx, Ly)
+    xy = np.array(fx["xy"], dtype=float)
+    assert np.array_equal(el, np.array(fx["el"]))
+    assert np.allclose(np.c_[x, y], xy, atol=1e-12)
+    # the product numbering is the same (no GPU needed: host mesh generation)
+    m = SimpleMesh(cx + 1, 2, Lx, Ly, order, orphan_nodes=orphan) if order == 2 else SimpleMesh(cx + 1, 2, Lx, Ly, 3)
+    px, py = m.get_nodes()
+    assert np.array_equal(np.asarray(m.get_elements()), np.array(fx["el"]))
+    assert np.allclose(np.c_[px, py], xy, atol=1e-12)
+
+
+# ---- FixedOpenSystemSolver lumped matrices, by hand from fixed_open_system_solver.pyx:230-247
+def test_fixed_matrices_hand_computed_2x2():
+    """One cell (nodes 0 (0,0), 1 (a,0), 2 (0,b), 3 (a,b); triangles [0,1,2], [1,3,2]),
+    mass m0 on the first triangle's centroid (a/3, b/3), m1 on the second's (2a/3, 2b/3),
+    V = g x at the centroid.  Per triangle of area A: diag hbar^2/2m A/3 + V A/3,
+    off-diagonal hbar^2/2m A/6; M diag A/6, off-diagonal A/12."""
+    from oracle import fixed_ref
+    HB = fixed_ref.HBAR
+    a, b = 10e-9, 6e-9
+    m0, m1, g = 0.067 * 9.1093837015e-31, 0.041 * 9.1093837015e-31, 0.3 * fixed_ref.EV / 10e-9
+    xs, ys, H, M = fixed_ref.matrices(2, 2, a, b, lambda x, y: m0 if x < a / 2 else m1, lambda x, y: g * x)
+    A = a * b / 2
+    c0, c1 = HB ** 2 / (2 * m0), HB ** 2 / (2 * m1)
+    v0, v1 = g * a / 3, g * 2 * a / 3
+    d0, d1 = (c0 + v0) * A / 3, (c1 + v1) * A / 3
+    Hh = np.array([[d0, c0 * A / 6, c0 * A / 6, 0],
+                   [c0 * A / 6, d0 + d1, (c0 + c1) * A / 6, c1 * A / 6],
+                   [c0 * A / 6, (c0 + c1) * A / 6, d0 + d1, c1 * A / 6],
+                   [0, c1 * A / 6, c1 * A / 6, d1]])
+    Mh = A / 12 * np.array([[2, 1, 1, 0], [1, 4, 2, 1], [1, 2, 4, 1], [0, 1, 1, 2]])
+    assert np.allclose(H.toarray(), Hh, rtol=1e-14, atol=0)
+    assert np.allclose(M.toarray(), Mh, rtol=1e-14, atol=0)
+
+
+def test_fixed_matrices_hand_computed_3x2():
+    """Two cells, uniform c = hbar^2/2m and V: diagonal (c + V) A/3 times the number of
+    triangles at the node, off-diagonal c A/6 times the triangles on the edge."""
+    from oracle import fixed_ref
+    a, b = 8e-9, 5e-9
+    m, V = 0.067 * 9.1093837015e-31, -0.05 * fixed_ref.EV
+    xs, ys, H, M = fixed_ref.matrices(3, 2, 2 * a, b, lambda x, y: m, lambda x, y: V)
+    A = a * b / 2
+    c = fixed_ref.HBAR ** 2 / (2 * m)
+    tri_at = [1, 3, 2, 2, 3, 1]
+    edges = {(0, 1): 1, (0, 3): 1, (1, 3): 2, (1, 4): 2, (3, 4): 1, (1, 2): 1, (2, 4): 2, (2, 5): 1, (4, 5): 1}
+    Hh = np.diag([(c + V) * A / 3 * t for t in tri_at])
+    Mh = np.diag([A / 6 * t for t in tri_at])
+    for (i, j), t in edges.items():
+        Hh[i, j] = Hh[j, i] = c * A / 6 * t
+        Mh[i, j] = Mh[j, i] = A / 12 * t
+    assert np.allclose(H.toarray(), Hh, rtol=1e-14, atol=0)
+    assert np.allclose(M.toarray(), Mh, rtol=1e-14, atol=0)
