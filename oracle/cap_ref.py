@@ -22,3 +22,19 @@ def cap_dense(nx, ny, Lx_nm, Ly_nm, order, mstar_func, v_func, cap_func):
     order_ = np.argsort(vals.real)
     m.update(vals=vals[order_], vecs_free=vecs[:, order_], free=free, A=A, Md=M)
     return m
+
+
+def cap_eigs(nx, ny, Lx_nm, Ly_nm, order, mstar_func, v_func, cap_func, k, sigma):
+    """The k eigenpairs of (H - i W, M) nearest sigma (eV) by ARPACK shift-invert with
+    a sparse LU, scipy.sparse.linalg.eigs(A, k, M, sigma): the CPU call the CAP
+    configuration names.  Sorted by distance to sigma.  Pinned against cap_dense on
+    small meshes (tests/test_oracle_pinning.py)."""
+    import scipy.sparse.linalg as spla
+    m = eig_ref.schrodinger_matrices(nx, ny, Lx_nm, Ly_nm, order, mstar_func, v_func, cap_func=cap_func)
+    free = eig_ref.free_dofs(m["x"], m["y"], m["el"], Lx_nm, Ly_nm)
+    A = (m["H"][free][:, free] - 1j * m["W"][free][:, free]).tocsc()
+    M = m["M"][free][:, free].tocsc().astype(np.complex128)
+    vals, vecs = spla.eigs(A, k=k, M=M, sigma=sigma, which="LM", tol=1e-12)
+    order_ = np.argsort(np.abs(vals - sigma))
+    m.update(vals=vals[order_], vecs_free=vecs[:, order_], free=free, A=A, Ms=M)
+    return m

@@ -313,3 +313,44 @@ def test_fixed_matrices_hand_computed_3x2():
         Mh[i, j] = Mh[j, i] = A / 12 * t
     assert np.allclose(H.toarray(), Hh, rtol=1e-14, atol=0)
     assert np.allclose(M.toarray(), Mh, rtol=1e-14, atol=0)
+
+
+def test_cap_eigs_matches_dense_zggev():
+    """The sparse shift-invert CAP oracle (eigs with sigma) returns the dense zggev
+    eigenvalues nearest sigma."""
+    from oracle import cap_ref
+
+    def v(x, y):
+        return np.where((np.abs(x - 20) > 4) & (np.abs(x - 20) < 6), 0.2, 0.0) + 0 * y
+
+    def w(x, y):
+        d = np.maximum(0.0, np.maximum(np.maximum(5 - x, x - 35), np.maximum(5 - y, y - 25)))
+        return 0.05 * (d / 5.0) ** 2
+    args = (13, 10, 40.0, 30.0, 2, lambda x, y: 0.067 + 0 * x, v, w)
+    dense = cap_ref.cap_dense(*args)
+    sig = 0.05
+    near = dense["vals"][np.argsort(np.abs(dense["vals"] - sig))[:6]]
+    sparse = cap_ref.cap_eigs(*args, k=6, sigma=sig)
+    assert np.allclose(sparse["vals"], near, rtol=1e-10, atol=1e-13)
+    assert np.all(sparse["vals"].imag <= 1e-12)
+
+
+def test_adjoint_ref_gradient_matches_central_differences():
+    """Hellmann-Feynman gradient of the gate loss (oracle quadrature, closed-form
+    dV/dtheta) against central differences (step 1e-6) of the eig_ref eigenvalues."""
+    from oracle import adjoint_ref
+    theta = np.array([-0.25, 18.0, 21.0, -0.12, 25.0, 14.0])
+    sigma = np.array([6.0, 5.0])
+    targets = np.array([-0.15, -0.05, 0.0])
+    loss, g, E = adjoint_ref.loss_and_gradient(10, 2, 40.0, 0.067, theta, sigma, targets)
+    h = 1e-6
+    fd = np.zeros_like(theta)
+    for j in range(len(theta)):
+        tp, tm = theta.copy(), theta.copy()
+        tp[j] += h
+        tm[j] -= h
+        fd[j] = (adjoint_ref.loss_and_gradient(10, 2, 40.0, 0.067, tp, sigma, targets)[0]
+                 - adjoint_ref.loss_and_gradient(10, 2, 40.0, 0.067, tm, sigma, targets)[0]) / (2 * h)
+    assert np.allclose(g, fd, rtol=1e-5, atol=1e-9), (g, fd)
+    # a dropped chain-rule factor (e.g. the amplitude in d/dcx) fails
+    assert np.abs(g[1]) > 1e-4 and np.abs(g[4]) > 1e-4
