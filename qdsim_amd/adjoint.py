@@ -17,9 +17,18 @@ sum, or any loss that weighs its members equally) have well-defined gradients,
 sum_k psi_k^2 being basis independent.  ``degenerate_clusters`` finds the
 clusters, and the backward warns when the incoming gradient differs inside one.
 
-The Poisson part of the self-consistent loop is linear: phi = K^-1 (f(theta) -
-K g), so d/dtheta of an objective through phi needs one adjoint solve
-K lambda = dJ/dphi with the same device CG (``poisson_adjoint``).
+Gate and doping parameters act through the Poisson step, which is linear:
+phi_f = K_ff^-1 P_f (s M rho - K g), phi_d = g_d (g: the Dirichlet values of the
+contacts, s = +1 for the physical sign, -1 for the reference's).  For an
+objective J(phi) one adjoint solve K_ff lambda = dJ/dphi_f (K symmetric, the same
+device CG as the forward solve) gives
+    dJ/drho = s M lambda_ext,   dJ/dg_d = dJ/dphi_d - (K lambda_ext)_d
+(lambda_ext: lambda on the free DOFs, 0 on the contacts).  ``poisson_potential``
+is that step as a torch.autograd.Function of (rho, V_p, V_n), and
+``electrostatic_levels`` chains it with the FE interpolation of phi to the
+quadrature points and the Hellmann-Feynman eigenvalues: the levels of V - q phi
+as a differentiable function of any torch expression for the charge density
+(doping profile) and the contact (gate) voltages.
 """
 import warnings
 
@@ -107,9 +116,44 @@ def quadrature_points_nm(space):
 
 def poisson_adjoint(poisson_problem, dJ_dphi_free, rtol=1e-12):
     """lambda = K_ff^-1 dJ/dphi_free (K symmetric), the adjoint of the Poisson step."""
-    from . import kernels
-    pp = poisson_problem
-    lam = torch.zeros_like(dJ_dphi_free)
-    kernels.cg(pp.r_indptr, pp.r_indices, pp.K_ff, pp.free.size, int(pp.r_indices.numel()),
-               dJ_dphi_free.contiguous(), lam, rtol=rtol, max_iter=20000)
-    return lam
+    return poisson_problem.solve_free(dJ_dphi_free, rtol=rtol)
+
+
+class _PoissonPotential(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, rho, V_p, V_n, pp, rtol):
+        ctx.pp, ctx.rtol = pp, rtol
+        return pp.solve(rho.detach().contiguous(), float(V_p), float(V_n), rtol=rtol)
+
+    @staticmethod
+    def backward(ctx, g_phi):
+        pp = ctx.pp
+        g_phi = g_phi.contiguous()
+        lam = torch.zeros_like(g_phi)
+        lam[pp.free_d] = poisson_adjoint(pp, g_phi[pp.free_d], ctx.rtol)
+        s = 1.0 if pp.sign == "physical" else -1.0
+        g_rho = s * pp.spmv(pp.M, lam)
+        g_dir = g_phi - pp.spmv(pp.K, lam)
+        g_vp = g_dir[pp.left_d].sum().reshape(())
+        g_vn = g_dir[pp.right_d].sum().reshape(())
+        return g_rho, g_vp, g_vn, None, None
+
+
+def poisson_potential(pp, rho_nodal, V_p, V_n, rtol=1e-12):
+    """Nodal phi (V) of the Poisson problem ``pp`` (a PoissonProblem) for the charge
+    density rho_nodal (C/m^3) and contact voltages V_p (x = 0), V_n (x = Lx),
+    differentiable in all three (tensors; plain numbers are taken as constants)."""
+    dev = rho_nodal.device
+    V_p = torch.as_tensor(V_p, dtype=torch.float64, device=dev)
+    V_n = torch.as_tensor(V_n, dtype=torch.float64, device=dev)
+    return _PoissonPotential.apply(rho_nodal, V_p, V_n, pp, rtol)
+
+
+def electrostatic_levels(prob, pp, v0_q, rho_nodal, V_p, V_n, k, mode="mixed", tol=1e-9, rtol=1e-12):
+    """Lowest k levels (eV) of V0 - q phi, phi from the Poisson problem ``pp`` on the
+    same mesh as the SchrodingerProblem ``prob`` (v0_q: the fixed potential at its
+    quadrature points, eV).  Differentiable in rho_nodal, V_p and V_n (and v0_q)."""
+    if pp.space.n != prob.space.n or pp.space.npe != prob.space.npe:
+        raise ValueError("Poisson and Schroedinger problems must share the mesh and element order")
+    phi = poisson_potential(pp, rho_nodal, V_p, V_n, rtol)
+    return eigenvalues(prob, v0_q - nodal_to_quadrature(prob.space, phi), k, mode, tol)

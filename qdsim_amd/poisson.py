@@ -103,7 +103,17 @@ class PoissonProblem:
         g[self.right_d] = V_n
         t = self.spmv(self.K, g)
         b = (rhs - t)[self.free_d].contiguous()
-        x = torch.zeros_like(b) if x0 is None else x0[self.free_d].contiguous()
+        x = self.solve_free(b, None if x0 is None else x0[self.free_d], rtol)
+        phi = g.clone()
+        phi[self.free_d] = x
+        return phi
+
+    def solve_free(self, b, x0=None, rtol=1e-12):
+        """x with K_ff x = b on the free DOFs (ordering of ``free``): device CG,
+        multigrid-preconditioned on structured meshes.  K_ff is symmetric, so the
+        adjoint of the Poisson step solves with the same call."""
+        b = b.contiguous()
+        x = torch.zeros_like(b) if x0 is None else x0.clone().contiguous()
         if self.mg is not None:
             it, rr = kernels.cg_mg(self.r_indptr, self.r_indices, self.K_ff, self.free.size,
                                    int(self.r_indices.numel()), b, x, self.mg._structs, self.mg.n_levels,
@@ -118,9 +128,7 @@ class PoissonProblem:
             # stops short is reported rather than passed on silently
             warnings.warn(f"Poisson CG stopped after {it} iterations at relative residual {rr:.3e} "
                           f"(rtol {rtol:.1e})", RuntimeWarning, stacklevel=2)
-        phi = g.clone()
-        phi[self.free_d] = x
-        return phi
+        return x
 
 
 class PoissonSolver:

@@ -354,3 +354,28 @@ def test_adjoint_ref_gradient_matches_central_differences():
     assert np.allclose(g, fd, rtol=1e-5, atol=1e-9), (g, fd)
     # a dropped chain-rule factor (e.g. the amplitude in d/dcx) fails
     assert np.abs(g[1]) > 1e-4 and np.abs(g[4]) > 1e-4
+
+
+def test_electrostatic_gradient_matches_central_differences():
+    """Gate and doping gradient of the level loss through the Poisson step (oracle
+    adjoint) against central differences of the oracle's own forward model."""
+    from oracle import adjoint_ref
+    nx, ny, Lx, Ly = 17, 9, 80.0, 40.0
+
+    def v0(x, y):
+        return -0.2 * np.exp(-((x - 40.0) ** 2 + (y - 20.0) ** 2) / (2 * 8.0 ** 2))
+    theta = np.array([0.05, -0.1, 0.02, 0.015, 43.0])
+    targets = np.array([-0.2, -0.1, -0.05])
+    for sign in ("physical", "reference"):
+        J, g, E = adjoint_ref.electrostatic_loss_and_gradient(nx, ny, Lx, Ly, 13.9, 0.067, v0, theta, targets, sign)
+        fd = np.zeros_like(theta)
+        for j in range(len(theta)):
+            h = 1e-6 * max(1.0, abs(theta[j]))
+            tp, tm = theta.copy(), theta.copy()
+            tp[j] += h
+            tm[j] -= h
+            fd[j] = (adjoint_ref.electrostatic_loss_and_gradient(nx, ny, Lx, Ly, 13.9, 0.067, v0, tp, targets, sign)[0]
+                     - adjoint_ref.electrostatic_loss_and_gradient(nx, ny, Lx, Ly, 13.9, 0.067, v0, tm, targets,
+                                                                   sign)[0]) / (2 * h)
+        assert np.allclose(g, fd, rtol=2e-5, atol=1e-9 * np.abs(fd).max()), (sign, g, fd)
+        assert np.all(np.abs(g) > 1e-6)
