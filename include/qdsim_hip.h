@@ -183,6 +183,16 @@ int qds_lattice_spmm(int32_t dtype, const qds_lattice_stencil* st /* host */, in
                      const void* vals_a, const void* vals_b, const void* X, int64_t ldx,
                      void* Y_a, int64_t ldy_a, void* Y_b, int64_t ldy_b, void* stream);
 
+/* The product for `batch` problems on one stencil in one launch (no reference
+ * counterpart: the reference solves one parameter set per call): problem i reads
+ * vals_a + i sva, vals_b + i svb (svb = 0: one shared matrix such as M),
+ * X + i sx and writes Y_a + i sya, Y_b + i syb (strides in elements, each keeping
+ * 16-byte alignment).                                                            */
+int qds_lattice_spmm_batched(int32_t dtype, const qds_lattice_stencil* st /* host */, int32_t b, int32_t batch,
+                             const void* vals_a, int64_t sva, const void* vals_b, int64_t svb,
+                             const void* X, int64_t ldx, int64_t sx, void* Y_a, int64_t ldy_a, int64_t sya,
+                             void* Y_b, int64_t ldy_b, int64_t syb, void* stream);
+
 /* ---- dense block kernels for the block eigensolver (device) --------------------
  * Replace the dense Gram / projection steps inside scipy's lobpcg
  * (eigen_solver.pyx:254-266) and ARPACK's orthogonalisation.                      */
@@ -230,6 +240,19 @@ int qds_block_update_blocks(int32_t dtype, int64_t n, const qds_colblocks* S /* 
                             const double* C1, void* Y1, int64_t ldy1, const double* C2, void* Y2,
                             int64_t ldy2, int32_t c2_first /* rows of C2 before it are zero */,
                             void* stream);
+/* Batched forms (one launch for `batch` problems of equal shape, no reference
+ * counterpart): every block of problem i is at its pointer + i * stride elements
+ * (one stride for all blocks of a call); outputs C / Y at + i m (k1 + k2) / + i
+ * y_stride; coefficients at + i c_stride.  Workspace of the Gram: batch times
+ * qds_gram_workspace.                                                             */
+int qds_gram_blocks_batched(int32_t dtype, int64_t n, const qds_colblocks* A /* host */,
+                            const qds_colblocks* B1 /* host */, const qds_colblocks* B2 /* host */, int32_t sym,
+                            double* C, int32_t batch, int64_t s_stride, void* workspace, int64_t ws_bytes,
+                            void* stream);
+int qds_block_update_blocks_batched(int32_t dtype, int64_t n, const qds_colblocks* S /* host */, int32_t k,
+                                    const double* C1, void* Y1, int64_t ldy1, const double* C2, void* Y2,
+                                    int64_t ldy2, int32_t c2_first, int32_t batch, int64_t s_stride,
+                                    int64_t c_stride, int64_t y_stride, void* stream);
 int qds_block_residual(int32_t dtype, int64_t n, int32_t b, const void* HX, int64_t ldh,
                        const void* MX, int64_t ldm, const double* lam, void* R, int64_t ldr,
                        void* stream);
@@ -241,10 +264,38 @@ int qds_residual_norms(int32_t dtype, int64_t n, int32_t b, const void* HX, int6
                        const double* colscale /* optional: R columns scaled after the norms */,
                        void* R, int64_t ldr, double* out, void* workspace, int64_t ws_bytes,
                        void* stream);
+/* Batched residual step: problem i's HX, MX, R at + i x_stride, lam and colscale
+ * at + i b, out at + i 2b.                                                        */
+int64_t qds_residual_norms_batched_workspace(int64_t n, int32_t b, int32_t batch);
+int qds_residual_norms_batched(int32_t dtype, int64_t n, int32_t b, int32_t batch, const void* HX, int64_t ldh,
+                               const void* MX, int64_t ldm, int64_t x_stride, const double* lam,
+                               const double* colscale, void* R, int64_t ldr, double* out, void* workspace,
+                               int64_t ws_bytes, void* stream);
 int64_t qds_coldot_workspace(int32_t b);
 int qds_coldot(int32_t dtype, int64_t n, int32_t b, const void* X, int64_t ldx, const void* Y,
                int64_t ldy, int32_t conj, double* out, void* workspace, int64_t ws_bytes,
                void* stream);
+
+/* ---- Rayleigh-Ritz on the device -------------------------------------------------
+ * Replaces the dense projected eigensolve inside scipy's lobpcg / ARPACK
+ * (qdsim_cython/solvers/eigen_solver.pyx:225-266): for each of `batch` problems,
+ * G = [S^T H S | S^T M S] (m x 2m FP64, row-major, problem stride g_stride;
+ * its M half is overwritten) gives the lowest b Ritz pairs: lam (in: the current
+ * Ritz values; out: the new ones), C (m x b: X_new = S C) and CP (C with rows
+ * < b zeroed, all zero after dropped directions: P_new = S CP; may be NULL), one
+ * workgroup per problem (m even, m <= 96).  norms ([batch][2b], NULL for the
+ * first projection) holds ||R_j||^2 and ||M x_j||^2 of the current block (from
+ * qds_residual_norms); with them the call also writes the locking mask of the next
+ * residual (colscale, may be NULL) and the status [batch][qds_rr_status_len(b)]:
+ * [0] worst relative residual of the first k columns, [1] converged (< tol),
+ * [2] dropped directions, [3] 0 Cholesky / 1 SVQB / 2 frozen, [4] Jacobi sweeps,
+ * [5] non-vanished basis columns, [8 + j] residual of column j.  With freeze, a
+ * converged problem keeps its block (C = [I; 0], lam unchanged).                */
+int32_t qds_rr_status_len(int32_t b);
+int qds_rayleigh_ritz(int64_t batch, int32_t m, int32_t b, int32_t k, double* G, int64_t g_stride,
+                      const double* norms, double* lam, double* C, double* CP, int64_t c_stride,
+                      double* colscale, double* status, double tol, double drop_tol, int32_t freeze,
+                      void* stream);
 
 /* ---- lattice multigrid preconditioner (device) ---------------------------------
  * No reference counterpart: replaces the sparse LU of ARPACK shift-invert
@@ -274,6 +325,16 @@ int qds_mg_vcycle(int32_t dtype, int32_t n_levels, const qds_mg_level* levels, i
 int qds_mg_vcycle_io(int32_t dtype, int32_t io_dtype, int32_t n_levels, const qds_mg_level* levels,
                      int32_t b, int32_t nu_pre, int32_t nu_post, int32_t coarse_sweeps, double omega,
                      void* stream);
+/* Batched V-cycle (no reference counterpart): `batch` problems on one hierarchy
+ * shape, problem i's arrays at the level pointers plus i times the level's strides
+ * (elements; 0 = shared, e.g. the edge weights when every problem has the same
+ * mass).  dtype / io_dtype as qds_mg_vcycle_io.                                    */
+typedef struct {
+  int64_t sd, sw, su, sf, st;   /* d; we and wn; u; f; t */
+} qds_mg_level_strides;
+int qds_mg_vcycle_batched(int32_t dtype, int32_t io_dtype, int32_t n_levels, const qds_mg_level* levels,
+                          const qds_mg_level_strides* strides /* host [n_levels] */, int32_t batch, int32_t b,
+                          int32_t nu_pre, int32_t nu_post, int32_t coarse_sweeps, double omega, void* stream);
 int qds_mg_apply(int32_t dtype, const qds_mg_level* level, int32_t b, const void* u, void* out,
                  void* stream);
 

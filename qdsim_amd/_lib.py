@@ -48,6 +48,17 @@ _SIGNATURES = {
     "qds_gram_blocks": ([_i32, _i64, _p, _p, _p, _i32, _p, _p, _i64, _p], _c.c_int),
     "qds_gram_noalloc_scratch_bytes": ([], _i64),
     "qds_gram_blocks_noalloc": ([_i32, _i64, _p, _p, _p, _i32, _p, _p], _c.c_int),
+    "qds_rr_status_len": ([_i32], _i32),
+    "qds_rayleigh_ritz": ([_i64, _i32, _i32, _i32, _p, _i64, _p, _p, _p, _p, _i64, _p, _p, _d, _d, _i32, _p],
+                          _c.c_int),
+    "qds_lattice_spmm_batched": ([_i32, _p, _i32, _i32, _p, _i64, _p, _i64, _p, _i64, _i64, _p, _i64, _i64, _p,
+                                  _i64, _i64, _p], _c.c_int),
+    "qds_gram_blocks_batched": ([_i32, _i64, _p, _p, _p, _i32, _p, _i32, _i64, _p, _i64, _p], _c.c_int),
+    "qds_block_update_blocks_batched": ([_i32, _i64, _p, _i32, _p, _p, _i64, _p, _p, _i64, _i32, _i32, _i64, _i64,
+                                         _i64, _p], _c.c_int),
+    "qds_residual_norms_batched_workspace": ([_i64, _i32, _i32], _i64),
+    "qds_residual_norms_batched": ([_i32, _i64, _i32, _i32, _p, _i64, _p, _i64, _i64, _p, _p, _p, _i64, _p, _p,
+                                    _i64, _p], _c.c_int),
     "qds_block_update_blocks": ([_i32, _i64, _p, _i32, _p, _p, _i64, _p, _p, _i64, _i32, _p], _c.c_int),
     "qds_block_residual": ([_i32, _i64, _i32, _p, _i64, _p, _i64, _p, _p, _i64, _p], _c.c_int),
     "qds_coldot_workspace": ([_i32], _i64),
@@ -57,6 +68,7 @@ _SIGNATURES = {
     "qds_coldot": ([_i32, _i64, _i32, _p, _i64, _p, _i64, _i32, _p, _p, _i64, _p], _c.c_int),
     "qds_mg_vcycle": ([_i32, _i32, _p, _i32, _i32, _i32, _i32, _d, _p], _c.c_int),
     "qds_mg_vcycle_io": ([_i32, _i32, _i32, _p, _i32, _i32, _i32, _i32, _d, _p], _c.c_int),
+    "qds_mg_vcycle_batched": ([_i32, _i32, _i32, _p, _p, _i32, _i32, _i32, _i32, _i32, _d, _p], _c.c_int),
     "qds_mg_apply": ([_i32, _p, _i32, _p, _p, _p], _c.c_int),
     "qds_interpolate_q": ([_i32, _i32, _i64, _p, _p, _p, _p, _p], _c.c_int),
     "qds_state_density_q": ([_i32, _i32, _i64, _p, _p, _p, _p, _p, _i64, _i32, _p, _p], _c.c_int),
@@ -95,6 +107,11 @@ class MGLevel(ctypes.Structure):
     _fields_ = [("nx", _i64), ("ny", _i64), ("d", _p), ("we", _p), ("wn", _p),
                 ("u", _p), ("f", _p), ("t", _p), ("ldu", _i64), ("ldf", _i64),
                 ("neumann_x", _i32), ("neumann_y", _i32)]
+
+
+class MGLevelStrides(ctypes.Structure):
+    """Mirror of qds_mg_level_strides."""
+    _fields_ = [("sd", _i64), ("sw", _i64), ("su", _i64), ("sf", _i64), ("st", _i64)]
 
 
 class QDSError(RuntimeError):

@@ -198,3 +198,63 @@ def gate_gradients(theta, sigma, targets, cells=256, order=2, L=100e-9, m_star=0
 
     out = run_batch(one, range(theta.shape[0]), workers=workers, setup=setup)
     return (np.array([o[0] for o in out]), np.stack([o[1] for o in out]), np.stack([o[2] for o in out]))
+
+
+def gate_potential_batched(theta, sigma, xq, yq):
+    """V (eV) of every parameter set at the quadrature points: theta (B, 3 G) torch
+    (differentiable), sigma (B, G) numpy; returns (B, E, Q)."""
+    sig = torch.as_tensor(np.asarray(sigma, dtype=np.float64), device=theta.device)
+    a = theta[:, 0::3][:, :, None, None]
+    cx = theta[:, 1::3][:, :, None, None]
+    cy = theta[:, 2::3][:, :, None, None]
+    s2 = (2.0 * sig ** 2)[:, :, None, None]
+    return (a * torch.exp(-((xq[None, None] - cx) ** 2 + (yq[None, None] - cy) ** 2) / s2)).sum(dim=1)
+
+
+class GateBatchSolver:
+    """All parameter sets of the gate-design objective on one mesh, solved together
+    (batched_eigen.BatchedSchrodinger): one launch per solver stage for the whole
+    batch.  The gradient of sum_k (E_k - E*_k)^2 with respect to every set's theta is
+    Hellmann-Feynman (dE_k/dV at the quadrature points, qds_state_density_q) chained
+    through the batched gate expression by one autograd call."""
+
+    def __init__(self, cells=256, order=2, L=100e-9, m_star=0.067):
+        from .batched_eigen import BatchedSchrodinger
+        from .mesh import SimpleMesh
+        m = m_star * M_E
+        self.bs = BatchedSchrodinger(SimpleMesh(cells + 1, cells + 1, L, L, order), lambda x, y: m + 0 * x)
+        self.xq, self.yq = self.bs.quad_points_nm()
+
+    def __call__(self, theta, sigma, targets, tol=1e-9, verbose=False):
+        from .adjoint import state_density_q
+        dev = self.xq.device
+        th = torch.as_tensor(np.asarray(theta, dtype=np.float64), device=dev).requires_grad_(True)
+        tgt = torch.as_tensor(np.asarray(targets, dtype=np.float64), device=dev)
+        k = len(targets)
+        v_q = gate_potential_batched(th, sigma, self.xq, self.yq)
+        v_det = v_q.detach()
+
+        def vfun(xq, yq):
+            if xq.shape == self.xq.shape and xq.data_ptr() == self.xq.data_ptr():
+                return v_det
+            with torch.no_grad():
+                return gate_potential_batched(th.detach(), sigma, xq, yq)
+        lam, psi, info = self.bs.solve(vfun, k, tol=tol, verbose=verbose)
+        E = torch.as_tensor(lam, device=dev)
+        gE = 2.0 * (E - tgt[None])
+        gV = torch.empty_like(v_det)
+        for i in range(E.shape[0]):
+            rho = state_density_q(self.bs.space, psi[i])          # (E, Q, k)
+            gV[i] = rho @ gE[i]
+        (g,) = torch.autograd.grad(v_q, th, grad_outputs=gV)
+        loss = ((E - tgt[None]) ** 2).sum(dim=1)
+        self.last_info = info
+        return loss.cpu().numpy(), g.cpu().numpy(), lam
+
+
+def gate_gradients_batched(theta, sigma, targets, cells=256, order=2, L=100e-9, m_star=0.067, tol=1e-9,
+                           solver=None):
+    """gate_gradients for all parameter sets in one batched solve.  Returns (loss (n,),
+    grad (n, p), E (n, k)) as numpy; ``solver``: a GateBatchSolver to reuse."""
+    solver = solver or GateBatchSolver(cells, order, L, m_star)
+    return solver(theta, sigma, targets, tol=tol)

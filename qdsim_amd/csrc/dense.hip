@@ -89,7 +89,7 @@ struct GramStage {
 template <typename T, bool VEC, int MAXP, int MAXT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
 gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, int g0, GramStage stg,
-             double* __restrict__ partial) {
+             double* __restrict__ partial, int64_t s_stride) {
   using L = GLoad<T, VEC>;
   using V = typename L::v;
   constexpr int VW = L::W;
@@ -162,7 +162,7 @@ gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, int g0, Gram
       pre[q] = V{};
       if (goff[q] < 0) continue;
       if (!full && r0 + loff[q] / ldsw >= r_end) continue;   // tail chunk: rows past the slab stay 0
-      const T* base = reinterpret_cast<const T*>(stg.src[q]) + r0 * stg.ld[q];
+      const T* base = reinterpret_cast<const T*>(stg.src[q]) + blockIdx.z * s_stride + r0 * stg.ld[q];
       pre[q] = *reinterpret_cast<const V*>(base + goff[q]);
     }
   };
@@ -195,7 +195,7 @@ gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, int g0, Gram
       for (int slot = 0; slot < MAXT; ++slot) acc[slot] = mfma_f64(av[slot], bv[slot], acc[slot]);
     }
   }
-  double* P = partial + (int64_t)blockIdx.x * g.mpad * g.kpad;
+  double* P = partial + ((int64_t)blockIdx.z * gridDim.x + blockIdx.x) * g.mpad * g.kpad;
 #pragma unroll
   for (int slot = 0; slot < MAXT; ++slot) {
     if (slot < nslots) {
@@ -218,6 +218,9 @@ constexpr int kRedOut = 32;
 constexpr int kRedGroups = kBlock / kRedOut;
 __global__ void __launch_bounds__(kBlock)
 gram_reduce(int nblk, GramShape g, const double* __restrict__ partial, double* __restrict__ C) {
+  // problem of a batch (blockIdx.y): its slabs' partials and its output
+  partial += (int64_t)blockIdx.y * nblk * g.mpad * g.kpad;
+  C += (int64_t)blockIdx.y * g.m * (g.k1 + g.k2);
   __shared__ double red[kBlock];
   const int ktot = g.k1 + g.k2;
   const int k1pad = 16 * g.tk1;
@@ -301,7 +304,7 @@ template <typename T, int NOUT, int OT, int MAXJ>
 __global__ void __launch_bounds__(kBlock)
 block_update_kernel(int64_t n, int m, int k, ColBlocks S, const double* __restrict__ C1,
                     const double* __restrict__ C2, T* __restrict__ Y1, int64_t ldy1, T* __restrict__ Y2,
-                    int64_t ldy2, int resident, int crows, int c2_first) {
+                    int64_t ldy2, int resident, int crows, int c2_first, int64_t sS, int64_t cstride, int64_t sY) {
   using M = MT<T>;
   constexpr int VEC = M::VEC;                  // elements of T per 16-byte load
   constexpr int KB = 4 * VEC;  // contraction indices covered by one vector load per lane
@@ -309,6 +312,13 @@ block_update_kernel(int64_t n, int m, int k, ColBlocks S, const double* __restri
   using Vt = typename MT<T>::vec_t;
   extern __shared__ unsigned char upd_smem[];
   T* sC = reinterpret_cast<T*>(upd_smem);   // [NOUT][crows][KP], zero padded
+  // problem of a batch (blockIdx.y): its S blocks, coefficients and outputs
+  const int64_t set = blockIdx.y;
+  C1 += set * cstride;
+  if (C2) C2 += set * cstride;
+  Y1 += set * sY;
+  if (NOUT == 2) Y2 += set * sY;
+  const int64_t s_off = set * sS;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int q = lane >> 4;      // 0..3
@@ -337,7 +347,7 @@ block_update_kernel(int64_t n, int m, int k, ColBlocks S, const double* __restri
     int64_t row = base + wave * 16 + rl;
     row = row < n ? row : n - 1;
     const int w = S.start[j + 1] - S.start[j];
-    const T* rp = reinterpret_cast<const T*>(S.p[j]) + row * S.ld[j];
+    const T* rp = reinterpret_cast<const T*>(S.p[j]) + s_off + row * S.ld[j];
 #pragma unroll
     for (int jj = 0; jj < MAXJ; ++jj) {
       const int c = jj * KB + q * VEC;
@@ -418,7 +428,8 @@ block_update_kernel(int64_t n, int m, int k, ColBlocks S, const double* __restri
 
 template <typename T, int NOUT>
 int launch_update(int64_t n, int m, int k, const ColBlocks& S, int c2_first, const double* C1, const double* C2,
-                  T* Y1, int64_t ldy1, T* Y2, int64_t ldy2, hipStream_t st) {
+                  T* Y1, int64_t ldy1, T* Y2, int64_t ldy2, hipStream_t st, int batch = 1, int64_t sS = 0,
+                  int64_t sC = 0, int64_t sY = 0) {
   constexpr int KB = 4 * MT<T>::VEC;
   const int ot = (k + 15) / 16;
   const int kp = ot * 16;
@@ -435,10 +446,10 @@ int launch_update(int64_t n, int m, int k, const ColBlocks& S, int c2_first, con
   QDS_CHECK_ARG((size_t)NOUT * crows * kp * sizeof(T) <= (size_t)kUpdLds, "qds_block_update: C too large");
   const size_t shm = (size_t)NOUT * crows * kp * sizeof(T);
   const int64_t passes = (n + kWaves * 16 - 1) / (kWaves * 16);
-  const unsigned blocks = qds_blocks(passes, 1, 4096);
+  const unsigned blocks = qds_blocks(passes, 1, batch > 1 ? (4096 + batch - 1) / batch * 4 : 4096);
 #define QDS_UPD(OTV, J)                                                                                       \
-  hipLaunchKernelGGL((block_update_kernel<T, NOUT, OTV, J>), dim3(blocks), dim3(kBlock), shm, st, n, m, k, S, \
-                     C1, C2, Y1, ldy1, Y2, ldy2, resident, crows, c2_first)
+  hipLaunchKernelGGL((block_update_kernel<T, NOUT, OTV, J>), dim3(blocks, (unsigned)batch), dim3(kBlock), shm, \
+                     st, n, m, k, S, C1, C2, Y1, ldy1, Y2, ldy2, resident, crows, c2_first, sS, sC, sY)
 #define QDS_UPD_J(OTV) do { if (J <= 2) QDS_UPD(OTV, 2); else if (J == 3) QDS_UPD(OTV, 3); else QDS_UPD(OTV, 4); } while (0)
   const int J = (wmax + KB - 1) / KB;
   switch (ot) {
@@ -631,7 +642,8 @@ namespace {
 // size check instead of the slab cap.
 int gram_blocks_impl(int32_t dtype, int64_t n, const qds_colblocks* A, const qds_colblocks* B1,
                      const qds_colblocks* B2, int32_t sym, double* C, double* partial, int64_t ws_bytes,
-                     int64_t partial_bytes, void* stream) {
+                     int64_t partial_bytes, void* stream, int batch = 1, int64_t s_stride = 0) {
+  QDS_CHECK_ARG(batch >= 1 && batch <= 65535, "qds_gram: batch %d out of range", batch);
   QDS_CHECK_ARG(dtype == qds::F32 || dtype == qds::F64, "qds_gram: dtype must be F32 or F64");
   QDS_CHECK_ARG(C && partial && A && B1, "qds_gram: null pointer");
   const int es = dtype == qds::F32 ? 4 : 8;
@@ -688,7 +700,7 @@ int gram_blocks_impl(int32_t dtype, int64_t n, const qds_colblocks* A, const qds
   const int ngroups = (tk + tpg - 1) / tpg;
   int nblk = gram_blocks(n);
   if (partial_bytes == 0) {
-    QDS_CHECK_ARG(ws_bytes >= qds_gram_workspace(n, m, k1, k2), "qds_gram: workspace too small");
+    QDS_CHECK_ARG(ws_bytes >= batch * qds_gram_workspace(n, m, k1, k2), "qds_gram: workspace too small");
   } else {
     const int64_t per_slab = (int64_t)g.mpad * g.kpad * (int64_t)sizeof(double);
     const int64_t cap = partial_bytes / per_slab;
@@ -712,11 +724,11 @@ int gram_blocks_impl(int32_t dtype, int64_t n, const qds_colblocks* A, const qds
     using T = decltype(tag);
     constexpr bool VV = decltype(vecc)::value;
     if (sg.n <= 9)
-      hipLaunchKernelGGL((gram_partial<T, VV, 9, 8>), dim3(nblk, ny), dim3(kBlock), shm, st, n, rows_per_blk,
-                         g, tpg, g0, sg, partial);
+      hipLaunchKernelGGL((gram_partial<T, VV, 9, 8>), dim3(nblk, ny, batch), dim3(kBlock), shm, st, n,
+                         rows_per_blk, g, tpg, g0, sg, partial, s_stride);
     else
-      hipLaunchKernelGGL((gram_partial<T, VV, kGramPieces, 8>), dim3(nblk, ny), dim3(kBlock), shm, st, n,
-                         rows_per_blk, g, tpg, g0, sg, partial);
+      hipLaunchKernelGGL((gram_partial<T, VV, kGramPieces, 8>), dim3(nblk, ny, batch), dim3(kBlock), shm, st, n,
+                         rows_per_blk, g, tpg, g0, sg, partial, s_stride);
   };
   auto launch_all = [&](auto tag, auto vecc) {
     if (stg.n <= 9 || ngroups == 1) {
@@ -748,7 +760,8 @@ int gram_blocks_impl(int32_t dtype, int64_t n, const qds_colblocks* A, const qds
   else launch_all(double(0), std::false_type());
   QDS_LAUNCH_CHECK();
   const int tot = m * (k1 + k2);
-  hipLaunchKernelGGL(gram_reduce, dim3((tot + kRedOut - 1) / kRedOut), dim3(kBlock), 0, st, nblk, g, partial, C);
+  hipLaunchKernelGGL(gram_reduce, dim3((tot + kRedOut - 1) / kRedOut, batch), dim3(kBlock), 0, st, nblk, g,
+                     partial, C);
   QDS_LAUNCH_CHECK();
   return QDS_OK;
 }
@@ -797,6 +810,19 @@ int static_scratch(double** out) {
 
 QDS_API int64_t qds_gram_noalloc_scratch_bytes(void) { return kStaticGramBytes; }
 
+// The Gram of `batch` problems in one launch: problem i's blocks of A, B1, B2 at
+// p[j] + i s_stride (elements, one stride for every block), its output at
+// C + i m (k1 + k2); workspace: batch * qds_gram_workspace(n, m, k1, k2).
+QDS_API int qds_gram_blocks_batched(int32_t dtype, int64_t n, const qds_colblocks* A, const qds_colblocks* B1,
+                                    const qds_colblocks* B2, int32_t sym, double* C, int32_t batch,
+                                    int64_t s_stride, void* workspace, int64_t ws_bytes, void* stream) {
+  QDS_CHECK_ARG(workspace, "qds_gram: null workspace");
+  const int es = dtype == qds::F32 ? 4 : 8;
+  QDS_CHECK_ARG(batch == 1 || (s_stride * es) % 16 == 0, "qds_gram: batch stride must keep 16-byte alignment");
+  return gram_blocks_impl(dtype, n, A, B1, B2, sym, C, static_cast<double*>(workspace), ws_bytes, 0, stream, batch,
+                          s_stride);
+}
+
 QDS_API int qds_gram_blocks_noalloc(int32_t dtype, int64_t n, const qds_colblocks* A, const qds_colblocks* B1,
                                     const qds_colblocks* B2, int32_t sym, double* C, void* stream) {
   double* partial = nullptr;
@@ -818,9 +844,25 @@ QDS_API int qds_gram(int32_t dtype, int64_t n, int32_t m, const void* A, int64_t
 // C1, C2: m x k row-major FP64 (rounded to the storage type on the device).  k <= 64.  Every block's width and
 // leading dimension must be multiples of 4 (F32) or 2 (F64), blocks 16-byte aligned.
 // c2_first: the caller guarantees rows [0, c2_first) of C2 are zero (0: none known).
+QDS_API int qds_block_update_blocks_batched(int32_t dtype, int64_t n, const qds_colblocks* S, int32_t k,
+                                            const double* C1, void* Y1, int64_t ldy1, const double* C2, void* Y2,
+                                            int64_t ldy2, int32_t c2_first, int32_t batch, int64_t s_stride,
+                                            int64_t c_stride, int64_t y_stride, void* stream);
+
 QDS_API int qds_block_update_blocks(int32_t dtype, int64_t n, const qds_colblocks* S, int32_t k,
                                     const double* C1, void* Y1, int64_t ldy1, const double* C2, void* Y2,
                                     int64_t ldy2, int32_t c2_first, void* stream) {
+  return qds_block_update_blocks_batched(dtype, n, S, k, C1, Y1, ldy1, C2, Y2, ldy2, c2_first, 1, 0, 0, 0, stream);
+}
+
+// The update for `batch` problems in one launch: problem i reads its S blocks at
+// S.p[j] + i s_stride, C1 / C2 at + i c_stride and writes Y1 / Y2 at + i y_stride
+// (strides in elements; the S blocks of all problems share one stride).
+QDS_API int qds_block_update_blocks_batched(int32_t dtype, int64_t n, const qds_colblocks* S, int32_t k,
+                                            const double* C1, void* Y1, int64_t ldy1, const double* C2, void* Y2,
+                                            int64_t ldy2, int32_t c2_first, int32_t batch, int64_t s_stride,
+                                            int64_t c_stride, int64_t y_stride, void* stream) {
+  QDS_CHECK_ARG(batch >= 1 && batch <= 65535, "qds_block_update: batch %d out of range", batch);
   QDS_CHECK_ARG(dtype == qds::F32 || dtype == qds::F64, "qds_block_update: dtype must be F32 or F64");
   const int es = dtype == qds::F32 ? 4 : 8;
   QDS_CHECK_ARG(S && blocks_vec(S, es), "qds_block_update: S blocks need 16-byte alignment and widths, "
@@ -835,14 +877,21 @@ QDS_API int qds_block_update_blocks(int32_t dtype, int64_t n, const qds_colblock
   hipStream_t st = qds_stream(stream);
   if (!C2) c2_first = 0;
   const double *c1 = C1, *c2 = C2;
+  if (batch > 1)
+    QDS_CHECK_ARG((s_stride * es) % 16 == 0 && (y_stride * es) % 16 == 0,
+                  "qds_block_update: batch strides must keep 16-byte alignment");
   if (dtype == qds::F32) {
     float *y1 = (float*)Y1, *y2 = (float*)Y2;
-    return C2 ? launch_update<float, 2>(n, m, k, dS, c2_first, c1, c2, y1, ldy1, y2, ldy2, st)
-              : launch_update<float, 1>(n, m, k, dS, 0, c1, nullptr, y1, ldy1, nullptr, ldy2, st);
+    return C2 ? launch_update<float, 2>(n, m, k, dS, c2_first, c1, c2, y1, ldy1, y2, ldy2, st, batch, s_stride,
+                                        c_stride, y_stride)
+              : launch_update<float, 1>(n, m, k, dS, 0, c1, nullptr, y1, ldy1, nullptr, ldy2, st, batch, s_stride,
+                                        c_stride, y_stride);
   }
   double *y1 = (double*)Y1, *y2 = (double*)Y2;
-  return C2 ? launch_update<double, 2>(n, m, k, dS, c2_first, c1, c2, y1, ldy1, y2, ldy2, st)
-            : launch_update<double, 1>(n, m, k, dS, 0, c1, nullptr, y1, ldy1, nullptr, ldy2, st);
+  return C2 ? launch_update<double, 2>(n, m, k, dS, c2_first, c1, c2, y1, ldy1, y2, ldy2, st, batch, s_stride,
+                                       c_stride, y_stride)
+            : launch_update<double, 1>(n, m, k, dS, 0, c1, nullptr, y1, ldy1, nullptr, ldy2, st, batch, s_stride,
+                                       c_stride, y_stride);
 }
 
 // Single-block form: S n x m with leading dimension lds.
@@ -951,7 +1000,18 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock)
 residual_norms_partial(int64_t n, int b, const T* __restrict__ HX, int64_t ldh, const T* __restrict__ MX,
                        int64_t ldm, const double* __restrict__ lam, const double* __restrict__ colscale,
-                       T* __restrict__ R, int64_t ldr, int64_t rows_per_blk, double* __restrict__ partial) {
+                       T* __restrict__ R, int64_t ldr, int64_t rows_per_blk, double* __restrict__ partial,
+                       int64_t sx) {
+  // problem of a batch (blockIdx.y): blocks at + set * sx, its own lam, colscale, partials
+  {
+    const int64_t set = blockIdx.y;
+    HX += set * sx;
+    MX += set * sx;
+    R += set * sx;
+    lam += set * b;
+    if (colscale) colscale += set * b;
+    partial += set * (int64_t)gridDim.x * 2 * b;
+  }
   const int groups = kBlock / b;
   const int c = threadIdx.x % b;
   const int gidx = threadIdx.x / b;
@@ -988,6 +1048,8 @@ residual_norms_partial(int64_t n, int b, const T* __restrict__ HX, int64_t ldh, 
 // t / 2b, t / 2b + groups, ...; the group sums are then combined in a fixed order.
 __global__ void __launch_bounds__(1024)
 residual_norms_reduce(int nblk, int b, const double* __restrict__ partial, double* __restrict__ out) {
+  partial += (int64_t)blockIdx.x * nblk * 2 * b;
+  out += (int64_t)blockIdx.x * 2 * b;
   __shared__ double red[1024];
   const int w = 2 * b;
   const int groups = 1024 / w;
@@ -1010,31 +1072,52 @@ QDS_API int64_t qds_residual_norms_workspace(int32_t b) {
   return (int64_t)kResBlocks * 2 * b * sizeof(double);
 }
 
+QDS_API int64_t qds_residual_norms_batched_workspace(int64_t n, int32_t b, int32_t batch) {
+  int64_t nblk = (n + 511) / 512;
+  if (nblk > kResBlocks) nblk = kResBlocks;
+  if (nblk < 1) nblk = 1;
+  return (int64_t)batch * nblk * 2 * b * sizeof(double);
+}
+
+// The residual step for `batch` problems in one launch: problem i's HX, MX, R at
+// + i x_stride (elements), lam / colscale at + i b, out at + i 2b.
+QDS_API int qds_residual_norms_batched(int32_t dtype, int64_t n, int32_t b, int32_t batch, const void* HX,
+                                       int64_t ldh, const void* MX, int64_t ldm, int64_t x_stride,
+                                       const double* lam, const double* colscale, void* R, int64_t ldr, double* out,
+                                       void* workspace, int64_t ws_bytes, void* stream) {
+  QDS_CHECK_ARG(HX && MX && lam && R && out && workspace && b >= 1 && b <= 128 && batch >= 1 && batch <= 65535,
+                "qds_residual_norms: bad arguments");
+  int64_t nblk = (n + 511) / 512;
+  if (nblk > kResBlocks) nblk = kResBlocks;
+  if (nblk < 1) nblk = 1;
+  QDS_CHECK_ARG(ws_bytes >= (int64_t)batch * nblk * 2 * b * (int64_t)sizeof(double),
+                "qds_residual_norms: workspace too small");
+  hipStream_t st = qds_stream(stream);
+  const int64_t rpb = (n + nblk - 1) / nblk;
+  double* part = static_cast<double*>(workspace);
+  if (dtype == qds::F32)
+    hipLaunchKernelGGL(residual_norms_partial<float>, dim3(nblk, batch), dim3(kBlock), 0, st, n, b,
+                       (const float*)HX, ldh, (const float*)MX, ldm, lam, colscale, (float*)R, ldr, rpb, part,
+                       x_stride);
+  else if (dtype == qds::F64)
+    hipLaunchKernelGGL(residual_norms_partial<double>, dim3(nblk, batch), dim3(kBlock), 0, st, n, b,
+                       (const double*)HX, ldh, (const double*)MX, ldm, lam, colscale, (double*)R, ldr, rpb, part,
+                       x_stride);
+  else
+    QDS_CHECK_ARG(false, "qds_residual_norms: dtype must be F32 or F64");
+  QDS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(residual_norms_reduce, dim3(batch), dim3(1024), 0, st, (int)nblk, b, part, out);
+  QDS_LAUNCH_CHECK();
+  return QDS_OK;
+}
+
 // colscale (optional, b doubles) multiplies the stored residual columns after the
 // norms are taken: the eigensolver zeroes the columns of converged pairs (soft locking).
 QDS_API int qds_residual_norms(int32_t dtype, int64_t n, int32_t b, const void* HX, int64_t ldh,
                                const void* MX, int64_t ldm, const double* lam, const double* colscale,
                                void* R, int64_t ldr, double* out, void* workspace, int64_t ws_bytes,
                                void* stream) {
-  QDS_CHECK_ARG(HX && MX && lam && R && out && workspace && b >= 1 && b <= 128,
-                "qds_residual_norms: bad arguments");
   QDS_CHECK_ARG(ws_bytes >= qds_residual_norms_workspace(b), "qds_residual_norms: workspace too small");
-  hipStream_t st = qds_stream(stream);
-  int64_t nblk = (n + 511) / 512;
-  if (nblk > kResBlocks) nblk = kResBlocks;
-  if (nblk < 1) nblk = 1;
-  const int64_t rpb = (n + nblk - 1) / nblk;
-  double* part = static_cast<double*>(workspace);
-  if (dtype == qds::F32)
-    hipLaunchKernelGGL(residual_norms_partial<float>, dim3(nblk), dim3(kBlock), 0, st, n, b, (const float*)HX,
-                       ldh, (const float*)MX, ldm, lam, colscale, (float*)R, ldr, rpb, part);
-  else if (dtype == qds::F64)
-    hipLaunchKernelGGL(residual_norms_partial<double>, dim3(nblk), dim3(kBlock), 0, st, n, b,
-                       (const double*)HX, ldh, (const double*)MX, ldm, lam, colscale, (double*)R, ldr, rpb, part);
-  else
-    QDS_CHECK_ARG(false, "qds_residual_norms: dtype must be F32 or F64");
-  QDS_LAUNCH_CHECK();
-  hipLaunchKernelGGL(residual_norms_reduce, dim3(1), dim3(1024), 0, st, (int)nblk, b, part, out);
-  QDS_LAUNCH_CHECK();
-  return QDS_OK;
+  return qds_residual_norms_batched(dtype, n, b, 1, HX, ldh, MX, ldm, 0, lam, colscale, R, ldr, out, workspace,
+                                    ws_bytes, stream);
 }

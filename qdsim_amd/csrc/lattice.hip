@@ -62,6 +62,10 @@ struct LatticeArgs {
   int32_t base[QDS_LATTICE_MAX_CLASS];  // class offset within a value tile (elements)
   int16_t off[QDS_LATTICE_MAX_CLASS][QDS_LATTICE_MAX_ENTRY];  // LDS point offset of each entry
   int8_t tasks[kWaves][kMaxTasks];  // task = class * nch + chunk, -1 terminated
+  // batch of problems on one stencil (blockIdx.y): element strides between the
+  // problems' value arrays, X and Y blocks (0: shared)
+  int64_t sva, svb, sx, sya, syb;
+  int32_t batch;
 };
 
 template <typename T, int N>
@@ -147,6 +151,14 @@ lattice_spmm_kernel(const LatticeArgs a, const T* __restrict__ va, const T* __re
   float4* lds4 = reinterpret_cast<float4*>(lds_raw);
   const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
   if (blk >= a.n_blocks) return;  // uniform over the workgroup
+  {
+    const int64_t set = blockIdx.y;
+    va += set * a.sva;
+    vb += set * a.svb;
+    X += set * a.sx;
+    Ya += set * a.sya;
+    if (FUSED) Yb += set * a.syb;
+  }
   const int64_t cr = blk / a.bx_per_row;
   const int64_t t0 = (blk - cr * a.bx_per_row) * a.m;  // first value tile of the block
   const int64_t cc0 = t0 * a.tile;                    // first cell of the block
@@ -325,7 +337,7 @@ template <typename T, int VEC, int P, bool FUSED>
 int launch(const LatticeArgs& a, size_t lds, const void* va, const void* vb, const void* X,
            int64_t ldx, int chunked, void* Ya, int64_t ldya, void* Yb, int64_t ldyb, hipStream_t st) {
   const int64_t grid = (a.n_blocks + 7) / 8 * 8;
-  hipLaunchKernelGGL((lattice_spmm_kernel<T, VEC, P, FUSED>), dim3((unsigned)grid), dim3(kThreads), lds,
+  hipLaunchKernelGGL((lattice_spmm_kernel<T, VEC, P, FUSED>), dim3((unsigned)grid, (unsigned)a.batch), dim3(kThreads), lds,
                      st, a, (const T*)va, (const T*)vb, (const T*)X, ldx, chunked, (T*)Ya, ldya,
                      (T*)Yb, ldyb);
   QDS_LAUNCH_CHECK();
@@ -353,9 +365,15 @@ int dispatch(int p, bool fused, const LatticeArgs& a, size_t lds, const void* va
 
 }  // namespace
 
-QDS_API int qds_lattice_spmm(int32_t dtype, const qds_lattice_stencil* s, int32_t b, const void* vals_a,
-                             const void* vals_b, const void* X, int64_t ldx, void* Y_a, int64_t ldy_a,
-                             void* Y_b, int64_t ldy_b, void* stream) {
+namespace {
+struct BatchStrides {
+  int32_t batch;
+  int64_t sva, svb, sx, sya, syb;
+};
+
+int lattice_spmm_impl(int32_t dtype, const qds_lattice_stencil* s, int32_t b, const void* vals_a,
+                      const void* vals_b, const void* X, int64_t ldx, void* Y_a, int64_t ldy_a,
+                      void* Y_b, int64_t ldy_b, const BatchStrides& bs, void* stream) {
   QDS_CHECK_ARG(s && vals_a && X && Y_a, "qds_lattice_spmm: null pointer");
   QDS_CHECK_ARG(!vals_b || Y_b, "qds_lattice_spmm: vals_b without Y_b");
   QDS_CHECK_ARG(dtype == qds::F32 || dtype == qds::F64, "qds_lattice_spmm: dtype must be F32 or F64");
@@ -384,9 +402,9 @@ QDS_API int qds_lattice_spmm(int32_t dtype, const qds_lattice_stencil* s, int32_
   if (b > bmax) {
     for (int j = 0; j < b; j += bmax) {
       const int w = std::min(bmax, b - j);
-      const int rc = qds_lattice_spmm(dtype, s, w, vals_a, vals_b, (const char*)X + (size_t)j * es, ldx,
-                                      (char*)Y_a + (size_t)j * es, ldy_a,
-                                      Y_b ? (char*)Y_b + (size_t)j * es : nullptr, ldy_b, stream);
+      const int rc = lattice_spmm_impl(dtype, s, w, vals_a, vals_b, (const char*)X + (size_t)j * es, ldx,
+                                       (char*)Y_a + (size_t)j * es, ldy_a,
+                                       Y_b ? (char*)Y_b + (size_t)j * es : nullptr, ldy_b, bs, stream);
       if (rc != QDS_OK) return rc;
     }
     return QDS_OK;
@@ -405,7 +423,18 @@ QDS_API int qds_lattice_spmm(int32_t dtype, const qds_lattice_stencil* s, int32_
   else vec = vec_ok(2) ? 2 : 1;
   QDS_CHECK_ARG(aligned(vals_a, 16) && (!fused || aligned(vals_b, 16)),
                 "qds_lattice_spmm: value arrays must be 16-byte aligned");
+  QDS_CHECK_ARG(bs.batch >= 1 && bs.batch <= 65535, "qds_lattice_spmm: batch %d out of range", bs.batch);
+  if (bs.batch > 1)
+    QDS_CHECK_ARG((bs.sva * es) % 16 == 0 && (bs.svb * es) % 16 == 0 && (bs.sx * es) % 16 == 0 &&
+                      (bs.sya * es) % 16 == 0 && (bs.syb * es) % 16 == 0,
+                  "qds_lattice_spmm: batch strides must keep 16-byte alignment");
   LatticeArgs a{};
+  a.batch = bs.batch;
+  a.sva = bs.sva;
+  a.svb = bs.svb;
+  a.sx = bs.sx;
+  a.sya = bs.sya;
+  a.syb = bs.syb;
   a.nx = s->nx;
   a.ny = s->ny;
   a.cells_x = s->cells_x;
@@ -480,4 +509,23 @@ QDS_API int qds_lattice_spmm(int32_t dtype, const qds_lattice_stencil* s, int32_
   }
   if (vec == 2) return dispatch<double, 2>(p, fused, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
   return dispatch<double, 1>(p, fused, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
+}
+}  // namespace
+
+QDS_API int qds_lattice_spmm(int32_t dtype, const qds_lattice_stencil* s, int32_t b, const void* vals_a,
+                             const void* vals_b, const void* X, int64_t ldx, void* Y_a, int64_t ldy_a,
+                             void* Y_b, int64_t ldy_b, void* stream) {
+  const BatchStrides one{1, 0, 0, 0, 0, 0};
+  return lattice_spmm_impl(dtype, s, b, vals_a, vals_b, X, ldx, Y_a, ldy_a, Y_b, ldy_b, one, stream);
+}
+
+// The same product for `batch` problems on one stencil in one launch: problem i
+// reads vals_a + i sva, vals_b + i svb (svb = 0: one shared matrix, e.g. M), X + i sx
+// and writes Y_a + i sya, Y_b + i syb (strides in elements).
+QDS_API int qds_lattice_spmm_batched(int32_t dtype, const qds_lattice_stencil* s, int32_t b, int32_t batch,
+                                     const void* vals_a, int64_t sva, const void* vals_b, int64_t svb,
+                                     const void* X, int64_t ldx, int64_t sx, void* Y_a, int64_t ldy_a,
+                                     int64_t sya, void* Y_b, int64_t ldy_b, int64_t syb, void* stream) {
+  const BatchStrides bs{batch, sva, svb, sx, sya, syb};
+  return lattice_spmm_impl(dtype, s, b, vals_a, vals_b, X, ldx, Y_a, ldy_a, Y_b, ldy_b, bs, stream);
 }

@@ -39,7 +39,22 @@ struct Level {
   T* t;
   int64_t ldu, ldf;  // per-point strides of u and f (t is dense, stride b)
   int32_t nbx, nby;  // 1: coarsening of this level keeps its boundary rows (Neumann), 0: Dirichlet
+  // batch of problems on one lattice (blockIdx.y): element strides between the
+  // problems' d, we / wn, u, f and t (0: shared by every problem)
+  int64_t sd, sw, su, sf, st;
 };
+
+// this workgroup's problem of the batch
+template <typename T>
+__device__ __forceinline__ void to_set(Level<T>& L) {
+  const int64_t y = blockIdx.y;
+  L.d += y * L.sd;
+  L.we += y * L.sw;
+  L.wn += y * L.sw;
+  L.u += y * L.su;
+  L.f += y * L.sf;
+  L.t += y * L.st;
+}
 
 // Column vectors: each thread owns VEC consecutive columns of one lattice point,
 // so every stencil access is a 16-byte load (32-bit index math throughout; the
@@ -260,19 +275,26 @@ __device__ __forceinline__ void copy_items(int64_t np, int b, const T* __restric
 template <typename T, int VEC>
 __global__ void __launch_bounds__(kBlock)
 jacobi_kernel(Level<T> L, int b, const T* __restrict__ u, int64_t ldu_in, T* __restrict__ out,
-              int64_t ldo, T omega, int zero_init) {
+              int64_t ldo, T omega, int zero_init, int64_t s_in, int64_t s_out) {
+  to_set(L);
+  u += blockIdx.y * s_in;
+  out += blockIdx.y * s_out;
   jacobi_items<T, VEC>(L, b, u, ldu_in, out, ldo, omega, zero_init, QDS_GRID_T0, QDS_GRID_STEP);
 }
 
 template <typename T, int VEC>
 __global__ void __launch_bounds__(kBlock)
 restrict_residual_kernel(Level<T> F, Level<T> C, int b) {
+  to_set(F);
+  to_set(C);
   restrict_items<T, VEC>(F, C, b, QDS_GRID_T0, QDS_GRID_STEP);
 }
 
 template <typename T, int VEC>
 __global__ void __launch_bounds__(kBlock)
 prolong_add_kernel(Level<T> F, Level<T> C, int b) {
+  to_set(F);
+  to_set(C);
   prolong_items<T, VEC>(F, C, b, QDS_GRID_T0, QDS_GRID_STEP);
 }
 
@@ -444,13 +466,21 @@ __device__ __forceinline__ void up_items(const Level<T>& F, const Level<T>& C, i
 
 template <typename T, int VEC, typename S>
 __global__ void __launch_bounds__(kBlock)
-mg_down_kernel(Level<T> F, Level<T> C, int b, T omega, const S* __restrict__ ff) {
+mg_down_kernel(Level<T> F, Level<T> C, int b, T omega, const S* __restrict__ ff, int64_t sff) {
+  to_set(F);
+  to_set(C);
+  ff += blockIdx.y * sff;
   down_items<T, VEC, S>(F, C, b, omega, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, ff);
 }
 
 template <typename T, int VEC, typename S>
 __global__ void __launch_bounds__(kBlock)
-mg_up_kernel(Level<T> F, Level<T> C, int b, T omega, const S* __restrict__ ff, S* __restrict__ uu) {
+mg_up_kernel(Level<T> F, Level<T> C, int b, T omega, const S* __restrict__ ff, S* __restrict__ uu, int64_t sff,
+             int64_t suu) {
+  to_set(F);
+  to_set(C);
+  ff += blockIdx.y * sff;
+  uu += blockIdx.y * suu;
   up_items<T, VEC, S>(F, C, b, omega, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, ff, uu);
 }
 
@@ -506,7 +536,10 @@ __device__ __forceinline__ void stage_window(T* dst, const S* __restrict__ src, 
 // down: coarse f on a kDX x kDY coarse tile
 template <typename T, int VEC, typename S>
 __global__ void __launch_bounds__(256)
-mg_down_tiled(Level<T> F, Level<T> C, int b, T omega, const S* __restrict__ ff) {
+mg_down_tiled(Level<T> F, Level<T> C, int b, T omega, const S* __restrict__ ff, int64_t sff) {
+  to_set(F);
+  to_set(C);
+  ff += blockIdx.y * sff;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* fs = reinterpret_cast<T*>(lds_raw);                  // kDFX * kDFY points x b
   T* cinv = fs + kDFX * kDFY * b;                          // 1 / d
@@ -595,7 +628,12 @@ mg_down_tiled(Level<T> F, Level<T> C, int b, T omega, const S* __restrict__ ff) 
 // up: fine u on a kUX x kUY fine tile
 template <typename T, int VEC, typename S>
 __global__ void __launch_bounds__(256)
-mg_up_tiled(Level<T> F, Level<T> C, int b, T omega, const S* __restrict__ ff, S* __restrict__ uu) {
+mg_up_tiled(Level<T> F, Level<T> C, int b, T omega, const S* __restrict__ ff, S* __restrict__ uu, int64_t sff,
+            int64_t suu) {
+  to_set(F);
+  to_set(C);
+  ff += blockIdx.y * sff;
+  uu += blockIdx.y * suu;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T* fs = reinterpret_cast<T*>(lds_raw);                  // kUFX * kUFY points x b
   T* us = fs + kUFX * kUFY * b;                            // u2 on the halo tile
@@ -742,34 +780,45 @@ __device__ void block_smooth(const Level<T>& L, int b, int sweeps, T omega, bool
 template <typename T, int VEC>
 __global__ void __launch_bounds__(kCoarseBlock)
 coarse_cycle_kernel(LevelPack<T> P, int nl, int b, int nu1, int nu2, int ncoarse, T omega) {
+  // this workgroup's problem of the batch: local copies of its levels
+  auto lev = [&](int l) {
+    Level<T> L = P.L[l];
+    to_set(L);
+    return L;
+  };
   if (nu1 == 1 && nu2 == 1) {
     for (int l = 0; l < nl - 1; ++l) {
-      down_items<T, VEC>(P.L[l], P.L[l + 1], b, omega, threadIdx.x, blockDim.x);
+      const Level<T> F = lev(l), C = lev(l + 1);
+      down_items<T, VEC>(F, C, b, omega, threadIdx.x, blockDim.x);
       __syncthreads();
     }
-    block_smooth<T, VEC>(P.L[nl - 1], b, ncoarse, omega, true);
+    block_smooth<T, VEC>(lev(nl - 1), b, ncoarse, omega, true);
     for (int l = nl - 2; l >= 0; --l) {
-      up_items<T, VEC>(P.L[l], P.L[l + 1], b, omega, threadIdx.x, blockDim.x);
+      const Level<T> F = lev(l), C = lev(l + 1);
+      up_items<T, VEC>(F, C, b, omega, threadIdx.x, blockDim.x);
       __syncthreads();
     }
     return;
   }
   for (int l = 0; l < nl - 1; ++l) {
-    block_smooth<T, VEC>(P.L[l], b, nu1, omega, true);
-    restrict_items<T, VEC>(P.L[l], P.L[l + 1], b, threadIdx.x, blockDim.x);
+    const Level<T> F = lev(l), C = lev(l + 1);
+    block_smooth<T, VEC>(F, b, nu1, omega, true);
+    restrict_items<T, VEC>(F, C, b, threadIdx.x, blockDim.x);
     __syncthreads();
   }
-  block_smooth<T, VEC>(P.L[nl - 1], b, ncoarse, omega, true);
+  block_smooth<T, VEC>(lev(nl - 1), b, ncoarse, omega, true);
   for (int l = nl - 2; l >= 0; --l) {
-    prolong_items<T, VEC>(P.L[l], P.L[l + 1], b, threadIdx.x, blockDim.x);
+    const Level<T> F = lev(l), C = lev(l + 1);
+    prolong_items<T, VEC>(F, C, b, threadIdx.x, blockDim.x);
     __syncthreads();
-    block_smooth<T, VEC>(P.L[l], b, nu2, omega, false);
+    block_smooth<T, VEC>(F, b, nu2, omega, false);
   }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
 apply_kernel(Level<T> L, int b, const T* __restrict__ u, T* __restrict__ out) {
+  to_set(L);
   const uint32_t total = (uint32_t)(L.nx * L.ny) * (uint32_t)b;
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     const uint32_t p = t / (uint32_t)b;
@@ -780,7 +829,10 @@ apply_kernel(Level<T> L, int b, const T* __restrict__ u, T* __restrict__ out) {
 }
 
 template <typename T>
-__global__ void copy_kernel(int64_t np, int b, const T* __restrict__ src, int64_t lds, T* __restrict__ dst) {
+__global__ void copy_kernel(int64_t np, int b, const T* __restrict__ src, int64_t lds, T* __restrict__ dst,
+                            int64_t s_src, int64_t s_dst) {
+  src += blockIdx.y * s_src;
+  dst += blockIdx.y * s_dst;
   const uint32_t total = (uint32_t)np * (uint32_t)b;
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     const uint32_t p = t / (uint32_t)b;
@@ -790,10 +842,11 @@ __global__ void copy_kernel(int64_t np, int b, const T* __restrict__ src, int64_
 }
 
 template <typename T, int VEC>
-int smooth(const Level<T>& L, int b, int sweeps, T omega, bool zero_first, hipStream_t st) {
+int smooth(const Level<T>& L, int b, int sweeps, T omega, bool zero_first, hipStream_t st, int batch) {
   // Jacobi sweeps ping-pong between u and t; the result always ends in u
   const int64_t work = L.nx * L.ny * (b / VEC);
-  const dim3 g(qds_blocks(work, kBlock, 65536)), blk(kBlock);
+  const dim3 g(qds_blocks(work, kBlock, 65536), batch), blk(kBlock);
+  auto s_of = [&](const T* q) { return q == L.u ? L.su : L.st; };
   // with an odd count the first sweep writes straight to u, so no final copy
   T* src = L.u;
   T* dst = (sweeps % 2 == 0) ? L.t : L.u;
@@ -802,13 +855,13 @@ int smooth(const Level<T>& L, int b, int sweeps, T omega, bool zero_first, hipSt
     const bool zi = zero_first && s == 0;
     if (!zi && src == dst) {
       // first sweep of an odd count on a live u: copy u into t (dense) first
-      hipLaunchKernelGGL(copy_kernel<T>, dim3(qds_blocks(L.nx * L.ny * b, kBlock, 65536)), blk, 0, st,
-                         L.nx * L.ny, b, (const T*)L.u, L.ldu, L.t);
+      hipLaunchKernelGGL(copy_kernel<T>, dim3(qds_blocks(L.nx * L.ny * b, kBlock, 65536), batch), blk, 0, st,
+                         L.nx * L.ny, b, (const T*)L.u, L.ldu, L.t, L.su, L.st);
       QDS_LAUNCH_CHECK();
       src = L.t;
     }
     hipLaunchKernelGGL((jacobi_kernel<T, VEC>), g, blk, 0, st, L, b, (const T*)src, ld_of(src), dst,
-                       ld_of(dst), omega, zi ? 1 : 0);
+                       ld_of(dst), omega, zi ? 1 : 0, s_of(src), s_of(dst));
     QDS_LAUNCH_CHECK();
     src = dst;
     dst = (dst == L.u) ? L.t : L.u;
@@ -821,7 +874,7 @@ int smooth(const Level<T>& L, int b, int sweeps, T omega, bool zero_first, hipSt
 // single-workgroup coarse cycle)
 template <typename T, int VEC, typename S = T>
 int vcycle(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, hipStream_t st,
-           const S* f0 = nullptr, S* u0 = nullptr) {
+           const S* f0 = nullptr, S* u0 = nullptr, int batch = 1, int64_t sf0 = 0, int64_t su0 = 0) {
   // first level small enough for the single-workgroup coarse cycle
   int lc = nl;
   for (int l = 0; l < nl; ++l)
@@ -844,35 +897,35 @@ int vcycle(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, h
       if (use_tiled(L[l], b, VEC) && (l > 0 || io16)) {
         const unsigned tiles = (unsigned)(((L[l + 1].nx + kDX - 1) / kDX) * ((L[l + 1].ny + kDY - 1) / kDY));
         if (l == 0 && mixed)
-          hipLaunchKernelGGL((mg_down_tiled<T, VEC, S>), dim3(tiles), dim3(256), down_tiled_lds<T>(b), st, L[l],
-                             L[l + 1], b, omega, f0);
+          hipLaunchKernelGGL((mg_down_tiled<T, VEC, S>), dim3(tiles, batch), dim3(256), down_tiled_lds<T>(b), st,
+                             L[l], L[l + 1], b, omega, f0, sf0);
         else
-          hipLaunchKernelGGL((mg_down_tiled<T, VEC, T>), dim3(tiles), dim3(256), down_tiled_lds<T>(b), st, L[l],
-                             L[l + 1], b, omega, (const T*)L[l].f);
+          hipLaunchKernelGGL((mg_down_tiled<T, VEC, T>), dim3(tiles, batch), dim3(256), down_tiled_lds<T>(b), st,
+                             L[l], L[l + 1], b, omega, (const T*)L[l].f, L[l].sf);
       } else if (l == 0 && mixed) {
-        hipLaunchKernelGGL((mg_down_kernel<T, VEC, S>), dim3(qds_blocks(work, kBlock, 65536)), dim3(kBlock), 0,
-                           st, L[l], L[l + 1], b, omega, f0);
+        hipLaunchKernelGGL((mg_down_kernel<T, VEC, S>), dim3(qds_blocks(work, kBlock, 65536), batch), dim3(kBlock),
+                           0, st, L[l], L[l + 1], b, omega, f0, sf0);
       } else {
-        hipLaunchKernelGGL((mg_down_kernel<T, VEC, T>), dim3(qds_blocks(work, kBlock, 65536)), dim3(kBlock), 0,
-                           st, L[l], L[l + 1], b, omega, (const T*)L[l].f);
+        hipLaunchKernelGGL((mg_down_kernel<T, VEC, T>), dim3(qds_blocks(work, kBlock, 65536), batch), dim3(kBlock),
+                           0, st, L[l], L[l + 1], b, omega, (const T*)L[l].f, L[l].sf);
       }
       QDS_LAUNCH_CHECK();
       continue;
     }
-    int rc = smooth<T, VEC>(L[l], b, nu1, omega, true, st);
+    int rc = smooth<T, VEC>(L[l], b, nu1, omega, true, st, batch);
     if (rc) return rc;
-    hipLaunchKernelGGL((restrict_residual_kernel<T, VEC>), dim3(qds_blocks(work, kBlock, 65536)),
+    hipLaunchKernelGGL((restrict_residual_kernel<T, VEC>), dim3(qds_blocks(work, kBlock, 65536), batch),
                        dim3(kBlock), 0, st, L[l], L[l + 1], b);
     QDS_LAUNCH_CHECK();
   }
   if (lc < nl) {
     LevelPack<T> P;
     for (int l = lc; l < nl; ++l) P.L[l - lc] = L[l];
-    hipLaunchKernelGGL((coarse_cycle_kernel<T, VEC>), dim3(1), dim3(kCoarseBlock), 0, st, P, nl - lc, b, nu1,
-                       nu2, ncoarse, omega);
+    hipLaunchKernelGGL((coarse_cycle_kernel<T, VEC>), dim3(1, batch), dim3(kCoarseBlock), 0, st, P, nl - lc, b,
+                       nu1, nu2, ncoarse, omega);
     QDS_LAUNCH_CHECK();
   } else {
-    int rc = smooth<T, VEC>(L[nl - 1], b, ncoarse, omega, true, st);
+    int rc = smooth<T, VEC>(L[nl - 1], b, ncoarse, omega, true, st, batch);
     if (rc) return rc;
   }
   const int top = lc < nl ? lc - 1 : nl - 2;
@@ -882,25 +935,25 @@ int vcycle(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, h
       if (use_tiled(L[l], b, VEC) && (l > 0 || io16)) {
         const unsigned tiles = (unsigned)(((L[l].nx + kUX - 1) / kUX) * ((L[l].ny + kUY - 1) / kUY));
         if (l == 0 && mixed)
-          hipLaunchKernelGGL((mg_up_tiled<T, VEC, S>), dim3(tiles), dim3(256), up_tiled_lds<T>(b), st, L[l],
-                             L[l + 1], b, omega, f0, u0);
+          hipLaunchKernelGGL((mg_up_tiled<T, VEC, S>), dim3(tiles, batch), dim3(256), up_tiled_lds<T>(b), st, L[l],
+                             L[l + 1], b, omega, f0, u0, sf0, su0);
         else
-          hipLaunchKernelGGL((mg_up_tiled<T, VEC, T>), dim3(tiles), dim3(256), up_tiled_lds<T>(b), st, L[l],
-                             L[l + 1], b, omega, (const T*)L[l].f, L[l].u);
+          hipLaunchKernelGGL((mg_up_tiled<T, VEC, T>), dim3(tiles, batch), dim3(256), up_tiled_lds<T>(b), st, L[l],
+                             L[l + 1], b, omega, (const T*)L[l].f, L[l].u, L[l].sf, L[l].su);
       } else if (l == 0 && mixed) {
-        hipLaunchKernelGGL((mg_up_kernel<T, VEC, S>), dim3(qds_blocks(work, kBlock, 65536)), dim3(kBlock), 0, st,
-                           L[l], L[l + 1], b, omega, f0, u0);
+        hipLaunchKernelGGL((mg_up_kernel<T, VEC, S>), dim3(qds_blocks(work, kBlock, 65536), batch), dim3(kBlock), 0,
+                           st, L[l], L[l + 1], b, omega, f0, u0, sf0, su0);
       } else {
-        hipLaunchKernelGGL((mg_up_kernel<T, VEC, T>), dim3(qds_blocks(work, kBlock, 65536)), dim3(kBlock), 0, st,
-                           L[l], L[l + 1], b, omega, (const T*)L[l].f, L[l].u);
+        hipLaunchKernelGGL((mg_up_kernel<T, VEC, T>), dim3(qds_blocks(work, kBlock, 65536), batch), dim3(kBlock), 0,
+                           st, L[l], L[l + 1], b, omega, (const T*)L[l].f, L[l].u, L[l].sf, L[l].su);
       }
       QDS_LAUNCH_CHECK();
       continue;
     }
-    hipLaunchKernelGGL((prolong_add_kernel<T, VEC>), dim3(qds_blocks(work, kBlock, 65536)), dim3(kBlock),
+    hipLaunchKernelGGL((prolong_add_kernel<T, VEC>), dim3(qds_blocks(work, kBlock, 65536), batch), dim3(kBlock),
                        0, st, L[l], L[l + 1], b);
     QDS_LAUNCH_CHECK();
-    int rc = smooth<T, VEC>(L[l], b, nu2, omega, false, st);
+    int rc = smooth<T, VEC>(L[l], b, nu2, omega, false, st, batch);
     if (rc) return rc;
   }
   return QDS_OK;
@@ -908,7 +961,7 @@ int vcycle(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, h
 
 template <typename T, typename S = T>
 int vcycle_any(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omega, hipStream_t st,
-               const S* f0 = nullptr, S* u0 = nullptr) {
+               const S* f0 = nullptr, S* u0 = nullptr, int batch = 1, int64_t sf0 = 0, int64_t su0 = 0) {
   constexpr int VW = 16 / sizeof(T);
   bool vec = b % VW == 0;
   for (int l = 0; l < nl && vec; ++l) {
@@ -916,8 +969,8 @@ int vcycle_any(int nl, Level<T>* L, int b, int nu1, int nu2, int ncoarse, T omeg
           reinterpret_cast<uintptr_t>(L[l].u) % 16 == 0 && reinterpret_cast<uintptr_t>(L[l].f) % 16 == 0 &&
           reinterpret_cast<uintptr_t>(L[l].t) % 16 == 0;
   }
-  if (vec) return vcycle<T, VW, S>(nl, L, b, nu1, nu2, ncoarse, omega, st, f0, u0);
-  return vcycle<T, 1, S>(nl, L, b, nu1, nu2, ncoarse, omega, st, f0, u0);
+  if (vec) return vcycle<T, VW, S>(nl, L, b, nu1, nu2, ncoarse, omega, st, f0, u0, batch, sf0, su0);
+  return vcycle<T, 1, S>(nl, L, b, nu1, nu2, ncoarse, omega, st, f0, u0, batch, sf0, su0);
 }
 
 }  // namespace
@@ -1023,6 +1076,70 @@ QDS_API int qds_mg_vcycle_io(int32_t dtype, int32_t io_dtype, int32_t n_levels, 
   L[0].u = L[0].t;
   return vcycle_any<float, double>(n_levels, L, b, nu_pre, nu_post, coarse_sweeps, (float)omega,
                                    qds_stream(stream), f0, u0);
+}
+
+extern "C" {
+// Element strides between the problems of a batch, per level (0: shared).
+typedef struct {
+  int64_t sd, sw, su, sf, st;   // d; we and wn; u; f; t
+} qds_mg_level_strides;
+}
+
+// One V-cycle for each of `batch` problems on the same lattice hierarchy in one
+// launch per pass: problem i's level arrays are the level's pointers plus i times
+// the level's strides.  dtype / io_dtype as qds_mg_vcycle_io (for an FP64 level 0
+// on an FP32 hierarchy, strides[0].sf / .su are the FP64 blocks' strides).
+QDS_API int qds_mg_vcycle_batched(int32_t dtype, int32_t io_dtype, int32_t n_levels, const qds_mg_level* levels,
+                                  const qds_mg_level_strides* strides, int32_t batch, int32_t b, int32_t nu_pre,
+                                  int32_t nu_post, int32_t coarse_sweeps, double omega, void* stream) {
+  QDS_CHECK_ARG(levels && strides && n_levels >= 1 && n_levels <= 32 && b >= 1 && batch >= 1 && batch <= 65535,
+                "qds_mg_vcycle_batched: bad arguments");
+  QDS_CHECK_ARG(coarsening_ok(levels, n_levels), "qds_mg_vcycle_batched: levels are not vertex-centred coarsenings");
+  for (int l = 0; l < n_levels; ++l) {
+    QDS_CHECK_ARG(levels[l].d && levels[l].we && levels[l].wn && levels[l].u && levels[l].f && levels[l].t,
+                  "qds_mg_vcycle_batched: null pointer on level %d", l);
+    QDS_CHECK_ARG(levels[l].nx * levels[l].ny * (int64_t)(levels[l].ldu ? levels[l].ldu : b) < (int64_t(1) << 31) &&
+                  levels[l].nx * levels[l].ny * (int64_t)(levels[l].ldf ? levels[l].ldf : b) < (int64_t(1) << 31),
+                  "qds_mg_vcycle_batched: level %d exceeds 32-bit indexing", l);
+    const qds_mg_level_strides& z = strides[l];
+    QDS_CHECK_ARG(z.sd >= 0 && z.sw >= 0 && z.su >= 0 && z.sf >= 0 && z.st >= 0,
+                  "qds_mg_vcycle_batched: negative stride on level %d", l);
+  }
+  hipStream_t stm = qds_stream(stream);
+  auto fill = [&](auto* L, auto tag) {
+    using T = decltype(tag);
+    for (int l = 0; l < n_levels; ++l) {
+      L[l] = Level<T>{levels[l].nx, levels[l].ny, (const T*)levels[l].d, (const T*)levels[l].we,
+                      (const T*)levels[l].wn, (T*)levels[l].u, (T*)levels[l].f, (T*)levels[l].t,
+                      levels[l].ldu ? levels[l].ldu : b, levels[l].ldf ? levels[l].ldf : b,
+                      levels[l].neumann_x ? 1 : 0, levels[l].neumann_y ? 1 : 0,
+                      strides[l].sd, strides[l].sw, strides[l].su, strides[l].sf, strides[l].st};
+    }
+  };
+  if (dtype == io_dtype && dtype == qds::F32) {
+    Level<float> L[32];
+    fill(L, 0.0f);
+    return vcycle_any<float, float>(n_levels, L, b, nu_pre, nu_post, coarse_sweeps, (float)omega, stm, nullptr,
+                                    nullptr, batch);
+  }
+  if (dtype == io_dtype && dtype == qds::F64) {
+    Level<double> L[32];
+    fill(L, 0.0);
+    return vcycle_any<double, double>(n_levels, L, b, nu_pre, nu_post, coarse_sweeps, omega, stm, nullptr, nullptr,
+                                      batch);
+  }
+  QDS_CHECK_ARG(dtype == qds::F32 && io_dtype == qds::F64 && n_levels >= 2,
+                "qds_mg_vcycle_batched: only an FP32 hierarchy with FP64 level-0 blocks");
+  Level<float> L[32];
+  fill(L, 0.0f);
+  const double* f0 = (const double*)levels[0].f;
+  double* u0 = (double*)levels[0].u;
+  L[0].f = L[0].t;
+  L[0].u = L[0].t;
+  L[0].sf = L[0].st;
+  L[0].su = L[0].st;
+  return vcycle_any<float, double>(n_levels, L, b, nu_pre, nu_post, coarse_sweeps, (float)omega, stm, f0, u0, batch,
+                                   strides[0].sf, strides[0].su);
 }
 
 // out = A u on one level (tests of the lattice operator).

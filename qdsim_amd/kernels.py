@@ -219,6 +219,100 @@ def gram_blocks(A, B1, B2=None, sym=False, out=None):
     return out
 
 
+# ---- batched forms: every operand (batch, n, b) contiguous, one launch per stage
+def _bcolblocks(blocks):
+    """qds_colblocks of set 0 for (batch, n, c_j) tensors and their common set stride."""
+    stride = blocks[0].stride(0)
+    assert all(B.dim() == 3 and B.stride(0) == stride and B.stride(2) == 1 for B in blocks)
+    cb = _lib.ColBlocks()
+    for j, B in enumerate(blocks):
+        cb.p[j] = B.data_ptr()
+        cb.ld[j] = B.stride(1)
+        cb.cols[j] = B.shape[2]
+    cb.nb = len(blocks)
+    return cb, stride
+
+
+def lattice_spmm_batched(stencil, A, X, YA, B=None, YB=None):
+    """YA[i] = A_i X[i] (and YB[i] = B_i X[i]) for a batch on one stencil; A, B are
+    (batch, slots) or (slots,) (shared by every problem)."""
+    batch, n, b = X.shape
+
+    def vstride(V):
+        return 0 if V.dim() == 1 else V.stride(0)
+    check(lib.qds_lattice_spmm_batched(code(X), ctypes.byref(stencil), b, batch, ptr(A), vstride(A), ptr(B),
+                                       vstride(B) if B is not None else 0, ptr(X), X.stride(1), X.stride(0),
+                                       ptr(YA), YA.stride(1), YA.stride(0), ptr(YB),
+                                       YB.stride(1) if YB is not None else 1, YB.stride(0) if YB is not None else 0,
+                                       stream_ptr()), "qds_lattice_spmm_batched")
+
+
+def gram_blocks_batched(A, B1, B2=None, sym=False, out=None):
+    """out[i] = [A_0[i] | ...]^T [B1[i] | B2[i]] in FP64 for lists of (batch, n, c) blocks."""
+    batch, n = A[0].shape[:2]
+    m = sum(a.shape[2] for a in A)
+    k1 = sum(x.shape[2] for x in B1)
+    k2 = sum(x.shape[2] for x in B2) if B2 else 0
+    if out is None:
+        out = torch.empty((batch, m, k1 + k2), dtype=torch.float64, device=A[0].device)
+    ca, st = _bcolblocks(A)
+    c1, st1 = _bcolblocks(B1)
+    c2, st2 = _bcolblocks(B2) if B2 else (None, st)
+    assert st == st1 == st2 and out.is_contiguous()
+    wsb = batch * lib.qds_gram_workspace(n, m, k1, k2)
+    wp, wn = workspace("gram").ptr(wsb)
+    check(lib.qds_gram_blocks_batched(code(A[0]), n, ctypes.byref(ca), ctypes.byref(c1),
+                                      ctypes.byref(c2) if c2 is not None else None, 1 if sym else 0, ptr(out), batch,
+                                      st, wp, wn, stream_ptr()), "qds_gram_blocks_batched")
+    return out
+
+
+def block_update_blocks_batched(S, C1, Y1, C2=None, Y2=None, c2_first=0):
+    """Y1[i] = [S_0[i] | ...] C1[i] (and Y2[i] = ... C2[i]); C (batch, m, k) FP64."""
+    batch, n = S[0].shape[:2]
+    k = C1.shape[2]
+    cs, st = _bcolblocks(S)
+    # coefficients: (batch, m, k) with contiguous rows; their set stride may be a
+    # larger buffer's (a row slice of it)
+    for Cx in (C1, C2):
+        assert Cx is None or (Cx.dtype == torch.float64 and Cx.stride(2) == 1 and Cx.stride(1) == k)
+    assert C2 is None or C2.stride(0) == C1.stride(0)
+    check(lib.qds_block_update_blocks_batched(code(S[0]), n, ctypes.byref(cs), k, ptr(C1), ptr(Y1), Y1.stride(1),
+                                              ptr(C2), ptr(Y2), Y2.stride(1) if Y2 is not None else 1,
+                                              int(c2_first) if C2 is not None else 0, batch, st, C1.stride(0),
+                                              Y1.stride(0), stream_ptr()), "qds_block_update_blocks_batched")
+
+
+def residual_norms_batched(HX, MX, lam, R, colscale, out):
+    """R[i] = (HX[i] - MX[i] diag(lam[i])) diag(colscale[i]); out[i] = (||r_j||^2, ||MX_j||^2)."""
+    batch, n, b = HX.shape
+    assert HX.stride(0) == MX.stride(0) == R.stride(0)
+    wsb = lib.qds_residual_norms_batched_workspace(n, b, batch)
+    wp, wn = workspace("resnorm").ptr(wsb)
+    check(lib.qds_residual_norms_batched(code(HX), n, b, batch, ptr(HX), HX.stride(1), ptr(MX), MX.stride(1),
+                                         HX.stride(0), ptr(lam), ptr(colscale), ptr(R), R.stride(1), ptr(out), wp, wn,
+                                         stream_ptr()), "qds_residual_norms_batched")
+
+
+def rr_status_len(b):
+    return int(lib.qds_rr_status_len(int(b)))
+
+
+def rayleigh_ritz(G, lam, C, status, k, tol, drop_tol, norms=None, CP=None, colscale=None, freeze=True):
+    """Device Rayleigh-Ritz (qds_rayleigh_ritz) of a batch of projected pencils.
+    G (batch, m, 2m) FP64 (its M half is overwritten), lam (batch, b) in/out,
+    C / CP (batch, m, b), norms (batch, 2b), colscale (batch, b), status
+    (batch, rr_status_len(b)); all contiguous float64 device tensors."""
+    batch, m, m2 = G.shape
+    b = lam.shape[-1]
+    assert m2 == 2 * m and C.shape[-2:] == (m, b) and status.shape[-1] == rr_status_len(b)
+    for t in (G, lam, C, status, norms, CP, colscale):
+        assert t is None or (t.dtype == torch.float64 and t.is_contiguous())
+    check(lib.qds_rayleigh_ritz(batch, m, b, int(k), ptr(G), m * 2 * m, ptr(norms), ptr(lam), ptr(C), ptr(CP), m * b,
+                                ptr(colscale), ptr(status), float(tol), float(drop_tol), 1 if freeze else 0,
+                                stream_ptr()), "qds_rayleigh_ritz")
+
+
 def block_update_blocks(S, C1, Y1, C2=None, Y2=None, c2_first=0):
     """Y1 = [S_0 | S_1 | ...] C1 (and Y2 = ... C2); C in the storage dtype of S.
     c2_first: rows of C2 before it are zero (the kernel skips them)."""
@@ -276,6 +370,13 @@ def mg_vcycle(dtype_code, levels_struct, n_levels, b, nu_pre, nu_post, coarse, o
 def mg_vcycle_io(dtype_code, io_code, levels_struct, n_levels, b, nu_pre, nu_post, coarse, omega):
     check(lib.qds_mg_vcycle_io(dtype_code, io_code, n_levels, ctypes.cast(levels_struct, ctypes.c_void_p), b,
                                nu_pre, nu_post, coarse, float(omega), stream_ptr()), "qds_mg_vcycle_io")
+
+
+def mg_vcycle_batched(dtype_code, io_code, levels_struct, strides_struct, n_levels, batch, b, nu_pre, nu_post, coarse,
+                      omega):
+    check(lib.qds_mg_vcycle_batched(dtype_code, io_code, n_levels, ctypes.cast(levels_struct, ctypes.c_void_p),
+                                    ctypes.cast(strides_struct, ctypes.c_void_p), batch, b, nu_pre, nu_post, coarse,
+                                    float(omega), stream_ptr()), "qds_mg_vcycle_batched")
 
 
 def mg_apply(dtype_code, level_struct, b, u, out):

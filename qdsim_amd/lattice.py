@@ -303,3 +303,127 @@ class PoissonLatticeMG:
         # coarse columns = fine columns 2j+1; coarse vertical edge R spans fine rows 2R, 2R+1
         WNc = hmean(WN[0::2, 1::2], WN[1::2, 1::2])
         return WEc.contiguous(), WNc.contiguous()
+
+
+class BatchedLatticeMG:
+    """LatticeMG for a batch of problems on one lattice that share the kinetic
+    coefficient (edge weights) and differ in their potential and shift: every level
+    keeps one we / wn and a diagonal per problem, and one qds_mg_vcycle_batched call
+    runs the V-cycles of all problems (one launch per pass).  Same construction as
+    LatticeMG per problem: level diagonal = edge sums + coarsened (V - sigma) lumped
+    mass; the coarse mass term is linear in V, so it is coarsened for all problems at
+    once."""
+
+    def __init__(self, lattice, c_elem, v_elems, sigmas, b, dtype=torch.float32, nu_pre=1, nu_post=1,
+                 coarse_sweeps=10, omega=0.8, min_coarse=3):
+        self.lat = lattice
+        self.b = int(b)
+        self.dtype = dtype
+        self.nu_pre, self.nu_post, self.coarse_sweeps, self.omega = nu_pre, nu_post, coarse_sweeps, omega
+        dev = device()
+        L = lattice
+        c_elem = c_elem.to(dev, torch.float64)
+        v_elems = v_elems.to(dev, torch.float64)
+        self.batch = B = v_elems.shape[0]
+        sig = torch.as_tensor(sigmas, dtype=torch.float64, device=dev).reshape(B)
+        f64 = dict(dtype=torch.float64, device=dev)
+        ix = torch.arange(L.NX, **f64)
+        iy = torch.arange(L.NY, **f64)
+        Y, X = torch.meshgrid(iy, ix, indexing="ij")
+        xm = (X[:, :-1] + 0.5) * L.hx
+        yy = Y[:, :-1] * L.hy
+        WE = 0.5 * (c_elem[L.element_at(xm, yy + 0.25 * L.hy)] + c_elem[L.element_at(xm, yy - 0.25 * L.hy)]) \
+            * (L.hy / L.hx)
+        ym = (Y[:-1, :] + 0.5) * L.hy
+        xx = X[:-1, :] * L.hx
+        WN = 0.5 * (c_elem[L.element_at(xx + 0.25 * L.hx, ym)] + c_elem[L.element_at(xx - 0.25 * L.hx, ym)]) \
+            * (L.hx / L.hy)
+        xp = X * L.hx
+        yp = Y * L.hy
+        vs = sum(v_elems[:, L.element_at(xp + sx * L.hx, yp + sy * L.hy)]
+                 for sx in (-0.25, 0.25) for sy in (-0.25, 0.25)) * 0.25          # (B, NY, NX)
+        Mterm = (vs - sig[:, None, None]) * (L.hx * L.hy)
+        Mass = torch.full((L.NY, L.NX), L.hx * L.hy, **f64)
+        self.sigma = sig.clone()
+        self.levels_data = []
+        NX, NY = L.NX, L.NY
+        while True:
+            edges = (WE[1:-1, :-1] + WE[1:-1, 1:] + WN[:-1, 1:-1] + WN[1:, 1:-1])
+            d64 = edges[None] + Mterm[:, 1:-1, 1:-1]
+            we = WE[1:-1, 1:].clone()
+            we[:, -1] = 0.0
+            wn = WN[1:, 1:-1].clone()
+            wn[-1, :] = 0.0
+            n = (NX - 2) * (NY - 2)
+            self.levels_data.append({
+                "nx": NX - 2, "ny": NY - 2, "n": n, "d64": d64.reshape(B, n).contiguous(),
+                "d": d64.reshape(B, n).to(dtype).contiguous(),
+                "we": we.contiguous().to(dtype).reshape(-1), "wn": wn.contiguous().to(dtype).reshape(-1),
+                "mass": Mass[1:-1, 1:-1].reshape(-1).contiguous()})
+            nxi, nyi = NX - 2, NY - 2
+            if not (nxi % 2 == 1 and nyi % 2 == 1 and (nxi - 1) // 2 >= min_coarse
+                    and (nyi - 1) // 2 >= min_coarse):
+                break
+            WEc, WNc, Massc, NXc, NYc = LatticeMG._coarsen(WE, WN, Mass, NX, NY)
+            Mterm = self._restrict_batch(Mterm, NXc, NYc)
+            WE, WN, Mass, NX, NY = WEc, WNc, Massc, NXc, NYc
+        self.n_levels = len(self.levels_data)
+        self._structs = (_lib.MGLevel * self.n_levels)()
+        self._strides = (_lib.MGLevelStrides * self.n_levels)()
+        for l, lv in enumerate(self.levels_data):
+            n = lv["n"]
+            for key in ("u", "f", "t"):
+                lv[key] = torch.zeros((B, n, self.b), dtype=dtype, device=dev)
+            s = self._structs[l]
+            s.nx, s.ny = lv["nx"], lv["ny"]
+            s.d, s.we, s.wn = lv["d"].data_ptr(), lv["we"].data_ptr(), lv["wn"].data_ptr()
+            s.u, s.f, s.t = lv["u"].data_ptr(), lv["f"].data_ptr(), lv["t"].data_ptr()
+            s.ldu = s.ldf = 0
+            z = self._strides[l]
+            z.sd, z.sw, z.su, z.sf, z.st = n, 0, n * self.b, n * self.b, n * self.b
+
+    @staticmethod
+    def _restrict_batch(M, NXc, NYc):
+        """R m with bilinear weights for (B, NY, NX) (LatticeMG._coarsen per problem)."""
+        Mp = torch.nn.functional.pad(M[:, None], (1, 1, 1, 1))[:, 0]
+        w = (0.5, 1.0, 0.5)
+        Mc = torch.zeros((M.shape[0], NYc, NXc), dtype=M.dtype, device=M.device)
+        for a in range(3):
+            for c in range(3):
+                Mc += w[a] * w[c] * Mp[:, a:a + 2 * NYc - 1:2, c:c + 2 * NXc - 1:2]
+        return Mc
+
+    def set_sigma(self, sigmas):
+        """Move each problem's shift (diagonals updated in place)."""
+        sig = torch.as_tensor(sigmas, dtype=torch.float64, device=self.sigma.device).reshape(self.batch)
+        delta = sig - self.sigma
+        for lv in self.levels_data:
+            lv["d64"] = lv["d64"] - delta[:, None] * lv["mass"][None, :]
+            lv["d"].copy_(lv["d64"].to(self.dtype))
+        self.sigma = sig.clone()
+
+    @property
+    def takes_fp64(self):
+        if self.dtype != torch.float32 or self.nu_pre != 1 or self.nu_post != 1 or self.n_levels < 2:
+            return False
+        lv = self.levels_data[0]
+        vec = 4 if self.b % 4 == 0 else 1
+        return lv["nx"] * lv["ny"] * (self.b // vec) > 24576
+
+    def __call__(self, R, W):
+        """W[i] = V-cycle_i(R[i]) for (batch, n, b) blocks R, W (FP64 blocks on an FP32
+        hierarchy when ``takes_fp64``)."""
+        assert R.shape == W.shape and R.shape[0] == self.batch and R.is_contiguous() and W.is_contiguous()
+        io = _lib.F32 if R.dtype == torch.float32 else _lib.F64
+        if R.dtype != self.dtype:
+            assert self.dtype == torch.float32 and R.dtype == torch.float64 and self.takes_fp64
+        s0, z0 = self._structs[0], self._strides[0]
+        saved = (s0.f, s0.u, s0.ldf, s0.ldu, z0.sf, z0.su)
+        s0.f, s0.ldf, s0.u, s0.ldu = R.data_ptr(), R.stride(1), W.data_ptr(), W.stride(1)
+        z0.sf, z0.su = R.stride(0), W.stride(0)
+        try:
+            kernels.mg_vcycle_batched(_lib.F32 if self.dtype == torch.float32 else _lib.F64, io, self._structs,
+                                      self._strides, self.n_levels, self.batch, self.b, self.nu_pre, self.nu_post,
+                                      self.coarse_sweeps, self.omega)
+        finally:
+            s0.f, s0.u, s0.ldf, s0.ldu, z0.sf, z0.su = saved

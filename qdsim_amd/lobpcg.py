@@ -213,11 +213,171 @@ class LOBPCGResult:
 
 
 def lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=None, seed=0,
-           verbose=False, stall_iters=None, refresh=None):
+           verbose=False, stall_iters=None, refresh=None, rr="host"):
+    """rr: "host" (LAPACK Rayleigh-Ritz after one device-to-host copy per iteration)
+    or "device" (qds_rayleigh_ritz on the GPU; the host only reads each
+    iteration's status one iteration behind).  For one problem the host path is
+    faster: the single-workgroup Jacobi of the device path takes ~2.3 ms for the
+    72 x 72 projected pencil against ~0.3 ms of LAPACK plus ~0.6 ms of round trip;
+    the device path pays off for batches (one launch for all problems)."""
+    if rr == "device" and 3 * b <= 96:
+        return _lobpcg_device(op, k, b, precond, X0, tol, maxiter, drop_tol, seed, verbose, stall_iters, refresh)
     # the host Rayleigh-Ritz works on matrices of at most 96 x 192: multithreaded
     # BLAS only adds thread wake-ups there (measured 2-70x slower triangular solves)
     with threadpool_limits(1, "blas"):
         return _lobpcg(op, k, b, precond, X0, tol, maxiter, drop_tol, seed, verbose, stall_iters, refresh)
+
+
+def _default_drop_tol(dt):
+    # directions whose M-Gram eigenvalue is below drop_tol carry rounding of the
+    # stored vectors amplified by drop_tol^-1/2 into the projected matrices.
+    # FP32: 1e-7 keeps that amplification near 2e-4 while dropping (and the P
+    # restart it forces) stays rare; looser thresholds (1e-5, 1e-6) dropped a
+    # direction every other iteration on the quantum-dot problems and slowed
+    # convergence 1.3-2x.
+    return 1e-7 if dt == torch.float32 else 1e-10
+
+
+class _StallTest:
+    """Stagnation close to the target (the first iterations from a random start are
+    not monotone); far from it only a long stall stops."""
+
+    def __init__(self, tol, stall_iters):
+        self.tol, self.n = tol, stall_iters
+        self.best = self.min_seen = math.inf
+        self.best_it = self.min_it = 0
+
+    def stalled(self, it, worst):
+        if self.n is None:
+            return False
+        if worst < self.min_seen:
+            self.min_seen, self.min_it = worst, it
+        if worst < 100.0 * self.tol:
+            if worst < 0.7 * self.best:
+                self.best, self.best_it = worst, it
+            elif it - self.best_it >= self.n:
+                return True
+        elif it - self.min_it >= 4 * self.n:
+            return True
+        return False
+
+
+def _lobpcg_device(op, k, b, precond, X0, tol, maxiter, drop_tol, seed, verbose, stall_iters, refresh):
+    """The block iteration of ``_lobpcg`` with the Rayleigh-Ritz step on the device
+    (qds_rayleigh_ritz): residual norms -> preconditioner -> product -> Gram ->
+    Rayleigh-Ritz (which also applies the residual test, the locking mask for the
+    next residual and, once converged, keeps the block) -> updates, all queued on
+    the stream without a host round trip.  The basis is always [X | W | P]; an
+    empty or restarted P block enters as zero columns, which the Rayleigh-Ritz
+    kernel leaves out.  Every iteration ends with an asynchronous copy of its status
+    to page-locked memory; the host queues iteration i + 1 before it reads the
+    status of iteration i, so the GPU never waits for the host, and stops one
+    iteration after the status says converged (that iteration keeps the block)."""
+    dev = op.H.device
+    dt = op.dtype
+    n = op.n
+    assert b % 4 == 0 and b >= k and 3 * b <= n, (n, b, k)
+    if drop_tol is None:
+        drop_tol = _default_drop_tol(dt)
+    t_start = time.perf_counter()
+
+    def blk():
+        return torch.zeros((n, b), dtype=dt, device=dev)
+
+    f64 = dict(dtype=torch.float64, device=dev)
+    X, HX, MX = [blk(), blk()], [blk(), blk()], [blk(), blk()]
+    P, HP, MP = [blk(), blk()], [blk(), blk()], [blk(), blk()]
+    W, HW, MW, R = blk(), blk(), blk(), blk()
+    m = 3 * b
+    nst = kernels.rr_status_len(b)
+    G = torch.zeros((1, m, 2 * m), **f64)
+    norms = torch.zeros((1, 2 * b), **f64)
+    lam = torch.zeros((1, b), **f64)
+    C = torch.zeros((1, m, b), **f64)
+    CP = torch.zeros((1, m, b), **f64)
+    colscale = torch.ones((1, b), **f64)
+    status = torch.zeros((1, nst), **f64)
+    status_h = torch.zeros((2, nst), dtype=torch.float64, pin_memory=dev.type == "cuda")
+    events = [torch.cuda.Event(), torch.cuda.Event()] if dev.type == "cuda" else None
+    c = 0
+    if X0 is None:
+        g = torch.Generator(device=dev).manual_seed(seed)
+        X[c].copy_(torch.rand((n, b), generator=g, dtype=dt, device=dev) - 0.5)
+        if precond is not None:
+            R.copy_(X[c])
+            precond(R, X[c])
+    else:
+        X[c].copy_(X0[:, :b].to(dt))
+    # initial Rayleigh-Ritz on X alone
+    op.apply(X[c], HX[c], MX[c])
+    G0 = torch.zeros((1, b, 2 * b), **f64)
+    C0 = torch.zeros((1, b, b), **f64)
+    kernels.gram_blocks([X[c]], [HX[c]], [MX[c]], sym=True, out=G0[0])
+    kernels.rayleigh_ritz(G0, lam, C0, status, k, tol, drop_tol)
+    for src in (X, HX, MX):
+        kernels.block_update_blocks([src[c]], C0[0], src[1 - c])
+    c = 1 - c
+    CPb = CP[0][b:]
+
+    def iterate(c):
+        kernels.residual_norms(HX[c], MX[c], lam[0], R, colscale[0], out=norms[0])
+        if precond is not None:
+            precond(R, W)
+        else:
+            W.copy_(R)
+        op.apply(W, HW, MW)
+        S, HS, MS = [X[c], W, P[c]], [HX[c], HW, HP[c]], [MX[c], MW, MP[c]]
+        kernels.gram_blocks(S, HS, MS, sym=True, out=G[0])
+        kernels.rayleigh_ritz(G, lam, C, status, k, tol, drop_tol, norms=norms, CP=CP, colscale=colscale,
+                              freeze=True)
+        kernels.block_update_blocks(S, C[0], X[1 - c], CP[0], P[1 - c], c2_first=b)
+        if refresh:
+            # images of X by product below; those of P from the W and P blocks only
+            # (the X rows of the P coefficients are zero)
+            for src, pimg in ((HS, HP), (MS, MP)):
+                kernels.block_update_blocks(src[1:], CPb, pimg[1 - c])
+            op.apply(X[1 - c], HX[1 - c], MX[1 - c])
+        else:
+            for src, img, pimg in ((HS, HX, HP), (MS, MX, MP)):
+                kernels.block_update_blocks(src, C[0], img[1 - c], CP[0], pimg[1 - c], c2_first=b)
+
+    def launch(it, c):
+        iterate(c)
+        slot = it % 2
+        status_h[slot].copy_(status[0], non_blocking=True)
+        if events is not None:
+            events[slot].record()
+
+    history, restarts = [], 0
+    converged = False
+    stall = _StallTest(tol, stall_iters)
+    res = None
+    t_iter = time.perf_counter()
+    launch(1, c)
+    c = 1 - c
+    it = 0
+    for it in range(1, maxiter + 1):
+        if it < maxiter:
+            launch(it + 1, c)
+            c = 1 - c
+        if events is not None:
+            events[it % 2].synchronize()
+        st = status_h[it % 2].numpy().copy()
+        worst = float(st[0])
+        history.append(worst)
+        restarts += int(st[2] > 0)
+        res = st[8:8 + b]
+        if verbose:
+            print(f"  lobpcg[{str(dt)[6:]}/dev] it {it:4d} worst rel. residual {worst:.3e}")
+        if st[1] > 0:
+            converged = True
+            break
+        if stall.stalled(it, worst):
+            break
+    sync()
+    t_end = time.perf_counter()
+    timings = {"setup_s": t_iter - t_start, "iterate_s": t_end - t_iter, "restarts": restarts}
+    return LOBPCGResult(lam[0].clone(), X[c], torch.from_numpy(res.copy()), it, history, converged, timings)
 
 
 def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=None, seed=0,
@@ -372,7 +532,8 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
 
 
 def solve_mixed(op64, k, b, precond32=None, precond64=None, tol=1e-6, tol32=1e-3, maxiter=400,
-                seed=0, verbose=False, drop32=None, stall32=8, refresh32=1, X0=None, b64=None, between=None):
+                seed=0, verbose=False, drop32=None, stall32=8, refresh32=1, X0=None, b64=None, between=None,
+                rr="host"):
     """FP32 LOBPCG to tol32 (or stagnation), then FP64 LOBPCG to tol.  X0: optional
     initial block for the FP32 phase (default: seeded random vectors).
 
@@ -386,12 +547,12 @@ def solve_mixed(op64, k, b, precond32=None, precond64=None, tol=1e-6, tol32=1e-3
     op32 = op64.astype(torch.float32)
     t0 = time.perf_counter()
     r32 = lobpcg(op32, k, b, precond32, X0=X0, tol=tol32, maxiter=maxiter, seed=seed, verbose=verbose,
-                 stall_iters=stall32, drop_tol=drop32, refresh=refresh32)
+                 stall_iters=stall32, drop_tol=drop32, refresh=refresh32, rr=rr)
     if between is not None:
         between(r32)
     t1 = time.perf_counter()
     X0 = r32.X.to(torch.float64)
-    r64 = lobpcg(op64, k, b64 or b, precond64, X0=X0, tol=tol, maxiter=maxiter, seed=seed, verbose=verbose)
+    r64 = lobpcg(op64, k, b64 or b, precond64, X0=X0, tol=tol, maxiter=maxiter, seed=seed, verbose=verbose, rr=rr)
     t2 = time.perf_counter()
     r64.timings.update({"fp32_s": t1 - t0, "fp64_s": t2 - t1, "fp32_iters": r32.iterations,
                         "fp64_iters": r64.iterations, "fp32_restarts": r32.timings["restarts"],

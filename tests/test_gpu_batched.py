@@ -1,0 +1,170 @@
+"""Batched stages (one launch for a batch of problems on one mesh) against the same
+stage run problem by problem, and the batched solve against per-problem solves and
+the FP64 oracle."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import eig_ref
+
+pytestmark = pytest.mark.gpu
+
+M_E = 9.1093837015e-31
+EV = 1.602176634e-19
+
+
+def _gates(B, L, seed):
+    rng = np.random.default_rng(seed)
+    amp = rng.uniform(-0.3, 0.1, (B, 4))
+    cen = rng.uniform(0.2 * L, 0.8 * L, (B, 4, 2))
+    wid = rng.uniform(4.0, 10.0, (B, 4))
+
+    def vfun(xq, yq):
+        out = torch.zeros((B,) + tuple(xq.shape), dtype=torch.float64, device=xq.device)
+        for i in range(B):
+            for g in range(4):
+                out[i] += amp[i, g] * torch.exp(-((xq - cen[i, g, 0]) ** 2 + (yq - cen[i, g, 1]) ** 2)
+                                               / (2 * wid[i, g] ** 2))
+        return out
+
+    def vnp(i):
+        return lambda x, y: sum(amp[i, g] * np.exp(-((x - cen[i, g, 0]) ** 2 + (y - cen[i, g, 1]) ** 2)
+                                                   / (2 * wid[i, g] ** 2)) for g in range(4))
+    return vfun, vnp
+
+
+def _setup(cells=64, order=2, L=60.0, B=3, seed=1):
+    from qdsim_amd.batched_eigen import BatchedSchrodinger
+    from qdsim_amd.mesh import SimpleMesh
+    mesh = SimpleMesh(cells + 1, cells + 1, L * 1e-9, L * 1e-9, order)
+    bs = BatchedSchrodinger(mesh, lambda x, y: 0.067 * M_E + 0 * x)
+    vfun, vnp = _gates(B, L, seed)
+    return bs, vfun, vnp
+
+
+def test_batched_stages_equal_per_problem():
+    from qdsim_amd import kernels
+    from qdsim_amd.lattice import BatchedLatticeMG, LatticeMG
+    print("batched stages, P2 64^2, 3 problems", flush=True)
+    bs, vfun, _ = _setup()
+    xq, yq = bs.quad_points_nm()
+    v_q = vfun(xq, yq)
+    B = v_q.shape[0]
+    op = bs._operator(v_q)
+    n, b = op.n, 16
+    g = torch.Generator(device="cuda").manual_seed(3)
+    X = torch.rand((B, n, b), generator=g, dtype=torch.float64, device="cuda") - 0.5
+    for dt in (torch.float32, torch.float64):
+        o = op.astype(dt)
+        Xd = X.to(dt)
+        HX, MX = torch.empty_like(Xd), torch.empty_like(Xd)
+        o.apply(Xd, HX, MX)
+        for i in range(B):
+            h, mm = torch.empty_like(Xd[i]), torch.empty_like(Xd[i])
+            kernels.lattice_spmm(o.stencil, o.H[i], Xd[i], h, o.M, mm)
+            assert torch.equal(h, HX[i]) and torch.equal(mm, MX[i])
+        # Gram, residual norms, update
+        G = kernels.gram_blocks_batched([Xd, HX], [HX, MX], [MX, Xd], sym=False)
+        lam = torch.rand((B, b), dtype=torch.float64, device="cuda")
+        cs = (torch.rand((B, b), device="cuda") > 0.3).double()
+        R = torch.empty_like(Xd)
+        out = torch.empty((B, 2 * b), dtype=torch.float64, device="cuda")
+        kernels.residual_norms_batched(HX, MX, lam, R, cs, out)
+        Cc = torch.rand((B, 2 * b, b), dtype=torch.float64, device="cuda")
+        Y1, Y2 = torch.empty_like(Xd), torch.empty_like(Xd)
+        kernels.block_update_blocks_batched([Xd, HX], Cc, Y1, Cc, Y2, c2_first=0)
+        for i in range(B):
+            Gi = kernels.gram_blocks([Xd[i], HX[i]], [HX[i], MX[i]], [MX[i], Xd[i]], sym=False)
+            assert torch.equal(Gi, G[i])
+            Ri = torch.empty_like(Xd[i])
+            oi = torch.empty(2 * b, dtype=torch.float64, device="cuda")
+            kernels.residual_norms(HX[i], MX[i], lam[i], Ri, cs[i], out=oi)
+            assert torch.equal(Ri, R[i]) and torch.equal(oi, out[i])
+            y = torch.empty_like(Xd[i])
+            kernels.block_update_blocks([Xd[i], HX[i]], Cc[i], y)
+            assert torch.equal(y, Y1[i]) and torch.equal(y, Y2[i])
+    # batched multigrid = per-problem multigrid
+    p = bs.prob
+    c_e = bs.c_q.mean(dim=1)
+    v_e = v_q.mean(dim=2)
+    sig = [float(v_e[i].min()) - 0.02 for i in range(B)]
+    bmg = BatchedLatticeMG(p.lat, c_e, v_e, sig, b)
+    Rf = X.float().contiguous()
+    Wb = torch.empty_like(Rf)
+    bmg(Rf, Wb)
+    bmg.set_sigma([s - 0.01 for s in sig])
+    Wb2 = torch.empty_like(Rf)
+    bmg(Rf, Wb2)
+    for i in range(B):
+        mg = LatticeMG(p.lat, c_e, v_e[i], sig[i], b)
+        assert bmg.n_levels == mg.n_levels
+        w = torch.empty_like(Rf[i])
+        mg(Rf[i].contiguous(), w)
+        err = (w - Wb[i]).abs().max().item() / w.abs().max().item()
+        mg.set_sigma(sig[i] - 0.01)
+        w2 = torch.empty_like(Rf[i])
+        mg(Rf[i].contiguous(), w2)
+        err2 = (w2 - Wb2[i]).abs().max().item() / w2.abs().max().item()
+        print("mg", i, err, err2, flush=True)
+        assert err <= 1e-6 and err2 <= 1e-6
+
+
+def test_batched_solve_matches_single_and_oracle():
+    from qdsim_amd.schrodinger import SchrodingerProblem
+    print("batched solve, P2 128^2 (nested start), 3 problems, k=8", flush=True)
+    bs, vfun, vnp = _setup(cells=128, L=60.0, B=3, seed=2)
+    lam, vecs, info = bs.solve(vfun, 8, tol=1e-9)
+    print("batched", info["fp32_iters"], info["fp64_iters"], info["nested_iters"], info["timings"], flush=True)
+    assert info["converged"].all()
+    for i in range(3):
+        prob = SchrodingerProblem(bs.mesh, lambda x, y: 0.067 * M_E + 0 * x,
+                                  lambda x, y, i=i: vnp(i)(x * 1e9, y * 1e9) * EV)
+        l1, v1, inf1 = prob.solve(8, tol=1e-9)
+        err = np.abs(lam[i] - l1) / np.abs(l1)
+        print("set", i, "batched vs single", err.max(), flush=True)
+        assert err.max() <= 1e-9
+        C = vecs[i].cpu().numpy().T @ (bs.space.to_scipy(bs.M_full) @ v1.cpu().numpy())
+        assert np.linalg.svd(C, compute_uv=False).min() > 1 - 1e-6
+    ref = eig_ref.solve_closed(129, 129, 60.0, 60.0, 2, lambda x, y: 0.067 + 0 * x, vnp(1), 8)
+    err = np.abs(lam[1] - ref["vals"]) / np.abs(ref["vals"])
+    print("oracle", err.max(), flush=True)
+    assert err.max() <= 1e-9
+
+
+def test_batched_gate_gradients_match_per_set_and_oracle():
+    """The batched config-5 driver (all sets in one solve) against the per-set driver
+    and the FP64 oracle gradient, 4 seeded sets at 48^2 P2 cells on 100 nm."""
+    from oracle import adjoint_ref
+    from qdsim_amd.batch import gate_gradients, gate_gradients_batched, gate_parameter_sets
+    theta, sigma, targets = gate_parameter_sets(64, seed=5)
+    kw = dict(cells=48, order=2, L=100e-9, tol=1e-10)
+    loss_b, g_b, E_b = gate_gradients_batched(theta[:4], sigma[:4], targets, **kw)
+    loss_s, g_s, E_s = gate_gradients(theta[:4], sigma[:4], targets, mode="fp64", **kw)
+    print("batched vs per-set", np.abs(E_b - E_s).max(), np.abs(g_b - g_s).max() / np.abs(g_s).max(), flush=True)
+    assert np.allclose(E_b, E_s, rtol=1e-9, atol=1e-12)
+    assert np.abs(g_b - g_s).max() <= 1e-7 * np.abs(g_s).max()
+    for i in range(4):
+        lo, go, Eo = adjoint_ref.loss_and_gradient(48, 2, 100.0, 0.067, theta[i], sigma[i], targets)
+        assert np.allclose(E_b[i], Eo, rtol=1e-9, atol=1e-12)
+        assert abs(loss_b[i] - lo) <= 1e-9 * lo
+        assert np.abs(g_b[i] - go).max() <= 1e-7 * np.abs(go).max()
+
+
+def test_batched_gate_gradients_full_config():
+    """All 64 seeded sets at the config-5 size (256^2 P2 cells) in one batched solve:
+    every set converges, and set 7 agrees with the per-set driver."""
+    import time
+    from qdsim_amd.batch import GateBatchSolver, gate_gradients, gate_parameter_sets
+    theta, sigma, targets = gate_parameter_sets(64, seed=5)
+    solver = GateBatchSolver()
+    t0 = time.perf_counter()
+    loss, g, E = solver(theta, sigma, targets)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    info = solver.last_info
+    print(f"64 sets in {t:.2f} s ({64 / t:.1f} sets/s)", info["fp32_iters"], info["fp64_iters"],
+          info["nested_iters"], info["timings"], flush=True)
+    assert info["converged"].all() and g.shape == (64, 24) and np.all(np.isfinite(g))
+    l7, g7, E7 = gate_gradients(theta[7:8], sigma[7:8], targets)
+    assert np.allclose(E[7], E7[0], rtol=1e-9, atol=1e-12)
+    assert np.abs(g[7] - g7[0]).max() <= 1e-6 * np.abs(g7[0]).max()
