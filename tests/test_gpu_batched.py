@@ -168,3 +168,27 @@ def test_batched_gate_gradients_full_config():
     l7, g7, E7 = gate_gradients(theta[7:8], sigma[7:8], targets)
     assert np.allclose(E[7], E7[0], rtol=1e-9, atol=1e-12)
     assert np.abs(g[7] - g7[0]).max() <= 1e-6 * np.abs(g7[0]).max()
+
+
+def test_batched_graph_replays_equal_eager():
+    """The graph-captured iteration gives the eager iteration's trajectory: every
+    replay's status (worst residual of each problem) and the final blocks match."""
+    from qdsim_amd.batched_eigen import lobpcg_batched
+    print("graph vs eager, P2 64^2, 3 problems", flush=True)
+    bs, vfun, _ = _setup(cells=64, B=3, seed=4)
+    xq, yq = bs.quad_points_nm()
+    v_q = vfun(xq, yq)
+    op = bs._operator(v_q)
+    runs = {}
+    for graph in (False, True, True):
+        pre, _ = bs._preconditioner(v_q, 16)
+        r = lobpcg_batched(op.astype(torch.float32), 8, 16, pre, tol=1e-4, maxiter=60, graph=graph, seed=3)
+        torch.cuda.synchronize()
+        runs.setdefault(graph, []).append(r)
+        print(graph, r.iterations, r.timings, flush=True)
+    e = runs[False][0]
+    for g in runs[True]:
+        assert g.timings["graphs"] == 2
+        assert g.iterations == e.iterations
+        assert all(np.array_equal(a, b) for a, b in zip(g.history, e.history))
+        assert torch.equal(g.X, e.X) and torch.equal(g.eigenvalues, e.eigenvalues)
