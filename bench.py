@@ -86,7 +86,10 @@ def _spmm_roofline(prob, b, iters=20):
     if type(op).__name__ == "LatticeOperator":
         # lattice form: FP32 H and M value slots (no indices) + X once + HX, MX
         nbytes = nnz * (4 + 4) + b * n * (4 + 2 * 4)
-        kernel = "lattice_spmm_kernel<float,4,3,fused> (H and M, b=%d)" % b
+        from qdsim_amd import autotune
+        ch = autotune.choice_for(op.stencil, torch.float32, b, True, 1, None)
+        kernel = "lattice_spmm_kernel<float,4,3,fused,%s> (H and M, b=%d, LDS budget %d KiB)" % (
+            "values in LDS" if ch[0] else "values from HBM", b, ch[1] // 1024)
     else:
         nbytes = csr_bytes
         kernel = "spmm_real<float,4,fused> (H and M, b=%d)" % b
@@ -404,6 +407,7 @@ def main():
         "assembly_kernels": asm,
         "setup_s": setup_s,
         "solver_timings": infos[-1]["timings"],
+        "spmm_tuning": __import__("qdsim_amd.autotune", fromlist=["report"]).report(),
     }
     print(json.dumps(out))
 

@@ -193,6 +193,21 @@ int qds_lattice_spmm_batched(int32_t dtype, const qds_lattice_stencil* st /* hos
                              const void* X, int64_t ldx, int64_t sx, void* Y_a, int64_t ldy_a, int64_t sya,
                              void* Y_b, int64_t ldy_b, int64_t syb, void* stream);
 
+/* Block shape of the lattice product (no reference counterpart), chosen per
+ * operator by the package's autotuner: values_in_lds = 1 stages each block's value
+ * runs in LDS with its X rows (the default), 0 reads them from HBM in the compute
+ * phase; lds_budget (4096 .. 65536 bytes) sizes the block (cells per workgroup),
+ * and with it how many workgroups share a CU.                                    */
+typedef struct {
+  int32_t values_in_lds;
+  int32_t lds_budget;
+} qds_lattice_tuning;
+int qds_lattice_spmm_tuned(int32_t dtype, const qds_lattice_stencil* st /* host */, int32_t b, int32_t batch,
+                           const void* vals_a, int64_t sva, const void* vals_b, int64_t svb,
+                           const void* X, int64_t ldx, int64_t sx, void* Y_a, int64_t ldy_a, int64_t sya,
+                           void* Y_b, int64_t ldy_b, int64_t syb, const qds_lattice_tuning* tune /* host */,
+                           void* stream);
+
 /* ---- dense block kernels for the block eigensolver (device) --------------------
  * Replace the dense Gram / projection steps inside scipy's lobpcg
  * (eigen_solver.pyx:254-266) and ARPACK's orthogonalisation.                      */

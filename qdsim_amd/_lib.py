@@ -53,6 +53,8 @@ _SIGNATURES = {
                           _c.c_int),
     "qds_lattice_spmm_batched": ([_i32, _p, _i32, _i32, _p, _i64, _p, _i64, _p, _i64, _i64, _p, _i64, _i64, _p,
                                   _i64, _i64, _p], _c.c_int),
+    "qds_lattice_spmm_tuned": ([_i32, _p, _i32, _i32, _p, _i64, _p, _i64, _p, _i64, _i64, _p, _i64, _i64, _p,
+                                _i64, _i64, _p, _p], _c.c_int),
     "qds_gram_blocks_batched": ([_i32, _i64, _p, _p, _p, _i32, _p, _i32, _i64, _p, _i64, _p], _c.c_int),
     "qds_block_update_blocks_batched": ([_i32, _i64, _p, _i32, _p, _p, _i64, _p, _p, _i64, _i32, _i32, _i64, _i64,
                                          _i64, _p], _c.c_int),
@@ -107,6 +109,11 @@ class MGLevel(ctypes.Structure):
     _fields_ = [("nx", _i64), ("ny", _i64), ("d", _p), ("we", _p), ("wn", _p),
                 ("u", _p), ("f", _p), ("t", _p), ("ldu", _i64), ("ldf", _i64),
                 ("neumann_x", _i32), ("neumann_y", _i32)]
+
+
+class LatticeTuning(ctypes.Structure):
+    """Mirror of qds_lattice_tuning."""
+    _fields_ = [("values_in_lds", _i32), ("lds_budget", _i32)]
 
 
 class MGLevelStrides(ctypes.Structure):

@@ -1,0 +1,120 @@
+"""Autotuner of the lattice product's block shape (qds_lattice_tuning).
+
+No reference counterpart.  The lattice product (qds_lattice_spmm_tuned) has two
+launch-shape choices that do not change its arithmetic: whether each block's
+value runs are staged in LDS with its X rows or read from HBM by the compute
+phase, and the LDS budget a block is sized for (the cells per workgroup, and with
+it how many workgroups share a CU).  Every candidate computes the same sums in the
+same order, so the choice only changes time.  On the first product of an operator
+shape (order, lattice size, dtype, block width, fused or not, batch) the tuner
+times every candidate with HIP events on the stream the product runs on, with the
+operator's own values, and keeps the fastest for the process; the choices are also
+written to a JSON file (``QDSIM_AMD_TUNE_CACHE``, default
+~/.cache/qdsim_amd/lattice_tuning.json) keyed by the device and the shape, and read
+back by later processes.  ``QDSIM_AMD_TUNE=0`` skips the timing and uses the
+default shape.
+"""
+import json
+import os
+import threading
+
+import torch
+
+from . import _lib
+
+# (values staged in LDS, LDS budget in bytes); the first is the untuned default
+CANDIDATES = [(1, 52 * 1024), (0, 52 * 1024), (0, 40 * 1024), (0, 32 * 1024), (0, 24 * 1024),
+              (0, 64 * 1024), (1, 40 * 1024)]
+DEFAULT = CANDIDATES[0]
+
+_lock = threading.Lock()
+_choices = {}
+_file_loaded = False
+
+
+def _cache_path():
+    return os.environ.get("QDSIM_AMD_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "qdsim_amd", "lattice_tuning.json"))
+
+
+def _device_tag():
+    p = torch.cuda.get_device_properties(torch.cuda.current_device())
+    return f"{p.gcnArchName}/{p.multi_processor_count}"
+
+
+def _key(stencil, dtype, b, fused, batch):
+    return f"{_device_tag()}|p{stencil.order}|{stencil.nx}x{stencil.ny}|{str(dtype)[6:]}|b{b}|f{int(fused)}|n{batch}"
+
+
+def _load_file():
+    global _file_loaded
+    if _file_loaded:
+        return
+    _file_loaded = True
+    try:
+        with open(_cache_path()) as f:
+            data = json.load(f)
+        for k, v in data.items():
+            _choices.setdefault(k, (int(v[0]), int(v[1])))
+    except (OSError, ValueError, TypeError, IndexError):
+        pass
+
+
+def _save_file():
+    path = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + f".tmp{os.getpid()}"
+        with open(tmp, "w") as f:
+            json.dump({k: list(v) for k, v in _choices.items()}, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass
+
+
+def to_struct(choice):
+    t = _lib.LatticeTuning()
+    t.values_in_lds, t.lds_budget = int(choice[0]), int(choice[1])
+    return t
+
+
+def choice_for(stencil, dtype, b, fused, batch, run):
+    """The tuned (values_in_lds, lds_budget) for this product shape.  ``run(choice)``
+    launches the product once with that choice on the current stream (it must not
+    synchronise); the first call for a shape times every candidate."""
+    key = _key(stencil, dtype, b, fused, batch)
+    with _lock:
+        _load_file()
+        if key in _choices:
+            return _choices[key]
+    if os.environ.get("QDSIM_AMD_TUNE", "1") == "0":
+        return DEFAULT
+    times = {}
+    st = torch.cuda.current_stream()
+    for cand in CANDIDATES:
+        try:
+            run(cand)                      # warm-up (and a shape the kernel rejects raises here)
+            run(cand)
+        except _lib.QDSError:
+            continue
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(5):
+            run(cand)
+        e1.record(st)
+        e1.synchronize()
+        times[cand] = e0.elapsed_time(e1) / 5
+    best = min(times, key=times.get) if times else DEFAULT
+    with _lock:
+        _choices[key] = best
+        _timings[key] = {f"{c[0]}:{c[1]}": t for c, t in times.items()}
+        _save_file()
+    return best
+
+
+_timings = {}
+
+
+def report():
+    """Chosen shapes and the candidate times (ms) of the shapes tuned in this process."""
+    return {k: {"choice": list(v), "times_ms": _timings.get(k)} for k, v in _choices.items()}

@@ -43,7 +43,7 @@ class BatchedLatticeOperator:
         self.dtype = dtype
 
     def apply(self, X, HX, MX):
-        kernels.lattice_spmm_batched(self.stencil, self.H, X, HX, self.M, MX)
+        kernels.lattice_spmm_auto(self.stencil, self.H, X, HX, self.M, MX)
 
     def astype(self, dtype):
         return BatchedLatticeOperator(self.stencil, self.n, self.H, self.M, dtype)

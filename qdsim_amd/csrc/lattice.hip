@@ -139,7 +139,11 @@ __device__ __forceinline__ void task_entries(const int16_t* __restrict__ off, co
 // shortly before.
 __device__ __forceinline__ int64_t xcd_block(int64_t g, int64_t G) { return (g % 8) * (G / 8) + g / 8; }
 
-template <typename T, int VEC, int P, bool FUSED>
+// VLDS: the block's value runs are staged in LDS with the X rows (true) or read by
+// the compute phase straight from HBM (false: every value is used once, by the
+// lanes of one row, so staging buys nothing but LDS footprint and a barrier's
+// worth of latency; without it more workgroups fit on a CU).
+template <typename T, int VEC, int P, bool FUSED, bool VLDS>
 __global__ void __launch_bounds__(kThreads)
 lattice_spmm_kernel(const LatticeArgs a, const T* __restrict__ va, const T* __restrict__ vb,
                     const T* __restrict__ X, int64_t ldx, int chunked, T* __restrict__ Ya,
@@ -176,7 +180,7 @@ lattice_spmm_kernel(const LatticeArgs a, const T* __restrict__ va, const T* __re
   // outside the interior), then the value runs.  Every thread issues all of its
   // 16-byte loads before the first LDS write, so one memory latency covers the
   // whole block.
-  if (chunked && a.row_chunks <= kThreads && NV * a.vchunks <= kThreads * kVB) {
+  if (chunked && a.row_chunks <= kThreads && (!VLDS || NV * a.vchunks <= kThreads * kVB)) {
     // common case: each thread owns one 16-byte column slot of the X rows
     // (decoded once, then one load per row) and at most kVB value chunks
     const int cpp = ps / CE;
@@ -196,26 +200,30 @@ lattice_spmm_kernel(const LatticeArgs a, const T* __restrict__ va, const T* __re
       if (iin && J >= 1 && J <= a.ny) xv[lr] = *reinterpret_cast<const float4*>(xsrc + (J - 1) * a.nx * ldx);
     }
     float4 vv[kVB];
+    if constexpr (VLDS) {
 #pragma unroll
-    for (int i = 0; i < kVB; ++i) {
-      const int f2 = i * kThreads + (int)threadIdx.x;
-      vv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-      const bool second = f2 >= a.vchunks;
-      const int f = second ? f2 - a.vchunks : f2;
-      if (f2 < NV * a.vchunks && f < vvalid) vv[i] = second ? vsrc_b[f] : vsrc_a[f];
+      for (int i = 0; i < kVB; ++i) {
+        const int f2 = i * kThreads + (int)threadIdx.x;
+        vv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        const bool second = f2 >= a.vchunks;
+        const int f = second ? f2 - a.vchunks : f2;
+        if (f2 < NV * a.vchunks && f < vvalid) vv[i] = second ? vsrc_b[f] : vsrc_a[f];
+      }
     }
     if (xact) {
 #pragma unroll
       for (int lr = 0; lr < 3 * P; ++lr) lds4[lr * a.row_chunks + xdst] = xv[lr];
     }
+    if constexpr (VLDS) {
 #pragma unroll
-    for (int i = 0; i < kVB; ++i) {
-      const int f2 = i * kThreads + (int)threadIdx.x;
-      if (f2 < NV * a.vchunks) lds4[a.vlds + f2] = vv[i];
+      for (int i = 0; i < kVB; ++i) {
+        const int f2 = i * kThreads + (int)threadIdx.x;
+        if (f2 < NV * a.vchunks) lds4[a.vlds + f2] = vv[i];
+      }
     }
   } else {
     const int xchunks = chunked ? a.xchunks : 0;
-    const int total = xchunks + NV * a.vchunks;
+    const int total = xchunks + (VLDS ? NV * a.vchunks : 0);
     for (int e0 = 0; e0 < total; e0 += kThreads * kBatch) {
       float4 v[kBatch];
       int dst[kBatch];
@@ -268,8 +276,8 @@ lattice_spmm_kernel(const LatticeArgs a, const T* __restrict__ va, const T* __re
 
   // ---- wave tasks: rpw rows of one class, lpr lanes (VEC columns each) per row,
   // operands from LDS only
-  const T* lva = lds + (int64_t)a.vlds * CE;
-  const T* lvb = lva + (int64_t)a.vchunks * CE;
+  const T* lva = VLDS ? lds + (int64_t)a.vlds * CE : reinterpret_cast<const T*>(vsrc_a);
+  const T* lvb = VLDS ? lva + (int64_t)a.vchunks * CE : reinterpret_cast<const T*>(vsrc_b);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int g = lane / a.lpr;
@@ -333,11 +341,11 @@ lattice_spmm_kernel(const LatticeArgs a, const T* __restrict__ va, const T* __re
 
 bool aligned(const void* p, int bytes) { return (reinterpret_cast<uintptr_t>(p) % bytes) == 0; }
 
-template <typename T, int VEC, int P, bool FUSED>
+template <typename T, int VEC, int P, bool FUSED, bool VLDS>
 int launch(const LatticeArgs& a, size_t lds, const void* va, const void* vb, const void* X,
            int64_t ldx, int chunked, void* Ya, int64_t ldya, void* Yb, int64_t ldyb, hipStream_t st) {
   const int64_t grid = (a.n_blocks + 7) / 8 * 8;
-  hipLaunchKernelGGL((lattice_spmm_kernel<T, VEC, P, FUSED>), dim3((unsigned)grid, (unsigned)a.batch), dim3(kThreads), lds,
+  hipLaunchKernelGGL((lattice_spmm_kernel<T, VEC, P, FUSED, VLDS>), dim3((unsigned)grid, (unsigned)a.batch), dim3(kThreads), lds,
                      st, a, (const T*)va, (const T*)vb, (const T*)X, ldx, chunked, (T*)Ya, ldya,
                      (T*)Yb, ldyb);
   QDS_LAUNCH_CHECK();
@@ -345,13 +353,16 @@ int launch(const LatticeArgs& a, size_t lds, const void* va, const void* vb, con
 }
 
 template <typename T, int VEC>
-int dispatch(int p, bool fused, const LatticeArgs& a, size_t lds, const void* va, const void* vb,
+int dispatch(int p, bool fused, bool vlds, const LatticeArgs& a, size_t lds, const void* va, const void* vb,
              const void* X, int64_t ldx, int chunked, void* Ya, int64_t ldya, void* Yb, int64_t ldyb,
              hipStream_t st) {
-#define QDS_LAT_CASE(PP)                                                                          \
-  case PP:                                                                                      \
-    return fused ? launch<T, VEC, PP, true>(a, lds, va, vb, X, ldx, chunked, Ya, ldya, Yb, ldyb, st) \
-                 : launch<T, VEC, PP, false>(a, lds, va, vb, X, ldx, chunked, Ya, ldya, Yb, ldyb, st);
+#define QDS_LAT_CASE(PP)                                                                                       \
+  case PP:                                                                                                   \
+    if (vlds)                                                                                                \
+      return fused ? launch<T, VEC, PP, true, true>(a, lds, va, vb, X, ldx, chunked, Ya, ldya, Yb, ldyb, st)   \
+                   : launch<T, VEC, PP, false, true>(a, lds, va, vb, X, ldx, chunked, Ya, ldya, Yb, ldyb, st); \
+    return fused ? launch<T, VEC, PP, true, false>(a, lds, va, vb, X, ldx, chunked, Ya, ldya, Yb, ldyb, st)    \
+                 : launch<T, VEC, PP, false, false>(a, lds, va, vb, X, ldx, chunked, Ya, ldya, Yb, ldyb, st);
   switch (p) {
     QDS_LAT_CASE(1)
     QDS_LAT_CASE(2)
@@ -371,9 +382,16 @@ struct BatchStrides {
   int64_t sva, svb, sx, sya, syb;
 };
 
+struct Tuning {
+  bool vlds;        // values staged in LDS
+  size_t budget;    // LDS bytes per workgroup the block is sized for
+};
+constexpr Tuning kDefaultTuning{true, kLdsBudget};
+
 int lattice_spmm_impl(int32_t dtype, const qds_lattice_stencil* s, int32_t b, const void* vals_a,
                       const void* vals_b, const void* X, int64_t ldx, void* Y_a, int64_t ldy_a,
-                      void* Y_b, int64_t ldy_b, const BatchStrides& bs, void* stream) {
+                      void* Y_b, int64_t ldy_b, const BatchStrides& bs, void* stream,
+                      const Tuning& tune = kDefaultTuning) {
   QDS_CHECK_ARG(s && vals_a && X && Y_a, "qds_lattice_spmm: null pointer");
   QDS_CHECK_ARG(!vals_b || Y_b, "qds_lattice_spmm: vals_b without Y_b");
   QDS_CHECK_ARG(dtype == qds::F32 || dtype == qds::F64, "qds_lattice_spmm: dtype must be F32 or F64");
@@ -404,7 +422,7 @@ int lattice_spmm_impl(int32_t dtype, const qds_lattice_stencil* s, int32_t b, co
       const int w = std::min(bmax, b - j);
       const int rc = lattice_spmm_impl(dtype, s, w, vals_a, vals_b, (const char*)X + (size_t)j * es, ldx,
                                        (char*)Y_a + (size_t)j * es, ldy_a,
-                                       Y_b ? (char*)Y_b + (size_t)j * es : nullptr, ldy_b, bs, stream);
+                                       Y_b ? (char*)Y_b + (size_t)j * es : nullptr, ldy_b, bs, stream, tune);
       if (rc != QDS_OK) return rc;
     }
     return QDS_OK;
@@ -450,11 +468,11 @@ int lattice_spmm_impl(int32_t dtype, const qds_lattice_stencil* s, int32_t b, co
   // value tiles per lattice block: enough cells to fill a wave's rows, within the
   // LDS budget
   auto x_bytes = [&](int tic) { return ((size_t)3 * p * (size_t)(p * tic + 2 * p) * point_bytes + 15) / 16 * 16; };
-  auto block_bytes = [&](int m) { return x_bytes(m * s->tile) + nval * m * tile_vbytes; };
+  auto block_bytes = [&](int m) { return x_bytes(m * s->tile) + (tune.vlds ? nval * m * tile_vbytes : 0); };
   // smallest m whose block gives every row slot of a wave a cell and every wave a task
   int m = (a.rpw + s->tile - 1) / s->tile;
   while (p * p * ((m * s->tile + a.rpw - 1) / a.rpw) < kWaves) ++m;
-  while (m > 1 && block_bytes(m) > kLdsBudget) --m;
+  while (m > 1 && block_bytes(m) > tune.budget) --m;
   QDS_CHECK_ARG(block_bytes(m) <= kLdsMax, "qds_lattice_spmm: block width %d too large for LDS", b);
   const int tic = m * s->tile;
   a.m = m;
@@ -503,12 +521,12 @@ int lattice_spmm_impl(int32_t dtype, const qds_lattice_stencil* s, int32_t b, co
   const size_t lds = block_bytes(m);
   hipStream_t st = qds_stream(stream);
   if (dtype == qds::F32) {
-    if (vec == 4) return dispatch<float, 4>(p, fused, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
-    if (vec == 2) return dispatch<float, 2>(p, fused, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
-    return dispatch<float, 1>(p, fused, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
+    if (vec == 4) return dispatch<float, 4>(p, fused, tune.vlds, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
+    if (vec == 2) return dispatch<float, 2>(p, fused, tune.vlds, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
+    return dispatch<float, 1>(p, fused, tune.vlds, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
   }
-  if (vec == 2) return dispatch<double, 2>(p, fused, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
-  return dispatch<double, 1>(p, fused, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
+  if (vec == 2) return dispatch<double, 2>(p, fused, tune.vlds, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
+  return dispatch<double, 1>(p, fused, tune.vlds, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
 }
 }  // namespace
 
@@ -522,6 +540,21 @@ QDS_API int qds_lattice_spmm(int32_t dtype, const qds_lattice_stencil* s, int32_
 // The same product for `batch` problems on one stencil in one launch: problem i
 // reads vals_a + i sva, vals_b + i svb (svb = 0: one shared matrix, e.g. M), X + i sx
 // and writes Y_a + i sya, Y_b + i syb (strides in elements).
+// The batched product with an explicit block shape (see qds_lattice_tuning): what
+// the package's autotuner times and then uses.
+QDS_API int qds_lattice_spmm_tuned(int32_t dtype, const qds_lattice_stencil* s, int32_t b, int32_t batch,
+                                   const void* vals_a, int64_t sva, const void* vals_b, int64_t svb,
+                                   const void* X, int64_t ldx, int64_t sx, void* Y_a, int64_t ldy_a,
+                                   int64_t sya, void* Y_b, int64_t ldy_b, int64_t syb,
+                                   const qds_lattice_tuning* tune, void* stream) {
+  QDS_CHECK_ARG(tune, "qds_lattice_spmm_tuned: null tuning");
+  QDS_CHECK_ARG(tune->lds_budget >= 4096 && tune->lds_budget <= (int32_t)kLdsMax,
+                "qds_lattice_spmm_tuned: lds_budget %d outside [4096, %d]", tune->lds_budget, (int)kLdsMax);
+  const BatchStrides bs{batch, sva, svb, sx, sya, syb};
+  const Tuning t{tune->values_in_lds != 0, (size_t)tune->lds_budget};
+  return lattice_spmm_impl(dtype, s, b, vals_a, vals_b, X, ldx, Y_a, ldy_a, Y_b, ldy_b, bs, stream, t);
+}
+
 QDS_API int qds_lattice_spmm_batched(int32_t dtype, const qds_lattice_stencil* s, int32_t b, int32_t batch,
                                      const void* vals_a, int64_t sva, const void* vals_b, int64_t svb,
                                      const void* X, int64_t ldx, int64_t sx, void* Y_a, int64_t ldy_a,

@@ -155,6 +155,39 @@ def lattice_spmm(st, vals_a, X, Ya, vals_b=None, Yb=None):
           "qds_lattice_spmm")
 
 
+def lattice_spmm_tuned(st, vals_a, X, Ya, vals_b=None, Yb=None, choice=None):
+    """The lattice product with an explicit block shape (autotune choice): X, Ya, Yb
+    (n, b) or (batch, n, b); vals (slots,) shared or (batch, slots)."""
+    assert vals_a.dtype == X.dtype == Ya.dtype
+    from . import autotune
+    t = autotune.to_struct(choice or autotune.DEFAULT)
+    if X.dim() == 2:
+        batch, sx, sya, syb = 1, 0, 0, 0
+        ldx, ldya, ldyb = _ld(X), _ld(Ya), _ld(Yb) if Yb is not None else 1
+        b = _width(X)
+    else:
+        batch, b = X.shape[0], X.shape[2]
+        sx, sya, syb = X.stride(0), Ya.stride(0), Yb.stride(0) if Yb is not None else 0
+        ldx, ldya, ldyb = X.stride(1), Ya.stride(1), Yb.stride(1) if Yb is not None else 1
+
+    def vstride(V):
+        return 0 if V is None or V.dim() == 1 else V.stride(0)
+    check(lib.qds_lattice_spmm_tuned(code(X), ctypes.byref(st), b, batch, ptr(vals_a), vstride(vals_a), ptr(vals_b),
+                                     vstride(vals_b), ptr(X), ldx, sx, ptr(Ya), ldya, sya, ptr(Yb), ldyb, syb,
+                                     ctypes.byref(t), stream_ptr()), "qds_lattice_spmm_tuned")
+
+
+def lattice_spmm_auto(st, vals_a, X, Ya, vals_b=None, Yb=None):
+    """The lattice product with the block shape the autotuner chose for this shape
+    (timed on first use, see qdsim_amd.autotune)."""
+    from . import autotune
+    b = X.shape[-1]
+    batch = 1 if X.dim() == 2 else X.shape[0]
+    choice = autotune.choice_for(st, X.dtype, b, vals_b is not None, batch,
+                                 lambda ch: lattice_spmm_tuned(st, vals_a, X, Ya, vals_b, Yb, ch))
+    lattice_spmm_tuned(st, vals_a, X, Ya, vals_b, Yb, choice)
+
+
 def spmm_cap(indptr, indices, n_rows, h, w, m, X, YH, YM):
     """YH = (H - iW) X, YM = M X for complex blocks, real value arrays."""
     b = _width(X)

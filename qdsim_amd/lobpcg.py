@@ -61,8 +61,9 @@ class CSROperator:
 
 class LatticeOperator:
     """H and M of a structured-mesh problem in the lattice (stencil) form: per-class
-    value arrays without column indices, products by qds_lattice_spmm (bitwise
-    equal to the CSR product of the same matrices)."""
+    value arrays without column indices, products by qds_lattice_spmm_tuned with the
+    autotuned block shape (bitwise equal to the CSR product of the same matrices
+    for every shape)."""
 
     def __init__(self, stencil, n, H, M, dtype):
         self.stencil = stencil
@@ -73,13 +74,13 @@ class LatticeOperator:
         self.dtype = dtype
 
     def apply(self, X, HX, MX):
-        kernels.lattice_spmm(self.stencil, self.H, X, HX, self.M, MX)
+        kernels.lattice_spmm_auto(self.stencil, self.H, X, HX, self.M, MX)
 
     def apply_h(self, X, HX):
-        kernels.lattice_spmm(self.stencil, self.H, X, HX)
+        kernels.lattice_spmm_auto(self.stencil, self.H, X, HX)
 
     def apply_m(self, X, MX):
-        kernels.lattice_spmm(self.stencil, self.M, X, MX)
+        kernels.lattice_spmm_auto(self.stencil, self.M, X, MX)
 
     def astype(self, dtype):
         return LatticeOperator(self.stencil, self.n, self.H, self.M, dtype)
