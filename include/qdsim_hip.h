@@ -372,6 +372,91 @@ int qds_cg_mg(int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* indi
               void* workspace, int64_t ws_bytes, int32_t* iters_out /* host */,
               double* relres_out /* host */, void* stream);
 
+/* ---- nonsymmetric Krylov (device) -------------------------------------------------
+ * Jacobi left-preconditioned BiCGStab in FP64.  Replaces the SimplicialLDLT
+ * factorisation and solve of the drift-diffusion systems
+ * (backend/src/full_poisson_dd_solver.cpp:473-480, :1018-1040; on the S matrices
+ * of the ldlt_lower mode it solves what that factorisation solves).  x holds the
+ * initial guess; stops at ||D^-1 (b - A x)|| <= rtol ||D^-1 b|| (D = diag A;
+ * recursive residual, read every check_every iterations) or max_iter; a
+ * breakdown restarts from the last iterate (restarts_out, host, optional).  A
+ * non-finite b gives a non-finite x and relres_out.                               */
+int64_t qds_bicgstab_workspace(int64_t n);
+int qds_bicgstab(int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* indices, const double* vals,
+                 const double* b, double* x, double rtol, int32_t max_iter, int32_t check_every,
+                 void* workspace, int64_t ws_bytes, int32_t* iters_out /* host */,
+                 double* relres_out /* host */, int32_t* restarts_out /* host */, void* stream);
+
+/* ---- Poisson-drift-diffusion (device) ----------------------------------------------
+ * FullPoissonDriftDiffusionSolver (backend/src/full_poisson_dd_solver.cpp) on a P1
+ * triangle mesh.  Device arrays, FP64 node-major; fields and currents [n][2].
+ * ne_ptr/ne_elem: the elements of each node in ascending id (the order of the
+ * reference's per-node element loops).  mat: [n_mat][8] = N_c, N_v, E_g, mu_n,
+ * mu_p, eps_r, n_i = sqrt(N_c N_v) exp(-E_g / 2kT), 0; mat_idx per node and
+ * emat_idx per element (centroid) select the material of get_material_at
+ * (:185-211).  contact: 0 interior, 1 x < tol (V_p contact), 2 x > Lx - tol
+ * (V_n contact), as the reference's boundary tests (:404-416, :790-838).         */
+typedef struct {
+  int64_t n_nodes, n_elem;
+  const int32_t* conn;                      /* [n_elem][3] */
+  const double* x;
+  const double* y;
+  const int32_t* ne_ptr;                    /* [n_nodes + 1] */
+  const int32_t* ne_elem;                   /* [3 n_elem] */
+  const int32_t* indptr;                    /* CSR pattern of the mesh (row kernels) */
+  const int32_t* indices;
+  const int32_t* mat_idx;                   /* [n_nodes] */
+  const int32_t* emat_idx;                  /* [n_elem] (Poisson rows) */
+  const double* mat;                        /* [n_mat][8] */
+  const int8_t* contact;                    /* [n_nodes] */
+} qds_dd_mesh;
+/* n, p from phi, phi_n, phi_p: Boltzmann N_c exp((phi_n - phi)/kT), N_v exp((phi -
+ * phi_p - E_g)/kT), or with fermi_dirac the F_1/2 approximation; floor n_min with
+ * std::max.  Replaces compute_carrier_concentrations_boltzmann / _fermi_dirac
+ * (:656-718).                                                                     */
+int qds_dd_carriers(const qds_dd_mesh* m, int32_t fermi_dirac, double kT, double n_min, const double* phi,
+                    const double* phi_n, const double* phi_p, double* n, double* p, void* stream);
+/* Ohmic contact values of apply_boundary_conditions (:778-839), N_a = N_d = n_contact;
+ * fn / fp (optional): the contact rows of the continuity loads get the new phi_n /
+ * phi_p (the Dirichlet values of :1026-1027).                                      */
+int qds_dd_contacts(const qds_dd_mesh* m, double V_p, double V_n, double kT, double n_contact, double* phi,
+                    double* phi_n, double* phi_p, double* n, double* p, double* fn, double* fp, void* stream);
+/* Initial carriers and quasi-Fermi levels from the doping
+ * (initialize_carrier_concentrations, :216-260).                                    */
+int qds_dd_initial(const qds_dd_mesh* m, double kT, double n_min, const double* doping, double* n, double* p,
+                   double* phi_n, double* phi_p, void* stream);
+/* rho = scale (p - n + N); compute_charge_density (:267-288) with scale = q.        */
+int qds_dd_charge(int64_t n_nodes, double scale, const double* n, const double* p, const double* doping,
+                  double* rho, void* stream);
+/* mode 0: phi = phi_old + damping (phi - phi_old) (apply_damping, :1168-1173).
+ * mode 1: non-finite phi -> phi_old and sums[3] = {|phi|^2, |phi - phi_old|^2,
+ * #non-finite} on the device (the Gummel test of solve, :1383-1408).
+ * workspace: qds_dd_workspace() bytes (mode 1).                                    */
+int64_t qds_dd_workspace(void);
+int qds_dd_potential_update(int64_t n_nodes, int32_t mode, double damping, const double* phi_old, double* phi,
+                            double* sums, void* workspace, int64_t ws_bytes, void* stream);
+/* E = -grad phi averaged over each node's elements (compute_electric_field, :524-583). */
+int qds_dd_field(const qds_dd_mesh* m, const double* phi, double* E, void* stream);
+/* Jn, Jp = mean over the node's elements of q mu_c n_c grad phi_n, q mu_c p_c grad
+ * phi_p (compute_current_densities, :1068-1159).                                     */
+int qds_dd_currents(const qds_dd_mesh* m, double q, const double* phi_n, const double* phi_p, const double* n,
+                    const double* p, const double* mu_n, const double* mu_p, double* Jn, double* Jp,
+                    void* stream);
+/* Continuity systems of solve_drift_diffusion_equations (:847-1040): values of
+ * K_n, K_p on the mesh pattern and loads f_n, f_p, Dirichlet rows replaced by the
+ * contact phi_n / phi_p.  G: per-node generation-recombination rate, or NULL for
+ * the default SRH model (:72-82) evaluated from n, p.  ldlt_lower = 1 writes the
+ * matrix the reference's SimplicialLDLT factors (lower triangle mirrored).         */
+int qds_dd_continuity(const qds_dd_mesh* m, int32_t ldlt_lower, double kT, const double* E, const double* mu_n,
+                      const double* mu_p, const double* G, const double* n, const double* p, const double* phi_n,
+                      const double* phi_p, double* Kn, double* Kp, double* fn, double* fp, void* stream);
+/* Poisson system of solve_poisson_equation (:296-471): K = eps_r eps0 grad.grad A,
+ * f_i -= sum_q rho_q lambda_i (1/6) A, rho = q (p - n + N), Dirichlet rows V_p / V_n;
+ * ldlt_lower as above.                                                               */
+int qds_dd_poisson_system(const qds_dd_mesh* m, int32_t ldlt_lower, double eps0, double q, const double* n,
+                          const double* p, const double* doping, double V_p, double V_n, double* K, double* f,
+                          void* stream);
+
 /* ---- adaptive refinement indicator (device) --------------------------------
  * Replaces AdaptiveMesh::computeRefinementFlags
  * (backend/src/adaptive_mesh.cpp:431-475): out[e] = |gradient of the least-squares

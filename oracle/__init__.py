@@ -33,6 +33,14 @@ Parity status per function:
                                    self_consistent_solver.pyx:257-286
   interp_ref ..................... pinned: exact reproduction of degree-p polynomials (values and
                                    gradients), brute-force nearest node outside the mesh
+  dd_ref ......................... pinned: Poisson step = nodally exact parabola for uniform charge
+                                   (sign, 1/6-weight factor, eps0, Dirichlet rows); field and currents
+                                   of linear fields; Boltzmann round trip of the mass-action carriers;
+                                   continuity operator: K 1 = 0, divergence identity for the drift
+                                   term, E = 0 against fem_ref's stiffness, load sums; continuity
+                                   step = nodally exact parabola; ldlt_lower against np.tril.  The
+                                   lower-triangle reading of SimplicialLDLT is Eigen's documented
+                                   default, not executed here
 """
 
 PINNING = __doc__

@@ -12,7 +12,8 @@ __all__ = ["SimpleMesh", "create_simple_mesh", "FEMSolver", "EigenSolver", "Self
            "FixedOpenSystemSolver", "PoissonSolver", "SchrodingerProblem", "create_fem_solver",
            "create_eigen_solver", "lifetimes_fs", "FEInterpolator", "create_interpolator",
            "FileMesh", "load_mesh", "save_mesh", "load_csr", "save_csr", "load_pattern", "save_pattern",
-           "bias_sweep", "bias_sweep_instances", "gate_gradients", "gate_parameter_sets"]
+           "bias_sweep", "bias_sweep_instances", "gate_gradients", "gate_parameter_sets",
+           "FullPoissonDriftDiffusionSolver", "Material"]
 
 
 def __getattr__(name):
@@ -38,6 +39,9 @@ def __getattr__(name):
     if name == "SchrodingerProblem":
         from .schrodinger import SchrodingerProblem
         return SchrodingerProblem
+    if name in ("FullPoissonDriftDiffusionSolver", "Material"):
+        from . import drift_diffusion
+        return getattr(drift_diffusion, name)
     if name in ("bias_sweep", "bias_sweep_instances", "gate_gradients", "gate_parameter_sets"):
         from . import batch
         return getattr(batch, name)

@@ -85,6 +85,19 @@ _SIGNATURES = {
     "qds_gradient_indicator": ([_i32, _i64, _p, _p, _p, _p, _d, _p, _p], _c.c_int),
     "qds_cg_mg": ([_i64, _i64, _p, _p, _p, _p, _p, _d, _i32, _i32, _i32, _p, _i32, _i32, _d, _p, _i64,
                    _c.POINTER(_i32), _c.POINTER(_d), _p], _c.c_int),
+    "qds_bicgstab_workspace": ([_i64], _i64),
+    "qds_bicgstab": ([_i64, _i64, _p, _p, _p, _p, _p, _d, _i32, _i32, _p, _i64, _c.POINTER(_i32),
+                      _c.POINTER(_d), _c.POINTER(_i32), _p], _c.c_int),
+    "qds_dd_carriers": ([_p, _i32, _d, _d, _p, _p, _p, _p, _p, _p], _c.c_int),
+    "qds_dd_contacts": ([_p, _d, _d, _d, _d, _p, _p, _p, _p, _p, _p, _p, _p], _c.c_int),
+    "qds_dd_initial": ([_p, _d, _d, _p, _p, _p, _p, _p, _p], _c.c_int),
+    "qds_dd_charge": ([_i64, _d, _p, _p, _p, _p, _p], _c.c_int),
+    "qds_dd_workspace": ([], _i64),
+    "qds_dd_potential_update": ([_i64, _i32, _d, _p, _p, _p, _p, _i64, _p], _c.c_int),
+    "qds_dd_field": ([_p, _p, _p, _p], _c.c_int),
+    "qds_dd_currents": ([_p, _d, _p, _p, _p, _p, _p, _p, _p, _p, _p], _c.c_int),
+    "qds_dd_continuity": ([_p, _i32, _d, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p], _c.c_int),
+    "qds_dd_poisson_system": ([_p, _i32, _d, _d, _p, _p, _p, _d, _d, _p, _p, _p], _c.c_int),
 }
 
 
@@ -119,6 +132,13 @@ class LatticeTuning(ctypes.Structure):
 class MGLevelStrides(ctypes.Structure):
     """Mirror of qds_mg_level_strides."""
     _fields_ = [("sd", _i64), ("sw", _i64), ("su", _i64), ("sf", _i64), ("st", _i64)]
+
+
+class DDMesh(ctypes.Structure):
+    """Mirror of qds_dd_mesh."""
+    _fields_ = [("n_nodes", _i64), ("n_elem", _i64), ("conn", _p), ("x", _p), ("y", _p), ("ne_ptr", _p),
+                ("ne_elem", _p), ("indptr", _p), ("indices", _p), ("mat_idx", _p), ("emat_idx", _p), ("mat", _p),
+                ("contact", _p)]
 
 
 class QDSError(RuntimeError):

@@ -300,3 +300,273 @@ QDS_API int qds_cg_mg(int64_t n, int64_t nnz, const int32_t* indptr, const int32
   if (relres_out) *relres_out = sqrt(rr / bb);
   return QDS_OK;
 }
+
+// ---- BiCGStab (nonsymmetric systems) -----------------------------------------------
+//
+// Jacobi left-preconditioned BiCGStab in FP64 for the drift-diffusion systems.
+// Reference: full_poisson_dd_solver.cpp:1018-1040 (and :473-480 for the Poisson
+// step) factorises those matrices with SimplicialLDLT and solves directly; here
+// the row-replaced system is solved iteratively on the device.  The iteration runs
+// on D^-1 A x = D^-1 b (D = diag A, 1 where it is zero), so the residual it
+// measures and stops on is the row-scaled one: a replaced Dirichlet row (diagonal
+// 1) and an interior row (diagonal of the operator's scale, e.g. eps ~ 1e-12 F/cm)
+// weigh alike.  Scalars live in device memory; a breakdown (rho, omega or <r0, v>
+// at zero, or a non-finite step) raises a flag that stops every later update in
+// the burst, and the host restarts from the last iterate with a fresh shadow
+// residual.
+namespace {
+
+// sc layout (double): 0 rho, 1 alpha, 2 omega, 3 beta, 4 rr, 5 flag, 6 r0v, 7 ts, 8 tt, 9 bb
+__device__ __forceinline__ double bs_block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sh[i];
+  return s;
+}
+
+// r = dinv (b - r) (r holds A x); r0 = r; p = v = 0; partial (r.r)
+__global__ void __launch_bounds__(kBlock)
+bs_init(int64_t n, const double* __restrict__ b, const double* __restrict__ dinv, double* __restrict__ r,
+        double* __restrict__ r0, double* __restrict__ p, double* __restrict__ v, double* __restrict__ partial) {
+  __shared__ double sh[kBlock / 64];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double ri = dinv[i] * (b[i] - r[i]);
+    r[i] = ri;
+    r0[i] = ri;
+    p[i] = 0.0;
+    v[i] = 0.0;
+    s += ri * ri;
+  }
+  s = bs_block_sum(s, sh);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = s;
+    partial[2 * blockIdx.x + 1] = s;
+  }
+}
+
+// stage 0 (restart): rho = r0.r = rr, alpha = omega = 1, beta = 0, flag = (rho == 0)
+// stage 1: r0v = s0, alpha = rho / r0v
+// stage 2: ts = s0, tt = s1, omega = ts / tt
+// stage 3: rho_new = s0, rr = s1, beta = (rho_new / rho) (alpha / omega), rho = rho_new
+// stage 4: bb = s0
+__global__ void __launch_bounds__(kBlock)
+bs_final(int nblk, const double* __restrict__ partial, double* __restrict__ sc, int stage) {
+  __shared__ double sh[kBlock / 64];
+  double s0 = 0.0, s1 = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+    s0 += partial[2 * i];
+    s1 += partial[2 * i + 1];
+  }
+  s0 = bs_block_sum(s0, sh);
+  s1 = bs_block_sum(s1, sh);
+  if (threadIdx.x != 0) return;
+  if (stage == 4) {
+    sc[9] = s1;
+    return;
+  }
+  if (stage == 0) {
+    sc[0] = s0;
+    sc[1] = 1.0;
+    sc[2] = 1.0;
+    sc[3] = 0.0;
+    sc[4] = s1;
+    sc[5] = s0 != 0.0 ? 0.0 : 1.0;
+    return;
+  }
+  if (sc[5] != 0.0) return;
+  if (stage == 1) {
+    sc[6] = s0;
+    const double a = sc[0] / s0;
+    if (s0 == 0.0 || !isfinite(a)) sc[5] = 1.0;
+    else sc[1] = a;
+  } else if (stage == 2) {
+    sc[7] = s0;
+    sc[8] = s1;
+    const double w = s0 / s1;
+    if (s1 == 0.0 || w == 0.0 || !isfinite(w)) sc[5] = 1.0;
+    else sc[2] = w;
+  } else {
+    sc[4] = s1;
+    const double bt = (s0 / sc[0]) * (sc[1] / sc[2]);
+    if (s0 == 0.0 || !isfinite(bt)) sc[5] = 1.0;
+    else {
+      sc[3] = bt;
+      sc[0] = s0;
+    }
+  }
+}
+
+// p = r + beta (p - omega v)
+__global__ void __launch_bounds__(kBlock)
+bs_update_p(int64_t n, const double* __restrict__ sc, const double* __restrict__ r, double* __restrict__ p,
+            const double* __restrict__ v) {
+  if (sc[5] != 0.0) return;
+  const double beta = sc[3], omega = sc[2];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = r[i] + beta * (p[i] - omega * v[i]);
+}
+
+// partial (a . (d b)) and, with c, ((d c) . (d c)) ; d = dinv scales the raw products
+__global__ void __launch_bounds__(kBlock)
+bs_dot_scaled(int64_t n, const double* __restrict__ sc, const double* __restrict__ a, const double* __restrict__ b,
+              const double* __restrict__ c, const double* __restrict__ dinv, double* __restrict__ partial) {
+  __shared__ double sh[kBlock / 64];
+  double s0 = 0.0, s1 = 0.0;
+  if (sc == nullptr || sc[5] == 0.0) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+      const double d = dinv[i];
+      s0 += a[i] * (d * b[i]);
+      if (c) {
+        const double dc = d * c[i];
+        s1 += dc * dc;
+      }
+    }
+  }
+  s0 = bs_block_sum(s0, sh);
+  s1 = bs_block_sum(s1, sh);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = s0;
+    partial[2 * blockIdx.x + 1] = s1;
+  }
+}
+
+// v = dinv w ; s = r - alpha v (in r)
+__global__ void __launch_bounds__(kBlock)
+bs_update_s(int64_t n, const double* __restrict__ sc, double* __restrict__ r, const double* __restrict__ w,
+            double* __restrict__ v, const double* __restrict__ dinv) {
+  if (sc[5] != 0.0) return;
+  const double alpha = sc[1];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double vi = dinv[i] * w[i];
+    v[i] = vi;
+    r[i] = r[i] - alpha * vi;
+  }
+}
+
+// x += alpha p + omega s ; r = s - omega dinv t ; partial (r0.r), (r.r)   (s in r)
+__global__ void __launch_bounds__(kBlock)
+bs_update_x(int64_t n, const double* __restrict__ sc, double* __restrict__ x, double* __restrict__ r,
+            const double* __restrict__ r0, const double* __restrict__ p, const double* __restrict__ t,
+            const double* __restrict__ dinv, double* __restrict__ partial) {
+  __shared__ double sh[kBlock / 64];
+  double s0 = 0.0, s1 = 0.0;
+  if (sc[5] == 0.0) {
+    const double alpha = sc[1], omega = sc[2];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+      const double si = r[i];
+      x[i] += alpha * p[i] + omega * si;
+      const double ri = si - omega * dinv[i] * t[i];
+      r[i] = ri;
+      s0 += r0[i] * ri;
+      s1 += ri * ri;
+    }
+  }
+  s0 = bs_block_sum(s0, sh);
+  s1 = bs_block_sum(s1, sh);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = s0;
+    partial[2 * blockIdx.x + 1] = s1;
+  }
+}
+
+}  // namespace
+
+QDS_API int64_t qds_bicgstab_workspace(int64_t n) {
+  // r, r0, p, v, w, t, dinv + partials + scalars
+  return (7 * n + 2 * kRedBlocks + 16) * (int64_t)sizeof(double);
+}
+
+// Solve A x = b (A general, CSR FP64); x holds the initial guess.  Stops when
+// ||D^-1 (b - A x)|| <= rtol ||D^-1 b|| (recursive residual, read every
+// check_every iterations) or after max_iter iterations; iters_out / relres_out
+// (host) report the result, restarts_out (host, optional) the breakdown restarts.
+QDS_API int qds_bicgstab(int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* indices, const double* vals,
+                         const double* b, double* x, double rtol, int32_t max_iter, int32_t check_every,
+                         void* workspace, int64_t ws_bytes, int32_t* iters_out, double* relres_out,
+                         int32_t* restarts_out, void* stream) {
+  QDS_CHECK_ARG(indptr && indices && vals && b && x && workspace && n > 0, "qds_bicgstab: null pointer");
+  QDS_CHECK_ARG(ws_bytes >= qds_bicgstab_workspace(n), "qds_bicgstab: workspace too small");
+  if (check_every < 1) check_every = 1;
+  hipStream_t st = qds_stream(stream);
+  double* r = static_cast<double*>(workspace);
+  double* r0 = r + n;
+  double* p = r0 + n;
+  double* v = p + n;
+  double* w = v + n;
+  double* t = w + n;
+  double* dinv = t + n;
+  double* partial = dinv + n;
+  double* sc = partial + 2 * kRedBlocks;
+  const dim3 g(qds_blocks(n, kBlock, kRedBlocks)), blk(kBlock);
+  const int nblk = (int)g.x;
+
+  hipLaunchKernelGGL(diag_inv, g, blk, 0, st, n, indptr, indices, vals, dinv);
+  hipLaunchKernelGGL(bs_dot_scaled, g, blk, 0, st, n, (const double*)nullptr, b, b, b, dinv, partial);
+  hipLaunchKernelGGL(bs_final, dim3(1), blk, 0, st, nblk, partial, sc, 4);
+  QDS_LAUNCH_CHECK();
+  double hs[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  QDS_HIP(hipMemcpyAsync(hs, sc, 10 * sizeof(double), hipMemcpyDeviceToHost, st));
+  QDS_HIP(hipStreamSynchronize(st));
+  const double bb = hs[9];   // ||D^-1 b||^2
+  int restarts = 0;
+  if (!(bb > 0.0)) {
+    // b = 0: x = 0.  A non-finite b gives a non-finite x (all-ones bytes are a NaN),
+    // as a direct factorisation would; relres_out reports NaN
+    QDS_HIP(hipMemsetAsync(x, bb == 0.0 ? 0 : 0xff, n * sizeof(double), st));
+    if (iters_out) *iters_out = 0;
+    if (relres_out) *relres_out = bb == 0.0 ? 0.0 : bb;
+    if (restarts_out) *restarts_out = 0;
+    return QDS_OK;
+  }
+  const double tol2 = rtol * rtol * bb;
+  auto restart = [&]() -> int {
+    int rc = qds_spmm(qds::F64, n, nnz, 1, indptr, indices, vals, nullptr, x, 1, r, 1, nullptr, 1, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(bs_init, g, blk, 0, st, n, b, dinv, r, r0, p, v, partial);
+    hipLaunchKernelGGL(bs_final, dim3(1), blk, 0, st, nblk, partial, sc, 0);
+    QDS_LAUNCH_CHECK();
+    return QDS_OK;
+  };
+  int rc = restart();
+  if (rc) return rc;
+  QDS_HIP(hipMemcpyAsync(hs, sc, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+  QDS_HIP(hipStreamSynchronize(st));
+  int it = 0;
+  while (hs[4] > tol2 && it < max_iter) {
+    for (int k = 0; k < check_every && it < max_iter; ++k, ++it) {
+      hipLaunchKernelGGL(bs_update_p, g, blk, 0, st, n, sc, r, p, v);
+      rc = qds_spmm(qds::F64, n, nnz, 1, indptr, indices, vals, nullptr, p, 1, w, 1, nullptr, 1, st);
+      if (rc) return rc;
+      hipLaunchKernelGGL(bs_dot_scaled, g, blk, 0, st, n, sc, r0, w, (const double*)nullptr, dinv, partial);
+      hipLaunchKernelGGL(bs_final, dim3(1), blk, 0, st, nblk, partial, sc, 1);
+      hipLaunchKernelGGL(bs_update_s, g, blk, 0, st, n, sc, r, w, v, dinv);
+      rc = qds_spmm(qds::F64, n, nnz, 1, indptr, indices, vals, nullptr, r, 1, t, 1, nullptr, 1, st);
+      if (rc) return rc;
+      hipLaunchKernelGGL(bs_dot_scaled, g, blk, 0, st, n, sc, r, t, t, dinv, partial);
+      hipLaunchKernelGGL(bs_final, dim3(1), blk, 0, st, nblk, partial, sc, 2);
+      hipLaunchKernelGGL(bs_update_x, g, blk, 0, st, n, sc, x, r, r0, p, t, dinv, partial);
+      hipLaunchKernelGGL(bs_final, dim3(1), blk, 0, st, nblk, partial, sc, 3);
+      QDS_LAUNCH_CHECK();
+    }
+    QDS_HIP(hipMemcpyAsync(hs, sc, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+    QDS_HIP(hipStreamSynchronize(st));
+    if (hs[5] != 0.0 && it < max_iter) {
+      // breakdown inside the burst: the updates after it were skipped and x is the
+      // last good iterate; restart from it
+      if (++restarts > 50) break;
+      if ((rc = restart())) return rc;
+      QDS_HIP(hipMemcpyAsync(hs, sc, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+      QDS_HIP(hipStreamSynchronize(st));
+    }
+  }
+  if (iters_out) *iters_out = it;
+  if (relres_out) *relres_out = sqrt(hs[4] / bb);
+  if (restarts_out) *restarts_out = restarts;
+  return QDS_OK;
+}

@@ -439,3 +439,15 @@ def cg(indptr, indices, vals, n, nnz, b, x, rtol=1e-10, max_iter=10000, check_ev
                      int(max_iter), int(check_every), wp, wn, ctypes.byref(it), ctypes.byref(rr),
                      stream_ptr()), "qds_cg")
     return it.value, rr.value
+
+
+def bicgstab(indptr, indices, vals, n, nnz, b, x, rtol=1e-12, max_iter=20000, check_every=10):
+    """Jacobi-preconditioned BiCGStab (qds_bicgstab); x holds the initial guess.
+    Returns (iterations, relative residual, breakdown restarts)."""
+    wsb = lib.qds_bicgstab_workspace(n)
+    wp, wn = workspace("bicgstab").ptr(wsb)
+    it, rr, rs = ctypes.c_int32(), ctypes.c_double(), ctypes.c_int32()
+    check(lib.qds_bicgstab(n, nnz, ptr(indptr), ptr(indices), ptr(vals), ptr(b), ptr(x), float(rtol), int(max_iter),
+                           int(check_every), wp, wn, ctypes.byref(it), ctypes.byref(rr), ctypes.byref(rs),
+                           stream_ptr()), "qds_bicgstab")
+    return it.value, rr.value, rs.value

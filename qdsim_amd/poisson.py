@@ -31,15 +31,23 @@ from .units import EPSILON_0, Q_E
 class PoissonProblem:
     """Device-resident Poisson operator of one mesh (values in SI)."""
 
-    def __init__(self, mesh, epsilon_r_func, sampling="centroid", sign="physical"):
+    def __init__(self, mesh, epsilon_r_func, sampling="centroid", sign="physical", eps_elem=None, contacts=None):
+        """eps_elem: per-element permittivity (F per length unit), used instead of
+        epsilon_r_func * EPSILON_0; contacts: (left, right) node-id arrays of the V_p
+        and V_n contacts instead of the first and last lattice columns."""
         if sign not in ("physical", "reference"):
             raise ValueError(f"sign must be 'physical' or 'reference', got {sign!r}")
         self.sign = sign
         self.mesh = mesh
         self.space = FEMSpace(mesh)
         sp_ = self.space
-        where = "centroid" if sampling == "centroid" else "quadrature"
-        eps = sp_.sample(epsilon_r_func, where, EPSILON_0)
+        if eps_elem is not None:
+            eps = torch.as_tensor(np.asarray(eps_elem, dtype=np.float64)).to(sp_.dev).contiguous()
+            if eps.shape != (sp_.n_elem,):
+                raise ValueError(f"eps_elem needs one value per element ({sp_.n_elem}), got {tuple(eps.shape)}")
+        else:
+            where = "centroid" if sampling == "centroid" else "quadrature"
+            eps = sp_.sample(epsilon_r_func, where, EPSILON_0)
         K, M, _ = sp_.assemble(c=eps, stiffness_mass=True, mass=True)
         self.K = K                       # SI (scale free in 2-D)
         self.M = M * (NM * NM)           # m^2
@@ -50,6 +58,10 @@ class PoissonProblem:
         if lattice:
             ry, rx = mesh.lattice_shape
             lx = np.asarray(mesh.lattice_x)
+        if contacts is not None:
+            self.left = np.asarray(contacts[0], dtype=np.int64)
+            self.right = np.asarray(contacts[1], dtype=np.int64)
+        elif lattice:
             self.left = np.nonzero((lx == 0) & used)[0]
             self.right = np.nonzero((lx == rx - 1) & used)[0]
         else:
