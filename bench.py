@@ -243,6 +243,89 @@ def _cpu_baseline(k=K_STATES, samples=(48, 96), sm_cells=16):
             **info}
 
 
+DD_NODES = (2049, 1025)
+
+
+def _dd_workload(nx, ny, steps, warmup, cpu=True):
+    """Poisson--drift--diffusion: the reference's pn-diode example (200 x 100 nm,
+    P1, acceptors 1e17 cm^-3 for x < 100 nm, donors 1e17 beyond, default SRH and
+    constant mobilities, V_p = 0, V_n = 0.5 V) on an nx x ny node mesh.  A step is
+    one solve() of the reference loop (initial carriers, contacts, field,
+    statistics, one continuity step: both continuity systems assembled and solved
+    by BiCGStab, statistics, currents), inputs resident in HBM; the Poisson step
+    (CG with a multigrid preconditioner) is timed on its own."""
+    import numpy as np
+    import torch
+    from qdsim_amd import SimpleMesh, kernels
+    from qdsim_amd.drift_diffusion import FullPoissonDriftDiffusionSolver
+
+    mesh = SimpleMesh(nx, ny, 200.0, 100.0)
+    dop = lambda x, y: np.where(x < 100.0, -1e17, 1e17)
+    t0 = time.perf_counter()
+    s = FullPoissonDriftDiffusionSolver(mesh, lambda x, y: 12.9, dop)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    for _ in range(warmup):
+        s.solve(0.0, 0.5)
+    times = []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.solve(0.0, 0.5)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    info = dict(s.last_info)
+    # the Poisson step (CG + multigrid on the free nodes) from the initial carriers
+    # (finite charge: the solved example state holds n = inf where phi_n ~ 1e24 V)
+    s.initialize_carrier_concentrations()
+    s._contacts(0.0, 0.5)
+    ptimes = []
+    for _ in range(3):
+        s._phi.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.solve_poisson_equation(0.0, 0.5)
+        torch.cuda.synchronize()
+        ptimes.append(time.perf_counter() - t0)
+    poisson_s = min(ptimes)
+    poisson_info = s.last_info.get("poisson")
+    # the BiCGStab matrix product (CSR FP64, one vector) on the electron matrix
+    st = torch.cuda.current_stream()
+    x = torch.ones(s.n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    kernels.spmm(s._indptr, s._indices, s.n, s.nnz, s._Kn, x, y)
+    e0.record(st)
+    for _ in range(20):
+        kernels.spmm(s._indptr, s._indices, s.n, s.nnz, s._Kn, x, y)
+    e1.record(st)
+    e1.synchronize()
+    spmv_us = e0.elapsed_time(e1) / 20 * 1e3
+    spmv_bytes = s.nnz * 12 + 4 * (s.n + 1) + 16 * s.n
+    out = {"nodes": s.n, "nnz": s.nnz, "setup_s": setup_s, "ms_per_solve": float(np.mean(times)) * 1e3,
+           "solves_per_s": 1.0 / float(np.mean(times)), "krylov": info, "poisson_step_ms": poisson_s * 1e3,
+           "poisson_step": poisson_info,
+           "spmv": {"kernel": "spmv_real<double> (BiCGStab product, one vector)", "avg_us": spmv_us,
+                    "bytes": spmv_bytes, "achieved_GBs": spmv_bytes / spmv_us * 1e-3,
+                    "frac": spmv_bytes / spmv_us * 1e-3 / HBM_PEAK_GBS}}
+    if cpu:
+        from oracle import dd_ref
+        cnx, cny = 257, 129
+        cm = SimpleMesh(cnx, cny, 200.0, 100.0)
+        cx, cy = cm.get_nodes()
+        import warnings
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            r = dd_ref.DDRef(cx, cy, cm.get_elements(), 200.0, dop(cx, cy))
+            t0 = time.perf_counter()
+            r.solve(0.0, 0.5)
+            tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cnx * cny / tc, "unit": "nodes/s (one solve)", "cores": 1, "kind": "port",
+                               "sample": f"oracle DDRef.solve on {cnx}x{cny} nodes (scipy spsolve), {tc:.2f} s"}
+        out["gpu_nodes_per_s"] = s.n / float(np.mean(times))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -260,6 +343,9 @@ def main():
                     help="only run the first N instances of the seeded bias sweep (self-consistent loop, "
                          "30 outer iterations each) and print their timings")
     ap.add_argument("--sweep-workers", type=int, default=4)
+    ap.add_argument("--workload", choices=["eigen", "dd"], default="eigen",
+                    help="dd: only the Poisson-drift-diffusion workload (reference pn-diode example)")
+    ap.add_argument("--dd-nodes", type=int, nargs=2, default=DD_NODES)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -272,6 +358,10 @@ def main():
     from qdsim_amd.schrodinger import SchrodingerProblem
 
     torch.cuda.set_device(0)
+    if args.workload == "dd":
+        dd = _dd_workload(args.dd_nodes[0], args.dd_nodes[1], args.steps, args.warmup, cpu=not args.no_cpu)
+        print(json.dumps({"drift_diffusion": dd}))
+        return
     if args.sweep_instances > 0:
         from qdsim_amd.batch import bias_sweep, bias_sweep_instances
         inst = bias_sweep_instances(256, seed=4)[:args.sweep_instances]

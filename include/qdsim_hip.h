@@ -386,6 +386,17 @@ int qds_bicgstab(int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* i
                  const double* b, double* x, double rtol, int32_t max_iter, int32_t check_every,
                  void* workspace, int64_t ws_bytes, int32_t* iters_out /* host */,
                  double* relres_out /* host */, int32_t* restarts_out /* host */, void* stream);
+/* The same BiCGStab right-preconditioned by one multigrid V(nu, nu)-cycle per
+ * product on an FP64 lattice hierarchy (levels[0] holds the n unknowns in lattice
+ * order, as for qds_cg_mg); stops at ||b - A x|| <= rtol ||b||.  Replaces the same
+ * factorisations for the continuity systems restricted to the free nodes of a
+ * structured mesh (full_poisson_dd_solver.cpp:1018-1040).  Workspace as
+ * qds_bicgstab_workspace.                                                         */
+int qds_bicgstab_mg(int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* indices, const double* vals,
+                    const double* b, double* x, double rtol, int32_t max_iter, int32_t check_every,
+                    int32_t n_levels, const qds_mg_level* levels, int32_t nu, int32_t coarse_sweeps, double omega,
+                    void* workspace, int64_t ws_bytes, int32_t* iters_out /* host */,
+                    double* relres_out /* host */, int32_t* restarts_out /* host */, void* stream);
 
 /* ---- Poisson-drift-diffusion (device) ----------------------------------------------
  * FullPoissonDriftDiffusionSolver (backend/src/full_poisson_dd_solver.cpp) on a P1

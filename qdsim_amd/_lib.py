@@ -88,6 +88,8 @@ _SIGNATURES = {
     "qds_bicgstab_workspace": ([_i64], _i64),
     "qds_bicgstab": ([_i64, _i64, _p, _p, _p, _p, _p, _d, _i32, _i32, _p, _i64, _c.POINTER(_i32),
                       _c.POINTER(_d), _c.POINTER(_i32), _p], _c.c_int),
+    "qds_bicgstab_mg": ([_i64, _i64, _p, _p, _p, _p, _p, _d, _i32, _i32, _i32, _p, _i32, _i32, _d, _p, _i64,
+                         _c.POINTER(_i32), _c.POINTER(_d), _c.POINTER(_i32), _p], _c.c_int),
     "qds_dd_carriers": ([_p, _i32, _d, _d, _p, _p, _p, _p, _p, _p], _c.c_int),
     "qds_dd_contacts": ([_p, _d, _d, _d, _d, _p, _p, _p, _p, _p, _p, _p, _p], _c.c_int),
     "qds_dd_initial": ([_p, _d, _d, _p, _p, _p, _p, _p, _p], _c.c_int),

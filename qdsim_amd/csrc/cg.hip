@@ -329,6 +329,11 @@ __device__ __forceinline__ double bs_block_sum(double v, double* sh) {
   return s;
 }
 
+// d_i of the left scaling (1 without one)
+__device__ __forceinline__ double dscale(const double* __restrict__ dinv, int64_t i) {
+  return dinv ? dinv[i] : 1.0;
+}
+
 // r = dinv (b - r) (r holds A x); r0 = r; p = v = 0; partial (r.r)
 __global__ void __launch_bounds__(kBlock)
 bs_init(int64_t n, const double* __restrict__ b, const double* __restrict__ dinv, double* __restrict__ r,
@@ -336,7 +341,7 @@ bs_init(int64_t n, const double* __restrict__ b, const double* __restrict__ dinv
   __shared__ double sh[kBlock / 64];
   double s = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const double ri = dinv[i] * (b[i] - r[i]);
+    const double ri = dscale(dinv, i) * (b[i] - r[i]);
     r[i] = ri;
     r0[i] = ri;
     p[i] = 0.0;
@@ -420,7 +425,7 @@ bs_dot_scaled(int64_t n, const double* __restrict__ sc, const double* __restrict
   double s0 = 0.0, s1 = 0.0;
   if (sc == nullptr || sc[5] == 0.0) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-      const double d = dinv[i];
+      const double d = dscale(dinv, i);
       s0 += a[i] * (d * b[i]);
       if (c) {
         const double dc = d * c[i];
@@ -443,25 +448,26 @@ bs_update_s(int64_t n, const double* __restrict__ sc, double* __restrict__ r, co
   if (sc[5] != 0.0) return;
   const double alpha = sc[1];
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const double vi = dinv[i] * w[i];
+    const double vi = dscale(dinv, i) * w[i];
     v[i] = vi;
     r[i] = r[i] - alpha * vi;
   }
 }
 
-// x += alpha p + omega s ; r = s - omega dinv t ; partial (r0.r), (r.r)   (s in r)
+// x += alpha P + omega S ; r = s - omega dinv t ; partial (r0.r), (r.r)   (s in r;
+// P, S = p, s with the left scaling, the preconditioned p^, s^ with a right one)
 __global__ void __launch_bounds__(kBlock)
-bs_update_x(int64_t n, const double* __restrict__ sc, double* __restrict__ x, double* __restrict__ r,
-            const double* __restrict__ r0, const double* __restrict__ p, const double* __restrict__ t,
-            const double* __restrict__ dinv, double* __restrict__ partial) {
+bs_update_x(int64_t n, const double* __restrict__ sc, double* __restrict__ x, double* r,
+            const double* __restrict__ r0, const double* __restrict__ P, const double* S,
+            const double* __restrict__ t, const double* __restrict__ dinv, double* __restrict__ partial) {
   __shared__ double sh[kBlock / 64];
   double s0 = 0.0, s1 = 0.0;
   if (sc[5] == 0.0) {
     const double alpha = sc[1], omega = sc[2];
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
       const double si = r[i];
-      x[i] += alpha * p[i] + omega * si;
-      const double ri = si - omega * dinv[i] * t[i];
+      x[i] += alpha * P[i] + omega * S[i];
+      const double ri = si - omega * dscale(dinv, i) * t[i];
       r[i] = ri;
       s0 += r0[i] * ri;
       s1 += ri * ri;
@@ -478,42 +484,43 @@ bs_update_x(int64_t n, const double* __restrict__ sc, double* __restrict__ x, do
 }  // namespace
 
 QDS_API int64_t qds_bicgstab_workspace(int64_t n) {
-  // r, r0, p, v, w, t, dinv + partials + scalars
-  return (7 * n + 2 * kRedBlocks + 16) * (int64_t)sizeof(double);
+  // r, r0, p, v, w, t, dinv or p^, s^ + partials + scalars
+  return (8 * n + 2 * kRedBlocks + 16) * (int64_t)sizeof(double);
 }
 
-// Solve A x = b (A general, CSR FP64); x holds the initial guess.  Stops when
-// ||D^-1 (b - A x)|| <= rtol ||D^-1 b|| (recursive residual, read every
-// check_every iterations) or after max_iter iterations; iters_out / relres_out
-// (host) report the result, restarts_out (host, optional) the breakdown restarts.
-QDS_API int qds_bicgstab(int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* indices, const double* vals,
-                         const double* b, double* x, double rtol, int32_t max_iter, int32_t check_every,
-                         void* workspace, int64_t ws_bytes, int32_t* iters_out, double* relres_out,
-                         int32_t* restarts_out, void* stream) {
-  QDS_CHECK_ARG(indptr && indices && vals && b && x && workspace && n > 0, "qds_bicgstab: null pointer");
-  QDS_CHECK_ARG(ws_bytes >= qds_bicgstab_workspace(n), "qds_bicgstab: workspace too small");
+namespace {
+
+// BiCGStab on A x = b.  dinv != NULL: left scaling D^-1 (Jacobi), residual measured
+// scaled; precondition != NULL: right preconditioner u = M^-1 f (f, u device
+// vectors of length n), residual measured unscaled.
+template <class Precond>
+int bicgstab_impl(int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* indices, const double* vals,
+                  const double* b, double* x, double rtol, int32_t max_iter, int32_t check_every, bool jacobi,
+                  Precond precondition, void* workspace, int32_t* iters_out, double* relres_out,
+                  int32_t* restarts_out, hipStream_t st) {
   if (check_every < 1) check_every = 1;
-  hipStream_t st = qds_stream(stream);
   double* r = static_cast<double*>(workspace);
   double* r0 = r + n;
   double* p = r0 + n;
   double* v = p + n;
   double* w = v + n;
   double* t = w + n;
-  double* dinv = t + n;
-  double* partial = dinv + n;
+  double* dinv = jacobi ? t + n : nullptr;
+  double* ph = jacobi ? p : t + n;         // the operand of the first product
+  double* shat = jacobi ? r : t + 2 * n;   // and of the second
+  double* partial = t + 3 * n;
   double* sc = partial + 2 * kRedBlocks;
   const dim3 g(qds_blocks(n, kBlock, kRedBlocks)), blk(kBlock);
   const int nblk = (int)g.x;
 
-  hipLaunchKernelGGL(diag_inv, g, blk, 0, st, n, indptr, indices, vals, dinv);
-  hipLaunchKernelGGL(bs_dot_scaled, g, blk, 0, st, n, (const double*)nullptr, b, b, b, dinv, partial);
+  if (jacobi) hipLaunchKernelGGL(diag_inv, g, blk, 0, st, n, indptr, indices, vals, dinv);
+  hipLaunchKernelGGL(bs_dot_scaled, g, blk, 0, st, n, (const double*)nullptr, b, b, b, (const double*)dinv, partial);
   hipLaunchKernelGGL(bs_final, dim3(1), blk, 0, st, nblk, partial, sc, 4);
   QDS_LAUNCH_CHECK();
   double hs[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   QDS_HIP(hipMemcpyAsync(hs, sc, 10 * sizeof(double), hipMemcpyDeviceToHost, st));
   QDS_HIP(hipStreamSynchronize(st));
-  const double bb = hs[9];   // ||D^-1 b||^2
+  const double bb = hs[9];   // ||D^-1 b||^2 (||b||^2 without the scaling)
   int restarts = 0;
   if (!(bb > 0.0)) {
     // b = 0: x = 0.  A non-finite b gives a non-finite x (all-ones bytes are a NaN),
@@ -528,7 +535,7 @@ QDS_API int qds_bicgstab(int64_t n, int64_t nnz, const int32_t* indptr, const in
   auto restart = [&]() -> int {
     int rc = qds_spmm(qds::F64, n, nnz, 1, indptr, indices, vals, nullptr, x, 1, r, 1, nullptr, 1, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(bs_init, g, blk, 0, st, n, b, dinv, r, r0, p, v, partial);
+    hipLaunchKernelGGL(bs_init, g, blk, 0, st, n, b, (const double*)dinv, r, r0, p, v, partial);
     hipLaunchKernelGGL(bs_final, dim3(1), blk, 0, st, nblk, partial, sc, 0);
     QDS_LAUNCH_CHECK();
     return QDS_OK;
@@ -541,16 +548,19 @@ QDS_API int qds_bicgstab(int64_t n, int64_t nnz, const int32_t* indptr, const in
   while (hs[4] > tol2 && it < max_iter) {
     for (int k = 0; k < check_every && it < max_iter; ++k, ++it) {
       hipLaunchKernelGGL(bs_update_p, g, blk, 0, st, n, sc, r, p, v);
-      rc = qds_spmm(qds::F64, n, nnz, 1, indptr, indices, vals, nullptr, p, 1, w, 1, nullptr, 1, st);
+      if (!jacobi && (rc = precondition(p, ph))) return rc;
+      rc = qds_spmm(qds::F64, n, nnz, 1, indptr, indices, vals, nullptr, ph, 1, w, 1, nullptr, 1, st);
       if (rc) return rc;
-      hipLaunchKernelGGL(bs_dot_scaled, g, blk, 0, st, n, sc, r0, w, (const double*)nullptr, dinv, partial);
+      hipLaunchKernelGGL(bs_dot_scaled, g, blk, 0, st, n, sc, r0, w, (const double*)nullptr, (const double*)dinv,
+                         partial);
       hipLaunchKernelGGL(bs_final, dim3(1), blk, 0, st, nblk, partial, sc, 1);
-      hipLaunchKernelGGL(bs_update_s, g, blk, 0, st, n, sc, r, w, v, dinv);
-      rc = qds_spmm(qds::F64, n, nnz, 1, indptr, indices, vals, nullptr, r, 1, t, 1, nullptr, 1, st);
+      hipLaunchKernelGGL(bs_update_s, g, blk, 0, st, n, sc, r, w, v, (const double*)dinv);
+      if (!jacobi && (rc = precondition(r, shat))) return rc;
+      rc = qds_spmm(qds::F64, n, nnz, 1, indptr, indices, vals, nullptr, shat, 1, t, 1, nullptr, 1, st);
       if (rc) return rc;
-      hipLaunchKernelGGL(bs_dot_scaled, g, blk, 0, st, n, sc, r, t, t, dinv, partial);
+      hipLaunchKernelGGL(bs_dot_scaled, g, blk, 0, st, n, sc, r, t, t, (const double*)dinv, partial);
       hipLaunchKernelGGL(bs_final, dim3(1), blk, 0, st, nblk, partial, sc, 2);
-      hipLaunchKernelGGL(bs_update_x, g, blk, 0, st, n, sc, x, r, r0, p, t, dinv, partial);
+      hipLaunchKernelGGL(bs_update_x, g, blk, 0, st, n, sc, x, r, r0, ph, shat, t, (const double*)dinv, partial);
       hipLaunchKernelGGL(bs_final, dim3(1), blk, 0, st, nblk, partial, sc, 3);
       QDS_LAUNCH_CHECK();
     }
@@ -569,4 +579,47 @@ QDS_API int qds_bicgstab(int64_t n, int64_t nnz, const int32_t* indptr, const in
   if (relres_out) *relres_out = sqrt(hs[4] / bb);
   if (restarts_out) *restarts_out = restarts;
   return QDS_OK;
+}
+
+}  // namespace
+
+// Solve A x = b (A general, CSR FP64); x holds the initial guess.  Stops when
+// ||D^-1 (b - A x)|| <= rtol ||D^-1 b|| (recursive residual, read every
+// check_every iterations) or after max_iter iterations; iters_out / relres_out
+// (host) report the result, restarts_out (host, optional) the breakdown restarts.
+QDS_API int qds_bicgstab(int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* indices, const double* vals,
+                         const double* b, double* x, double rtol, int32_t max_iter, int32_t check_every,
+                         void* workspace, int64_t ws_bytes, int32_t* iters_out, double* relres_out,
+                         int32_t* restarts_out, void* stream) {
+  QDS_CHECK_ARG(indptr && indices && vals && b && x && workspace && n > 0, "qds_bicgstab: null pointer");
+  QDS_CHECK_ARG(ws_bytes >= qds_bicgstab_workspace(n), "qds_bicgstab: workspace too small");
+  auto none = [](const double*, double*) { return QDS_OK; };
+  return bicgstab_impl(n, nnz, indptr, indices, vals, b, x, rtol, max_iter, check_every, true, none, workspace,
+                       iters_out, relres_out, restarts_out, qds_stream(stream));
+}
+
+// The same iteration right-preconditioned by one multigrid V(nu, nu)-cycle per
+// product (qds_mg_vcycle, FP64 hierarchy, one column; levels[0] holds the n
+// unknowns in lattice order, as for qds_cg_mg); the residual is measured unscaled:
+// ||b - A x|| <= rtol ||b||.
+QDS_API int qds_bicgstab_mg(int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* indices,
+                            const double* vals, const double* b, double* x, double rtol, int32_t max_iter,
+                            int32_t check_every, int32_t n_levels, const qds_mg_level* levels, int32_t nu,
+                            int32_t coarse_sweeps, double omega, void* workspace, int64_t ws_bytes,
+                            int32_t* iters_out, double* relres_out, int32_t* restarts_out, void* stream) {
+  QDS_CHECK_ARG(indptr && indices && vals && b && x && workspace && levels && n > 0, "qds_bicgstab_mg: null pointer");
+  QDS_CHECK_ARG(n_levels >= 1 && n_levels <= 32 && levels[0].nx * levels[0].ny == n,
+                "qds_bicgstab_mg: level 0 (%lld x %lld) does not hold the %lld unknowns", (long long)levels[0].nx,
+                (long long)levels[0].ny, (long long)n);
+  QDS_CHECK_ARG(ws_bytes >= qds_bicgstab_workspace(n), "qds_bicgstab_mg: workspace too small");
+  qds_mg_level lv[32];
+  for (int l = 0; l < n_levels; ++l) lv[l] = levels[l];
+  lv[0].ldf = lv[0].ldu = 1;
+  auto vcycle = [&](const double* f, double* u) {
+    lv[0].f = const_cast<double*>(f);
+    lv[0].u = u;
+    return qds_mg_vcycle(qds::F64, n_levels, lv, 1, nu, nu, coarse_sweeps, omega, stream);
+  };
+  return bicgstab_impl(n, nnz, indptr, indices, vals, b, x, rtol, max_iter, check_every, false, vcycle, workspace,
+                       iters_out, relres_out, restarts_out, qds_stream(stream));
 }

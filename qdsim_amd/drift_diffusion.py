@@ -29,11 +29,14 @@ contacts and charge; node averages for the field and the currents; row kernels t
 write the continuity and Poisson matrices straight into the P1 pattern with the
 Dirichlet rows replaced.  Linear systems:
 
-* default (``ldlt_lower=False``): the Dirichlet problems as the code writes them.
-  The Poisson step is the SPD system on the free nodes, solved by CG with a
-  multigrid V-cycle preconditioner on structured meshes (PoissonProblem); the
-  continuity systems are solved by BiCGStab (qds_bicgstab) on the row-replaced
-  matrices.
+* default (``ldlt_lower=False``): the Dirichlet problems as the code writes them,
+  restricted to the free (non-contact) nodes with the contact values lifted into
+  the load.  The Poisson step is solved by CG and the continuity systems by
+  BiCGStab, both preconditioned by a multigrid V-cycle on structured meshes whose
+  contacts are the first and last lattice columns (PoissonProblem, qds_cg_mg,
+  qds_bicgstab_mg; the continuity hierarchies discretise the diffusion part
+  D grad.grad).  Other meshes: Jacobi-CG, and Jacobi BiCGStab on the row-replaced
+  matrices (qds_bicgstab).
 * ``ldlt_lower=True``: the systems the reference's SimplicialLDLT (UpLo = Lower)
   factors, i.e. the lower triangle of each row-replaced matrix mirrored, solved by
   BiCGStab.  This reproduces the reference's numbers; the default solves the
@@ -175,6 +178,7 @@ class FullPoissonDriftDiffusionSolver:
         self.error = float("nan")
         self.last_info = {}
         self._poisson = None
+        self._cont_mg = None
         self._set_materials()
         self.initialize_carrier_concentrations()
 
@@ -215,12 +219,28 @@ class FullPoissonDriftDiffusionSolver:
         self._poisson = None
 
     def _poisson_problem(self):
-        """Free-node Poisson operator (default mode), built on first use."""
+        """Free-node Poisson operator (default mode), built on first use; its free-node
+        numbering, restricted pattern and slot map also serve the continuity systems."""
         if self._poisson is None:
             left = np.nonzero(self.contact_host == 1)[0]
             right = np.nonzero(self.contact_host == 2)[0]
             self._poisson = PoissonProblem(self.mesh, None, sign="reference", eps_elem=self._eps_elem,
                                            contacts=(left, right))
+            self._cont_mg = None
+            if self._poisson.mg is not None:
+                # preconditioners of the continuity systems: multigrid hierarchies of
+                # their diffusion parts D_e grad.grad (D_e = kT x the element mean of
+                # the materials' mobilities); the drift part, and any field-dependent
+                # mobility model, is left to the Krylov solver
+                from .lattice import Lattice, PoissonLatticeMG
+                lat = Lattice(self.mesh)
+                c = self.conn_host
+                mats = np.array([[m.mu_n, m.mu_p] for m in self._mats])
+                mu = mats[self.node_mat]
+                d = KT * (mu[c[:, 0]] + mu[c[:, 1]] + mu[c[:, 2]]) / 3.0
+                self._cont_mg = tuple(PoissonLatticeMG(lat, torch.from_numpy(d[:, k].copy()).to(self.dev))
+                                      for k in range(2))
+            self._mask = torch.from_numpy((self.contact_host != 0).astype(np.float64)).to(self.dev)
         return self._poisson
 
     # -- setters (:97-176) ------------------------------------------------------------------
@@ -311,6 +331,9 @@ class FullPoissonDriftDiffusionSolver:
     def _krylov(self, vals, b, x, what):
         it, rr, rs = kernels.bicgstab(self._indptr, self._indices, vals, self.n, self.nnz, b, x, rtol=self.rtol,
                                       max_iter=self.max_krylov)
+        return self._record(what, it, rr, rs)
+
+    def _record(self, what, it, rr, rs):
         info = {"iterations": it, "relres": rr, "restarts": rs}
         self.last_info[what] = info
         if not rr <= self.rtol:
@@ -374,8 +397,26 @@ class FullPoissonDriftDiffusionSolver:
         """Continuity step (:847-1063)."""
         mu_n, mu_p = self._mobilities()
         self.continuity_system(V_p, V_n, (mu_n, mu_p))
-        self._krylov(self._Kn, self._fn, self._phi_n, "electrons")
-        self._krylov(self._Kp, self._fp, self._phi_p, "holes")
+        prob = None if self.ldlt_lower else self._poisson_problem()
+        if prob is not None and self._cont_mg is not None:
+            # the Dirichlet problems on the free nodes (lattice order), the contact
+            # values lifted into the load: K_ff x = f_f - K_fc x_c
+            for K, f, x, mg, what in ((self._Kn, self._fn, self._phi_n, self._cont_mg[0], "electrons"),
+                                      (self._Kp, self._fp, self._phi_p, self._cont_mg[1], "holes")):
+                t = torch.empty_like(x)
+                kernels.spmm(self._indptr, self._indices, self.n, self.nnz, K, x * self._mask, t)
+                b = (f - t)[prob.free_d].contiguous()
+                xf = x[prob.free_d].contiguous()
+                K_ff = kernels.csr_take(prob.r_src, K)
+                it, rr, rs = kernels.bicgstab_mg(prob.r_indptr, prob.r_indices, K_ff, prob.free.size,
+                                                 int(prob.r_indices.numel()), b, xf, mg._structs, mg.n_levels,
+                                                 mg.nu, mg.coarse_sweeps, mg.omega, rtol=self.rtol,
+                                                 max_iter=self.max_krylov)
+                x[prob.free_d] = xf
+                self._record(what, it, rr, rs)
+        else:
+            self._krylov(self._Kn, self._fn, self._phi_n, "electrons")
+            self._krylov(self._Kp, self._fp, self._phi_p, "holes")
         self._carriers()
         self.compute_current_densities((mu_n, mu_p))
 

@@ -451,3 +451,16 @@ def bicgstab(indptr, indices, vals, n, nnz, b, x, rtol=1e-12, max_iter=20000, ch
                            int(check_every), wp, wn, ctypes.byref(it), ctypes.byref(rr), ctypes.byref(rs),
                            stream_ptr()), "qds_bicgstab")
     return it.value, rr.value, rs.value
+
+
+def bicgstab_mg(indptr, indices, vals, n, nnz, b, x, levels_struct, n_levels, nu=1, coarse=10, omega=0.8,
+                rtol=1e-12, max_iter=2000, check_every=5):
+    """BiCGStab right-preconditioned by a multigrid V-cycle (qds_bicgstab_mg)."""
+    wsb = lib.qds_bicgstab_workspace(n)
+    wp, wn = workspace("bicgstab").ptr(wsb)
+    it, rr, rs = ctypes.c_int32(), ctypes.c_double(), ctypes.c_int32()
+    check(lib.qds_bicgstab_mg(n, nnz, ptr(indptr), ptr(indices), ptr(vals), ptr(b), ptr(x), float(rtol),
+                              int(max_iter), int(check_every), int(n_levels), levels_struct, int(nu), int(coarse),
+                              float(omega), wp, wn, ctypes.byref(it), ctypes.byref(rr), ctypes.byref(rs),
+                              stream_ptr()), "qds_bicgstab_mg")
+    return it.value, rr.value, rs.value

@@ -92,7 +92,8 @@ class PoissonProblem:
         dev = sp_.dev
         self.r_indptr = torch.from_numpy(r_ip).to(dev)
         self.r_indices = torch.from_numpy(r_ix).to(dev)
-        self.K_ff = kernels.csr_take(torch.from_numpy(r_src).to(dev), K)
+        self.r_src = torch.from_numpy(r_src).to(dev)   # restricted CSR slot -> full CSR slot
+        self.K_ff = kernels.csr_take(self.r_src, K)
         self.free_d = torch.from_numpy(free).to(dev)
         self.left_d = torch.from_numpy(self.left).to(dev)
         self.right_d = torch.from_numpy(self.right).to(dev)
