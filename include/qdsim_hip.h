@@ -350,6 +350,26 @@ typedef struct {
 int qds_mg_vcycle_batched(int32_t dtype, int32_t io_dtype, int32_t n_levels, const qds_mg_level* levels,
                           const qds_mg_level_strides* strides /* host [n_levels] */, int32_t batch, int32_t b,
                           int32_t nu_pre, int32_t nu_post, int32_t coarse_sweeps, double omega, void* stream);
+/* Construction of the eigensolver's lattice hierarchy (no reference counterpart: the
+ * reference has no preconditioner).  Full-lattice FP64 arrays, row-major:
+ * WE [NY][NX-1], WN [NY-1][NX], Mterm and Mass [NY][NX].
+ * qds_mg_lattice_coefficients: level-0 edge weights (mean kinetic coefficient of
+ * the two elements touching an edge x h_perp/h_par), mass term ((mean V over four
+ * quarter points) - sigma) h_x h_y and lumped mass, from per-element c and V
+ * (element 2(j cx + i) + upper of the cell holding a point; 1/dxc, 1/dyc given).
+ * qds_mg_lattice_coarsen: NX x NY (odd) -> (NX+1)/2 x (NY+1)/2, harmonic-mean edge
+ * weights, bilinear restriction of Mterm and Mass.
+ * qds_mg_lattice_interior: the level's unknowns (interior points): d, we, wn in
+ * dtype (qds_mg_level arrays), FP64 d and lumped mass (n = (NX-2)(NY-2)).          */
+int qds_mg_lattice_coefficients(int64_t NX, int64_t NY, int64_t cx, int64_t cy, double hx, double hy,
+                                double inv_dxc, double inv_dyc, const double* c_elem, const double* v_elem,
+                                double sigma, double* WE, double* WN, double* Mterm, double* Mass, void* stream);
+int qds_mg_lattice_coarsen(int64_t NX, int64_t NY, const double* WE, const double* WN, const double* Mterm,
+                           const double* Mass, double* WEc, double* WNc, double* Mterm_c, double* Mass_c,
+                           void* stream);
+int qds_mg_lattice_interior(int32_t dtype, int64_t NX, int64_t NY, const double* WE, const double* WN,
+                            const double* Mterm, const double* Mass, void* d, void* we, void* wn, double* d64,
+                            double* mass, void* stream);
 int qds_mg_apply(int32_t dtype, const qds_mg_level* level, int32_t b, const void* u, void* out,
                  void* stream);
 

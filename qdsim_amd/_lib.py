@@ -85,6 +85,10 @@ _SIGNATURES = {
     "qds_gradient_indicator": ([_i32, _i64, _p, _p, _p, _p, _d, _p, _p], _c.c_int),
     "qds_cg_mg": ([_i64, _i64, _p, _p, _p, _p, _p, _d, _i32, _i32, _i32, _p, _i32, _i32, _d, _p, _i64,
                    _c.POINTER(_i32), _c.POINTER(_d), _p], _c.c_int),
+    "qds_mg_lattice_coefficients": ([_i64, _i64, _i64, _i64, _d, _d, _d, _d, _p, _p, _d, _p, _p, _p, _p, _p],
+                                    _c.c_int),
+    "qds_mg_lattice_coarsen": ([_i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p], _c.c_int),
+    "qds_mg_lattice_interior": ([_i32, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p], _c.c_int),
     "qds_bicgstab_workspace": ([_i64], _i64),
     "qds_bicgstab": ([_i64, _i64, _p, _p, _p, _p, _p, _d, _i32, _i32, _p, _i64, _c.POINTER(_i32),
                       _c.POINTER(_d), _c.POINTER(_i32), _p], _c.c_int),

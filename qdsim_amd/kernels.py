@@ -464,3 +464,20 @@ def bicgstab_mg(indptr, indices, vals, n, nnz, b, x, levels_struct, n_levels, nu
                               float(omega), wp, wn, ctypes.byref(it), ctypes.byref(rr), ctypes.byref(rs),
                               stream_ptr()), "qds_bicgstab_mg")
     return it.value, rr.value, rs.value
+
+
+def mg_lattice_coefficients(NX, NY, cx, cy, hx, hy, inv_dxc, inv_dyc, c_elem, v_elem, sigma, WE, WN, Mterm, Mass):
+    check(lib.qds_mg_lattice_coefficients(int(NX), int(NY), int(cx), int(cy), float(hx), float(hy), float(inv_dxc),
+                                          float(inv_dyc), ptr(c_elem), ptr(v_elem), float(sigma), ptr(WE), ptr(WN),
+                                          ptr(Mterm), ptr(Mass), stream_ptr()), "qds_mg_lattice_coefficients")
+
+
+def mg_lattice_coarsen(NX, NY, WE, WN, Mterm, Mass, WEc, WNc, Mc, Massc):
+    check(lib.qds_mg_lattice_coarsen(int(NX), int(NY), ptr(WE), ptr(WN), ptr(Mterm), ptr(Mass), ptr(WEc), ptr(WNc),
+                                     ptr(Mc), ptr(Massc), stream_ptr()), "qds_mg_lattice_coarsen")
+
+
+def mg_lattice_interior(dtype, NX, NY, WE, WN, Mterm, Mass, d, we, wn, d64, mass):
+    code_ = _lib.F32 if dtype == torch.float32 else _lib.F64
+    check(lib.qds_mg_lattice_interior(code_, int(NX), int(NY), ptr(WE), ptr(WN), ptr(Mterm), ptr(Mass), ptr(d),
+                                      ptr(we), ptr(wn), ptr(d64), ptr(mass), stream_ptr()), "qds_mg_lattice_interior")

@@ -81,52 +81,55 @@ class LatticeMG:
         self.nu_pre, self.nu_post, self.coarse_sweeps, self.omega = nu_pre, nu_post, coarse_sweeps, omega
         dev = device()
         L = lattice
-        c_elem = c_elem.to(dev, torch.float64)
-        v_elem = v_elem.to(dev, torch.float64)
+        c_elem = c_elem.to(dev, torch.float64).contiguous()
+        v_elem = v_elem.to(dev, torch.float64).contiguous()
         f64 = dict(dtype=torch.float64, device=dev)
-        # fine full-lattice edge weights and mass term
-        ix = torch.arange(L.NX, **f64)
-        iy = torch.arange(L.NY, **f64)
-        Y, X = torch.meshgrid(iy, ix, indexing="ij")
-        # horizontal edges (NY, NX-1)
-        xm = (X[:, :-1] + 0.5) * L.hx
-        yy = Y[:, :-1] * L.hy
-        ca = c_elem[L.element_at(xm, yy + 0.25 * L.hy)]
-        cb = c_elem[L.element_at(xm, yy - 0.25 * L.hy)]
-        WE = 0.5 * (ca + cb) * (L.hy / L.hx)
-        # vertical edges (NY-1, NX)
-        ym = (Y[:-1, :] + 0.5) * L.hy
-        xx = X[:-1, :] * L.hx
-        ca = c_elem[L.element_at(xx + 0.25 * L.hx, ym)]
-        cb = c_elem[L.element_at(xx - 0.25 * L.hx, ym)]
-        WN = 0.5 * (ca + cb) * (L.hx / L.hy)
-        xp = X * L.hx
-        yp = Y * L.hy
-        vs = sum(v_elem[L.element_at(xp + sx * L.hx, yp + sy * L.hy)]
-                 for sx in (-0.25, 0.25) for sy in (-0.25, 0.25)) * 0.25
-        Mterm = (vs - sigma) * (L.hx * L.hy)
-        # lumped mass of each level (the coefficient of -sigma in its diagonal)
-        Mass = torch.full_like(Mterm, L.hx * L.hy)
         self.sigma = float(sigma)
-        self.levels_data = []
+        # fine full-lattice edge weights, mass term and lumped mass (one kernel), then
+        # per level the unknowns' diagonals and weights and the next coarser lattice
         NX, NY = L.NX, L.NY
+        WE = torch.empty((NY, NX - 1), **f64)
+        WN = torch.empty((NY - 1, NX), **f64)
+        Mterm = torch.empty((NY, NX), **f64)
+        Mass = torch.empty((NY, NX), **f64)
+        kernels.mg_lattice_coefficients(NX, NY, L.cx, L.cy, L.hx, L.hy, 1.0 / L.dxc, 1.0 / L.dyc, c_elem, v_elem,
+                                        self.sigma, WE, WN, Mterm, Mass)
+        shapes = []
         while True:
-            lv = self._interior(WE, WN, Mterm, NX, NY, dev)
-            lv["mass"] = Mass[1:-1, 1:-1].contiguous()
-            self.levels_data.append(lv)
+            shapes.append((NX, NY))
             nxi, nyi = NX - 2, NY - 2
             if not (nxi % 2 == 1 and nyi % 2 == 1 and (nxi - 1) // 2 >= min_coarse
                     and (nyi - 1) // 2 >= min_coarse):
                 break
-            Mass = self._coarsen(WE, WN, Mass, NX, NY)[2]
-            WE, WN, Mterm, NX, NY = self._coarsen(WE, WN, Mterm, NX, NY)
-        self.n_levels = len(self.levels_data)
+            NX, NY = (NX + 1) // 2, (NY + 1) // 2
+        self.n_levels = len(shapes)
+        # every level's u, f, t in one zeroed buffer
+        sizes = [-(-(nx - 2) * (ny - 2) * self.b // 64) * 64 for nx, ny in shapes]   # 256-byte aligned slices
+        work = torch.zeros(3 * sum(sizes), dtype=dtype, device=dev)
+        self.levels_data = []
+        off = 0
+        for l, (NX, NY) in enumerate(shapes):
+            n = (NX - 2) * (NY - 2)
+            lv = {"nx": NX - 2, "ny": NY - 2,
+                  "d": torch.empty(n, dtype=dtype, device=dev), "we": torch.empty(n, dtype=dtype, device=dev),
+                  "wn": torch.empty(n, dtype=dtype, device=dev), "d64": torch.empty((NY - 2, NX - 2), **f64),
+                  "mass": torch.empty((NY - 2, NX - 2), **f64)}
+            kernels.mg_lattice_interior(dtype, NX, NY, WE, WN, Mterm, Mass, lv["d"], lv["we"], lv["wn"],
+                                        lv["d64"], lv["mass"])
+            for name in ("u", "f", "t"):
+                lv[name] = work[off:off + n * self.b].view(n, self.b)
+                off += sizes[l]
+            self.levels_data.append(lv)
+            if l + 1 < len(shapes):
+                NXc, NYc = shapes[l + 1]
+                WEc = torch.empty((NYc, NXc - 1), **f64)
+                WNc = torch.empty((NYc - 1, NXc), **f64)
+                Mc = torch.empty((NYc, NXc), **f64)
+                Massc = torch.empty((NYc, NXc), **f64)
+                kernels.mg_lattice_coarsen(NX, NY, WE, WN, Mterm, Mass, WEc, WNc, Mc, Massc)
+                WE, WN, Mterm, Mass = WEc, WNc, Mc, Massc
         self._structs = (_lib.MGLevel * self.n_levels)()
         for l, lv in enumerate(self.levels_data):
-            n = lv["nx"] * lv["ny"]
-            lv["u"] = torch.zeros((n, self.b), dtype=dtype, device=dev)
-            lv["f"] = torch.zeros((n, self.b), dtype=dtype, device=dev)
-            lv["t"] = torch.zeros((n, self.b), dtype=dtype, device=dev)
             s = self._structs[l]
             s.nx, s.ny = lv["nx"], lv["ny"]
             s.d, s.we, s.wn = lv["d"].data_ptr(), lv["we"].data_ptr(), lv["wn"].data_ptr()
@@ -154,18 +157,6 @@ class LatticeMG:
         lv = self.levels_data[0]
         vec = 4 if self.b % 4 == 0 else 1
         return lv["nx"] * lv["ny"] * (self.b // vec) > 24576
-
-    def _interior(self, WE, WN, Mterm, NX, NY, dev):
-        d = (WE[1:-1, :-1] + WE[1:-1, 1:] + WN[:-1, 1:-1] + WN[1:, 1:-1] + Mterm[1:-1, 1:-1])
-        we = WE[1:-1, 1:].clone()
-        we[:, -1] = 0.0
-        wn = WN[1:, 1:-1].clone()
-        wn[-1, :] = 0.0
-        return {"nx": NX - 2, "ny": NY - 2,
-                "d": d.contiguous().to(self.dtype).reshape(-1),
-                "we": we.contiguous().to(self.dtype).reshape(-1),
-                "wn": wn.contiguous().to(self.dtype).reshape(-1),
-                "d64": d}
 
     @staticmethod
     def _coarsen(WE, WN, Mterm, NX, NY):

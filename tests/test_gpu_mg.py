@@ -219,3 +219,59 @@ def test_poisson_vcycle_neumann_rows_matches_numpy(order, cells):
     ref = _np_vcycle_neumann_y(levels, f[:, None], mg.coarse_sweeps, mg.omega)[:, 0]
     err = np.abs(z.cpu().numpy() - ref).max() / np.abs(ref).max()
     assert err < 1e-11, err
+
+
+def _hierarchy_tensor_form(lat, c_elem, v_elem, sigma, min_coarse=3):
+    """The hierarchy's arrays restated with tensor operations (element lookup of
+    the quarter points, harmonic-mean edges, bilinear mass restriction)."""
+    from qdsim_amd.lattice import LatticeMG
+    L = lat
+    f64 = dict(dtype=torch.float64, device="cuda")
+    Y, X = torch.meshgrid(torch.arange(L.NY, **f64), torch.arange(L.NX, **f64), indexing="ij")
+    xm, yy = (X[:, :-1] + 0.5) * L.hx, Y[:, :-1] * L.hy
+    WE = 0.5 * (c_elem[L.element_at(xm, yy + 0.25 * L.hy)] + c_elem[L.element_at(xm, yy - 0.25 * L.hy)]) * (L.hy / L.hx)
+    ym, xx = (Y[:-1, :] + 0.5) * L.hy, X[:-1, :] * L.hx
+    WN = 0.5 * (c_elem[L.element_at(xx + 0.25 * L.hx, ym)] + c_elem[L.element_at(xx - 0.25 * L.hx, ym)]) * (L.hx / L.hy)
+    xp, yp = X * L.hx, Y * L.hy
+    vs = sum(v_elem[L.element_at(xp + sx * L.hx, yp + sy * L.hy)] for sx in (-0.25, 0.25) for sy in (-0.25, 0.25)) * 0.25
+    Mterm = (vs - sigma) * (L.hx * L.hy)
+    Mass = torch.full_like(Mterm, L.hx * L.hy)
+    levels, NX, NY = [], L.NX, L.NY
+    while True:
+        d = WE[1:-1, :-1] + WE[1:-1, 1:] + WN[:-1, 1:-1] + WN[1:, 1:-1] + Mterm[1:-1, 1:-1]
+        we = WE[1:-1, 1:].clone(); we[:, -1] = 0.0
+        wn = WN[1:, 1:-1].clone(); wn[-1, :] = 0.0
+        levels.append((d, we, wn, Mass[1:-1, 1:-1]))
+        nxi, nyi = NX - 2, NY - 2
+        if not (nxi % 2 == 1 and nyi % 2 == 1 and (nxi - 1) // 2 >= min_coarse and (nyi - 1) // 2 >= min_coarse):
+            break
+        Mass = LatticeMG._coarsen(WE, WN, Mass, NX, NY)[2]
+        WE, WN, Mterm, NX, NY = LatticeMG._coarsen(WE, WN, Mterm, NX, NY)
+    return levels
+
+
+@pytest.mark.parametrize("order,cells,dtype", [(1, (64, 48), torch.float64), (3, (32, 32), torch.float32),
+                                               (2, (40, 24), torch.float32)])
+def test_hierarchy_kernels_match_tensor_form(order, cells, dtype):
+    """The device-built hierarchy (qds_mg_lattice_*) equals the tensor restatement
+    bitwise, level by level, including the FP64 diagonals that shift changes use."""
+    from qdsim_amd.lattice import Lattice, LatticeMG
+    from qdsim_amd.mesh import SimpleMesh
+    m = SimpleMesh(cells[0] + 1, cells[1] + 1, 60e-9, 60e-9 * cells[1] / cells[0], order)
+    lat = Lattice(m)
+    rng = np.random.default_rng(3)
+    c = torch.from_numpy(0.5 + 0.5 * rng.random(m.num_elements)).cuda()
+    v = torch.from_numpy(0.2 * rng.random(m.num_elements) - 0.1).cuda()
+    mg = LatticeMG(lat, c, v, sigma=-0.15, b=8, dtype=dtype)
+    ref = _hierarchy_tensor_form(lat, c, v, -0.15)
+    assert mg.n_levels == len(ref) >= 2
+    for lv, (d, we, wn, mass) in zip(mg.levels_data, ref):
+        assert torch.equal(lv["d64"], d)
+        assert torch.equal(lv["mass"], mass)
+        assert torch.equal(lv["d"], d.reshape(-1).to(dtype))
+        assert torch.equal(lv["we"], we.reshape(-1).to(dtype))
+        assert torch.equal(lv["wn"], wn.reshape(-1).to(dtype))
+    # a shift change moves every diagonal by (old - new shift) x lumped mass
+    mg.set_sigma(-0.2)
+    for lv, (d, we, wn, mass) in zip(mg.levels_data, ref):
+        assert torch.allclose(lv["d64"], d + 0.05 * mass, rtol=1e-14, atol=0)
