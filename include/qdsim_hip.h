@@ -303,9 +303,11 @@ int qds_coldot(int32_t dtype, int64_t n, int32_t b, const void* X, int64_t ldx, 
  * qds_residual_norms); with them the call also writes the locking mask of the next
  * residual (colscale, may be NULL) and the status [batch][qds_rr_status_len(b)]:
  * [0] worst relative residual of the first k columns, [1] converged (< tol),
- * [2] dropped directions, [3] 0 Cholesky / 1 SVQB / 2 frozen, [4] Jacobi sweeps,
- * [5] non-vanished basis columns, [8 + j] residual of column j.  With freeze, a
- * converged problem keeps its block (C = [I; 0], lam unchanged).                */
+ * [2] dropped directions, [3] 0 Cholesky / 1 SVQB / 2 frozen, [4] Jacobi sweeps
+ * (0 when the tridiagonal eigensolver produced the pairs), [5] non-vanished basis
+ * columns, [6] 1 when the tridiagonal eigensolver produced the pairs (Cholesky
+ * path, m >= 10, 2 b <= m + 1, every check passed), [8 + j] residual of column j.
+ * With freeze, a converged problem keeps its block (C = [I; 0], lam unchanged). */
 int32_t qds_rr_status_len(int32_t b);
 int qds_rayleigh_ritz(int64_t batch, int32_t m, int32_t b, int32_t k, double* G, int64_t g_stride,
                       const double* norms, double* lam, double* C, double* CP, int64_t c_stride,
