@@ -103,7 +103,9 @@ def _assembly_roofline(prob, iters=10):
     and M in one element pass, then the segmented gather into CSR) and their
     algorithmic bytes: element pass = connectivity + 3 vertex coordinates + the c and
     v samples per quadrature point + two packed-symmetric FP64 outputs; gather =
-    row pointer + per contribution an index and two FP64 reads + two FP64 outputs."""
+    row pointer + per contribution its 4-byte index + the two packed element arrays
+    read once (the two triangles of an element matrix share their packed entries) +
+    two FP64 outputs per nonzero."""
     import torch
     from qdsim_amd import kernels
     sp = prob.space
@@ -131,7 +133,7 @@ def _assembly_roofline(prob, iters=10):
     nsym = sp.npe * (sp.npe + 1) // 2
     be = E * (sp.npe * 4 + 6 * 8 + 2 * sp.nq * 8 + 2 * nsym * 8)
     ncontrib = int(sp.seg_src.numel())
-    bg = 4 * (sp.nnz + 1) + ncontrib * (4 + 2 * 8) + sp.nnz * 2 * 8
+    bg = 4 * (sp.nnz + 1) + ncontrib * 4 + 2 * E * nsym * 8 + sp.nnz * 2 * 8
     return {"element_kernel": "element_mfma_kernel<%d,%d> (H and M)" % (sp.npe, sp.nq),
             "element_us": te * 1e6, "element_bytes": be, "element_GBs": be / te / 1e9,
             "gather_us": tg * 1e6, "gather_bytes": bg, "gather_GBs": bg / tg / 1e9,

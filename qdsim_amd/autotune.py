@@ -37,9 +37,22 @@ def _cache_path():
                           os.path.join(os.path.expanduser("~"), ".cache", "qdsim_amd", "lattice_tuning.json"))
 
 
+_tags = {}
+
+
 def _device_tag():
-    p = torch.cuda.get_device_properties(torch.cuda.current_device())
-    return f"{p.gcnArchName}/{p.multi_processor_count}"
+    """Architecture and CU count of the current device, queried once per device
+    under the lock: the batch drivers tune from several host threads at once, and
+    concurrent property queries failed there ("Invalid device id")."""
+    idx = torch.cuda.current_device()
+    tag = _tags.get(idx)
+    if tag is None:
+        with _lock:
+            if idx not in _tags:
+                p = torch.cuda.get_device_properties(idx)
+                _tags[idx] = f"{p.gcnArchName}/{p.multi_processor_count}"
+            tag = _tags[idx]
+    return tag
 
 
 def _key(stencil, dtype, b, fused, batch):

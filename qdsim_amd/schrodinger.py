@@ -299,13 +299,24 @@ class SchrodingerProblem:
             elif breakdown is not None and breakdown != "not converged":
                 raise np.linalg.LinAlgError(f"mixed-precision solve broke down ({breakdown}) with the retry off")
         elif mode == "fp64":
+            # the same start as the mixed solve (warm block or nested coarse solve), so
+            # that the two modes differ in precision only
             breakdown, retried = None, False
             pre = self.preconditioner(b, torch.float64, cycle=dict(nu_pre=2, nu_post=2, coarse_sweeps=40))
             sync()
             t_prep = time.perf_counter() - t0
-            t_nested = 0.0
+            t_n = time.perf_counter()
             self._nested_iters = 0
-            X0 = start["X_block"] if _warm(start, self.n_free, b) else None
+            self._nested_theta = None
+            sigma0 = pre.sigma
+            if _warm(start, self.n_free, b):
+                X0 = start["X_block"][:, :b].contiguous()
+            else:
+                X0 = self._nested_start(k, b, **(nested_opts or {}))
+            if self._nested_theta is not None:
+                pre.set_sigma(shift_below(self._nested_theta, sigma0, 0.25))
+            sync()
+            t_nested = time.perf_counter() - t_n
             res = lobpcg(op64, k, b, pre, X0=X0, tol=tol, maxiter=maxiter, seed=seed, verbose=verbose)
         else:
             raise ValueError(f"unknown mode {mode}")
