@@ -218,9 +218,10 @@ def lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=None
     """rr: "host" (LAPACK Rayleigh-Ritz after one device-to-host copy per iteration)
     or "device" (qds_rayleigh_ritz on the GPU; the host only reads each
     iteration's status one iteration behind).  For one problem the host path is
-    faster: the single-workgroup Jacobi of the device path takes ~2.3 ms for the
-    72 x 72 projected pencil against ~0.3 ms of LAPACK plus ~0.6 ms of round trip;
-    the device path pays off for batches (one launch for all problems)."""
+    faster: the single-workgroup kernel of the device path takes ~0.58 ms for the
+    72 x 72 projected pencil (a chain of ~5m dependent steps) against ~0.3 ms of
+    single-threaded LAPACK plus a ~25 us round trip; the device path pays off for
+    batches (one launch for all problems)."""
     if rr == "device" and 3 * b <= 96:
         return _lobpcg_device(op, k, b, precond, X0, tol, maxiter, drop_tol, seed, verbose, stall_iters, refresh)
     # the host Rayleigh-Ritz works on matrices of at most 96 x 192: multithreaded
