@@ -35,6 +35,13 @@ extern "C" {
 const char* qds_last_error(void);
 int qds_version(void);
 int qds_device_info(char* arch /* host */, int32_t arch_len, int32_t* cu_count /* host */);
+/* No counterpart in the reference.  For callers that load the library themselves
+ * (dlopen, ctypes) and could end up with two HIP runtimes in one process: launches a
+ * probe kernel through the library's runtime and reads it back; QDS_EHIP, with the
+ * mapped libamdhip64 images named in qds_last_error(), when that fails.  Allocates a
+ * 4-byte device buffer.  qds_runtime_images: number of libamdhip64 images mapped. */
+int qds_runtime_check(void);
+int qds_runtime_images(void);
 
 /* ---- CSR pattern and scatter maps (host) -------------------------------------
  * Replaces the COO->CSR conversion of the element triplets:

@@ -29,6 +29,8 @@ _SIGNATURES = {
     "qds_last_error": ([], _c.c_char_p),
     "qds_version": ([], _c.c_int),
     "qds_device_info": ([_c.c_char_p, _i32, _c.POINTER(_i32)], _c.c_int),
+    "qds_runtime_check": ([], _c.c_int),
+    "qds_runtime_images": ([], _c.c_int),
     "qds_pattern_count": ([_p, _i64, _i32, _i64, _c.POINTER(_i64)], _c.c_int),
     "qds_pattern_fill": ([_p, _i64, _i32, _i64, _i64, _p, _p, _p, _p, _p], _c.c_int),
     "qds_pattern_restrict_count": ([_p, _p, _i64, _p, _i64, _c.POINTER(_i64)], _c.c_int),

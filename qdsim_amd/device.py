@@ -14,9 +14,20 @@ DTYPE_CODE = {torch.float32: _lib.F32, torch.float64: _lib.F64,
               torch.complex64: _lib.C64, torch.complex128: _lib.C128}
 
 
+_runtime_checked = False
+
+
 def device():
+    """The current GPU.  The first call also checks that the library's kernels run
+    through the HIP runtime torch uses (qds_runtime_check: a probe kernel launched
+    and read back), so that a process holding two HIP runtimes fails here with the
+    runtime images named instead of at the first kernel."""
+    global _runtime_checked
     if not torch.cuda.is_available():
         raise RuntimeError("qdsim_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    if not _runtime_checked:
+        _lib.check(_lib.lib.qds_runtime_check(), "qds_runtime_check")
+        _runtime_checked = True
     return torch.device("cuda", torch.cuda.current_device())
 
 

@@ -217,3 +217,11 @@ def test_residual_norms(dtype, b):
     Rd = R.double().cpu().numpy()
     assert np.allclose(rn.cpu().numpy(), (Rd ** 2).sum(0), rtol=1e-10)
     assert np.allclose(mn.cpu().numpy(), (m ** 2).sum(0), rtol=1e-10)
+
+
+def test_runtime_check_runs_a_probe_kernel():
+    """The library's kernels run through the same HIP runtime as torch's buffers (one
+    libamdhip64 image mapped in this process)."""
+    from qdsim_amd import _lib
+    assert _lib.lib.qds_runtime_check() == 0, _lib.lib.qds_last_error()
+    assert _lib.lib.qds_runtime_images() == 1
