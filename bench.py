@@ -424,8 +424,10 @@ def main():
         times.append(time.perf_counter() - t0)
         info["eigenvalues"] = lam
         infos.append(info)
-        print(f"[bench] step {s}: {times[-1]*1e3:.1f} ms, fp32 iters {info['timings'].get('fp32_iters')}, "
-              f"fp64 iters {info['timings'].get('fp64_iters')}, converged {info['converged']}", flush=True)
+        tm = info["timings"]
+        print(f"[bench] step {s}: {times[-1]*1e3:.1f} ms, nested iters {tm.get('nested_iters')}, mixed iters "
+              f"{tm.get('mixed_iters')}, fp32 iters {tm.get('fp32_iters')}, fp64 iters {tm.get('fp64_iters')}, "
+              f"converged {info['converged']}", flush=True)
     total = sum(times)
 
     # assembly alone (element kernel + gather), HIP events on the launch stream
@@ -499,7 +501,8 @@ def main():
         "assembly_kernels": asm,
         "setup_s": setup_s,
         "solver_timings": infos[-1]["timings"],
-        "spmm_tuning": __import__("qdsim_amd.autotune", fromlist=["report"]).report(),
+        "spmm_tuning": {k: v for k, v in __import__("qdsim_amd.autotune", fromlist=["report"]).report().items()
+                        if f"|{args.cells * 3 - 1}x{args.cells * 3 - 1}|" in k},
     }
     print(json.dumps(out))
 

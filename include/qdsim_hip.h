@@ -293,6 +293,28 @@ int qds_residual_norms_batched(int32_t dtype, int64_t n, int32_t b, int32_t batc
                                const void* MX, int64_t ldm, int64_t x_stride, const double* lam,
                                const double* colscale, void* R, int64_t ldr, double* out, void* workspace,
                                int64_t ws_bytes, void* stream);
+/* Mixed-precision forms (no reference counterpart; they replace the same dense
+ * steps of scipy's lobpcg, eigen_solver.pyx:254-266, for a basis whose blocks
+ * differ in storage type: X and its images in FP64, the corrections W, P and
+ * their images in FP32).
+ * qds_residual_norms_to: qds_residual_norms with R stored in r_dtype (F32 from
+ *   F64 images, or the images' type); the norms are FP64 sums of the FP64 residual.
+ * qds_gram_blocks_typed: qds_gram_blocks for operands whose block j has storage
+ *   type *_dt[j] (F32 / F64); rows converted to FP64 when staged, FP64 products
+ *   and sums; workspace qds_gram_workspace.
+ * qds_block_update_typed: Y1 = S C1 in FP64 and Y2 = S C2 in y2_dtype (F32 / F64)
+ *   for blocks of storage type s_dt[j]; FP64 products and sums, m <= 96, k <= 64,
+ *   blocks of at most 32 (F64) / 64 (F32) columns, 16-byte aligned.              */
+int qds_residual_norms_to(int32_t dtype, int32_t r_dtype, int64_t n, int32_t b, const void* HX, int64_t ldh,
+                          const void* MX, int64_t ldm, const double* lam, const double* colscale, void* R,
+                          int64_t ldr, double* out, void* workspace, int64_t ws_bytes, void* stream);
+int qds_gram_blocks_typed(int64_t n, const qds_colblocks* A /* host */, const int32_t* a_dt /* host */,
+                          const qds_colblocks* B1 /* host */, const int32_t* b1_dt /* host */,
+                          const qds_colblocks* B2 /* host */, const int32_t* b2_dt /* host */, int32_t sym,
+                          double* C, void* workspace, int64_t ws_bytes, void* stream);
+int qds_block_update_typed(int64_t n, const qds_colblocks* S /* host */, const int32_t* s_dt /* host */, int32_t k,
+                           const double* C1, void* Y1, int64_t ldy1, const double* C2, int32_t y2_dtype, void* Y2,
+                           int64_t ldy2, int32_t c2_first, void* stream);
 int64_t qds_coldot_workspace(int32_t b);
 int qds_coldot(int32_t dtype, int64_t n, int32_t b, const void* X, int64_t ldx, const void* Y,
                int64_t ldy, int32_t conj, double* out, void* workspace, int64_t ws_bytes,

@@ -70,6 +70,10 @@ _SIGNATURES = {
     "qds_residual_norms": ([_i32, _i64, _i32, _p, _i64, _p, _i64, _p, _p, _p, _i64, _p, _p, _i64, _p],
                            _c.c_int),
     "qds_coldot": ([_i32, _i64, _i32, _p, _i64, _p, _i64, _i32, _p, _p, _i64, _p], _c.c_int),
+    "qds_residual_norms_to": ([_i32, _i32, _i64, _i32, _p, _i64, _p, _i64, _p, _p, _p, _i64, _p, _p, _i64, _p],
+                              _c.c_int),
+    "qds_gram_blocks_typed": ([_i64, _p, _p, _p, _p, _p, _p, _i32, _p, _p, _i64, _p], _c.c_int),
+    "qds_block_update_typed": ([_i64, _p, _p, _i32, _p, _p, _i64, _p, _i32, _p, _i64, _i32, _p], _c.c_int),
     "qds_mg_vcycle": ([_i32, _i32, _p, _i32, _i32, _i32, _i32, _d, _p], _c.c_int),
     "qds_mg_vcycle_io": ([_i32, _i32, _i32, _p, _i32, _i32, _i32, _i32, _d, _p], _c.c_int),
     "qds_mg_vcycle_batched": ([_i32, _i32, _i32, _p, _p, _i32, _i32, _i32, _i32, _i32, _d, _p], _c.c_int),

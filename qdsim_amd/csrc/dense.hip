@@ -84,14 +84,17 @@ struct GramStage {
   int w[kGramPieces];              // columns
   int op[kGramPieces];             // 0 A, 1 B1, 2 B2
   int col0[kGramPieces];           // first column of the piece inside its operand
+  int es[kGramPieces];             // TYPED kernels: bytes per element of the piece (4 FP32, 8 FP64)
 };
 
-template <typename T, bool VEC, int MAXP, int MAXT>
+// TYPED: every piece has its own storage type (stg.es), read with 16-byte loads
+// (4 FP32 or 2 FP64 values per load) and converted to FP64 in LDS; T is unused.
+template <typename T, bool VEC, int MAXP, int MAXT, bool TYPED = false>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
 gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, int g0, GramStage stg,
              double* __restrict__ partial, int64_t s_stride) {
   using L = GLoad<T, VEC>;
-  using V = typename L::v;
+  using V = typename std::conditional<TYPED, uint4, typename L::v>::type;
   constexpr int VW = L::W;
   extern __shared__ double gsm[];
   const int tk = g.tk1 + g.tk2;
@@ -125,10 +128,11 @@ gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, int g0, Gram
     loff[q] = -1;
     if (q >= stg.n) continue;
     const int w = stg.w[q];
-    const int vpr = w / VW;
+    const int vwq = TYPED ? 16 / stg.es[q] : VW;
+    const int vpr = w / vwq;
     const int v = threadIdx.x;
     if (v >= kRB * vpr) continue;
-    const int rr = v / vpr, cc = (v - rr * vpr) * VW;
+    const int rr = v / vpr, cc = (v - rr * vpr) * vwq;
     int lcol;
     if (stg.op[q] == 0) {
       lcol = stg.col0[q] + cc;
@@ -162,17 +166,34 @@ gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, int g0, Gram
       pre[q] = V{};
       if (goff[q] < 0) continue;
       if (!full && r0 + loff[q] / ldsw >= r_end) continue;   // tail chunk: rows past the slab stay 0
-      const T* base = reinterpret_cast<const T*>(stg.src[q]) + blockIdx.z * s_stride + r0 * stg.ld[q];
-      pre[q] = *reinterpret_cast<const V*>(base + goff[q]);
+      if constexpr (TYPED) {
+        const char* base = reinterpret_cast<const char*>(stg.src[q]) + (r0 * stg.ld[q] + goff[q]) * stg.es[q];
+        pre[q] = *reinterpret_cast<const V*>(base);
+      } else {
+        const T* base = reinterpret_cast<const T*>(stg.src[q]) + blockIdx.z * s_stride + r0 * stg.ld[q];
+        pre[q] = *reinterpret_cast<const V*>(base + goff[q]);
+      }
     }
   };
   auto store = [&]() {
 #pragma unroll
     for (int q = 0; q < MAXP; ++q) {
       if (loff[q] < 0) continue;
-      const T* v = reinterpret_cast<const T*>(&pre[q]);
+      if constexpr (TYPED) {
+        if (stg.es[q] == 4) {
+          const float* v = reinterpret_cast<const float*>(&pre[q]);
 #pragma unroll
-      for (int e = 0; e < VW; ++e) gsm[loff[q] + e] = (double)v[e];
+          for (int e = 0; e < 4; ++e) gsm[loff[q] + e] = (double)v[e];
+        } else {
+          const double* v = reinterpret_cast<const double*>(&pre[q]);
+          gsm[loff[q]] = v[0];
+          gsm[loff[q] + 1] = v[1];
+        }
+      } else {
+        const T* v = reinterpret_cast<const T*>(&pre[q]);
+#pragma unroll
+        for (int e = 0; e < VW; ++e) gsm[loff[q] + e] = (double)v[e];
+      }
     }
   };
   if (r_begin < r_end) load(r_begin);
@@ -464,6 +485,126 @@ int launch_update(int64_t n, int m, int k, const ColBlocks& S, int c2_first, con
   return QDS_OK;
 }
 
+// ---------------------------------------------------------------- typed update
+// Y1 = S C1 in FP64 and Y2 = S C2 in TY2 for column blocks of their own storage
+// type (es[j] bytes per element): the update of the mixed-precision iteration, whose
+// basis [X | W | P] keeps X in FP64 and the corrections W, P in FP32.  The layout
+// and pipelining follow block_update_kernel; every product runs on the FP64 MFMA
+// (FP32 operands are converted exactly), C is staged in LDS as FP64.  A lane's
+// 16-byte load holds VEC = 4 (FP32) or 2 (FP64) consecutive columns of its row,
+// and the contraction index of MFMA step e is the load's column e.
+struct TypedBlocks {
+  const void* p[4];
+  int64_t ld[4];
+  int32_t start[5];
+  int32_t es[4];
+  int32_t nb;
+};
+
+template <int OT, int MAXJ, typename TY2, bool TWO>
+__global__ void __launch_bounds__(kBlock)
+block_update_typed_kernel(int64_t n, int m, int k, TypedBlocks S, const double* __restrict__ C1,
+                          const double* __restrict__ C2, double* __restrict__ Y1, int64_t ldy1,
+                          TY2* __restrict__ Y2, int64_t ldy2, int crows, int c2_first) {
+  constexpr int KP = OT * 16;
+  extern __shared__ unsigned char upd_smem[];
+  double* sC = reinterpret_cast<double*>(upd_smem);   // [NOUT][crows][KP], zero padded
+  constexpr int NOUT = TWO ? 2 : 1;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int q = lane >> 4;
+  const int rl = lane & 15;
+  for (int t = threadIdx.x; t < NOUT * crows * KP; t += kBlock) {
+    const int o = t / (crows * KP);
+    const int rem = t - o * crows * KP;
+    const int i = rem / KP, j = rem - i * KP;
+    const double* C = o == 0 ? C1 : C2;
+    sC[t] = (i < m && j < k) ? C[i * k + j] : 0.0;
+  }
+  __syncthreads();
+  const int64_t rows_per_pass = (int64_t)kWaves * 16;
+  const int64_t first = (int64_t)blockIdx.x * rows_per_pass;
+  const int64_t stride = (int64_t)gridDim.x * rows_per_pass;
+  uint4 nxt[MAXJ];
+  auto fetch = [&](int64_t base, int j) {
+    int64_t row = base + wave * 16 + rl;
+    row = row < n ? row : n - 1;
+    const int w = S.start[j + 1] - S.start[j];
+    const int es = S.es[j];
+    const int vec = 16 / es;
+    const char* rp = reinterpret_cast<const char*>(S.p[j]) + row * S.ld[j] * es;
+#pragma unroll
+    for (int jj = 0; jj < MAXJ; ++jj) {
+      const int c = jj * 4 * vec + q * vec;
+      nxt[jj] = *reinterpret_cast<const uint4*>(rp + (size_t)(c < w ? c : 0) * es);
+    }
+  };
+  if (first < n) fetch(first, 0);
+  for (int64_t base = first; base < n; base += stride) {
+    v4d acc[NOUT][OT];
+#pragma unroll
+    for (int o = 0; o < NOUT; ++o)
+#pragma unroll
+      for (int t = 0; t < OT; ++t) acc[o][t] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < S.nb; ++j) {
+      uint4 cur[MAXJ];
+#pragma unroll
+      for (int jj = 0; jj < MAXJ; ++jj) cur[jj] = nxt[jj];
+      const int w = S.start[j + 1] - S.start[j];
+      const int cj = S.start[j];
+      const bool f32 = S.es[j] == 4;
+      if (j + 1 < S.nb) fetch(base, j + 1);
+      else if (base + stride < n) fetch(base + stride, 0);
+      const bool two = TWO && cj + w > c2_first;
+      const int vec = f32 ? 4 : 2;
+      const int kb = 4 * vec;
+#pragma unroll
+      for (int jj = 0; jj < MAXJ; ++jj) {
+        if (jj * kb >= w) break;
+        __builtin_amdgcn_sched_barrier(0);
+        const double keep = jj * kb + q * vec < w ? 1.0 : 0.0;
+        double sv[4];
+        if (f32) {
+          const float* fv = reinterpret_cast<const float*>(&cur[jj]);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) sv[e] = (double)fv[e] * keep;
+        } else {
+          const double* dv = reinterpret_cast<const double*>(&cur[jj]);
+          sv[0] = dv[0] * keep;
+          sv[1] = dv[1] * keep;
+          sv[2] = sv[3] = 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          if (e >= vec) break;
+          const int kl = cj + jj * kb + q * vec + e;   // C row (zero past m)
+#pragma unroll
+          for (int t = 0; t < OT; ++t) acc[0][t] = mfma_f64(sv[e], sC[kl * KP + t * 16 + rl], acc[0][t]);
+          if (two) {
+#pragma unroll
+            for (int t = 0; t < OT; ++t)
+              acc[NOUT - 1][t] = mfma_f64(sv[e], sC[(crows + kl) * KP + t * 16 + rl], acc[NOUT - 1][t]);
+          }
+        }
+      }
+    }
+    const int64_t r0 = base + wave * 16;
+#pragma unroll
+    for (int t = 0; t < OT; ++t) {
+      const int col = t * 16 + rl;
+      if (col >= k) continue;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t orow = r0 + q + 4 * e;   // v_mfma_f64_16x16x4f64 result row
+        if (orow < n) {
+          Y1[orow * ldy1 + col] = acc[0][t][e];
+          if (TWO) Y2[orow * ldy2 + col] = (TY2)acc[NOUT - 1][t][e];
+        }
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- residual / dots
 template <typename T>
 __global__ void residual_kernel(int64_t n, int b, const T* __restrict__ HX, int64_t ldh,
@@ -640,36 +781,80 @@ namespace {
 // the per-slab partial sums in `partial`.  The row slabs are gram_blocks(n), at
 // most max_slabs of them; partial_bytes == 0 takes the caller's qds_gram_workspace
 // size check instead of the slab cap.
+// Pieces of one operand whose blocks each have their own storage type (dts[j]):
+// 16-byte vector access, pieces of at most 16 vectors (64 FP32 / 32 FP64 columns).
+bool typed_pieces(const qds_colblocks* h, const int32_t* dts, int op, GramStage& stg, int& total) {
+  if (!h || !dts || h->nb < 1 || h->nb > 4) return false;
+  int acc = 0;
+  for (int j = 0; j < h->nb; ++j) {
+    if (dts[j] != qds::F32 && dts[j] != qds::F64) return false;
+    const int es = qds::dtype_size(dts[j]);
+    const int vw = 16 / es, wcap = 16 * vw;
+    if (!h->p[j] || h->cols[j] < 1 || h->ld[j] < h->cols[j] || (reinterpret_cast<uintptr_t>(h->p[j]) % 16) ||
+        (h->ld[j] * es) % 16 || h->cols[j] % vw)
+      return false;
+    for (int c = 0; c < h->cols[j]; c += wcap) {
+      if (stg.n == kGramPieces) return false;
+      const int w = h->cols[j] - c < wcap ? h->cols[j] - c : wcap;
+      stg.src[stg.n] = static_cast<const char*>(h->p[j]) + (size_t)c * es;
+      stg.ld[stg.n] = h->ld[j];
+      stg.w[stg.n] = w;
+      stg.op[stg.n] = op;
+      stg.col0[stg.n] = acc;
+      stg.es[stg.n] = es;
+      acc += w;
+      ++stg.n;
+    }
+  }
+  total = acc;
+  return true;
+}
+
+// dts (optional): per-operand arrays of per-block storage types (F32 / F64) for
+// operands whose blocks differ in type; then dtype is ignored, batch must be 1
+// and every block needs 16-byte vector access.
 int gram_blocks_impl(int32_t dtype, int64_t n, const qds_colblocks* A, const qds_colblocks* B1,
                      const qds_colblocks* B2, int32_t sym, double* C, double* partial, int64_t ws_bytes,
-                     int64_t partial_bytes, void* stream, int batch = 1, int64_t s_stride = 0) {
+                     int64_t partial_bytes, void* stream, int batch = 1, int64_t s_stride = 0,
+                     const int32_t* const* dts = nullptr) {
   QDS_CHECK_ARG(batch >= 1 && batch <= 65535, "qds_gram: batch %d out of range", batch);
-  QDS_CHECK_ARG(dtype == qds::F32 || dtype == qds::F64, "qds_gram: dtype must be F32 or F64");
+  const bool typed = dts != nullptr;
+  QDS_CHECK_ARG(typed || dtype == qds::F32 || dtype == qds::F64, "qds_gram: dtype must be F32 or F64");
+  QDS_CHECK_ARG(!typed || batch == 1, "qds_gram: typed blocks need batch 1");
   QDS_CHECK_ARG(C && partial && A && B1, "qds_gram: null pointer");
   const int es = dtype == qds::F32 ? 4 : 8;
-  const bool vec = blocks_vec(A, es) && blocks_vec(B1, es) && (!B2 || blocks_vec(B2, es));
-  const int wcap = vec ? 16 * (16 / es) : 16;   // piece: at most one vector per thread for kRB rows
-  ColBlocks dA, dB1, dB2;
+  bool vec = true;
   int m = 0, k1 = 0, k2 = 0;
-  QDS_CHECK_ARG(to_device_blocks(A, dA, m, es, wcap) && to_device_blocks(B1, dB1, k1, es, wcap),
-                "qds_gram: bad or too wide A / B1 blocks (at most %d pieces of %d columns)", kMaxPieces, wcap);
-  dB2.nb = 0;
-  if (B2) QDS_CHECK_ARG(to_device_blocks(B2, dB2, k2, es, wcap), "qds_gram: bad or too wide B2 blocks");
+  GramStage stg;
+  stg.n = 0;
+  if (typed) {
+    QDS_CHECK_ARG(typed_pieces(A, dts[0], 0, stg, m) && typed_pieces(B1, dts[1], 1, stg, k1) &&
+                      (!B2 || typed_pieces(B2, dts[2], 2, stg, k2)),
+                  "qds_gram: typed blocks need F32/F64 types, 16-byte alignment, widths and leading dimensions "
+                  "a multiple of the vector width, at most %d pieces", kGramPieces);
+  } else {
+    vec = blocks_vec(A, es) && blocks_vec(B1, es) && (!B2 || blocks_vec(B2, es));
+    const int wcap = vec ? 16 * (16 / es) : 16;   // piece: at most one vector per thread for kRB rows
+    ColBlocks dA, dB1, dB2;
+    QDS_CHECK_ARG(to_device_blocks(A, dA, m, es, wcap) && to_device_blocks(B1, dB1, k1, es, wcap),
+                  "qds_gram: bad or too wide A / B1 blocks (at most %d pieces of %d columns)", kMaxPieces, wcap);
+    dB2.nb = 0;
+    if (B2) QDS_CHECK_ARG(to_device_blocks(B2, dB2, k2, es, wcap), "qds_gram: bad or too wide B2 blocks");
+    const ColBlocks* ops[3] = {&dA, &dB1, &dB2};
+    for (int o = 0; o < 3; ++o)
+      for (int j = 0; j < ops[o]->nb; ++j) {
+        stg.src[stg.n] = ops[o]->p[j];
+        stg.ld[stg.n] = ops[o]->ld[j];
+        stg.w[stg.n] = ops[o]->start[j + 1] - ops[o]->start[j];
+        stg.op[stg.n] = o;
+        stg.col0[stg.n] = ops[o]->start[j];
+        stg.es[stg.n] = es;
+        ++stg.n;
+      }
+  }
   QDS_CHECK_ARG(!sym || (m == k1 && (k2 == 0 || k2 == m)), "qds_gram: sym needs square halves");
   GramShape g = make_shape(m, k1, k2, sym);
   QDS_CHECK_ARG(g.tm <= 255 && g.tk1 + g.tk2 <= 255, "qds_gram: block too wide");
-  GramStage stg;
-  stg.n = 0;
-  const ColBlocks* ops[3] = {&dA, &dB1, &dB2};
-  for (int o = 0; o < 3; ++o)
-    for (int j = 0; j < ops[o]->nb; ++j) {
-      stg.src[stg.n] = ops[o]->p[j];
-      stg.ld[stg.n] = ops[o]->ld[j];
-      stg.w[stg.n] = ops[o]->start[j + 1] - ops[o]->start[j];
-      stg.op[stg.n] = o;
-      stg.col0[stg.n] = ops[o]->start[j];
-      ++stg.n;
-    }
   const int tk = g.tk1 + g.tk2;
   const int k1pad = 16 * g.tk1;
   // column groups: at most 16 accumulator tiles per wave, the staged rows in 64 KiB
@@ -723,12 +908,20 @@ int gram_blocks_impl(int32_t dtype, int64_t n, const qds_colblocks* A, const qds
   auto launch = [&](auto tag, auto vecc, const GramStage& sg, int g0, int ny) {
     using T = decltype(tag);
     constexpr bool VV = decltype(vecc)::value;
-    if (sg.n <= 9)
+    if (typed) {
+      if (sg.n <= 9)
+        hipLaunchKernelGGL((gram_partial<double, true, 9, 8, true>), dim3(nblk, ny, batch), dim3(kBlock), shm, st,
+                           n, rows_per_blk, g, tpg, g0, sg, partial, s_stride);
+      else
+        hipLaunchKernelGGL((gram_partial<double, true, kGramPieces, 8, true>), dim3(nblk, ny, batch), dim3(kBlock),
+                           shm, st, n, rows_per_blk, g, tpg, g0, sg, partial, s_stride);
+    } else if (sg.n <= 9) {
       hipLaunchKernelGGL((gram_partial<T, VV, 9, 8>), dim3(nblk, ny, batch), dim3(kBlock), shm, st, n,
                          rows_per_blk, g, tpg, g0, sg, partial, s_stride);
-    else
+    } else {
       hipLaunchKernelGGL((gram_partial<T, VV, kGramPieces, 8>), dim3(nblk, ny, batch), dim3(kBlock), shm, st, n,
                          rows_per_blk, g, tpg, g0, sg, partial, s_stride);
+    }
   };
   auto launch_all = [&](auto tag, auto vecc) {
     if (stg.n <= 9 || ngroups == 1) {
@@ -749,6 +942,7 @@ int gram_blocks_impl(int32_t dtype, int64_t n, const qds_colblocks* A, const qds
         sg.w[sg.n] = stg.w[q];
         sg.op[sg.n] = stg.op[q];
         sg.col0[sg.n] = stg.col0[q];
+        sg.es[sg.n] = stg.es[q];
         ++sg.n;
       }
       launch(tag, vecc, sg, gi, 1);
@@ -766,6 +960,20 @@ int gram_blocks_impl(int32_t dtype, int64_t n, const qds_colblocks* A, const qds
   return QDS_OK;
 }
 }  // namespace
+
+// The Gram of operands whose column blocks differ in storage type: a_dt, b1_dt,
+// b2_dt give the type (F32 / F64) of each block of A, B1, B2.  Rows are converted
+// to FP64 when staged; the products and sums are FP64 as in qds_gram_blocks (an
+// all-F64 or all-F32 call gives the same result as qds_gram_blocks).  Workspace:
+// qds_gram_workspace.
+QDS_API int qds_gram_blocks_typed(int64_t n, const qds_colblocks* A, const int32_t* a_dt, const qds_colblocks* B1,
+                                  const int32_t* b1_dt, const qds_colblocks* B2, const int32_t* b2_dt, int32_t sym,
+                                  double* C, void* workspace, int64_t ws_bytes, void* stream) {
+  QDS_CHECK_ARG(workspace && a_dt && b1_dt && (!B2 || b2_dt), "qds_gram_blocks_typed: null pointer");
+  const int32_t* dts[3] = {a_dt, b1_dt, b2_dt};
+  return gram_blocks_impl(qds::F64, n, A, B1, B2, sym, C, static_cast<double*>(workspace), ws_bytes, 0, stream, 1,
+                          0, dts);
+}
 
 // C (m x (k1+k2), row-major FP64) = A^T [B1 | B2] for column-block matrices.
 // B2 may be NULL.  sym = 1 asserts each half is symmetric (m == k1 == k2) and
@@ -903,6 +1111,78 @@ QDS_API int qds_block_update(int32_t dtype, int64_t n, int32_t m, int32_t k, con
   return qds_block_update_blocks(dtype, n, &s, k, C1, Y1, ldy1, C2, Y2, ldy2, 0, stream);
 }
 
+// Y1 = S C1 (FP64) and, when C2 != NULL, Y2 = S C2 (stored in y2_dtype, F32 or
+// F64) for column blocks whose storage types differ (s_dt[j] for block j, F32 or
+// F64): the update of the mixed-precision iteration (X_new in FP64, P_new in FP32
+// from [X | W | P]).  FP64 products and sums; m <= 96 (C resident in LDS), k <= 64;
+// blocks 16-byte aligned with widths and leading dimensions a multiple of the
+// vector width (4 F32, 2 F64), at most 32 (F64) / 64 (F32) columns each.
+// c2_first: rows of C2 before it are zero.
+QDS_API int qds_block_update_typed(int64_t n, const qds_colblocks* S, const int32_t* s_dt, int32_t k,
+                                   const double* C1, void* Y1, int64_t ldy1, const double* C2, int32_t y2_dtype,
+                                   void* Y2, int64_t ldy2, int32_t c2_first, void* stream) {
+  QDS_CHECK_ARG(S && s_dt && S->nb >= 1 && S->nb <= 4, "qds_block_update_typed: bad S");
+  QDS_CHECK_ARG(C1 && Y1 && (!C2 || Y2), "qds_block_update_typed: null pointer");
+  QDS_CHECK_ARG(!C2 || y2_dtype == qds::F32 || y2_dtype == qds::F64, "qds_block_update_typed: bad y2_dtype");
+  QDS_CHECK_ARG(k >= 1 && k <= 16 * kMaxOutTiles, "qds_block_update_typed: bad shape");
+  TypedBlocks d;
+  int m = 0, maxj = 1;
+  for (int j = 0; j < 4; ++j) { d.p[j] = nullptr; d.ld[j] = 0; d.es[j] = 8; }
+  for (int j = 0; j < S->nb; ++j) {
+    QDS_CHECK_ARG(s_dt[j] == qds::F32 || s_dt[j] == qds::F64, "qds_block_update_typed: block %d type", j);
+    const int es = qds::dtype_size(s_dt[j]);
+    const int vw = 16 / es;
+    QDS_CHECK_ARG(S->p[j] && S->cols[j] >= 1 && S->cols[j] <= 16 * vw && S->ld[j] >= S->cols[j] &&
+                      (reinterpret_cast<uintptr_t>(S->p[j]) % 16) == 0 && (S->ld[j] * es) % 16 == 0 &&
+                      S->cols[j] % vw == 0,
+                  "qds_block_update_typed: block %d needs 16-byte alignment, width and leading dimension a "
+                  "multiple of %d, at most %d columns", j, vw, 16 * vw);
+    d.p[j] = S->p[j];
+    d.ld[j] = S->ld[j];
+    d.es[j] = es;
+    d.start[j] = m;
+    m += S->cols[j];
+    const int loads = (S->cols[j] + 4 * vw - 1) / (4 * vw);
+    maxj = loads > maxj ? loads : maxj;
+  }
+  for (int j = S->nb; j <= 4; ++j) d.start[j] = m;
+  d.nb = S->nb;
+  QDS_CHECK_ARG(c2_first >= 0 && c2_first <= m, "qds_block_update_typed: bad c2_first");
+  const int ot = (k + 15) / 16;
+  const int kp = ot * 16;
+  const int crows = m + 16;   // rows past m read zeros
+  const int nout = C2 ? 2 : 1;
+  const size_t shm = (size_t)nout * crows * kp * sizeof(double);
+  QDS_CHECK_ARG(shm <= (size_t)kUpdLds, "qds_block_update_typed: C of %d x %d too large for LDS", m, k);
+  if (n == 0) return QDS_OK;
+  hipStream_t st = qds_stream(stream);
+  const int64_t passes = (n + kWaves * 16 - 1) / (kWaves * 16);
+  const unsigned blocks = qds_blocks(passes, 1, 4096);
+  QDS_CHECK_ARG(maxj <= 4, "qds_block_update_typed: blocks too wide");
+#define QDS_UPDT(OTV, J, TY, TW)                                                                            \
+  hipLaunchKernelGGL((block_update_typed_kernel<OTV, J, TY, TW>), dim3(blocks), dim3(kBlock), shm, st, n, m, k, d, \
+                     C1, C2, (double*)Y1, ldy1, (TY*)Y2, ldy2, crows, C2 ? c2_first : 0)
+#define QDS_UPDT_Y(OTV, J)                                                                                  \
+  do {                                                                                                       \
+    if (!C2) QDS_UPDT(OTV, J, double, false);                                                                \
+    else if (y2_dtype == qds::F32) QDS_UPDT(OTV, J, float, true);                                            \
+    else QDS_UPDT(OTV, J, double, true);                                                                     \
+  } while (0)
+#define QDS_UPDT_J(OTV) \
+  do { if (maxj <= 2) QDS_UPDT_Y(OTV, 2); else if (maxj == 3) QDS_UPDT_Y(OTV, 3); else QDS_UPDT_Y(OTV, 4); } while (0)
+  switch (ot) {
+    case 1: QDS_UPDT_J(1); break;
+    case 2: QDS_UPDT_J(2); break;
+    case 3: QDS_UPDT_J(3); break;
+    default: QDS_UPDT_J(4); break;
+  }
+#undef QDS_UPDT_J
+#undef QDS_UPDT_Y
+#undef QDS_UPDT
+  QDS_LAUNCH_CHECK();
+  return QDS_OK;
+}
+
 // R = HX - MX diag(lambda).  Real dtypes: lambda has b doubles.  Complex dtypes:
 // lambda has b interleaved complex doubles, products without conjugation.
 QDS_API int qds_block_residual(int32_t dtype, int64_t n, int32_t b, const void* HX, int64_t ldh,
@@ -996,11 +1276,13 @@ QDS_API int qds_density(int64_t n, int32_t k, const double* psi, int64_t ld, con
 namespace {
 constexpr int kResBlocks = 2048;
 
-template <typename T>
+// R is stored in TR: the storage type of the images, or FP32 from FP64 images (the
+// mixed-precision iteration keeps its corrections in FP32).
+template <typename T, typename TR>
 __global__ void __launch_bounds__(kBlock)
 residual_norms_partial(int64_t n, int b, const T* __restrict__ HX, int64_t ldh, const T* __restrict__ MX,
                        int64_t ldm, const double* __restrict__ lam, const double* __restrict__ colscale,
-                       T* __restrict__ R, int64_t ldr, int64_t rows_per_blk, double* __restrict__ partial,
+                       TR* __restrict__ R, int64_t ldr, int64_t rows_per_blk, double* __restrict__ partial,
                        int64_t sx) {
   // problem of a batch (blockIdx.y): blocks at + set * sx, its own lam, colscale, partials
   {
@@ -1025,7 +1307,7 @@ residual_norms_partial(int64_t n, int b, const T* __restrict__ HX, int64_t ldh, 
     for (int64_t r = r_begin + gidx; r < r_end; r += groups) {
       const T m = MX[r * ldm + c];
       const T v = HX[r * ldh + c] - l * m;
-      R[r * ldr + c] = v * cs;
+      R[r * ldr + c] = (TR)(v * cs);
       sr += (double)v * (double)v;
       sm += (double)m * (double)m;
     }
@@ -1081,12 +1363,15 @@ QDS_API int64_t qds_residual_norms_batched_workspace(int64_t n, int32_t b, int32
 
 // The residual step for `batch` problems in one launch: problem i's HX, MX, R at
 // + i x_stride (elements), lam / colscale at + i b, out at + i 2b.
-QDS_API int qds_residual_norms_batched(int32_t dtype, int64_t n, int32_t b, int32_t batch, const void* HX,
-                                       int64_t ldh, const void* MX, int64_t ldm, int64_t x_stride,
-                                       const double* lam, const double* colscale, void* R, int64_t ldr, double* out,
-                                       void* workspace, int64_t ws_bytes, void* stream) {
+namespace {
+int residual_norms_impl(int32_t dtype, int32_t r_dtype, int64_t n, int32_t b, int32_t batch, const void* HX,
+                        int64_t ldh, const void* MX, int64_t ldm, int64_t x_stride, const double* lam,
+                        const double* colscale, void* R, int64_t ldr, double* out, void* workspace,
+                        int64_t ws_bytes, void* stream) {
   QDS_CHECK_ARG(HX && MX && lam && R && out && workspace && b >= 1 && b <= 128 && batch >= 1 && batch <= 65535,
                 "qds_residual_norms: bad arguments");
+  QDS_CHECK_ARG(r_dtype == dtype || (dtype == qds::F64 && r_dtype == qds::F32 && batch == 1),
+                "qds_residual_norms: R must be stored in the images' type, or in F32 from F64 images");
   int64_t nblk = (n + 511) / 512;
   if (nblk > kResBlocks) nblk = kResBlocks;
   if (nblk < 1) nblk = 1;
@@ -1096,12 +1381,16 @@ QDS_API int qds_residual_norms_batched(int32_t dtype, int64_t n, int32_t b, int3
   const int64_t rpb = (n + nblk - 1) / nblk;
   double* part = static_cast<double*>(workspace);
   if (dtype == qds::F32)
-    hipLaunchKernelGGL(residual_norms_partial<float>, dim3(nblk, batch), dim3(kBlock), 0, st, n, b,
+    hipLaunchKernelGGL((residual_norms_partial<float, float>), dim3(nblk, batch), dim3(kBlock), 0, st, n, b,
                        (const float*)HX, ldh, (const float*)MX, ldm, lam, colscale, (float*)R, ldr, rpb, part,
                        x_stride);
-  else if (dtype == qds::F64)
-    hipLaunchKernelGGL(residual_norms_partial<double>, dim3(nblk, batch), dim3(kBlock), 0, st, n, b,
+  else if (dtype == qds::F64 && r_dtype == qds::F64)
+    hipLaunchKernelGGL((residual_norms_partial<double, double>), dim3(nblk, batch), dim3(kBlock), 0, st, n, b,
                        (const double*)HX, ldh, (const double*)MX, ldm, lam, colscale, (double*)R, ldr, rpb, part,
+                       x_stride);
+  else if (dtype == qds::F64)
+    hipLaunchKernelGGL((residual_norms_partial<double, float>), dim3(nblk, batch), dim3(kBlock), 0, st, n, b,
+                       (const double*)HX, ldh, (const double*)MX, ldm, lam, colscale, (float*)R, ldr, rpb, part,
                        x_stride);
   else
     QDS_CHECK_ARG(false, "qds_residual_norms: dtype must be F32 or F64");
@@ -1109,6 +1398,26 @@ QDS_API int qds_residual_norms_batched(int32_t dtype, int64_t n, int32_t b, int3
   hipLaunchKernelGGL(residual_norms_reduce, dim3(batch), dim3(1024), 0, st, (int)nblk, b, part, out);
   QDS_LAUNCH_CHECK();
   return QDS_OK;
+}
+}  // namespace
+
+QDS_API int qds_residual_norms_batched(int32_t dtype, int64_t n, int32_t b, int32_t batch, const void* HX,
+                                       int64_t ldh, const void* MX, int64_t ldm, int64_t x_stride,
+                                       const double* lam, const double* colscale, void* R, int64_t ldr, double* out,
+                                       void* workspace, int64_t ws_bytes, void* stream) {
+  return residual_norms_impl(dtype, dtype, n, b, batch, HX, ldh, MX, ldm, x_stride, lam, colscale, R, ldr, out,
+                             workspace, ws_bytes, stream);
+}
+
+// The residual step of the mixed-precision iteration: FP64 images, R stored in
+// r_dtype (F32 or F64); the norms are FP64 sums of the FP64 residual.
+QDS_API int qds_residual_norms_to(int32_t dtype, int32_t r_dtype, int64_t n, int32_t b, const void* HX,
+                                  int64_t ldh, const void* MX, int64_t ldm, const double* lam,
+                                  const double* colscale, void* R, int64_t ldr, double* out, void* workspace,
+                                  int64_t ws_bytes, void* stream) {
+  QDS_CHECK_ARG(ws_bytes >= qds_residual_norms_workspace(b), "qds_residual_norms: workspace too small");
+  return residual_norms_impl(dtype, r_dtype, n, b, 1, HX, ldh, MX, ldm, 0, lam, colscale, R, ldr, out, workspace,
+                             ws_bytes, stream);
 }
 
 // colscale (optional, b doubles) multiplies the stored residual columns after the

@@ -252,6 +252,49 @@ def gram_blocks(A, B1, B2=None, sym=False, out=None):
     return out
 
 
+def _dtypes(blocks):
+    arr = (ctypes.c_int32 * 4)()
+    for j, B in enumerate(blocks):
+        arr[j] = code(B)
+    return arr
+
+
+def gram_blocks_typed(A, B1, B2=None, sym=False, out=None):
+    """gram_blocks for column blocks of different storage types (FP32 / FP64):
+    FP64 products and sums (qds_gram_blocks_typed)."""
+    n = A[0].shape[0]
+    m = sum(a.shape[1] for a in A)
+    k1 = sum(x.shape[1] for x in B1)
+    k2 = sum(x.shape[1] for x in B2) if B2 else 0
+    if out is None:
+        out = torch.empty((m, k1 + k2), dtype=torch.float64, device=A[0].device)
+    ca, c1 = _colblocks(A), _colblocks(B1)
+    c2 = _colblocks(B2) if B2 else None
+    ta, t1 = _dtypes(A), _dtypes(B1)
+    t2 = _dtypes(B2) if B2 else None
+    wsb = lib.qds_gram_workspace(n, m, k1, k2)
+    wp, wn = workspace("gram").ptr(wsb)
+    check(lib.qds_gram_blocks_typed(n, ctypes.byref(ca), ta, ctypes.byref(c1), t1,
+                                    ctypes.byref(c2) if c2 is not None else None, t2, 1 if sym else 0, ptr(out),
+                                    wp, wn, stream_ptr()), "qds_gram_blocks_typed")
+    return out
+
+
+def block_update_typed(S, C1, Y1, C2=None, Y2=None, c2_first=0):
+    """Y1 = [S_0 | S_1 | ...] C1 in FP64 (and Y2 = ... C2 in Y2's type) for blocks of
+    different storage types; FP64 products (qds_block_update_typed).  C1, C2:
+    contiguous FP64 (m, k)."""
+    n = S[0].shape[0]
+    k = C1.shape[1]
+    assert Y1.dtype == torch.float64 and C1.dtype == torch.float64 and C1.is_contiguous()
+    assert C2 is None or (C2.dtype == torch.float64 and C2.is_contiguous())
+    cs = _colblocks(S)
+    check(lib.qds_block_update_typed(n, ctypes.byref(cs), _dtypes(S), k, ptr(C1), ptr(Y1), _ld(Y1), ptr(C2),
+                                     code(Y2) if Y2 is not None else 0, ptr(Y2),
+                                     _ld(Y2) if Y2 is not None else 1, int(c2_first) if C2 is not None else 0,
+                                     stream_ptr()), "qds_block_update_typed")
+
+
 # ---- batched forms: every operand (batch, n, b) contiguous, one launch per stage
 def _bcolblocks(blocks):
     """qds_colblocks of set 0 for (batch, n, c_j) tensors and their common set stride."""
@@ -379,6 +422,12 @@ def residual_norms(HX, MX, lam, R, colscale=None, out=None):
     wsb = lib.qds_residual_norms_workspace(b)
     wp, wn = workspace("resnorm").ptr(wsb)
     cs = colscale.contiguous() if colscale is not None else None
+    if R.dtype != HX.dtype:
+        # FP32 residual from FP64 images (mixed-precision iteration)
+        check(lib.qds_residual_norms_to(code(HX), code(R), n, b, ptr(HX), _ld(HX), ptr(MX), _ld(MX),
+                                        ptr(lam.contiguous()), ptr(cs), ptr(R), _ld(R), ptr(out), wp, wn,
+                                        stream_ptr()), "qds_residual_norms_to")
+        return out[:b], out[b:]
     check(lib.qds_residual_norms(code(HX), n, b, ptr(HX), _ld(HX), ptr(MX), _ld(MX), ptr(lam.contiguous()),
                                  ptr(cs), ptr(R), _ld(R), ptr(out), wp, wn, stream_ptr()), "qds_residual_norms")
     return out[:b], out[b:]

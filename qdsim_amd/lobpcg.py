@@ -410,7 +410,10 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
     t_start = time.perf_counter()
 
     def blk():
-        return torch.zeros((n, b), dtype=dt, device=dev)
+        # not cleared: every block is written before it is read (P and its images
+        # enter the basis only after the first update wrote them), and a fill of
+        # the 16 blocks cost ~1.4 ms per FP64 solve at 2.4 M DOF
+        return torch.empty((n, b), dtype=dt, device=dev)
 
     # every block is its own contiguous (n, b) buffer: a row of a block is b
     # consecutive elements, so the block kernels read whole cache lines
@@ -446,6 +449,7 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
     lam_h = theta
     have_p = False
     history = []
+    col_history = []   # relative residual of every block column, per iteration
     drops = []
     converged = False
     it = 0
@@ -484,6 +488,7 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
         res = torch.from_numpy(res_h)
         worst = float(res_h[:k].max())
         history.append(worst)
+        col_history.append(res_h.copy())
         # lock converged columns from the next residual on (staged with the
         # Rayleigh-Ritz coefficients below)
         mask = (res_h >= tol).astype(np.float64)
@@ -530,7 +535,156 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
     t_end = time.perf_counter()
     X = X[c]
     timings = {"setup_s": t_iter - t_start, "iterate_s": t_end - t_iter, "restarts": sum(d > 0 for d in drops)}
-    return LOBPCGResult(lam.clone(), X, res, it, history, converged, timings)
+    out = LOBPCGResult(lam.clone(), X, res, it, history, converged, timings)
+    out.col_history = np.array(col_history)
+    return out
+
+
+def _gram_x_first(Gg, m, b):
+    """The [W | P | X]-ordered Gram pair of the mixed iteration in [X | W | P]
+    order.  Entries of an X row and a W / P column are taken from their transposes
+    (W / P row, X column: products with the FP64 images), which also replaces the
+    ones that a diagonal tile straddling the W/P and X blocks computed from the FP32
+    images."""
+    wp = m - b
+    out = np.empty_like(Gg)
+    for h in (0, 1):
+        A = Gg[:, h * m:(h + 1) * m]
+        O = out[:, h * m:(h + 1) * m]
+        O[:b, :b] = A[wp:, wp:]
+        O[b:, b:] = A[:wp, :wp]
+        O[b:, :b] = A[:wp, wp:]
+        O[:b, b:] = A[:wp, wp:].T
+    return out
+
+
+def lobpcg_mixed(op64, k, b, precond32, X0=None, tol=1e-6, maxiter=400, drop_tol=1e-10, verbose=False,
+                 retune=None, retune_at=None, op32=None, seed=0):
+    """Block LOBPCG whose iterate X and images HX, MX are FP64 while the search
+    directions W, P and their images are FP32 (mixed-precision defect correction).
+
+    Per iteration: the residual R = HX - MX diag(lambda) is formed in FP64 and stored
+    in FP32; W = T R by the FP32 multigrid; HW, MW by the FP32 product; the Gram
+    [X | W | P]^T [HS | MS] in FP64 from the mixed-type blocks; Rayleigh-Ritz in
+    FP64 (host LAPACK); X_new = S C in FP64 and P_new = [W | P] CP in FP32 in one
+    pass; the images of P_new by the FP32 product and those of X_new by the FP64
+    product (no image is updated from older images).  The FP32 rounding of the corrections perturbs each step
+    relative to the correction's own size, which shrinks with the residual, so the
+    iteration converges to the FP64 tolerance without the ~1e-3 residual floor of an
+    all-FP32 iteration; only X's product is FP64.
+
+    X0: initial (n, >= b) block (FP64 or FP32; default a seeded random block after
+    one preconditioner application).  retune(theta): optional callable run
+    once, when the worst residual first falls below retune_at (e.g. to move the
+    multigrid shift closer to the lowest Ritz value).  Stops when the relative
+    residual ||H x - lambda M x|| / (s ||M x||), s = max(|lambda|, max_j<k |lambda_j|),
+    of the first k columns is below tol, or after maxiter iterations."""
+    with threadpool_limits(1, "blas"):
+        return _lobpcg_mixed(op64, k, b, precond32, X0, tol, maxiter, drop_tol, verbose, retune, retune_at,
+                             op32, seed)
+
+
+def _lobpcg_mixed(op64, k, b, precond32, X0, tol, maxiter, drop_tol, verbose, retune, retune_at, op32, seed):
+    dev = op64.H.device
+    n = op64.n
+    assert b % 4 == 0 and b >= k and 3 * b <= n, (n, b, k)
+    if op32 is None:
+        op32 = op64.astype(torch.float32)
+    f64, f32 = torch.float64, torch.float32
+    t_start = time.perf_counter()
+
+    def blk(dt):
+        # written before read, as in _lobpcg
+        return torch.empty((n, b), dtype=dt, device=dev)
+
+    X, HX, MX = [blk(f64), blk(f64)], [blk(f64), blk(f64)], [blk(f64), blk(f64)]
+    P, HP, MP = [blk(f32), blk(f32)], [blk(f32), blk(f32)], [blk(f32), blk(f32)]
+    W, HW, MW, R = blk(f32), blk(f32), blk(f32), blk(f32)
+    c = 0
+    if X0 is None:
+        # seeded random block, smoothed by one preconditioner application
+        g = torch.Generator(device=dev).manual_seed(seed)
+        R.copy_(torch.rand((n, b), generator=g, dtype=f32, device=dev) - 0.5)
+        precond32(R, W)
+        X[c].copy_(W)
+    else:
+        X[c].copy_(X0[:, :b])
+    op64.apply(X[c], HX[c], MX[c])
+    G = kernels.gram_blocks([X[c]], [HX[c]], [MX[c]], sym=True)
+    stage_rr = _Staging(2 * 3 * b * b + 2 * b, dev)
+    stage_cs = _Staging(b, dev)
+    xfer = torch.empty(2 * b + 18 * b * b, dtype=f64, device=dev)
+    xfer_h = torch.empty(xfer.numel(), dtype=f64, pin_memory=dev.type == "cuda")
+    theta, C, _ = _svqb_rr(G, b, b, drop_tol)
+    C_d, lam = stage_rr.put([C, theta])
+    for src in (X, HX, MX):
+        kernels.block_update_blocks([src[c]], C_d, src[1 - c])
+    c = 1 - c
+    lam_h = theta
+    have_p = False
+    history, col_history, drops = [], [], []
+    converged = False
+    retuned = retune is None
+    it = 0
+    res = None
+    t_iter = time.perf_counter()
+    (colscale,) = stage_cs.put([np.ones(b)])
+    for it in range(1, maxiter + 1):
+        kernels.residual_norms(HX[c], MX[c], lam, R, colscale, out=xfer[:2 * b])
+        precond32(R, W)
+        op32.apply(W, HW, MW)
+        S = [X[c], W] + ([P[c]] if have_p else [])
+        m = len(S) * b
+        tot = 2 * b + 2 * m * m
+        # Gram over the basis ordered [W | P | X]: its upper tiles then take every
+        # X entry from the FP64 images, (HX)^T W rather than X^T (H W) with the FP32
+        # product (whose FP32 values of H put an error of ~eps32 ||H|| |X| |W| into
+        # the cross terms, a residual floor that does not shrink with W)
+        Sg, HSg, MSg = S[1:] + S[:1], [HW] + ([HP[c]] if have_p else []) + [HX[c]], \
+            [MW] + ([MP[c]] if have_p else []) + [MX[c]]
+        kernels.gram_blocks_typed(Sg, HSg, MSg, sym=True, out=xfer[2 * b:tot].view(m, 2 * m))
+        xfer_h[:tot].copy_(xfer[:tot], non_blocking=True)
+        sync()
+        rn_h, mn_h = xfer_h[:b].numpy(), xfer_h[b:2 * b].numpy()
+        G = _gram_x_first(xfer_h[2 * b:tot].numpy().reshape(m, 2 * m), m, b)
+        scale = max(np.abs(lam_h[:k]).max(), 1e-300)
+        res_h = np.sqrt(np.maximum(rn_h, 0.0)) / (np.maximum(np.abs(lam_h), scale) *
+                                                  np.sqrt(np.maximum(mn_h, 1e-300)))
+        res = torch.from_numpy(res_h)
+        worst = float(res_h[:k].max())
+        history.append(worst)
+        col_history.append(res_h.copy())
+        mask = (res_h >= tol).astype(np.float64)
+        if verbose:
+            print(f"  lobpcg[mixed] it {it:4d} worst rel. residual {worst:.3e}")
+        if worst < tol:
+            converged = True
+            break
+        theta, C_h, dropped = _rayleigh_ritz(G, m, b, drop_tol)
+        drops.append(dropped)
+        CP_h = C_h.copy()
+        CP_h[:b] = 0.0
+        C, CP, lam_next, colscale = stage_rr.put([C_h, CP_h, theta, mask])
+        # X_new (FP64) and P_new (FP32) in one pass over [X | W | P]
+        kernels.block_update_typed(S, C, X[1 - c], CP, P[1 - c], c2_first=b)
+        c = 1 - c
+        # every image from its own block: P_new is a small difference of larger
+        # terms near convergence, so FP32 images updated from those of W and P
+        # lose accuracy relative to P_new at every step (the residual then grows
+        # below ~1e-4); one FP32 product costs what the two image updates did
+        op32.apply(P[c], HP[c], MP[c])
+        op64.apply(X[c], HX[c], MX[c])
+        lam, lam_h = lam_next, theta
+        have_p = dropped == 0
+        if not retuned and retune_at is not None and worst < retune_at:
+            retune(theta)
+            retuned = True
+    sync()
+    t_end = time.perf_counter()
+    timings = {"setup_s": t_iter - t_start, "iterate_s": t_end - t_iter, "restarts": sum(d > 0 for d in drops)}
+    out = LOBPCGResult(lam.clone(), X[c], res, it, history, converged, timings)
+    out.col_history = np.array(col_history)
+    return out
 
 
 def solve_mixed(op64, k, b, precond32=None, precond64=None, tol=1e-6, tol32=1e-3, maxiter=400,
@@ -560,4 +714,5 @@ def solve_mixed(op64, k, b, precond32=None, precond64=None, tol=1e-6, tol32=1e-3
                         "fp64_iters": r64.iterations, "fp32_restarts": r32.timings["restarts"],
                         "fp64_restarts": r64.timings["restarts"]})
     r64.history = {"fp32": r32.history, "fp64": r64.history}
+    r64.col_history = {"fp32": getattr(r32, "col_history", None), "fp64": getattr(r64, "col_history", None)}
     return r64

@@ -27,7 +27,7 @@ from .device import sync
 from .fem import FEMSpace
 from .mesh import SimpleMesh
 from .lattice import Lattice, LatticeMG
-from .lobpcg import CSROperator, LatticeOperator, lobpcg, solve_mixed
+from .lobpcg import CSROperator, LatticeOperator, lobpcg, lobpcg_mixed, solve_mixed
 from .units import EV_TO_J, HBAR, NM
 
 KIN_SCALE = HBAR * HBAR / 2.0 / EV_TO_J / (NM * NM)   # hbar^2/2 in eV nm^2 kg
@@ -114,6 +114,9 @@ class SchrodingerProblem:
         # the shift below min V (info["breakdown"] names the reason, info["retried"]
         # says whether it was solved again)
         self.retry = True
+        # mixed-mode iteration: "mixed" (X in FP64, corrections W, P in FP32, one
+        # phase) or "two-phase" (all-FP32 phase to its floor, then all-FP64)
+        self.scheme = "two-phase"
         self.setup_time = time.perf_counter() - t0
         if m_star_func is not None:
             self.set_mass(m_star_func)
@@ -281,9 +284,26 @@ class SchrodingerProblem:
                 # fine Ritz values of the FP32 phase: the shift of the FP64 phase
                 pre32_64.set_sigma(shift_below(r32.eigenvalues, sigma0, 0.1))
 
+            scheme = mixed_opts.pop("scheme", self.scheme)
+
+            def run(tuned):
+                if scheme == "mixed":
+                    # one phase: the shift moves once, from the Ritz values of the
+                    # iteration itself, when the residual reaches the switch point
+                    # of the two-phase scheme
+                    t_m = time.perf_counter()
+                    r = lobpcg_mixed(op64, k, b, pre32, X0, tol=tol, maxiter=maxiter, verbose=verbose,
+                                     retune=(lambda th: pre32.set_sigma(shift_below(th, sigma0, 0.1))) if tuned
+                                     else None, retune_at=tol32, seed=seed)
+                    r.timings.update({"mixed_s": time.perf_counter() - t_m, "mixed_iters": r.iterations})
+                    r.history = {"mixed": r.history}
+                    r.col_history = {"mixed": r.col_history}
+                    return r
+                return solve_mixed(op64, k, b, pre32, pre64, tol=tol, tol32=tol32, maxiter=maxiter, seed=seed,
+                                   verbose=verbose, X0=X0, b64=b64, between=retune if tuned else None, **mixed_opts)
+
             try:
-                res = solve_mixed(op64, k, b, pre32, pre64, tol=tol, tol32=tol32, maxiter=maxiter, seed=seed,
-                                  verbose=verbose, X0=X0, b64=b64, between=retune, **mixed_opts)
+                res = run(True)
                 breakdown = breakdown_reason(res.eigenvalues.cpu().numpy(), res.converged)
             except np.linalg.LinAlgError as e:
                 breakdown = breakdown_reason(None, False, e)
@@ -294,8 +314,7 @@ class SchrodingerProblem:
                 # multigrid operator indefinite; solve again with the shift below min V
                 pre32.set_sigma(sigma0)
                 pre32_64.set_sigma(sigma0)
-                res = solve_mixed(op64, k, b, pre32, pre64, tol=tol, tol32=tol32, maxiter=maxiter, seed=seed,
-                                  verbose=verbose, X0=X0, b64=b64, **mixed_opts)
+                res = run(False)
             elif breakdown is not None and breakdown != "not converged":
                 raise np.linalg.LinAlgError(f"mixed-precision solve broke down ({breakdown}) with the retry off")
         elif mode == "fp64":
@@ -330,6 +349,6 @@ class SchrodingerProblem:
                 "residuals": res.residuals[:k].cpu().numpy() if res.residuals is not None else None,
                 "timings": dict(res.timings, prep_s=t_prep, nested_s=t_nested,
                                 nested_iters=self._nested_iters),
-                "history": res.history, "block": b, "X_block": res.X, "ritz": res.eigenvalues,
+                "history": res.history, "col_history": getattr(res, "col_history", None), "block": b, "X_block": res.X, "ritz": res.eigenvalues,
                 "solve_s": time.perf_counter() - t0}
         return lam.cpu().numpy(), full, info
