@@ -337,6 +337,16 @@ int qds_coldot(int32_t dtype, int64_t n, int32_t b, const void* X, int64_t ldx, 
  * columns, [6] 1 when the tridiagonal eigensolver produced the pairs (Cholesky
  * path, m >= 10, 2 b <= m + 1, every check passed), [8 + j] residual of column j.
  * With freeze, a converged problem keeps its block (C = [I; 0], lam unchanged). */
+/* The same projected eigensolve on the host CPU, for one problem between two GPU
+ * launches: G (m x 2m row-major FP64, host memory, m <= 192) gives theta (b
+ * ascending Ritz values) and C (m x b row-major, host).  Cholesky of the
+ * diagonally scaled M-Gram (vanished columns left out), Hager condition estimate,
+ * T = L^-1 H L^-T, Householder tridiagonalisation, bisection, inverse iteration.
+ * Returns 0, 1 when the basis is too close to dependent for the Cholesky path
+ * (reciprocal condition below drop_tol; the caller selects a basis otherwise), or
+ * a negative error code.                                                          */
+int qds_host_rayleigh_ritz(int32_t m, int32_t b, const double* G /* host */, double drop_tol,
+                           double* theta /* host */, double* C /* host */);
 int32_t qds_rr_status_len(int32_t b);
 int qds_rayleigh_ritz(int64_t batch, int32_t m, int32_t b, int32_t k, double* G, int64_t g_stride,
                       const double* norms, double* lam, double* C, double* CP, int64_t c_stride,

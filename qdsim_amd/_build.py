@@ -24,6 +24,12 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
           "-munsafe-fp-atomics"]
 
 
+# host-only numerical code (the per-iteration Rayleigh-Ritz solve) is built for
+# x86-64 with AVX2 and FMA, which every EPYC host of an MI355X node provides
+HOST_VECTOR = {"host_rr.cpp"}
+HOST_VECTOR_FLAGS = ["-mavx2", "-mfma"]
+
+
 def _sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
@@ -33,6 +39,8 @@ def _compile(src):
     cmd = [HIPCC] + COMMON + ["-I", CSRC]
     if src.endswith(".hip"):
         cmd += DEVICE + ["-x", "hip"]
+    elif os.path.basename(src) in HOST_VECTOR:
+        cmd += HOST_VECTOR_FLAGS
     cmd += ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -72,6 +72,7 @@ _SIGNATURES = {
     "qds_coldot": ([_i32, _i64, _i32, _p, _i64, _p, _i64, _i32, _p, _p, _i64, _p], _c.c_int),
     "qds_residual_norms_to": ([_i32, _i32, _i64, _i32, _p, _i64, _p, _i64, _p, _p, _p, _i64, _p, _p, _i64, _p],
                               _c.c_int),
+    "qds_host_rayleigh_ritz": ([_i32, _i32, _p, _d, _p, _p], _c.c_int),
     "qds_gram_blocks_typed": ([_i64, _p, _p, _p, _p, _p, _p, _i32, _p, _p, _i64, _p], _c.c_int),
     "qds_block_update_typed": ([_i64, _p, _p, _i32, _p, _p, _i64, _p, _i32, _p, _i64, _i32, _p], _c.c_int),
     "qds_mg_vcycle": ([_i32, _i32, _p, _i32, _i32, _i32, _i32, _d, _p], _c.c_int),

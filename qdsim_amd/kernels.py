@@ -252,6 +252,22 @@ def gram_blocks(A, B1, B2=None, sym=False, out=None):
     return out
 
 
+def host_rayleigh_ritz(G, m, b, drop_tol):
+    """Lowest b Ritz pairs of the projected pencil G = [S^T H S | S^T M S] (host
+    FP64 array, m x 2m) by qds_host_rayleigh_ritz: (theta (b,), C (m, b)), or None
+    when the Cholesky path does not apply (near-dependent basis)."""
+    G = np.ascontiguousarray(G, dtype=np.float64)
+    assert G.shape == (m, 2 * m)
+    theta = np.empty(b)
+    C = np.empty((m, b))
+    rc = lib.qds_host_rayleigh_ritz(int(m), int(b), G.ctypes.data, float(drop_tol), theta.ctypes.data,
+                                    C.ctypes.data)
+    if rc == 1:
+        return None
+    check(rc, "qds_host_rayleigh_ritz")
+    return theta, C
+
+
 def _dtypes(blocks):
     arr = (ctypes.c_int32 * 4)()
     for j, B in enumerate(blocks):

@@ -176,8 +176,14 @@ def _chol_rr(G, m, nb, drop_tol):
 
 
 def _rayleigh_ritz(G, m, nb, drop_tol):
-    out = _chol_rr(G, m, nb, drop_tol)
-    return out if out is not None else _svqb_rr(G, m, nb, drop_tol)
+    """Rayleigh-Ritz of one iteration: the library's host solver
+    (qds_host_rayleigh_ritz: Cholesky of the scaled M-Gram, tridiagonalisation,
+    bisection, inverse iteration; ~3-5x faster than the LAPACK calls of _chol_rr
+    for m = 72) when the basis is well conditioned, SVQB otherwise."""
+    out = kernels.host_rayleigh_ritz(G, m, nb, drop_tol)
+    if out is not None:
+        return out[0], out[1], 0
+    return _svqb_rr(G, m, nb, drop_tol)
 
 
 class _Staging:
@@ -214,10 +220,15 @@ class LOBPCGResult:
 
 
 def lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=None, seed=0,
-           verbose=False, stall_iters=None, refresh=None, rr="host"):
-    """rr: "host" (LAPACK Rayleigh-Ritz after one device-to-host copy per iteration)
-    or "device" (qds_rayleigh_ritz on the GPU; the host only reads each
-    iteration's status one iteration behind).  For one problem the host path is
+           verbose=False, stall_iters=None, refresh=None, rr="host", smooth=1, retune=None, retune_iter=None):
+    """rr: "host" (Rayleigh-Ritz on the host after one device-to-host copy per
+    iteration) or "device" (qds_rayleigh_ritz on the GPU; the host only reads each
+    iteration's status one iteration behind).  smooth: preconditioner applications
+    to the seeded random start (no X0): each one is a step of inverse iteration
+    with the shifted operator, which concentrates the start on the low spectrum.
+    retune(theta): optional callable run once after iteration retune_iter with the
+    Ritz values (host path), e.g. to move the preconditioner's shift once the Ritz
+    values bound the wanted eigenvalues.  For one problem the host path is
     faster: the single-workgroup kernel of the device path takes ~0.58 ms for the
     72 x 72 projected pencil (a chain of ~5m dependent steps) against ~0.3 ms of
     single-threaded LAPACK plus a ~25 us round trip; the device path pays off for
@@ -227,7 +238,8 @@ def lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=None
     # the host Rayleigh-Ritz works on matrices of at most 96 x 192: multithreaded
     # BLAS only adds thread wake-ups there (measured 2-70x slower triangular solves)
     with threadpool_limits(1, "blas"):
-        return _lobpcg(op, k, b, precond, X0, tol, maxiter, drop_tol, seed, verbose, stall_iters, refresh)
+        return _lobpcg(op, k, b, precond, X0, tol, maxiter, drop_tol, seed, verbose, stall_iters, refresh, smooth,
+                       retune, retune_iter)
 
 
 def _default_drop_tol(dt):
@@ -383,7 +395,7 @@ def _lobpcg_device(op, k, b, precond, X0, tol, maxiter, drop_tol, seed, verbose,
 
 
 def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=None, seed=0,
-            verbose=False, stall_iters=None, refresh=None):
+            verbose=False, stall_iters=None, refresh=None, smooth=1, retune=None, retune_iter=None):
     """Lowest k eigenpairs of (op.H, op.M) with block size b >= k.
 
     precond(R, W) writes W = T R (R, W are (n, b) blocks of op.dtype).
@@ -421,18 +433,6 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
     X, HX, MX = [blk(), blk()], [blk(), blk()], [blk(), blk()]
     P, HP, MP = [blk(), blk()], [blk(), blk()], [blk(), blk()]
     W, HW, MW, R = blk(), blk(), blk(), blk()
-    c = 0
-    if X0 is None:
-        g = torch.Generator(device=dev).manual_seed(seed)
-        X[c].copy_(torch.rand((n, b), generator=g, dtype=dt, device=dev) - 0.5)
-        if precond is not None:
-            R.copy_(X[c])
-            precond(R, X[c])
-    else:
-        X[c].copy_(X0[:, :b].to(dt))
-    # initial Rayleigh-Ritz on X alone
-    op.apply(X[c], HX[c], MX[c])
-    G = kernels.gram_blocks([X[c]], [HX[c]], [MX[c]], sym=True)
     # per iteration one copy to the device: Rayleigh-Ritz coefficients,
     # eigenvalues and the locking mask of the next residual
     stage_rr = _Staging(2 * 3 * b * b + 2 * b, dev)
@@ -441,11 +441,37 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
     # buffer [norms (2b) | Gram (m x 2m)] and its page-locked host twin
     xfer = torch.empty(2 * b + 18 * b * b, dtype=torch.float64, device=dev)
     xfer_h = torch.empty(xfer.numel(), dtype=torch.float64, pin_memory=dev.type == "cuda")
-    theta, C, _ = _svqb_rr(G, b, b, drop_tol)
-    C_d, lam = stage_rr.put([C, theta])
-    for src in (X, HX, MX):
-        kernels.block_update_blocks([src[c]], C_d, src[1 - c])
-    c = 1 - c
+
+    def rr_x(c):
+        # Rayleigh-Ritz on the X block alone (SVQB): X, HX, MX <- their Ritz
+        # combinations in the other half of the double buffer
+        op.apply(X[c], HX[c], MX[c])
+        G = kernels.gram_blocks([X[c]], [HX[c]], [MX[c]], sym=True)
+        theta, C, _ = _svqb_rr(G, b, b, drop_tol)
+        C_d, lam = stage_rr.put([C, theta])
+        for src in (X, HX, MX):
+            kernels.block_update_blocks([src[c]], C_d, src[1 - c])
+        return 1 - c, theta, lam
+
+    c = 0
+    if X0 is None:
+        g = torch.Generator(device=dev).manual_seed(seed)
+        X[c].copy_(torch.rand((n, b), generator=g, dtype=dt, device=dev) - 0.5)
+        if precond is not None:
+            nsm = max(int(smooth), 1)
+            for s_ in range(nsm):
+                R.copy_(X[c])
+                precond(R, X[c])
+                if (s_ + 1) % 4 == 0 and s_ + 1 < nsm:
+                    # subspace iteration: four steps of inverse iteration pull the
+                    # columns towards the lowest eigenvectors by up to
+                    # ((lambda_1 - sigma) / (lambda_b - sigma))^4; the Rayleigh-Ritz
+                    # step re-orthonormalises before the basis loses rank in FP32
+                    c, _, _ = rr_x(c)
+    else:
+        X[c].copy_(X0[:, :b].to(dt))
+    # initial Rayleigh-Ritz on X alone
+    c, theta, lam = rr_x(c)
     lam_h = theta
     have_p = False
     history = []
@@ -527,6 +553,8 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
         c = 1 - c
         lam, lam_h = lam_next, theta
         have_p = dropped == 0
+        if retune is not None and it == retune_iter:
+            retune(theta)
         if fresh:
             # recompute the images of X: the updated ones carry the rounding of every
             # update since the last product (FP32 storage)
