@@ -303,8 +303,9 @@ int qds_residual_norms_batched(int32_t dtype, int64_t n, int32_t b, int32_t batc
  *   type *_dt[j] (F32 / F64); rows converted to FP64 when staged, FP64 products
  *   and sums; workspace qds_gram_workspace.
  * qds_block_update_typed: Y1 = S C1 in FP64 and Y2 = S C2 in y2_dtype (F32 / F64)
- *   for blocks of storage type s_dt[j]; FP64 products and sums, m <= 96, k <= 64,
- *   blocks of at most 32 (F64) / 64 (F32) columns, 16-byte aligned.              */
+ *   for blocks of storage type s_dt[j]: FP64 blocks with FP64 products and sums,
+ *   FP32 blocks on the FP32 MFMA (C rounded to FP32), the two parts added in FP64;
+ *   k <= 64, blocks of at most 32 (F64) / 64 (F32) columns, 16-byte aligned.     */
 int qds_residual_norms_to(int32_t dtype, int32_t r_dtype, int64_t n, int32_t b, const void* HX, int64_t ldh,
                           const void* MX, int64_t ldm, const double* lam, const double* colscale, void* R,
                           int64_t ldr, double* out, void* workspace, int64_t ws_bytes, void* stream);

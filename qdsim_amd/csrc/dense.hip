@@ -486,18 +486,25 @@ int launch_update(int64_t n, int m, int k, const ColBlocks& S, int c2_first, con
 }
 
 // ---------------------------------------------------------------- typed update
-// Y1 = S C1 in FP64 and Y2 = S C2 in TY2 for column blocks of their own storage
-// type (es[j] bytes per element): the update of the mixed-precision iteration, whose
-// basis [X | W | P] keeps X in FP64 and the corrections W, P in FP32.  The layout
-// and pipelining follow block_update_kernel; every product runs on the FP64 MFMA
-// (FP32 operands are converted exactly), C is staged in LDS as FP64.  A lane's
-// 16-byte load holds VEC = 4 (FP32) or 2 (FP64) consecutive columns of its row,
-// and the contraction index of MFMA step e is the load's column e.
+// Y1 = S C1 (FP64) and Y2 = S C2 (TY2) for column blocks of their own storage
+// type (es[j] bytes per element): the update of the mixed-precision iteration,
+// whose basis [X | W | P] keeps X in FP64 and the corrections W, P in FP32.  The
+// layout and pipelining follow block_update_kernel.  FP64 blocks run on the FP64
+// MFMA with C in FP64; FP32 blocks on the FP32 MFMA (twice the rate; gfx950's FP64
+// MFMA bounds this kernel otherwise) with C rounded to FP32, into their own
+// accumulators that are added in FP64 at the end: the FP32 blocks' contribution
+// (the correction) is then exact to FP32 rounding relative to its own size.  The
+// two MFMAs place result rows differently (FP64: lane l, register e holds row
+// l/16 + 4e; FP32: row 4(l/16) + e), so an FP32 block's lane loads tile row
+// pi(l % 16), pi(i) = i/4 + 4 (i % 4), which puts both results of a row in the same
+// lane and register.  C is staged in LDS per precision: rows of FP64 blocks as
+// FP64 (cbase[j] = first staged row of block j), rows of FP32 blocks as FP32.
 struct TypedBlocks {
   const void* p[4];
   int64_t ld[4];
   int32_t start[5];
   int32_t es[4];
+  int32_t cbase[4];
   int32_t nb;
 };
 
@@ -505,21 +512,41 @@ template <int OT, int MAXJ, typename TY2, bool TWO>
 __global__ void __launch_bounds__(kBlock)
 block_update_typed_kernel(int64_t n, int m, int k, TypedBlocks S, const double* __restrict__ C1,
                           const double* __restrict__ C2, double* __restrict__ Y1, int64_t ldy1,
-                          TY2* __restrict__ Y2, int64_t ldy2, int crows, int c2_first) {
+                          TY2* __restrict__ Y2, int64_t ldy2, int rows64, int rows32, int c2_first) {
   constexpr int KP = OT * 16;
-  extern __shared__ unsigned char upd_smem[];
-  double* sC = reinterpret_cast<double*>(upd_smem);   // [NOUT][crows][KP], zero padded
   constexpr int NOUT = TWO ? 2 : 1;
+  extern __shared__ unsigned char upd_smem[];
+  double* sC64 = reinterpret_cast<double*>(upd_smem);              // [NOUT][rows64][KP]
+  float* sC32 = reinterpret_cast<float*>(sC64 + NOUT * rows64 * KP);  // [NOUT][rows32][KP]
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int q = lane >> 4;
   const int rl = lane & 15;
-  for (int t = threadIdx.x; t < NOUT * crows * KP; t += kBlock) {
-    const int o = t / (crows * KP);
-    const int rem = t - o * crows * KP;
+  const int prl = (rl >> 2) + 4 * (rl & 3);   // pi(rl)
+  // staged row r of a region -> C row (or -1: zero)
+  auto crow_of = [&](int r, int es) {
+    for (int j = 0; j < S.nb; ++j) {
+      if (S.es[j] != es) continue;
+      const int w = S.start[j + 1] - S.start[j];
+      if (r >= S.cbase[j] && r < S.cbase[j] + w) return S.start[j] + (r - S.cbase[j]);
+    }
+    return -1;
+  };
+  for (int t = threadIdx.x; t < NOUT * rows64 * KP; t += kBlock) {
+    const int o = t / (rows64 * KP);
+    const int rem = t - o * rows64 * KP;
     const int i = rem / KP, j = rem - i * KP;
+    const int cr = crow_of(i, 8);
     const double* C = o == 0 ? C1 : C2;
-    sC[t] = (i < m && j < k) ? C[i * k + j] : 0.0;
+    sC64[t] = (cr >= 0 && j < k) ? C[cr * k + j] : 0.0;
+  }
+  for (int t = threadIdx.x; t < NOUT * rows32 * KP; t += kBlock) {
+    const int o = t / (rows32 * KP);
+    const int rem = t - o * rows32 * KP;
+    const int i = rem / KP, j = rem - i * KP;
+    const int cr = crow_of(i, 4);
+    const double* C = o == 0 ? C1 : C2;
+    sC32[t] = (cr >= 0 && j < k) ? (float)C[cr * k + j] : 0.f;
   }
   __syncthreads();
   const int64_t rows_per_pass = (int64_t)kWaves * 16;
@@ -527,10 +554,10 @@ block_update_typed_kernel(int64_t n, int m, int k, TypedBlocks S, const double* 
   const int64_t stride = (int64_t)gridDim.x * rows_per_pass;
   uint4 nxt[MAXJ];
   auto fetch = [&](int64_t base, int j) {
-    int64_t row = base + wave * 16 + rl;
+    const int es = S.es[j];
+    int64_t row = base + wave * 16 + (es == 4 ? prl : rl);
     row = row < n ? row : n - 1;
     const int w = S.start[j + 1] - S.start[j];
-    const int es = S.es[j];
     const int vec = 16 / es;
     const char* rp = reinterpret_cast<const char*>(S.p[j]) + row * S.ld[j] * es;
 #pragma unroll
@@ -542,48 +569,62 @@ block_update_typed_kernel(int64_t n, int m, int k, TypedBlocks S, const double* 
   if (first < n) fetch(first, 0);
   for (int64_t base = first; base < n; base += stride) {
     v4d acc[NOUT][OT];
+    v4f acc32[NOUT][OT];
 #pragma unroll
     for (int o = 0; o < NOUT; ++o)
 #pragma unroll
-      for (int t = 0; t < OT; ++t) acc[o][t] = v4d{0.0, 0.0, 0.0, 0.0};
+      for (int t = 0; t < OT; ++t) {
+        acc[o][t] = v4d{0.0, 0.0, 0.0, 0.0};
+        acc32[o][t] = v4f{0.f, 0.f, 0.f, 0.f};
+      }
     for (int j = 0; j < S.nb; ++j) {
       uint4 cur[MAXJ];
 #pragma unroll
       for (int jj = 0; jj < MAXJ; ++jj) cur[jj] = nxt[jj];
       const int w = S.start[j + 1] - S.start[j];
-      const int cj = S.start[j];
+      const int cb = S.cbase[j];
       const bool f32 = S.es[j] == 4;
       if (j + 1 < S.nb) fetch(base, j + 1);
       else if (base + stride < n) fetch(base + stride, 0);
-      const bool two = TWO && cj + w > c2_first;
-      const int vec = f32 ? 4 : 2;
-      const int kb = 4 * vec;
+      const bool two = TWO && S.start[j] + w > c2_first;
+      if (f32) {
 #pragma unroll
-      for (int jj = 0; jj < MAXJ; ++jj) {
-        if (jj * kb >= w) break;
-        __builtin_amdgcn_sched_barrier(0);
-        const double keep = jj * kb + q * vec < w ? 1.0 : 0.0;
-        double sv[4];
-        if (f32) {
+        for (int jj = 0; jj < MAXJ; ++jj) {
+          if (jj * 16 >= w) break;
+          __builtin_amdgcn_sched_barrier(0);
+          const float keep = jj * 16 + q * 4 < w ? 1.f : 0.f;
           const float* fv = reinterpret_cast<const float*>(&cur[jj]);
 #pragma unroll
-          for (int e = 0; e < 4; ++e) sv[e] = (double)fv[e] * keep;
-        } else {
-          const double* dv = reinterpret_cast<const double*>(&cur[jj]);
-          sv[0] = dv[0] * keep;
-          sv[1] = dv[1] * keep;
-          sv[2] = sv[3] = 0.0;
+          for (int e = 0; e < 4; ++e) {
+            const float a = fv[e] * keep;
+            const int kl = cb + jj * 16 + q * 4 + e;   // staged FP32 row (zero past the block)
+#pragma unroll
+            for (int t = 0; t < OT; ++t) acc32[0][t] = mfma_f32(a, sC32[kl * KP + t * 16 + rl], acc32[0][t]);
+            if (two) {
+#pragma unroll
+              for (int t = 0; t < OT; ++t)
+                acc32[NOUT - 1][t] = mfma_f32(a, sC32[(rows32 + kl) * KP + t * 16 + rl], acc32[NOUT - 1][t]);
+            }
+          }
         }
+      } else {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          if (e >= vec) break;
-          const int kl = cj + jj * kb + q * vec + e;   // C row (zero past m)
+        for (int jj = 0; jj < MAXJ; ++jj) {
+          if (jj * 8 >= w) break;
+          __builtin_amdgcn_sched_barrier(0);
+          const double keep = jj * 8 + q * 2 < w ? 1.0 : 0.0;
+          const double* dv = reinterpret_cast<const double*>(&cur[jj]);
 #pragma unroll
-          for (int t = 0; t < OT; ++t) acc[0][t] = mfma_f64(sv[e], sC[kl * KP + t * 16 + rl], acc[0][t]);
-          if (two) {
+          for (int e = 0; e < 2; ++e) {
+            const double a = dv[e] * keep;
+            const int kl = cb + jj * 8 + q * 2 + e;    // staged FP64 row (zero past the block)
 #pragma unroll
-            for (int t = 0; t < OT; ++t)
-              acc[NOUT - 1][t] = mfma_f64(sv[e], sC[(crows + kl) * KP + t * 16 + rl], acc[NOUT - 1][t]);
+            for (int t = 0; t < OT; ++t) acc[0][t] = mfma_f64(a, sC64[kl * KP + t * 16 + rl], acc[0][t]);
+            if (two) {
+#pragma unroll
+              for (int t = 0; t < OT; ++t)
+                acc[NOUT - 1][t] = mfma_f64(a, sC64[(rows64 + kl) * KP + t * 16 + rl], acc[NOUT - 1][t]);
+            }
           }
         }
       }
@@ -595,10 +636,10 @@ block_update_typed_kernel(int64_t n, int m, int k, TypedBlocks S, const double* 
       if (col >= k) continue;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int64_t orow = r0 + q + 4 * e;   // v_mfma_f64_16x16x4f64 result row
+        const int64_t orow = r0 + q + 4 * e;   // FP64 result row (the FP32 rows were permuted to match)
         if (orow < n) {
-          Y1[orow * ldy1 + col] = acc[0][t][e];
-          if (TWO) Y2[orow * ldy2 + col] = (TY2)acc[NOUT - 1][t][e];
+          Y1[orow * ldy1 + col] = acc[0][t][e] + (double)acc32[0][t][e];
+          if (TWO) Y2[orow * ldy2 + col] = (TY2)(acc[NOUT - 1][t][e] + (double)acc32[NOUT - 1][t][e]);
         }
       }
     }
@@ -1114,10 +1155,11 @@ QDS_API int qds_block_update(int32_t dtype, int64_t n, int32_t m, int32_t k, con
 // Y1 = S C1 (FP64) and, when C2 != NULL, Y2 = S C2 (stored in y2_dtype, F32 or
 // F64) for column blocks whose storage types differ (s_dt[j] for block j, F32 or
 // F64): the update of the mixed-precision iteration (X_new in FP64, P_new in FP32
-// from [X | W | P]).  FP64 products and sums; m <= 96 (C resident in LDS), k <= 64;
-// blocks 16-byte aligned with widths and leading dimensions a multiple of the
-// vector width (4 F32, 2 F64), at most 32 (F64) / 64 (F32) columns each.
-// c2_first: rows of C2 before it are zero.
+// from [X | W | P]).  The FP64 blocks' products and sums are FP64; the FP32 blocks'
+// run on the FP32 MFMA with C rounded to FP32 and are added in FP64.  C resident in
+// LDS (k <= 64); blocks 16-byte aligned with widths and leading dimensions a
+// multiple of the vector width (4 F32, 2 F64), at most 32 (F64) / 64 (F32) columns
+// each.  c2_first: rows of C2 before it are zero.
 QDS_API int qds_block_update_typed(int64_t n, const qds_colblocks* S, const int32_t* s_dt, int32_t k,
                                    const double* C1, void* Y1, int64_t ldy1, const double* C2, int32_t y2_dtype,
                                    void* Y2, int64_t ldy2, int32_t c2_first, void* stream) {
@@ -1126,8 +1168,8 @@ QDS_API int qds_block_update_typed(int64_t n, const qds_colblocks* S, const int3
   QDS_CHECK_ARG(!C2 || y2_dtype == qds::F32 || y2_dtype == qds::F64, "qds_block_update_typed: bad y2_dtype");
   QDS_CHECK_ARG(k >= 1 && k <= 16 * kMaxOutTiles, "qds_block_update_typed: bad shape");
   TypedBlocks d;
-  int m = 0, maxj = 1;
-  for (int j = 0; j < 4; ++j) { d.p[j] = nullptr; d.ld[j] = 0; d.es[j] = 8; }
+  int m = 0, maxj = 1, w64 = 0, w32 = 0;
+  for (int j = 0; j < 4; ++j) { d.p[j] = nullptr; d.ld[j] = 0; d.es[j] = 8; d.cbase[j] = 0; }
   for (int j = 0; j < S->nb; ++j) {
     QDS_CHECK_ARG(s_dt[j] == qds::F32 || s_dt[j] == qds::F64, "qds_block_update_typed: block %d type", j);
     const int es = qds::dtype_size(s_dt[j]);
@@ -1141,6 +1183,8 @@ QDS_API int qds_block_update_typed(int64_t n, const qds_colblocks* S, const int3
     d.ld[j] = S->ld[j];
     d.es[j] = es;
     d.start[j] = m;
+    d.cbase[j] = es == 8 ? w64 : w32;
+    (es == 8 ? w64 : w32) += S->cols[j];
     m += S->cols[j];
     const int loads = (S->cols[j] + 4 * vw - 1) / (4 * vw);
     maxj = loads > maxj ? loads : maxj;
@@ -1150,9 +1194,10 @@ QDS_API int qds_block_update_typed(int64_t n, const qds_colblocks* S, const int3
   QDS_CHECK_ARG(c2_first >= 0 && c2_first <= m, "qds_block_update_typed: bad c2_first");
   const int ot = (k + 15) / 16;
   const int kp = ot * 16;
-  const int crows = m + 16;   // rows past m read zeros
+  // each region padded by one load of rows (the last load of a block may run past it)
+  const int rows64 = w64 ? w64 + 8 : 0, rows32 = w32 ? w32 + 16 : 0;
   const int nout = C2 ? 2 : 1;
-  const size_t shm = (size_t)nout * crows * kp * sizeof(double);
+  const size_t shm = (size_t)nout * kp * ((size_t)rows64 * sizeof(double) + (size_t)rows32 * sizeof(float));
   QDS_CHECK_ARG(shm <= (size_t)kUpdLds, "qds_block_update_typed: C of %d x %d too large for LDS", m, k);
   if (n == 0) return QDS_OK;
   hipStream_t st = qds_stream(stream);
@@ -1161,7 +1206,7 @@ QDS_API int qds_block_update_typed(int64_t n, const qds_colblocks* S, const int3
   QDS_CHECK_ARG(maxj <= 4, "qds_block_update_typed: blocks too wide");
 #define QDS_UPDT(OTV, J, TY, TW)                                                                            \
   hipLaunchKernelGGL((block_update_typed_kernel<OTV, J, TY, TW>), dim3(blocks), dim3(kBlock), shm, st, n, m, k, d, \
-                     C1, C2, (double*)Y1, ldy1, (TY*)Y2, ldy2, crows, C2 ? c2_first : 0)
+                     C1, C2, (double*)Y1, ldy1, (TY*)Y2, ldy2, rows64, rows32, C2 ? c2_first : 0)
 #define QDS_UPDT_Y(OTV, J)                                                                                  \
   do {                                                                                                       \
     if (!C2) QDS_UPDT(OTV, J, double, false);                                                                \
