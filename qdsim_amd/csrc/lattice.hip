@@ -109,25 +109,51 @@ __device__ __forceinline__ void ld4(const T* p, T (&v)[4]) {
 }
 
 // Entries of one task row, LEN known at compile time (fully unrolled; summation
-// order as in the generic loop: entry by entry).
+// order as in the generic loop: entry by entry).  The task's LDS offsets are read
+// into scalar registers before any LDS read: scalar loads and LDS reads share one
+// wait counter, so an offset fetched inside the loop drained every LDS read in
+// flight.  Entries then go in groups of 8 whose value and X reads are all issued
+// before their FMAs (the compiler otherwise kept two X reads in flight and exposed
+// the LDS latency at every entry).
 template <typename T, int VEC, bool FUSED, int LEN>
 __device__ __forceinline__ void task_entries(const int16_t* __restrict__ off, const T* xl, int ps,
                                              const T* lva, const T* lvb, int64_t vrow, int vstep,
                                              T (&acc_a)[VEC], T (&acc_b)[VEC]) {
+  constexpr int G = 8;
+  int o[LEN];
 #pragma unroll
-  for (int k = 0; k < LEN; k += 4) {
-    T av[4], bv[4];
-    const int64_t vk = vrow + (int64_t)(k >> 2) * vstep;
-    ld4<T>(lva + vk, av);
-    if (FUSED) ld4<T>(lvb + vk, bv);
+  for (int k = 0; k < LEN; ++k) o[k] = __builtin_amdgcn_readfirstlane((int)off[k]);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      T x[VEC];
-      ld<T, VEC>(xl + (int)off[k + q] * ps, x);
+  for (int k0 = 0; k0 < LEN; k0 += G) {
+    const int g = LEN - k0 < G ? LEN - k0 : G;   // a multiple of 4
+    T av[G], bv[G];
+    T x[G][VEC];
+#pragma unroll
+    for (int q = 0; q < G / 4; ++q) {
+      if (q * 4 >= g) break;
+      const int64_t vk = vrow + (int64_t)((k0 >> 2) + q) * vstep;
+      T t4[4];
+      ld4<T>(lva + vk, t4);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) av[4 * q + e] = t4[e];
+      if (FUSED) {
+        ld4<T>(lvb + vk, t4);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) bv[4 * q + e] = t4[e];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < G; ++e) {
+      if (e >= g) break;
+      ld<T, VEC>(xl + o[k0 + e] * ps, x[e]);
+    }
+#pragma unroll
+    for (int e = 0; e < G; ++e) {
+      if (e >= g) break;
 #pragma unroll
       for (int v = 0; v < VEC; ++v) {
-        acc_a[v] = fma(av[q], x[v], acc_a[v]);
-        if (FUSED) acc_b[v] = fma(bv[q], x[v], acc_b[v]);
+        acc_a[v] = fma(av[e], x[e][v], acc_a[v]);
+        if (FUSED) acc_b[v] = fma(bv[e], x[e][v], acc_b[v]);
       }
     }
   }
