@@ -345,6 +345,8 @@ def main():
                     help="only run the first N instances of the seeded bias sweep (self-consistent loop, "
                          "30 outer iterations each) and print their timings")
     ap.add_argument("--sweep-workers", type=int, default=4)
+    ap.add_argument("--sweep-processes", type=int, default=0,
+                    help="run the sweep instances in this many worker processes instead of threads")
     ap.add_argument("--workload", choices=["eigen", "dd"], default="eigen",
                     help="dd: only the Poisson-drift-diffusion workload (reference pn-diode example)")
     ap.add_argument("--dd-nodes", type=int, nargs=2, default=DD_NODES)
@@ -359,22 +361,31 @@ def main():
     from qdsim_amd.mesh import SimpleMesh
     from qdsim_amd.schrodinger import SchrodingerProblem
 
+    if args.sweep_instances > 0:
+        from qdsim_amd.batch import bias_sweep, bias_sweep_instances
+        inst = bias_sweep_instances(256, seed=4)[:args.sweep_instances]
+        if args.sweep_processes <= 0:
+            torch.cuda.set_device(0)
+        # in process mode this process never touches the GPU: the workers own it
+        t0 = time.perf_counter()
+        def progress(done, total):
+            if done % 16 == 0 or done == total:
+                print(f"sweep: {done}/{total} instances, {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+                      flush=True)
+        out = bias_sweep(inst, workers=args.sweep_workers, processes=args.sweep_processes, progress=progress)
+        if args.sweep_processes <= 0:
+            torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        print(json.dumps({"sweep": {"instances": len(out), "workers": args.sweep_workers,
+                                    "processes": args.sweep_processes, "seconds": t,
+                                    "instances_per_s": len(out) / t,
+                                    "per_instance": [{k: o[k] for k in ("index", "cells", "order", "dofs",
+                                                                         "solve_time")} for o in out]}}))
+        return
     torch.cuda.set_device(0)
     if args.workload == "dd":
         dd = _dd_workload(args.dd_nodes[0], args.dd_nodes[1], args.steps, args.warmup, cpu=not args.no_cpu)
         print(json.dumps({"drift_diffusion": dd}))
-        return
-    if args.sweep_instances > 0:
-        from qdsim_amd.batch import bias_sweep, bias_sweep_instances
-        inst = bias_sweep_instances(256, seed=4)[:args.sweep_instances]
-        t0 = time.perf_counter()
-        out = bias_sweep(inst, workers=args.sweep_workers)
-        torch.cuda.synchronize()
-        t = time.perf_counter() - t0
-        print(json.dumps({"sweep": {"instances": len(out), "workers": args.sweep_workers, "seconds": t,
-                                    "instances_per_s": len(out) / t,
-                                    "per_instance": [{k: o[k] for k in ("index", "cells", "order", "dofs",
-                                                                         "solve_time")} for o in out]}}))
         return
     m_star, potential = _funcs()
     t0 = time.perf_counter()

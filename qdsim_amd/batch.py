@@ -4,7 +4,9 @@ differentiable eigenvalues.
 
 The instances of a sweep never communicate, so a batch runs them on a small pool
 of host threads, each with its own HIP stream (default priority) and its own
-scratch workspaces.  Every instance synchronises with the host once per block
+scratch workspaces, or in worker processes (``run_processes``), each with its
+own interpreter and HIP context, so that the instances' host loops do not share
+one interpreter lock.  Every instance synchronises with the host once per block
 iteration (Rayleigh-Ritz on the host, convergence test); while one waits, the
 others' kernels keep the GPU busy, and small meshes, whose kernels are too short
 to fill the chip, overlap with each other.  An instance gives the same result
@@ -27,11 +29,12 @@ from .units import EV_TO_J
 M_E = 9.1093837015e-31
 
 
-def run_batch(fn, items, workers=4, setup=None):
+def run_batch(fn, items, workers=4, setup=None, progress=None):
     """[fn(state, item) for item in items] on ``workers`` host threads, each with
     its own stream; ``state = setup()`` is built once per thread (None without
     setup).  Items are taken in order from a shared queue; the results keep the
-    order of ``items``.  The first exception stops the batch and is re-raised."""
+    order of ``items``.  The first exception stops the batch and is re-raised.
+    progress(done, total), if given, is called as each item finishes."""
     items = list(items)
     results = [None] * len(items)
     if not items:
@@ -39,6 +42,7 @@ def run_batch(fn, items, workers=4, setup=None):
     dev = device()
     lock = threading.Lock()
     nxt = [0]
+    done = [0]
     errors = []
 
     def worker():
@@ -53,6 +57,10 @@ def run_batch(fn, items, workers=4, setup=None):
                         i = nxt[0]
                         nxt[0] += 1
                     results[i] = fn(state, items[i])
+                    if progress is not None:
+                        with lock:
+                            done[0] += 1
+                            progress(done[0], len(items))
         except BaseException as e:   # noqa: BLE001 -- re-raised in the caller's thread
             with lock:
                 errors.append(e)
@@ -126,8 +134,55 @@ def _sc_instance(spec, num_states, iterations, mixing, keep_fields, poisson_sign
     return out
 
 
+def _process_init():
+    # a worker process of a sweep: its own interpreter and, at its first kernel,
+    # its own HIP context and queues on the current GPU; host LAPACK, OpenMP and
+    # torch's CPU ops on one thread each (the workers already run side by side)
+    threadpool_limits(1)
+    torch.set_num_threads(1)
+
+
+def _sc_instance_task(args):
+    spec, num_states, iterations, mixing, keep_fields, poisson_sign, retry = args
+    out = _sc_instance(spec, num_states, iterations, mixing, keep_fields, poisson_sign, retry)
+    sync()
+    return out
+
+
+def run_processes(fn, items, processes, progress=None):
+    """[fn(item) for item in items] in ``processes`` worker processes (spawned: fresh
+    interpreters, each with its own HIP context on the same GPU), one item at a
+    time, results in the order of ``items``.  Unlike threads, the workers' host
+    work (Python per block iteration) does not share one interpreter lock, and the
+    GPU runs their kernels concurrently from separate queues.  fn must be a
+    module-level function; its exceptions are re-raised here.  The workers are
+    started as new interpreters (fork + exec, as subprocess does), never as forks
+    of a process that may hold a HIP context.  progress(done, total), if given,
+    is called in this process as each item finishes."""
+    import concurrent.futures
+    import multiprocessing
+    items = list(items)
+    if not items:
+        return []
+    ctx = multiprocessing.get_context("spawn")
+    n = max(1, min(int(processes), len(items)))
+    results = [None] * len(items)
+    with concurrent.futures.ProcessPoolExecutor(max_workers=n, mp_context=ctx, initializer=_process_init) as ex:
+        futures = {ex.submit(fn, item): i for i, item in enumerate(items)}
+        try:
+            for done, f in enumerate(concurrent.futures.as_completed(futures), 1):
+                results[futures[f]] = f.result()
+                if progress is not None:
+                    progress(done, len(items))
+        except BaseException:
+            for f in futures:
+                f.cancel()
+            raise
+    return results
+
+
 def bias_sweep(instances, num_quantum_states=8, iterations=30, mixing=0.3, workers=4, keep_fields=False,
-               poisson_sign="physical", retry=True):
+               poisson_sign="physical", retry=True, processes=0, progress=None):
     """Self-consistent loop of every instance (``bias_sweep_instances``): exactly
     ``iterations`` outer iterations with linear mixing, CG Poisson and the lowest
     ``num_quantum_states`` states.  Returns one summary dict per instance (fields
@@ -135,10 +190,26 @@ def bias_sweep(instances, num_quantum_states=8, iterations=30, mixing=0.3, worke
     down or did not converge at first (``breakdown_solves``), were redone with the
     safe shift (``retried_solves``; with retry=False a breakdown raises instead and
     a solve that does not converge is kept) and ended unconverged.  poisson_sign: "physical" or "reference" (the
-    reference's K phi = -M rho)."""
-    return run_batch(lambda _, spec: _sc_instance(spec, num_quantum_states, iterations, mixing, keep_fields,
-                                                  poisson_sign, retry),
-                     instances, workers=workers)
+    reference's K phi = -M rho).  processes > 0 runs the instances in that many
+    worker processes (``run_processes``) instead of ``workers`` threads;
+    progress(done, total) is called as each instance ends.  Instances start largest mesh first, so that
+    no large one is left to run alone at the end; the results keep the order of
+    ``instances``."""
+    instances = list(instances)
+    order = sorted(range(len(instances)),
+                   key=lambda i: -(instances[i]["cells"] * instances[i]["order"]) ** 2)
+    if processes and processes > 0:
+        out = run_processes(_sc_instance_task, [(instances[i], num_quantum_states, iterations, mixing,
+                                                 keep_fields, poisson_sign, retry) for i in order],
+                            processes, progress)
+    else:
+        out = run_batch(lambda _, spec: _sc_instance(spec, num_quantum_states, iterations, mixing, keep_fields,
+                                                     poisson_sign, retry),
+                        [instances[i] for i in order], workers=workers, progress=progress)
+    results = [None] * len(instances)
+    for j, i in enumerate(order):
+        results[i] = out[j]
+    return results
 
 
 # ---------------------------------------------------------------- gate gradients

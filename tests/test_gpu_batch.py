@@ -24,6 +24,22 @@ def test_bias_sweep_concurrent_equals_sequential():
     assert not np.allclose(seq[0]["potential"][:10], seq[1]["potential"][:10])
 
 
+def test_bias_sweep_processes_equal_threads():
+    # worker processes (own interpreter and HIP context each) give the thread
+    # workers' results: same kernels, same seeded starts (each process tunes its
+    # own launch shapes, so compare to FP64 rounding, not bits)
+    from qdsim_amd.batch import bias_sweep, bias_sweep_instances
+    inst = bias_sweep_instances(n=3, seed=6, cells=(16, 32), orders=(1, 2))
+    print("bias sweep instances (processes)", [(s["cells"], s["order"]) for s in inst], flush=True)
+    thr = bias_sweep(inst, num_quantum_states=4, iterations=2, workers=2, keep_fields=True)
+    prc = bias_sweep(inst, num_quantum_states=4, iterations=2, keep_fields=True, processes=2)
+    assert [o["index"] for o in prc] == [s["index"] for s in inst]
+    for a, b in zip(thr, prc):
+        assert a["iterations"] == b["iterations"] == 2
+        assert np.allclose(a["eigenvalues_eV"], b["eigenvalues_eV"], rtol=1e-12, atol=0)
+        assert np.allclose(a["potential"], b["potential"], rtol=0, atol=1e-12)
+
+
 def test_gate_gradients_concurrent_equal_sequential_and_fd():
     from qdsim_amd.batch import gate_gradients, gate_parameter_sets
     theta, sigma, targets = gate_parameter_sets(n=3, seed=5, n_gates=2, L=40.0, k=3)
