@@ -31,6 +31,10 @@ Parity status per function:
   sc_ref ......................... pinned by composition: eig_ref and poisson_ref (each pinned
                                    above) joined by the density formulas of
                                    self_consistent_solver.pyx:257-286
+  sc_adjoint_ref ................. pinned: the map at the ramp equals sc_ref.one_iteration; the
+                                   dense Jacobians against central differences of the map; the
+                                   reverse-mode gradients (v0, V_p, V_n, N_A, N_D) against central
+                                   differences of the Newton-converged fixed point
   interp_ref ..................... pinned: exact reproduction of degree-p polynomials (values and
                                    gradients), brute-force nearest node outside the mesh
   dd_ref ......................... pinned: Poisson step = nodally exact parabola for uniform charge

@@ -379,3 +379,81 @@ def test_electrostatic_gradient_matches_central_differences():
                                                                    sign)[0]) / (2 * h)
         assert np.allclose(g, fd, rtol=2e-5, atol=1e-9 * np.abs(fd).max()), (sign, g, fd)
         assert np.all(np.abs(g) > 1e-6)
+
+
+# ---- self-consistent fixed point and its gradients (sc_adjoint_ref) -------------
+def _sc_case():
+    from oracle import sc_adjoint_ref
+    P = sc_adjoint_ref.SCProblem(13, 9, 60.0, 40.0, 12.9, 0.067, 4)
+
+    def v_nm(x, y):
+        return -0.15 * np.exp(-((x - 30.0) ** 2 + (y - 20.0) ** 2) / (2 * 8.0 ** 2))
+    return P, v_nm
+
+
+def test_sc_adjoint_ref_map_is_one_iteration_of_sc_ref():
+    # F at the initial ramp is sc_ref's Poisson potential (same map, general phi)
+    from oracle import sc_ref
+    P, v_nm = _sc_case()
+    V_p, V_n, NA, ND = 0.0, 0.2, 2e22, 3e22
+    ramp = V_p + (V_n - V_p) * P.x / P.Lx
+    st = P.evaluate(ramp, v_nm(P.xq, P.yq), V_p, V_n, NA, ND)
+    one = sc_ref.one_iteration(P.nx, P.ny, P.Lx, P.Ly, 12.9, lambda x, y: 0.067 + 0 * x, v_nm, V_p, V_n, 4, NA, ND)
+    assert np.abs(st["E"] - one["vals"]).max() <= 1e-12
+    assert np.abs(st["F"] - one["phi_poisson"]).max() <= 1e-12 * np.abs(one["phi_poisson"]).max()
+
+
+def test_sc_adjoint_ref_jacobian_matches_central_differences():
+    P, v_nm = _sc_case()
+    v0 = v_nm(P.xq, P.yq)
+    V_p, V_n, NA, ND = -0.05, 0.2, 2e22, 3e22
+    phi, st, _ = P.fixed_point(v0, V_p, V_n, NA, ND)
+    jac = P.jacobians(st)
+    rng = np.random.default_rng(3)
+    d = np.zeros(P.n)
+    d[P.pfree] = rng.standard_normal(P.pfree.size)
+    h = 1e-6
+    fd = (P.evaluate(phi + h * d, v0, V_p, V_n, NA, ND)["F"] - P.evaluate(phi - h * d, v0, V_p, V_n, NA, ND)["F"]) / (2 * h)
+    an = jac["J"] @ d
+    assert np.abs(fd - an).max() <= 1e-6 * np.abs(an).max()
+    dv = rng.standard_normal(v0.shape)
+    fd = (P.evaluate(phi, v0 + h * dv, V_p, V_n, NA, ND)["F"] - P.evaluate(phi, v0 - h * dv, V_p, V_n, NA, ND)["F"]) / (2 * h)
+    an = jac["Fv"] @ dv.ravel()
+    assert np.abs(fd - an).max() <= 1e-6 * np.abs(an).max()
+
+
+def test_sc_adjoint_ref_gradients_match_central_differences():
+    """Reverse-mode gradients of J = g_phi . phi* + g_E . E* against central
+    differences of the Newton-converged fixed point, for every parameter kind;
+    the fixed point's feedback changes them by far more than the tolerance."""
+    P, v_nm = _sc_case()
+    v0 = v_nm(P.xq, P.yq)
+    V_p, V_n = -0.05, 0.2
+    NA = np.full(P.n, 2e22)
+    ND = np.full(P.n, 3e22)
+    phi, st, _ = P.fixed_point(v0, V_p, V_n, NA, ND)
+    rng = np.random.default_rng(1)
+    g_phi, g_E = rng.standard_normal(P.n), rng.standard_normal(4)
+    gr = P.gradients(st, g_phi, g_E)
+
+    def J(v0_=v0, Vp=V_p, Vn=V_n, NA_=NA, ND_=ND):
+        ph, s, _ = P.fixed_point(v0_, Vp, Vn, NA_, ND_, phi0=phi)
+        return g_phi @ ph + g_E @ s["E"]
+    dv = np.cos(P.xq / 13.0) * np.sin(P.yq / 7.0)
+    dNA = rng.uniform(0.5, 1.5, P.n) * 2e22
+    dND = rng.uniform(0.5, 1.5, P.n) * 3e22
+    h = 1e-6
+    cases = [((J(v0_=v0 + h * dv) - J(v0_=v0 - h * dv)) / (2 * h), float((gr["v0_q"] * dv).sum())),
+             ((J(Vp=V_p + h) - J(Vp=V_p - h)) / (2 * h), gr["V_p"]),
+             ((J(Vn=V_n + h) - J(Vn=V_n - h)) / (2 * h), gr["V_n"]),
+             ((J(NA_=NA + 1e-5 * dNA) - J(NA_=NA - 1e-5 * dNA)) / 2e-5, float(gr["N_A"] @ dNA)),
+             ((J(ND_=ND + 1e-5 * dND) - J(ND_=ND - 1e-5 * dND)) / 2e-5, float(gr["N_D"] @ dND))]
+    for fd, an in cases:
+        assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+    # without the feedback (mu = g) the doping and potential gradients would be
+    # off by more than 10% (18%, 31%, 24% here)
+    jac = P.jacobians(st)
+    g = g_phi + jac["Ephi"].T @ g_E
+    for a, b in [(jac["FNA"].T @ g, gr["N_A"]), (jac["FND"].T @ g, gr["N_D"]),
+                 ((jac["Fv"].T @ g + jac["Ev"].T @ g_E).reshape(P.xq.shape), gr["v0_q"])]:
+        assert np.linalg.norm(a - b) > 0.1 * np.linalg.norm(b)
