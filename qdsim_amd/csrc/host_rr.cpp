@@ -15,12 +15,14 @@
 //      then uses an eigenvalue-based basis selection instead);
 //   3. T = L^-1 H L^-T by two triangular solves (no explicit inverse);
 //   4. Householder reduction of T to tridiagonal form (Golub & Van Loan, Alg. 8.3.1);
-//   5. the lowest b eigenvalues by bisection on Sturm counts, the b intervals
-//      advanced together so that their division chains overlap;
+//   5. the lowest b eigenvalues by bisection on Sturm counts to 1e-12 of the
+//      spectrum's span, the b intervals advanced together so that their division
+//      chains overlap;
 //   6. their eigenvectors by inverse iteration on the tridiagonal matrix (one LU
 //      with partial pivoting per eigenvalue, three solves), vectors of eigenvalues
-//      closer than 1e-3 ||T|| orthogonalised against each other, then the
-//      Householder back-transformation;
+//      closer than 1e-3 ||T|| orthogonalised against each other; the eigenvalues
+//      as the vectors' Rayleigh quotients (rounding-level accurate); then the
+//      Householder back-transformation, all b vectors per pass;
 //   7. C = diag(d) L^-T Z.
 #include "qds_common.h"
 
@@ -34,16 +36,31 @@ namespace {
 constexpr int kIllcond = 1;   // the Cholesky path does not apply
 constexpr double kEps = 2.220446049250313e-16;
 
+// x . y with four partial sums (fixed order: deterministic, and vectorised)
+inline double dot(int n, const double* __restrict__ x, const double* __restrict__ y) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int i = 0;
+  for (; i + 4 <= n; i += 4) {
+    s0 += x[i] * y[i];
+    s1 += x[i + 1] * y[i + 1];
+    s2 += x[i + 2] * y[i + 2];
+    s3 += x[i + 3] * y[i + 3];
+  }
+  for (; i < n; ++i) s0 += x[i] * y[i];
+  return (s0 + s1) + (s2 + s3);
+}
+
+// y -= a x over n contiguous entries (distinct arrays)
+inline void axpy_sub(int n, double a, const double* __restrict__ x, double* __restrict__ y) {
+  for (int i = 0; i < n; ++i) y[i] -= a * x[i];
+}
+
 // forward substitution L Y = R for a row-major block Y (n x c), in place
 void lower_solve_rows(int n, int c, const double* L, double* Y) {
   for (int i = 0; i < n; ++i) {
     double* yi = Y + (size_t)i * c;
     const double* li = L + (size_t)i * n;
-    for (int k = 0; k < i; ++k) {
-      const double lik = li[k];
-      const double* yk = Y + (size_t)k * c;
-      for (int j = 0; j < c; ++j) yi[j] -= lik * yk[j];
-    }
+    for (int k = 0; k < i; ++k) axpy_sub(c, li[k], Y + (size_t)k * c, yi);
     const double inv = 1.0 / li[i];
     for (int j = 0; j < c; ++j) yi[j] *= inv;
   }
@@ -56,34 +73,41 @@ void upper_t_solve_rows(int n, int c, const double* L, double* Y) {
     const double inv = 1.0 / L[(size_t)i * n + i];
     for (int j = 0; j < c; ++j) yi[j] *= inv;
     const double* li = L + (size_t)i * n;
-    for (int k = 0; k < i; ++k) {
-      const double lik = li[k];
-      double* yk = Y + (size_t)k * c;
-      for (int j = 0; j < c; ++j) yk[j] -= lik * yi[j];
-    }
+    for (int k = 0; k < i; ++k) axpy_sub(c, li[k], yi, Y + (size_t)k * c);
   }
 }
 
-// A^-1 x with A = L L^T for one vector
-void chol_solve(int n, const double* L, double* x) {
-  lower_solve_rows(n, 1, L, x);
-  upper_t_solve_rows(n, 1, L, x);
+// A^-1 x with A = L L^T for one vector; LT = L^T (row-major), so that both
+// substitutions run as contiguous axpy updates (vectorised, no reassociation)
+void chol_solve(int n, const double* L, const double* LT, double* x) {
+  for (int k = 0; k < n; ++k) {            // L y = x: column k of L is row k of LT
+    const double yk = x[k] / LT[(size_t)k * n + k];
+    x[k] = yk;
+    const double* c = LT + (size_t)k * n;
+    for (int i = k + 1; i < n; ++i) x[i] -= yk * c[i];
+  }
+  for (int i = n - 1; i >= 0; --i) {       // L^T z = y: column i of L^T is row i of L
+    const double zi = x[i] / L[(size_t)i * n + i];
+    x[i] = zi;
+    const double* r = L + (size_t)i * n;
+    for (int k = 0; k < i; ++k) x[k] -= zi * r[k];
+  }
 }
 
 // Hager's estimate of ||A^-1||_1 for A = L L^T (A^-T = A^-1), with Higham's
 // alternating-sign vector as a second lower bound
-double inv_norm1_estimate(int n, const double* L) {
+double inv_norm1_estimate(int n, const double* L, const double* LT) {
   std::vector<double> x(n, 1.0 / n), y(n), s(n);
   double est = 0.0;
   for (int iter = 0; iter < 5; ++iter) {
     y = x;
-    chol_solve(n, L, y.data());
+    chol_solve(n, L, LT, y.data());
     double nrm = 0.0;
     for (int i = 0; i < n; ++i) nrm += std::fabs(y[i]);
     if (iter > 0 && nrm <= est) break;
     est = nrm;
     for (int i = 0; i < n; ++i) s[i] = y[i] >= 0.0 ? 1.0 : -1.0;
-    chol_solve(n, L, s.data());
+    chol_solve(n, L, LT, s.data());
     int jmax = 0;
     double zmax = -1.0, ztx = 0.0;
     for (int i = 0; i < n; ++i) {
@@ -98,7 +122,7 @@ double inv_norm1_estimate(int n, const double* L) {
     x[jmax] = 1.0;
   }
   for (int i = 0; i < n; ++i) y[i] = (i % 2 ? -1.0 : 1.0) * (1.0 + (n > 1 ? (double)i / (n - 1) : 0.0));
-  chol_solve(n, L, y.data());
+  chol_solve(n, L, LT, y.data());
   double alt = 0.0;
   for (int i = 0; i < n; ++i) alt += std::fabs(y[i]);
   return std::max(est, 2.0 * alt / (3.0 * n));
@@ -132,12 +156,16 @@ void tridiagonalize(int n, double* T, double* a, double* e, std::vector<double>&
     for (int i = 1; i < len; ++i) v[i] = T[(size_t)(k + 1 + i) * n + k] / v0;
     e[k] = mu;
     // T22 <- H T22 H = T22 - v w^T - w v^T, p = beta T22 v, w = p - (beta/2)(p.v) v
+    // p = beta T22 v as a sum of rows (T22 is symmetric): contiguous axpys
+    for (int i = 0; i < len; ++i) p[i] = 0.0;
+    for (int j = 0; j < len; ++j) {
+      const double* tj = T + (size_t)(k + 1 + j) * n + (k + 1);
+      const double vj = v[j];
+      for (int i = 0; i < len; ++i) p[i] += tj[i] * vj;
+    }
     double pv = 0.0;
     for (int i = 0; i < len; ++i) {
-      const double* ti = T + (size_t)(k + 1 + i) * n + (k + 1);
-      double s = 0.0;
-      for (int j = 0; j < len; ++j) s += ti[j] * v[j];
-      p[i] = bk * s;
+      p[i] *= bk;
       pv += p[i] * v[i];
     }
     const double half = 0.5 * bk * pv;
@@ -155,7 +183,8 @@ void tridiagonalize(int n, double* T, double* a, double* e, std::vector<double>&
   e[n - 2] = T[(size_t)(n - 1) * n + (n - 2)];
 }
 
-// The lowest b eigenvalues of the tridiagonal (a, e) by bisection.  Interval j
+// The lowest b eigenvalues of the tridiagonal (a, e) by bisection, to 1e-12 of
+// the spectrum's span.  Interval j
 // holds eigenvalue j: (number of eigenvalues below L_j) <= j < (number below H_j);
 // the count at x is the number of negative pivots of the LDL^T factorisation of
 // T - x I.  All b counts run in one pass over the rows.
@@ -174,7 +203,9 @@ void lowest_eigenvalues(int n, int b, const double* a, const double* e, double* 
   hi += 2.0 * kEps * span + pivmin;
   std::vector<double> L(b, lo), H(b, hi), x(b), q(b);
   std::vector<int> cnt(b);
-  const double tol = 2.0 * kEps * span;
+  // to 1e-12 of the spectrum's span: the shifts of the inverse iteration, whose
+  // vectors then give the eigenvalues to rounding as Rayleigh quotients
+  const double tol = std::max(2.0 * kEps * span, 1e-12 * span);
   for (int it = 0; it < 80; ++it) {
     double width = 0.0;
     for (int j = 0; j < b; ++j) {
@@ -291,24 +322,29 @@ QDS_API int qds_host_rayleigh_ritz(int32_t m, int32_t b, const double* G, double
     anorm = std::max(anorm, s);
   }
   // 2. Cholesky factor (lower, in place) and the condition estimate
+  // right-looking: column j is final once the rank-1 updates of columns < j are
+  // in; the updates run over contiguous rows of the trailing lower triangle
   std::vector<double>& L = A;
+  std::vector<double> col(n);
   for (int j = 0; j < n; ++j) {
     double* lj = L.data() + (size_t)j * n;
-    double s = lj[j];
-    for (int k = 0; k < j; ++k) s -= lj[k] * lj[k];
+    const double s = lj[j];
     if (!(s > 0.0)) return kIllcond;
     const double ljj = std::sqrt(s);
     lj[j] = ljj;
     const double inv = 1.0 / ljj;
+    for (int i = j + 1; i < n; ++i) col[i] = (L[(size_t)i * n + j] *= inv);
     for (int i = j + 1; i < n; ++i) {
       double* li = L.data() + (size_t)i * n;
-      double t = li[j];
-      for (int k = 0; k < j; ++k) t -= li[k] * lj[k];
-      li[j] = t * inv;
+      const double lij = col[i];
+      for (int c = j + 1; c <= i; ++c) li[c] -= lij * col[c];
     }
     for (int k = j + 1; k < n; ++k) lj[k] = 0.0;
   }
-  const double rcond = 1.0 / (anorm * inv_norm1_estimate(n, L.data()));
+  std::vector<double> LT((size_t)n * n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) LT[(size_t)j * n + i] = L[(size_t)i * n + j];
+  const double rcond = 1.0 / (anorm * inv_norm1_estimate(n, L.data(), LT.data()));
   if (!(rcond >= drop_tol)) return kIllcond;
   // 3. T = L^-1 Hs L^-T: Y = L^-1 Hs, then T = L^-1 Y^T, symmetrised
   lower_solve_rows(n, n, L.data(), Hs.data());
@@ -329,12 +365,13 @@ QDS_API int qds_host_rayleigh_ritz(int32_t m, int32_t b, const double* G, double
   }
   if (!(tnorm < 1e300)) return kIllcond;
   std::vector<double> lam(b), Z((size_t)b * n);   // row j of Z: eigenvector j of T
+  std::vector<double> V, beta;                      // Householder reflectors of step 4
   if (n == 1) {
     lam[0] = T[0];
     Z[0] = 1.0;
   } else {
     // 4.-5. tridiagonal form, lowest b eigenvalues
-    std::vector<double> a(n), e(n, 0.0), V, beta;
+    std::vector<double> a(n), e(n, 0.0);
     tridiagonalize(n, T.data(), a.data(), e.data(), V, beta);
     lowest_eigenvalues(n, b, a.data(), e.data(), lam.data());
     // 6. inverse iteration; a cluster is a run of eigenvalues closer than 1e-3 ||T||
@@ -360,35 +397,47 @@ QDS_API int qds_host_rayleigh_ritz(int32_t m, int32_t b, const double* G, double
         lu.solve(n, z);
         for (int q = cstart; q < j; ++q) {
           const double* zq = Z.data() + (size_t)q * n;
-          double dot = 0.0;
-          for (int i = 0; i < n; ++i) dot += zq[i] * z[i];
-          for (int i = 0; i < n; ++i) z[i] -= dot * zq[i];
+          axpy_sub(n, dot(n, zq, z), zq, z);
         }
-        double nrm = 0.0;
-        for (int i = 0; i < n; ++i) nrm += z[i] * z[i];
-        nrm = std::sqrt(nrm);
+        const double nrm = std::sqrt(dot(n, z, z));
         if (!(nrm > 0.0) || !std::isfinite(nrm)) return kIllcond;
         const double inv = 1.0 / nrm;
         for (int i = 0; i < n; ++i) z[i] *= inv;
       }
+      // the eigenvalue as the Rayleigh quotient of the unit vector
+      double rq = a[n - 1] * z[n - 1] * z[n - 1];
+      for (int i = 0; i + 1 < n; ++i) rq += z[i] * (a[i] * z[i] + 2.0 * e[i] * z[i + 1]);
+      lam[j] = rq;
     }
-    // back-transformation Q z, the last reflector first
-    for (int k = n - 3; k >= 0; --k) {
-      if (beta[k] == 0.0) continue;
-      const double* vk = V.data() + (size_t)k * n;
-      for (int j = 0; j < b; ++j) {
-        double* z = Z.data() + (size_t)j * n;
-        double dot = 0.0;
-        for (int i = k + 1; i < n; ++i) dot += vk[i] * z[i];
-        dot *= beta[k];
-        for (int i = k + 1; i < n; ++i) z[i] -= dot * vk[i];
+    // ascending order (a Rayleigh quotient may pass its neighbour's inside a cluster)
+    for (int j = 1; j < b; ++j)
+      for (int q = j; q > 0 && lam[q] < lam[q - 1]; --q) {
+        std::swap(lam[q], lam[q - 1]);
+        std::swap_ranges(Z.begin() + (size_t)q * n, Z.begin() + (size_t)(q + 1) * n, Z.begin() + (size_t)(q - 1) * n);
       }
-    }
   }
-  // 7. C = diag(d) L^-T Z, scattered into the kept rows
+  // Y = Z^T (n x b, row-major), back-transformed by the reflectors, the last one
+  // first: per reflector one contiguous pass computing the b dot products and one
+  // applying the rank-1 update
   std::vector<double> Y((size_t)n * b);
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < b; ++j) Y[(size_t)i * b + j] = Z[(size_t)j * n + i];
+  if (n > 1) {
+    std::vector<double> w(b);
+    for (int k = n - 3; k >= 0; --k) {
+      if (beta[k] == 0.0) continue;
+      const double* vk = V.data() + (size_t)k * n;
+      std::fill(w.begin(), w.end(), 0.0);
+      for (int i = k + 1; i < n; ++i) {
+        const double vi = vk[i];
+        const double* yi = Y.data() + (size_t)i * b;
+        for (int j = 0; j < b; ++j) w[j] += vi * yi[j];
+      }
+      for (int j = 0; j < b; ++j) w[j] *= beta[k];
+      for (int i = k + 1; i < n; ++i) axpy_sub(b, vk[i], w.data(), Y.data() + (size_t)i * b);
+    }
+  }
+  // 7. C = diag(d) L^-T Z, scattered into the kept rows
   upper_t_solve_rows(n, b, L.data(), Y.data());
   std::memset(C, 0, sizeof(double) * (size_t)m * b);
   for (int i = 0; i < n; ++i)
