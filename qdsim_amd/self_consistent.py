@@ -18,6 +18,16 @@ with the FE basis instead of a floor-based nearest-node lookup, the nodal charge
 uses each node's own density, and the new potential is mixed linearly,
 phi <- phi + alpha (phi_Poisson - phi), alpha = 0.3 (the damping of
 backend/src/self_consistent.cpp:1270-1272).
+
+``anderson=m > 0`` replaces the linear mixing by Anderson mixing over the last m
+iterates (no reference counterpart): with f_i = F(phi_i) - phi_i,
+gamma = argmin || f_i - sum_j gamma_j (f_j+1 - f_j) || over the history and
+phi <- phi_i + alpha f_i - sum_j gamma_j ((phi_j+1 - phi_j) + alpha (f_j+1 - f_j)).
+Strong charge feedback (high doping, large domains) makes the linear map's
+Jacobian eigenvalues reach far below -1, where linear mixing with any fixed alpha
+oscillates or diverges; Anderson mixing (a secant method, GMRES-like on the
+linearised map) converges there.  The history is restarted when its
+least-squares system is singular.
 """
 import time
 import warnings
@@ -32,18 +42,50 @@ from .schrodinger import SchrodingerProblem
 from .units import EV_TO_J, K_B, Q_E
 
 
+def mix_step(phi, phi_new, hist, alpha, depth):
+    """Next potential from phi and its Poisson image phi_new (device or host
+    tensors): linear mixing phi + alpha (phi_new - phi) for depth 0, else Anderson
+    mixing over ``hist`` (the last depth + 1 iterates and residuals, updated in
+    place).  Contacts hold V_p, V_n in phi and phi_new alike, so they never move."""
+    f = phi_new - phi
+    if depth <= 0:
+        return phi + alpha * f
+    hist.append((phi, f))
+    if len(hist) > depth + 1:
+        hist.pop(0)
+    if len(hist) == 1:
+        return phi + alpha * f
+    dX = torch.stack([hist[j + 1][0] - hist[j][0] for j in range(len(hist) - 1)], 1)
+    dF = torch.stack([hist[j + 1][1] - hist[j][1] for j in range(len(hist) - 1)], 1)
+    # min || f - dF gamma || by the normal equations with a relative Tikhonov
+    # floor; a singular or non-finite system drops the history
+    A = (dF.T @ dF).cpu().numpy()
+    r = (dF.T @ f).cpu().numpy()
+    reg = 1e-12 * max(float(np.trace(A)), 1e-300)
+    try:
+        gamma = np.linalg.solve(A + reg * np.eye(A.shape[0]), r)
+    except np.linalg.LinAlgError:
+        gamma = None
+    if gamma is None or not np.all(np.isfinite(gamma)):
+        del hist[:-1]
+        return phi + alpha * f
+    g = torch.from_numpy(gamma).to(device=phi.device, dtype=phi.dtype)
+    return phi + alpha * f - (dX + alpha * dF) @ g
+
+
 def _mean(a):
     return float(a.mean().item()) if torch.is_tensor(a) else float(a)
 
 
 class SelfConsistentSolver:
     def __init__(self, mesh, epsilon_r_func, m_star_func, potential_func, temperature=300.0,
-                 mixing=0.3, poisson_sign="physical"):
+                 mixing=0.3, poisson_sign="physical", anderson=0):
         self.mesh = mesh
         self.temperature = float(temperature)
         self.max_iterations = 100
         self.tolerance = 1e-6
         self.mixing = float(mixing)
+        self.anderson = int(anderson)   # Anderson history depth (0: linear mixing)
         self.warm_start_dphi = 1e-2
         self.eig_tol = 1e-6           # relative residual of each eigensolve
         self.fermi_level = 0.0
@@ -124,6 +166,7 @@ class SelfConsistentSolver:
         n = torch.zeros_like(phi)
         p = torch.zeros_like(phi)
         info = None
+        hist = []
         while it < self.max_iterations and residual > self.tolerance:
             it += 1
             # Schroedinger with V_total = V - q phi  (eV: V - phi)
@@ -147,7 +190,7 @@ class SelfConsistentSolver:
             n, p = self._densities(lam, psi_si, phi, N_A, N_D)
             rho = Q_E * (p - n)
             phi_new = self.poisson.solve(rho, V_p, V_n, x0=phi)
-            upd = phi + self.mixing * (phi_new - phi)
+            upd = self._mix(phi, phi_new, hist)
             residual = float((upd - phi).abs().max().item())
             phi = upd
             self.convergence_history.append(residual)
@@ -162,6 +205,9 @@ class SelfConsistentSolver:
         self.eigenvalues = np.asarray(lam) * EV_TO_J
         self.eigenvectors = [c.copy() for c in (psi * 1e9).cpu().numpy().T] if psi is not None else []
         return self._create_solution_dict()
+
+    def _mix(self, phi, phi_new, hist):
+        return mix_step(phi, phi_new, hist, self.mixing, self.anderson)
 
     def _create_solution_dict(self):
         return {
