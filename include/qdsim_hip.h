@@ -215,6 +215,22 @@ int qds_lattice_spmm_tuned(int32_t dtype, const qds_lattice_stencil* st /* host 
                            void* Y_b, int64_t ldy_b, int64_t syb, const qds_lattice_tuning* tune /* host */,
                            void* stream);
 
+/* The fused product of qds_lattice_spmm_tuned (batch 1, both matrices) with the
+ * block iteration's residual step in its epilogue: R = (Y_a - Y_b diag(lam))
+ * diag(colscale) (colscale may be NULL), stored as r_dtype (F32, or the product's
+ * type), and norms (device, 2b FP64) = the column sums of (Y_a - Y_b diag(lam))^2,
+ * then of Y_b^2 -- what qds_residual_norms_to computes from the stored products
+ * (same R values; the sums in another fixed order), without reading Y_a, Y_b back.
+ * Replaces the residual / convergence step of scipy's lobpcg
+ * (eigen_solver.pyx:254-266) fused into the operator apply.  b <= 128; workspace:
+ * qds_lattice_spmm_residual_workspace bytes.                                      */
+int64_t qds_lattice_spmm_residual_workspace(const qds_lattice_stencil* st /* host */, int32_t b);
+int qds_lattice_spmm_residual(int32_t dtype, const qds_lattice_stencil* st /* host */, int32_t b,
+                              const void* vals_a, const void* vals_b, const void* X, int64_t ldx, void* Y_a,
+                              int64_t ldy_a, void* Y_b, int64_t ldy_b, const double* lam, const double* colscale,
+                              int32_t r_dtype, void* R, int64_t ldr, double* norms, void* workspace,
+                              int64_t ws_bytes, const qds_lattice_tuning* tune /* host */, void* stream);
+
 /* ---- dense block kernels for the block eigensolver (device) --------------------
  * Replace the dense Gram / projection steps inside scipy's lobpcg
  * (eigen_solver.pyx:254-266) and ARPACK's orthogonalisation.                      */

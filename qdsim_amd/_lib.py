@@ -57,6 +57,9 @@ _SIGNATURES = {
                                   _i64, _i64, _p], _c.c_int),
     "qds_lattice_spmm_tuned": ([_i32, _p, _i32, _i32, _p, _i64, _p, _i64, _p, _i64, _i64, _p, _i64, _i64, _p,
                                 _i64, _i64, _p, _p], _c.c_int),
+    "qds_lattice_spmm_residual_workspace": ([_p, _i32], _i64),
+    "qds_lattice_spmm_residual": ([_i32, _p, _i32, _p, _p, _p, _i64, _p, _i64, _p, _i64, _p, _p, _i32, _p, _i64,
+                                   _p, _p, _i64, _p, _p], _c.c_int),
     "qds_gram_blocks_batched": ([_i32, _i64, _p, _p, _p, _i32, _p, _i32, _i64, _p, _i64, _p], _c.c_int),
     "qds_block_update_blocks_batched": ([_i32, _i64, _p, _i32, _p, _p, _i64, _p, _p, _i64, _i32, _i32, _i64, _i64,
                                          _i64, _p], _c.c_int),

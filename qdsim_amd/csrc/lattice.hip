@@ -25,6 +25,13 @@
 // Entries are summed in ascending column order with fused multiply-adds, the order
 // of the CSR kernel; the padding and boundary entries add exact zeros, so both
 // kernels give bitwise equal products.
+//
+// Optional residual epilogue (qds_lattice_spmm_residual): with Y_a = H X and
+// Y_b = M X in registers, each row also stores R = (Y_a - lambda Y_b) diag(colscale)
+// and adds ||r_j||^2 and ||(M X)_j||^2 of its row to per-lane sums; the workgroup
+// reduces them in a fixed order into one partial row per lattice block, and two
+// small kernels sum the partials (fixed order: deterministic).  The block iteration
+// otherwise reads H X and M X back from HBM for exactly this.
 #include "qds_common.h"
 #include "../../include/qdsim_hip.h"
 
@@ -41,6 +48,16 @@ constexpr int kBatch = 12;               // 16-byte loads in flight per thread w
 constexpr int kVB = 4;                   // value chunks per thread in the common staging path
 constexpr size_t kLdsBudget = 52 * 1024; // LDS per workgroup: 3 workgroups per CU
 constexpr size_t kLdsMax = 64 * 1024;
+
+// residual epilogue arguments (R == nullptr: plain product)
+struct ResEpilogue {
+  const double* lam;       // b eigenvalue estimates
+  const double* colscale;  // b column scales or nullptr (1)
+  void* R;                 // residual block, element size r_es
+  int64_t ldr;
+  int32_t r_es;
+  double* part;            // per lattice block: b sums of r^2, then b sums of (M x)^2
+};
 
 struct LatticeArgs {
   int64_t nx, ny, cells_x;
@@ -173,7 +190,7 @@ template <typename T, int VEC, int P, bool FUSED, bool VLDS>
 __global__ void __launch_bounds__(kThreads)
 lattice_spmm_kernel(const LatticeArgs a, const T* __restrict__ va, const T* __restrict__ vb,
                     const T* __restrict__ X, int64_t ldx, int chunked, T* __restrict__ Ya,
-                    int64_t ldya, T* __restrict__ Yb, int64_t ldyb) {
+                    int64_t ldya, T* __restrict__ Yb, int64_t ldyb, const ResEpilogue res) {
   constexpr int CE = 16 / sizeof(T);  // elements per 16-byte chunk
   constexpr int NV = FUSED ? 2 : 1;
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -311,6 +328,16 @@ lattice_spmm_kernel(const LatticeArgs a, const T* __restrict__ va, const T* __re
   const bool lane_ok = g < a.rpw && li * VEC < a.b;
   const int c0 = lane_ok ? li * VEC : 0;
   const int gg = g < a.rpw ? g : 0;
+  const bool resid = FUSED && res.R != nullptr;   // uniform
+  T rlam[VEC], rcs[VEC];
+  double rsum[VEC], msum[VEC];
+#pragma unroll
+  for (int v = 0; v < VEC; ++v) {
+    rlam[v] = resid ? (T)res.lam[c0 + v] : T(0);
+    rcs[v] = resid && res.colscale ? (T)res.colscale[c0 + v] : T(1);
+    rsum[v] = 0.0;
+    msum[v] = 0.0;
+  }
   for (int s = 0; s < kMaxTasks; ++s) {
     const int task = a.tasks[wave][s];
     if (task < 0) break;
@@ -361,7 +388,84 @@ lattice_spmm_kernel(const LatticeArgs a, const T* __restrict__ va, const T* __re
       const int64_t row = (J - 1) * a.nx + (I - 1);
       st<T, VEC>(Ya + row * ldya + c0, acc_a);
       if (FUSED) st<T, VEC>(Yb + row * ldyb + c0, acc_b);
+      if (resid) {
+        // the formula of the residual-norms kernel: v = hx - lambda mx, R = v cs
+        T rv[VEC];
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) {
+          const T m = acc_b[v];
+          rv[v] = acc_a[v] - rlam[v] * m;
+          rsum[v] += (double)rv[v] * (double)rv[v];
+          msum[v] += (double)m * (double)m;
+        }
+        if (res.r_es == 4) {
+          float* rp = reinterpret_cast<float*>(res.R) + row * res.ldr + c0;
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) rp[v] = (float)(rv[v] * rcs[v]);
+        } else {
+          double* rp = reinterpret_cast<double*>(res.R) + row * res.ldr + c0;
+#pragma unroll
+          for (int v = 0; v < VEC; ++v) rp[v] = (double)(rv[v] * rcs[v]);
+        }
+      }
     }
+  }
+  if (resid) {
+    // every wave is done with the staged block: reuse LDS for the reduction
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(lds_raw);
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) {
+      red[threadIdx.x * 2 * VEC + v] = rsum[v];
+      red[threadIdx.x * 2 * VEC + VEC + v] = msum[v];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < a.b; t += kThreads) {
+      const int tli = t / VEC, tv = t - tli * VEC;
+      double s0 = 0.0, s1 = 0.0;
+      for (int w = 0; w < kWaves; ++w)
+        for (int q = 0; q < a.rpw; ++q) {
+          const int tid = w * 64 + q * a.lpr + tli;
+          s0 += red[tid * 2 * VEC + tv];
+          s1 += red[tid * 2 * VEC + VEC + tv];
+        }
+      res.part[blk * 2 * a.b + t] = s0;
+      res.part[blk * 2 * a.b + a.b + t] = s1;
+    }
+  }
+}
+
+// Sums of the per-block residual partials (rows of width w = 2b) in a fixed
+// order: grid kResGroups blocks, block i sums rows i, i + kResGroups, ... per
+// column group of threads; then one block sums the kResGroups rows in order.
+constexpr int kResGroups = 256;
+__global__ void __launch_bounds__(kThreads)
+res_reduce_rows(int64_t nrows, int w, const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double red[kThreads];
+  const int c = threadIdx.x % w;
+  const int g = threadIdx.x / w;
+  const int groups = kThreads / w;
+  double a = 0.0;
+  if (g < groups) {
+    // four independent sums so that the loads of a thread overlap
+    const int64_t step = (int64_t)groups * gridDim.x;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int64_t r = (int64_t)blockIdx.x + (int64_t)g * gridDim.x;
+    for (; r + 3 * step < nrows; r += 4 * step) {
+      a0 += part[r * w + c];
+      a1 += part[(r + step) * w + c];
+      a2 += part[(r + 2 * step) * w + c];
+      a3 += part[(r + 3 * step) * w + c];
+    }
+    for (; r < nrows; r += step) a0 += part[r * w + c];
+    a = (a0 + a1) + (a2 + a3);
+  }
+  red[threadIdx.x] = a;
+  __syncthreads();
+  if (threadIdx.x < w) {
+    double s = 0.0;
+    for (int q = 0; q < groups; ++q) s += red[q * w + threadIdx.x];
+    out[(int64_t)blockIdx.x * w + threadIdx.x] = s;
   }
 }
 
@@ -369,11 +473,12 @@ bool aligned(const void* p, int bytes) { return (reinterpret_cast<uintptr_t>(p) 
 
 template <typename T, int VEC, int P, bool FUSED, bool VLDS>
 int launch(const LatticeArgs& a, size_t lds, const void* va, const void* vb, const void* X,
-           int64_t ldx, int chunked, void* Ya, int64_t ldya, void* Yb, int64_t ldyb, hipStream_t st) {
+           int64_t ldx, int chunked, void* Ya, int64_t ldya, void* Yb, int64_t ldyb, const ResEpilogue& res,
+           hipStream_t st) {
   const int64_t grid = (a.n_blocks + 7) / 8 * 8;
   hipLaunchKernelGGL((lattice_spmm_kernel<T, VEC, P, FUSED, VLDS>), dim3((unsigned)grid, (unsigned)a.batch), dim3(kThreads), lds,
                      st, a, (const T*)va, (const T*)vb, (const T*)X, ldx, chunked, (T*)Ya, ldya,
-                     (T*)Yb, ldyb);
+                     (T*)Yb, ldyb, res);
   QDS_LAUNCH_CHECK();
   return QDS_OK;
 }
@@ -381,14 +486,14 @@ int launch(const LatticeArgs& a, size_t lds, const void* va, const void* vb, con
 template <typename T, int VEC>
 int dispatch(int p, bool fused, bool vlds, const LatticeArgs& a, size_t lds, const void* va, const void* vb,
              const void* X, int64_t ldx, int chunked, void* Ya, int64_t ldya, void* Yb, int64_t ldyb,
-             hipStream_t st) {
+             const ResEpilogue& res, hipStream_t st) {
 #define QDS_LAT_CASE(PP)                                                                                       \
   case PP:                                                                                                   \
     if (vlds)                                                                                                \
-      return fused ? launch<T, VEC, PP, true, true>(a, lds, va, vb, X, ldx, chunked, Ya, ldya, Yb, ldyb, st)   \
-                   : launch<T, VEC, PP, false, true>(a, lds, va, vb, X, ldx, chunked, Ya, ldya, Yb, ldyb, st); \
-    return fused ? launch<T, VEC, PP, true, false>(a, lds, va, vb, X, ldx, chunked, Ya, ldya, Yb, ldyb, st)    \
-                 : launch<T, VEC, PP, false, false>(a, lds, va, vb, X, ldx, chunked, Ya, ldya, Yb, ldyb, st);
+      return fused ? launch<T, VEC, PP, true, true>(a, lds, va, vb, X, ldx, chunked, Ya, ldya, Yb, ldyb, res, st)   \
+                   : launch<T, VEC, PP, false, true>(a, lds, va, vb, X, ldx, chunked, Ya, ldya, Yb, ldyb, res, st); \
+    return fused ? launch<T, VEC, PP, true, false>(a, lds, va, vb, X, ldx, chunked, Ya, ldya, Yb, ldyb, res, st)    \
+                 : launch<T, VEC, PP, false, false>(a, lds, va, vb, X, ldx, chunked, Ya, ldya, Yb, ldyb, res, st);
   switch (p) {
     QDS_LAT_CASE(1)
     QDS_LAT_CASE(2)
@@ -414,10 +519,13 @@ struct Tuning {
 };
 constexpr Tuning kDefaultTuning{true, kLdsBudget};
 
+// res (optional): the residual epilogue; norms gets the 2b sums (r^2 then (M x)^2)
+// and part1 holds kResGroups rows of 2b partial sums.
 int lattice_spmm_impl(int32_t dtype, const qds_lattice_stencil* s, int32_t b, const void* vals_a,
                       const void* vals_b, const void* X, int64_t ldx, void* Y_a, int64_t ldy_a,
                       void* Y_b, int64_t ldy_b, const BatchStrides& bs, void* stream,
-                      const Tuning& tune = kDefaultTuning) {
+                      const Tuning& tune = kDefaultTuning, const ResEpilogue* res = nullptr,
+                      double* part1 = nullptr, double* norms = nullptr) {
   QDS_CHECK_ARG(s && vals_a && X && Y_a, "qds_lattice_spmm: null pointer");
   QDS_CHECK_ARG(!vals_b || Y_b, "qds_lattice_spmm: vals_b without Y_b");
   QDS_CHECK_ARG(dtype == qds::F32 || dtype == qds::F64, "qds_lattice_spmm: dtype must be F32 or F64");
@@ -444,6 +552,8 @@ int lattice_spmm_impl(int32_t dtype, const qds_lattice_stencil* s, int32_t b, co
   QDS_CHECK_ARG(nval * tile_vbytes + 4 * xrow1 <= kLdsMax, "qds_lattice_spmm: stencil too large for LDS");
   const int bmax = (int)((kLdsMax - nval * tile_vbytes) / xrow1) / 4 * 4;
   if (b > bmax) {
+    QDS_CHECK_ARG(res == nullptr, "qds_lattice_spmm_residual: block width %d above the single-pass limit %d", b,
+                  bmax);
     for (int j = 0; j < b; j += bmax) {
       const int w = std::min(bmax, b - j);
       const int rc = lattice_spmm_impl(dtype, s, w, vals_a, vals_b, (const char*)X + (size_t)j * es, ldx,
@@ -544,15 +654,29 @@ int lattice_spmm_impl(int32_t dtype, const qds_lattice_stencil* s, int32_t b, co
     a.tasks[w][cnt[w]++] = (int8_t)t;
     load[w] += a.len[t / a.nch];
   }
-  const size_t lds = block_bytes(m);
-  hipStream_t st = qds_stream(stream);
-  if (dtype == qds::F32) {
-    if (vec == 4) return dispatch<float, 4>(p, fused, tune.vlds, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
-    if (vec == 2) return dispatch<float, 2>(p, fused, tune.vlds, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
-    return dispatch<float, 1>(p, fused, tune.vlds, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
+  const ResEpilogue none{nullptr, nullptr, nullptr, 0, 0, nullptr};
+  const ResEpilogue& rs = res ? *res : none;
+  size_t lds = block_bytes(m);
+  if (res) {
+    QDS_CHECK_ARG(fused && bs.batch == 1, "qds_lattice_spmm_residual: needs both matrices and batch 1");
+    lds = std::max(lds, (size_t)kThreads * 2 * vec * sizeof(double));
   }
-  if (vec == 2) return dispatch<double, 2>(p, fused, tune.vlds, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
-  return dispatch<double, 1>(p, fused, tune.vlds, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, st);
+  hipStream_t st = qds_stream(stream);
+  int rc;
+  if (dtype == qds::F32)
+    rc = vec == 4 ? dispatch<float, 4>(p, fused, tune.vlds, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, rs, st)
+       : vec == 2 ? dispatch<float, 2>(p, fused, tune.vlds, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, rs, st)
+                  : dispatch<float, 1>(p, fused, tune.vlds, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, rs, st);
+  else
+    rc = vec == 2 ? dispatch<double, 2>(p, fused, tune.vlds, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, rs, st)
+                  : dispatch<double, 1>(p, fused, tune.vlds, a, lds, vals_a, vals_b, X, ldx, chunked, Y_a, ldy_a, Y_b, ldy_b, rs, st);
+  if (rc != QDS_OK || !res) return rc;
+  const int w = 2 * b;
+  hipLaunchKernelGGL(res_reduce_rows, dim3(kResGroups), dim3(kThreads), 0, st, a.n_blocks, w, res->part, part1);
+  QDS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(res_reduce_rows, dim3(1), dim3(kThreads), 0, st, (int64_t)kResGroups, w, part1, norms);
+  QDS_LAUNCH_CHECK();
+  return QDS_OK;
 }
 }  // namespace
 
@@ -579,6 +703,42 @@ QDS_API int qds_lattice_spmm_tuned(int32_t dtype, const qds_lattice_stencil* s, 
   const BatchStrides bs{batch, sva, svb, sx, sya, syb};
   const Tuning t{tune->values_in_lds != 0, (size_t)tune->lds_budget};
   return lattice_spmm_impl(dtype, s, b, vals_a, vals_b, X, ldx, Y_a, ldy_a, Y_b, ldy_b, bs, stream, t);
+}
+
+// Workspace of qds_lattice_spmm_residual: one partial row of 2b sums per lattice
+// block (bounded by one block per value tile) and kResGroups rows for the
+// second reduction stage.
+QDS_API int64_t qds_lattice_spmm_residual_workspace(const qds_lattice_stencil* s, int32_t b) {
+  if (!s || b < 1) return 0;
+  return ((int64_t)s->cells_y * s->tiles_x + kResGroups) * 2 * b * (int64_t)sizeof(double);
+}
+
+// The fused product Y_a = H X, Y_b = M X of qds_lattice_spmm_tuned (batch 1) with
+// the residual step of the block iteration in its epilogue:
+// R = (Y_a - Y_b diag(lam)) diag(colscale) stored as r_dtype (F32 or the product's
+// type), norms[j] = ||(Y_a - Y_b diag(lam))_j||^2 and norms[b + j] = ||(Y_b)_j||^2
+// (FP64, device).  R's values equal qds_residual_norms_to's for the same products;
+// the norms are the same sums in another order.  colscale may be NULL.
+QDS_API int qds_lattice_spmm_residual(int32_t dtype, const qds_lattice_stencil* s, int32_t b,
+                                      const void* vals_a, const void* vals_b, const void* X, int64_t ldx,
+                                      void* Y_a, int64_t ldy_a, void* Y_b, int64_t ldy_b, const double* lam,
+                                      const double* colscale, int32_t r_dtype, void* R, int64_t ldr,
+                                      double* norms, void* workspace, int64_t ws_bytes,
+                                      const qds_lattice_tuning* tune, void* stream) {
+  QDS_CHECK_ARG(tune && vals_b && Y_b && lam && R && norms && workspace, "qds_lattice_spmm_residual: null pointer");
+  QDS_CHECK_ARG(tune->lds_budget >= 4096 && tune->lds_budget <= (int32_t)kLdsMax,
+                "qds_lattice_spmm_residual: lds_budget %d outside [4096, %d]", tune->lds_budget, (int)kLdsMax);
+  QDS_CHECK_ARG(r_dtype == qds::F32 || r_dtype == dtype, "qds_lattice_spmm_residual: R must be F32 or the product's type");
+  QDS_CHECK_ARG(ws_bytes >= qds_lattice_spmm_residual_workspace(s, b), "qds_lattice_spmm_residual: workspace too small");
+  QDS_CHECK_ARG(ldr >= b && b <= 128, "qds_lattice_spmm_residual: b %d must be at most 128 and ldr %lld at least b",
+                b, (long long)ldr);
+  double* part = static_cast<double*>(workspace);
+  double* part1 = part + (int64_t)s->cells_y * s->tiles_x * 2 * b;
+  const ResEpilogue res{lam, colscale, R, ldr, r_dtype == qds::F32 ? 4 : 8, part};
+  const BatchStrides one{1, 0, 0, 0, 0, 0};
+  const Tuning t{tune->values_in_lds != 0, (size_t)tune->lds_budget};
+  return lattice_spmm_impl(dtype, s, b, vals_a, vals_b, X, ldx, Y_a, ldy_a, Y_b, ldy_b, one, stream, t, &res, part1,
+                           norms);
 }
 
 QDS_API int qds_lattice_spmm_batched(int32_t dtype, const qds_lattice_stencil* s, int32_t b, int32_t batch,

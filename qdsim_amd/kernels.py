@@ -188,6 +188,28 @@ def lattice_spmm_auto(st, vals_a, X, Ya, vals_b=None, Yb=None):
     lattice_spmm_tuned(st, vals_a, X, Ya, vals_b, Yb, choice)
 
 
+def lattice_spmm_residual(st, vals_a, vals_b, X, Ya, Yb, lam, R, colscale=None, out=None, choice=None):
+    """Ya = H X, Yb = M X (lattice form, the block shape ``choice``) with the residual
+    step fused in: R = (Ya - Yb diag(lam)) diag(colscale) in R's dtype, and
+    (||Ya - Yb lam||_j^2, ||Yb_j||^2) returned as views of ``out`` (2b FP64 device
+    values, allocated when None) -- the results of ``residual_norms``."""
+    from . import autotune
+    assert vals_a.dtype == vals_b.dtype == X.dtype == Ya.dtype == Yb.dtype
+    n, b = X.shape
+    if out is None:
+        out = torch.empty(2 * b, dtype=torch.float64, device=X.device)
+    assert out.dtype == torch.float64 and out.is_contiguous() and out.numel() >= 2 * b
+    t = autotune.to_struct(choice or autotune.DEFAULT)
+    wsb = lib.qds_lattice_spmm_residual_workspace(ctypes.byref(st), b)
+    wp, wn = workspace("lattice_res").ptr(wsb)
+    cs = colscale.contiguous() if colscale is not None else None
+    check(lib.qds_lattice_spmm_residual(code(X), ctypes.byref(st), b, ptr(vals_a), ptr(vals_b), ptr(X), _ld(X),
+                                        ptr(Ya), _ld(Ya), ptr(Yb), _ld(Yb), ptr(lam.contiguous()), ptr(cs), code(R),
+                                        ptr(R), _ld(R), ptr(out), wp, wn, ctypes.byref(t), stream_ptr()),
+          "qds_lattice_spmm_residual")
+    return out[:b], out[b:2 * b]
+
+
 def spmm_cap(indptr, indices, n_rows, h, w, m, X, YH, YM):
     """YH = (H - iW) X, YM = M X for complex blocks, real value arrays."""
     b = _width(X)

@@ -49,6 +49,11 @@ class CSROperator:
     def apply(self, X, HX, MX):
         kernels.spmm(self.indptr, self.indices, self.n, self.nnz, self.H, X, HX, self.M, MX)
 
+    def apply_residual(self, X, HX, MX, lam, R, colscale, out):
+        """apply, then the residual step (residual_norms) from the stored products."""
+        self.apply(X, HX, MX)
+        return kernels.residual_norms(HX, MX, lam, R, colscale, out=out)
+
     def apply_h(self, X, HX):
         kernels.spmm(self.indptr, self.indices, self.n, self.nnz, self.H, X, HX)
 
@@ -75,6 +80,16 @@ class LatticeOperator:
 
     def apply(self, X, HX, MX):
         kernels.lattice_spmm_auto(self.stencil, self.H, X, HX, self.M, MX)
+
+    def apply_residual(self, X, HX, MX, lam, R, colscale, out):
+        """apply with the residual step of the block iteration in the product's
+        epilogue (qds_lattice_spmm_residual): R = (HX - MX diag(lam)) diag(colscale)
+        and the column norms (views of ``out``) of residual_norms, without reading
+        HX and MX back.  The block shape is the one the autotuner chose for apply."""
+        from . import autotune
+        ch = autotune.choice_for(self.stencil, X.dtype, X.shape[1], True, 1,
+                                 lambda c: kernels.lattice_spmm_tuned(self.stencil, self.H, X, HX, self.M, MX, c))
+        return kernels.lattice_spmm_residual(self.stencil, self.H, self.M, X, HX, MX, lam, R, colscale, out, ch)
 
     def apply_h(self, X, HX):
         kernels.lattice_spmm_auto(self.stencil, self.H, X, HX)
@@ -657,8 +672,10 @@ def _lobpcg_mixed(op64, k, b, precond32, X0, tol, maxiter, drop_tol, verbose, re
     res = None
     t_iter = time.perf_counter()
     (colscale,) = stage_cs.put([np.ones(b)])
+    have_res = False     # R and its norms already produced by the product of X
     for it in range(1, maxiter + 1):
-        kernels.residual_norms(HX[c], MX[c], lam, R, colscale, out=xfer[:2 * b])
+        if not have_res:
+            kernels.residual_norms(HX[c], MX[c], lam, R, colscale, out=xfer[:2 * b])
         precond32(R, W)
         op32.apply(W, HW, MW)
         S = [X[c], W] + ([P[c]] if have_p else [])
@@ -701,7 +718,9 @@ def _lobpcg_mixed(op64, k, b, precond32, X0, tol, maxiter, drop_tol, verbose, re
         # lose accuracy relative to P_new at every step (the residual then grows
         # below ~1e-4); one FP32 product costs what the two image updates did
         op32.apply(P[c], HP[c], MP[c])
-        op64.apply(X[c], HX[c], MX[c])
+        # the FP64 product of X with the next iteration's residual step fused in
+        op64.apply_residual(X[c], HX[c], MX[c], lam_next, R, colscale, xfer[:2 * b])
+        have_res = True
         lam, lam_h = lam_next, theta
         have_p = dropped == 0
         if not retuned and retune_at is not None and worst < retune_at:

@@ -144,3 +144,36 @@ def test_every_tuning_candidate_gives_the_same_product(p, ncx, ncy, dtype, fused
     kernels.lattice_spmm_auto(st, la, X[1].contiguous(), ya, lb if fused else None,
                               torch.empty_like(ya) if fused else None)
     assert torch.equal(ya, ref_a)
+
+
+@pytest.mark.parametrize("p,ncx,ncy", [(1, 37, 23), (2, 19, 13), (3, 24, 11), (3, 64, 40), (3, 1, 3)])
+@pytest.mark.parametrize("dtype,rtype", [(torch.float64, torch.float32), (torch.float64, torch.float64),
+                                         (torch.float32, torch.float32)])
+@pytest.mark.parametrize("b", [4, 24])
+@pytest.mark.parametrize("choice", [(1, 53248), (0, 32768), (1, 24576)])
+def test_lattice_spmm_residual_epilogue(p, ncx, ncy, dtype, rtype, b, choice):
+    """The fused residual step gives the plain product bit for bit, R bit for bit
+    as the residual-norms kernel computes it from the stored products, and the same
+    column norms (another summation order)."""
+    from qdsim_amd import kernels
+    lat, r_ip, r_ix, st, slot, va, vb = _setup(p, ncx, ncy)
+    n = lat.n_free
+    print(f"lattice residual p={p} cells={ncx}x{ncy} n={n} b={b} {dtype}->{rtype} choice={choice}", flush=True)
+    _, (la, lb) = _device_operator(r_ip, r_ix, slot, va, vb, dtype)
+    dev = la.device
+    rng = np.random.default_rng(11)
+    X = torch.from_numpy(rng.standard_normal((n, b))).to(dev, dtype)
+    lam = torch.from_numpy(rng.uniform(0.5, 2.0, b)).to(dev)
+    cs = torch.from_numpy((np.arange(b) % 3 != 1).astype(np.float64)).to(dev)
+    HX, MX = torch.empty_like(X), torch.empty_like(X)
+    kernels.lattice_spmm_tuned(st, la, X, HX, lb, MX, choice)
+    R_ref = torch.empty((n, b), dtype=rtype, device=dev)
+    for colscale in (cs, None):
+        rn_ref, mn_ref = (t.clone() for t in kernels.residual_norms(HX, MX, lam, R_ref, colscale))
+        HX2 = torch.full_like(X, float("nan"))
+        MX2 = torch.full_like(X, float("nan"))
+        R = torch.full((n, b), float("nan"), dtype=rtype, device=dev)
+        rn, mn = kernels.lattice_spmm_residual(st, la, lb, X, HX2, MX2, lam, R, colscale, choice=choice)
+        assert torch.equal(HX2, HX) and torch.equal(MX2, MX)
+        assert torch.equal(R, R_ref)
+        assert torch.allclose(rn, rn_ref, rtol=1e-12, atol=0) and torch.allclose(mn, mn_ref, rtol=1e-12, atol=0)
