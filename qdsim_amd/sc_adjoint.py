@@ -37,6 +37,8 @@ rule), the acceptor and donor profiles N_A, N_D (nodal) and the contact voltages
 V_p, V_n.  Requires E_k < E_{k+1} for the states that carry occupation (the
 density of k states is otherwise not a differentiable function of V).
 """
+import warnings
+
 import numpy as np
 import torch
 
@@ -175,6 +177,10 @@ class SCLinearization:
         rho = Q_E * (self.p - self.n_e)
         self.F = pp.solve(rho, self.V_p, self.V_n, x0=self.phi, rtol=1e-12)
         self.fixed_point_residual = float((self.F - self.phi).abs().max())
+        scale = max(abs(self.V_p), abs(self.V_n), float(self.phi.abs().max()), 1e-3)
+        if self.fixed_point_residual > 1e-6 * scale:
+            warnings.warn(f"phi is {self.fixed_point_residual:.2e} V from a fixed point of the self-consistent map: "
+                          f"the adjoint gives the gradient of a fixed point", RuntimeWarning, stacklevel=2)
         # Sternheimer set: states that carry occupation
         self.active = [s for s in range(self.k) if self.occ[s] > occ_floor * self.occ.max()]
         self.D = _pair_weights(self.lam, self.occ, self.docc)
@@ -263,6 +269,9 @@ class SCLinearization:
             rz = rz_new
             Pd.mul_(beta).add_(Z)
         self.sternheimer_iterations.append(its)
+        if bool(live.any()):
+            warnings.warn(f"Sternheimer block CG stopped after {its} iterations above its tolerance "
+                          f"{self.rtol_st:.1e}: the gradient is less accurate", RuntimeWarning, stacklevel=3)
         self._project_vec(Y, st)
         return Y
 
@@ -334,6 +343,9 @@ class SCLinearization:
             gE_V = self.hf @ gE                                                  # dJ/dV from the levels
             g = g - self._interp_T(gE_V)
         mu, its, rel = _gmres(lambda v: v - self.jt(v), g, rtol)
+        if rel > 10 * rtol:
+            warnings.warn(f"adjoint GMRES stopped at relative residual {rel:.2e} (rtol {rtol:.1e})",
+                          RuntimeWarning, stacklevel=2)
         lam, g_rho, g_n, g_V = self._pieces(mu)
         pp = self.pp
         g_dir = mu - pp.spmv(pp.K, lam)
