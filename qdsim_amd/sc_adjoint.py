@@ -206,7 +206,6 @@ class SCLinearization:
         MP = torch.empty_like(Pf)
         op.apply_m(Pf, MP)
         lam = self.lam
-        sigma0 = None
         pre = sp.preconditioner(ka, torch.float64)
         sigma0 = pre.sigma
         gap = lam[min(self.k - 1, max(self.active) + 1)] - lam[0] if self.k > 1 else 0.0
