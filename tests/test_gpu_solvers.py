@@ -175,6 +175,34 @@ def test_self_consistent_zero_mixing_keeps_initial_ramp():
 
 
 # ---- CAP / open system ------------------------------------------------------------
+def test_fixed_open_system_fem_open_matches_dense_oracle():
+    """The FEM open path of FixedOpenSystemSolver (use_open_boundaries=True, the
+    default formulation): the reference's CAP profile (fixed_open_system_solver.pyx:
+    124-142) as the absorbing term of the P1 pencil, complex eigenvalues in J and
+    lifetimes hbar / (2 |Im E|), against dense zggev of the oracle pencil with the
+    same profile."""
+    from qdsim_amd.solvers import FixedOpenSystemSolver, lifetimes_fs
+    nx, ny, Lx, Ly, k = 17, 13, 40e-9, 30e-9, 4
+
+    def well(x, y):
+        return -0.08 * EV if abs(x - 20e-9) < 8e-9 and abs(y - 15e-9) < 8e-9 else 0.0
+    s = FixedOpenSystemSolver(nx, ny, Lx, Ly, _mass_si, well, True)
+    s.configure_device_specific_solver("quantum_dot", {"cap_strength": 0.02 * EV, "cap_length_ratio": 0.2})
+    vals, vecs = s.solve(k)
+    assert len(vals) == k and np.iscomplexobj(vals) and len(vecs) == k
+    ref = cap_ref.cap_dense(nx, ny, Lx * 1e9, Ly * 1e9, 1, lambda x, y: 0.067 + 0 * x,
+                            np.vectorize(lambda x, y: well(x * 1e-9, y * 1e-9) / EV),
+                            lambda x, y: s.cap_absorption(x * 1e-9, y * 1e-9) / EV)
+    rv = ref["vals"][:k]
+    e_ev = vals / EV
+    err = np.abs(e_ev - rv) / np.abs(rv)
+    print("open FEM eigenvalues (eV)", e_ev, "max rel err", err.max(), flush=True)
+    assert err.max() < 1e-9
+    assert np.all(e_ev.imag < 0)
+    tau = lifetimes_fs(vals)
+    assert np.allclose(tau, 1.054571817e-34 / (2 * np.abs(rv.imag) * EV) * 1e15, rtol=1e-8)
+
+
 def _cap_nm(Lx, Ly, eta, width):
     def f(x, y):
         dx = np.maximum(0.0, np.maximum(width - x, x - (Lx - width)))
