@@ -17,11 +17,12 @@ def _mstar(x, y):
 
 
 @pytest.mark.parametrize("order", [1, 2, 3])
-def test_schrodinger_assembly_matches_oracle(order):
+@pytest.mark.parametrize("nx,ny", [(9, 7), (65, 49)])
+def test_schrodinger_assembly_matches_oracle(order, nx, ny):
     from qdsim_amd.mesh import SimpleMesh
     from qdsim_amd.schrodinger import SchrodingerProblem
     from qdsim_amd.units import EV_TO_J, M_E
-    nx, ny = 9, 7
+    print(f"assembly P{order} {nx - 1}x{ny - 1} cells", flush=True)
     m = SimpleMesh(nx, ny, 10e-9, 8e-9, order)
     prob = SchrodingerProblem(m, lambda x, y: _mstar(x * 1e9, y * 1e9) * M_E,
                               lambda x, y: _gauss_v(x * 1e9, y * 1e9) * EV_TO_J)
