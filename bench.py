@@ -58,8 +58,10 @@ def _funcs():
     return m_star, potential
 
 
-def _spmm_roofline(prob, b, iters=20):
-    """Time the fused FP32 H+M product (the eigensolver's operator apply) with HIP events."""
+def _spmm_roofline(prob, b, iters=50, warmup=20):
+    """Time the fused FP32 H+M product (the eigensolver's operator apply) with HIP
+    events: ``warmup`` untimed launches first (launches on an idle GPU run ~15%
+    slower until its clocks ramp up), then ``iters`` back to back."""
     import torch
     op = prob.operator(torch.float32)
     n, nnz = op.n, op.nnz
@@ -67,7 +69,7 @@ def _spmm_roofline(prob, b, iters=20):
     X = torch.randn((n, b), dtype=torch.float32, device="cuda")
     HX = torch.empty((n, b), dtype=torch.float32, device="cuda")
     MX = torch.empty_like(HX)
-    for _ in range(3):
+    for _ in range(warmup):
         op.apply(X, HX, MX)
     torch.cuda.synchronize()
     st = torch.cuda.current_stream()
@@ -401,7 +403,7 @@ def main():
         return
     if args.profile_kernels:
         prob.assemble()
-        roof = _spmm_roofline(prob, 24, iters=20)
+        roof = _spmm_roofline(prob, 24)
         src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")   # 1 GiB
         dst = torch.empty_like(src)
         for _ in range(5):

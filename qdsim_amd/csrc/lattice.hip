@@ -186,7 +186,9 @@ __device__ __forceinline__ int64_t xcd_block(int64_t g, int64_t G) { return (g %
 // the compute phase straight from HBM (false: every value is used once, by the
 // lanes of one row, so staging buys nothing but LDS footprint and a barrier's
 // worth of latency; without it more workgroups fit on a CU).
-template <typename T, int VEC, int P, bool FUSED, bool VLDS>
+// RES: the residual epilogue is compiled in (its registers would otherwise cost the
+// plain product occupancy and scheduling freedom).
+template <typename T, int VEC, int P, bool FUSED, bool VLDS, bool RES>
 __global__ void __launch_bounds__(kThreads)
 lattice_spmm_kernel(const LatticeArgs a, const T* __restrict__ va, const T* __restrict__ vb,
                     const T* __restrict__ X, int64_t ldx, int chunked, T* __restrict__ Ya,
@@ -328,7 +330,7 @@ lattice_spmm_kernel(const LatticeArgs a, const T* __restrict__ va, const T* __re
   const bool lane_ok = g < a.rpw && li * VEC < a.b;
   const int c0 = lane_ok ? li * VEC : 0;
   const int gg = g < a.rpw ? g : 0;
-  const bool resid = FUSED && res.R != nullptr;   // uniform
+  constexpr bool resid = FUSED && RES;
   T rlam[VEC], rcs[VEC];
   double rsum[VEC], msum[VEC];
 #pragma unroll
@@ -420,17 +422,28 @@ lattice_spmm_kernel(const LatticeArgs a, const T* __restrict__ va, const T* __re
       red[threadIdx.x * 2 * VEC + VEC + v] = msum[v];
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < a.b; t += kThreads) {
-      const int tli = t / VEC, tv = t - tli * VEC;
-      double s0 = 0.0, s1 = 0.0;
-      for (int w = 0; w < kWaves; ++w)
-        for (int q = 0; q < a.rpw; ++q) {
-          const int tid = w * 64 + q * a.lpr + tli;
-          s0 += red[tid * 2 * VEC + tv];
-          s1 += red[tid * 2 * VEC + VEC + tv];
-        }
-      res.part[blk * 2 * a.b + t] = s0;
-      res.part[blk * 2 * a.b + a.b + t] = s1;
+    // the kWaves * rpw row lanes of each column: every thread sums one share
+    // (fixed assignment), then one thread per column adds the shares in order
+    const int nrow = kWaves * a.rpw;
+    const int shares = min(kThreads / (2 * a.b), nrow);
+    double share = 0.0;
+    const int t2 = threadIdx.x % (2 * a.b), sh = threadIdx.x / (2 * a.b);
+    if (sh < shares) {
+      const int col = t2 < a.b ? t2 : t2 - a.b;
+      const int half = t2 < a.b ? 0 : VEC;
+      const int tli = col / VEC, tv = col - tli * VEC;
+      for (int rr = sh; rr < nrow; rr += shares) {
+        const int w = rr / a.rpw, q = rr - w * a.rpw;
+        share += red[(w * 64 + q * a.lpr + tli) * 2 * VEC + half + tv];
+      }
+    }
+    __syncthreads();
+    red[threadIdx.x] = share;
+    __syncthreads();
+    if (threadIdx.x < 2 * a.b) {
+      double s0 = 0.0;
+      for (int q = 0; q < shares; ++q) s0 += red[q * 2 * a.b + threadIdx.x];
+      res.part[blk * 2 * a.b + threadIdx.x] = s0;
     }
   }
 }
@@ -471,12 +484,12 @@ res_reduce_rows(int64_t nrows, int w, const double* __restrict__ part, double* _
 
 bool aligned(const void* p, int bytes) { return (reinterpret_cast<uintptr_t>(p) % bytes) == 0; }
 
-template <typename T, int VEC, int P, bool FUSED, bool VLDS>
+template <typename T, int VEC, int P, bool FUSED, bool VLDS, bool RES = false>
 int launch(const LatticeArgs& a, size_t lds, const void* va, const void* vb, const void* X,
            int64_t ldx, int chunked, void* Ya, int64_t ldya, void* Yb, int64_t ldyb, const ResEpilogue& res,
            hipStream_t st) {
   const int64_t grid = (a.n_blocks + 7) / 8 * 8;
-  hipLaunchKernelGGL((lattice_spmm_kernel<T, VEC, P, FUSED, VLDS>), dim3((unsigned)grid, (unsigned)a.batch), dim3(kThreads), lds,
+  hipLaunchKernelGGL((lattice_spmm_kernel<T, VEC, P, FUSED, VLDS, RES>), dim3((unsigned)grid, (unsigned)a.batch), dim3(kThreads), lds,
                      st, a, (const T*)va, (const T*)vb, (const T*)X, ldx, chunked, (T*)Ya, ldya,
                      (T*)Yb, ldyb, res);
   QDS_LAUNCH_CHECK();
@@ -489,6 +502,9 @@ int dispatch(int p, bool fused, bool vlds, const LatticeArgs& a, size_t lds, con
              const ResEpilogue& res, hipStream_t st) {
 #define QDS_LAT_CASE(PP)                                                                                       \
   case PP:                                                                                                   \
+    if (fused && res.R != nullptr)                                                                           \
+      return vlds ? launch<T, VEC, PP, true, true, true>(a, lds, va, vb, X, ldx, chunked, Ya, ldya, Yb, ldyb, res, st) \
+                  : launch<T, VEC, PP, true, false, true>(a, lds, va, vb, X, ldx, chunked, Ya, ldya, Yb, ldyb, res, st); \
     if (vlds)                                                                                                \
       return fused ? launch<T, VEC, PP, true, true>(a, lds, va, vb, X, ldx, chunked, Ya, ldya, Yb, ldyb, res, st)   \
                    : launch<T, VEC, PP, false, true>(a, lds, va, vb, X, ldx, chunked, Ya, ldya, Yb, ldyb, res, st); \
