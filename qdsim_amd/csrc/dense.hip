@@ -508,8 +508,12 @@ struct TypedBlocks {
   int32_t nb;
 };
 
+// Four waves per SIMD (128 registers, a few spilled outside the MFMA loop): the
+// kernel is bound by the loads in flight, one column block ahead per wave; at the
+// compiler's own choice (164 registers, three waves) it took 472 us on the
+// benchmark basis against 439 us.
 template <int OT, int MAXJ, typename TY2, bool TWO>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
 block_update_typed_kernel(int64_t n, int m, int k, TypedBlocks S, const double* __restrict__ C1,
                           const double* __restrict__ C2, double* __restrict__ Y1, int64_t ldy1,
                           TY2* __restrict__ Y2, int64_t ldy2, int rows64, int rows32, int c2_first) {
