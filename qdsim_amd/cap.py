@@ -28,10 +28,9 @@ complex64 storage for the bulk of the iterations, complex128 refinement.
 """
 import numpy as np
 import torch
-from threadpoolctl import threadpool_limits
 
 from . import kernels
-from .device import sync
+from .device import blas_single_thread, sync
 from .units import EV_TO_J
 
 
@@ -169,7 +168,7 @@ def cap_lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, seed=0, t
     """constraint: optional in-place projection of a complex (n, b) block applied to
     the initial block and to every preconditioned residual (deflation)."""
     # small host eigenproblems: single-threaded BLAS avoids thread wake-up costs
-    with threadpool_limits(1, "blas"):
+    with blas_single_thread():
         return _cap_lobpcg(op, k, b, precond, X0, tol, maxiter, seed, target, drop_tol, stall_iters, verbose,
                            constraint)
 

@@ -15,6 +15,20 @@ DTYPE_CODE = {torch.float32: _lib.F32, torch.float64: _lib.F64,
 
 
 _runtime_checked = False
+_blas_ctl = None
+
+
+def blas_single_thread():
+    """Context manager keeping host BLAS / LAPACK on one thread (the block
+    iteration's small dense solves lose to thread start-up otherwise).  The
+    library scan of threadpoolctl runs once per process (~1-6 ms); entering the
+    cached controller's limit costs microseconds, where a fresh
+    ``threadpool_limits`` per solve cost ~1 ms."""
+    global _blas_ctl
+    if _blas_ctl is None:
+        from threadpoolctl import ThreadpoolController
+        _blas_ctl = ThreadpoolController()
+    return _blas_ctl.limit(limits=1, user_api="blas")
 
 
 def device():

@@ -28,10 +28,9 @@ import numpy as np
 import scipy.linalg as sla
 import scipy.linalg.lapack as sla_lapack
 import torch
-from threadpoolctl import threadpool_limits
 
 from . import kernels
-from .device import device, sync
+from .device import blas_single_thread, device, sync
 
 
 class CSROperator:
@@ -252,7 +251,7 @@ def lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=None
         return _lobpcg_device(op, k, b, precond, X0, tol, maxiter, drop_tol, seed, verbose, stall_iters, refresh)
     # the host Rayleigh-Ritz works on matrices of at most 96 x 192: multithreaded
     # BLAS only adds thread wake-ups there (measured 2-70x slower triangular solves)
-    with threadpool_limits(1, "blas"):
+    with blas_single_thread():
         return _lobpcg(op, k, b, precond, X0, tol, maxiter, drop_tol, seed, verbose, stall_iters, refresh, smooth,
                        retune, retune_iter)
 
@@ -458,11 +457,12 @@ def _lobpcg(op, k, b, precond=None, X0=None, tol=1e-6, maxiter=500, drop_tol=Non
     xfer_h = torch.empty(xfer.numel(), dtype=torch.float64, pin_memory=dev.type == "cuda")
 
     def rr_x(c):
-        # Rayleigh-Ritz on the X block alone (SVQB): X, HX, MX <- their Ritz
-        # combinations in the other half of the double buffer
+        # Rayleigh-Ritz on the X block alone (the host solver, SVQB when the block
+        # is near dependent): X, HX, MX <- their Ritz combinations in the other
+        # half of the double buffer
         op.apply(X[c], HX[c], MX[c])
         G = kernels.gram_blocks([X[c]], [HX[c]], [MX[c]], sym=True)
-        theta, C, _ = _svqb_rr(G, b, b, drop_tol)
+        theta, C, _ = _rayleigh_ritz(G.cpu().numpy(), b, b, drop_tol)
         C_d, lam = stage_rr.put([C, theta])
         for src in (X, HX, MX):
             kernels.block_update_blocks([src[c]], C_d, src[1 - c])
@@ -622,7 +622,7 @@ def lobpcg_mixed(op64, k, b, precond32, X0=None, tol=1e-6, maxiter=400, drop_tol
     multigrid shift closer to the lowest Ritz value).  Stops when the relative
     residual ||H x - lambda M x|| / (s ||M x||), s = max(|lambda|, max_j<k |lambda_j|),
     of the first k columns is below tol, or after maxiter iterations."""
-    with threadpool_limits(1, "blas"):
+    with blas_single_thread():
         return _lobpcg_mixed(op64, k, b, precond32, X0, tol, maxiter, drop_tol, verbose, retune, retune_at,
                              op32, seed)
 
@@ -658,7 +658,7 @@ def _lobpcg_mixed(op64, k, b, precond32, X0, tol, maxiter, drop_tol, verbose, re
     stage_cs = _Staging(b, dev)
     xfer = torch.empty(2 * b + 18 * b * b, dtype=f64, device=dev)
     xfer_h = torch.empty(xfer.numel(), dtype=f64, pin_memory=dev.type == "cuda")
-    theta, C, _ = _svqb_rr(G, b, b, drop_tol)
+    theta, C, _ = _rayleigh_ritz(G.cpu().numpy(), b, b, drop_tol)
     C_d, lam = stage_rr.put([C, theta])
     for src in (X, HX, MX):
         kernels.block_update_blocks([src[c]], C_d, src[1 - c])
