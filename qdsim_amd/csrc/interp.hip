@@ -217,6 +217,53 @@ interp_points_kernel(Mesh M, int64_t n, const double* __restrict__ px, const dou
   }
 }
 
+// Values only (no gradients, no element ids), many fields: a block locates
+// kVPts points (one thread each) and keeps their element and shape values in
+// LDS; then all threads produce the (point, field) outputs with consecutive
+// threads on consecutive fields of a point, so the stores and the reads of the
+// field rows are contiguous.  Same arithmetic per output as interp_points_kernel
+// (the nodal sum in local-node order; the nearest node outside the mesh).
+constexpr int kVPts = 64;
+__global__ void __launch_bounds__(kBlock)
+interp_values_kernel(Mesh M, int64_t n, const double* __restrict__ px, const double* __restrict__ py,
+                     double* __restrict__ val) {
+  __shared__ int64_t s_e[kVPts];
+  __shared__ int32_t s_nd[kVPts];
+  __shared__ double s_N[kVPts][10];
+  const int nf = M.nf;
+  for (int64_t p0 = (int64_t)blockIdx.x * kVPts; p0 < n; p0 += (int64_t)gridDim.x * kVPts) {
+    const int np = (int)min((int64_t)kVPts, n - p0);
+    if ((int)threadIdx.x < np) {
+      const int64_t t = p0 + threadIdx.x;
+      double l[3];
+      const int64_t e = locate(M, px[t], py[t], l);
+      s_e[threadIdx.x] = e;
+      if (e >= 0) {
+        double N[10], dN[10][3];
+        shape(M.order, l, N, dN);
+        for (int a = 0; a < M.npe; ++a) s_N[threadIdx.x][a] = N[a];
+      } else {
+        s_nd[threadIdx.x] = nearest_node(M, px[t], py[t]);
+      }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < np * nf; q += kBlock) {
+      const int pt = q / nf, f = q - pt * nf;
+      const int64_t e = s_e[pt];
+      double v;
+      if (e >= 0) {
+        const int32_t* c = M.conn + e * M.npe;
+        v = 0.0;
+        for (int a = 0; a < M.npe; ++a) v += M.f[(int64_t)c[a] * nf + f] * s_N[pt][a];
+      } else {
+        v = M.f[(int64_t)s_nd[pt] * nf + f];
+      }
+      val[(p0 + pt) * nf + f] = v;
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 QDS_API int qds_interpolate_points(int32_t order, int64_t cx, int64_t cy, double dx, double dy,
@@ -231,6 +278,12 @@ QDS_API int qds_interpolate_points(int32_t order, int64_t cx, int64_t cy, double
   QDS_CHECK_ARG((grad_x == nullptr) == (grad_y == nullptr), "qds_interpolate_points: give both gradients or none");
   if (n_points == 0) return QDS_OK;
   Mesh M{order, order == 1 ? 3 : (order == 2 ? 6 : 10), cx, cy, dx, dy, conn, x, y, lattice_node, field, nfield};
+  if (grad_x == nullptr && elem == nullptr && nfield >= 4) {
+    hipLaunchKernelGGL(interp_values_kernel, dim3(qds_blocks((n_points + kVPts - 1) / kVPts, 1, 65536)),
+                       dim3(kBlock), 0, qds_stream(stream), M, n_points, px, py, val);
+    QDS_LAUNCH_CHECK();
+    return QDS_OK;
+  }
   hipLaunchKernelGGL(interp_points_kernel, dim3(qds_blocks(n_points, kBlock, 65536)), dim3(kBlock), 0,
                      qds_stream(stream), M, n_points, px, py, val, grad_x, grad_y, elem);
   QDS_LAUNCH_CHECK();

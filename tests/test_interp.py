@@ -85,3 +85,28 @@ def test_device_interpolation_matches_oracle(order):
     assert inside and abs(sum(lam) - 1.0) < 1e-12
     N = fi.evaluate_shape_functions(lam)
     assert abs(np.dot(N, field[el[e[0]]]) - v[0]) < 1e-12 * scale
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("nf", [4, 24, 37])
+def test_device_interpolation_many_fields_values_only(order, nf):
+    """The values-only path for many fields (one point located per thread, fields
+    written by consecutive threads) gives the per-point kernel's values bit for bit,
+    inside and outside the mesh, and the oracle's to 1e-12."""
+    from qdsim_amd.interpolator import FEInterpolator
+    rng = np.random.default_rng(20 + order + nf)
+    m, x, y, el = _mesh(order)
+    field = rng.standard_normal((len(x), nf))
+    pts = rng.uniform([-5, -5], [55, 35], (1000, 2))
+    px, py = pts[:, 0], pts[:, 1]
+    fi = FEInterpolator(m)
+    v = fi.interpolate_points(px, py, field)
+    v_ref, _, _ = fi.interpolate_points(px, py, field, gradient=True)
+    assert torch.equal(v, v_ref)
+    v = v.cpu().numpy()
+    scale = np.abs(field).max()
+    for i in range(0, len(px), 97):
+        for f in (0, nf - 1):
+            rv = interp_ref.interpolate(px[i], py[i], x, y, el, order, field[:, f])
+            assert abs(v[i, f] - rv) <= 1e-12 * scale
