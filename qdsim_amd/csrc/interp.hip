@@ -223,7 +223,7 @@ interp_points_kernel(Mesh M, int64_t n, const double* __restrict__ px, const dou
 // threads on consecutive fields of a point, so the stores and the reads of the
 // field rows are contiguous.  Same arithmetic per output as interp_points_kernel
 // (the nodal sum in local-node order; the nearest node outside the mesh).
-constexpr int kVPts = 64;
+constexpr int kVPts = kBlock;   // every thread locates one point
 __global__ void __launch_bounds__(kBlock)
 interp_values_kernel(Mesh M, int64_t n, const double* __restrict__ px, const double* __restrict__ py,
                      double* __restrict__ val) {
