@@ -13,7 +13,7 @@ round of each), and keeps the fastest for the process; the choices are also
 written to a JSON file (``QDSIM_AMD_TUNE_CACHE``, default
 ~/.cache/qdsim_amd/lattice_tuning.json) keyed by the device and the shape, and read
 back by later processes.  ``QDSIM_AMD_TUNE=0`` skips the timing and uses the
-default shape.
+default shape, as do products below MIN_TUNED_ELEMENTS (n b batch) elements.
 """
 import json
 import os
@@ -27,6 +27,10 @@ from . import _lib
 CANDIDATES = [(1, 52 * 1024), (0, 52 * 1024), (0, 40 * 1024), (0, 32 * 1024), (0, 24 * 1024),
               (0, 64 * 1024), (1, 40 * 1024)]
 DEFAULT = CANDIDATES[0]
+# n * b * batch below which the default shape is used untimed: a sweep of uneven
+# meshes (256 self-consistent instances, 75 shapes of up to 591k DOF x 16
+# columns, eight worker processes) spent ~9 of 78 s tuning them
+MIN_TUNED_ELEMENTS = 16 << 20
 
 _lock = threading.Lock()
 _choices = {}
@@ -60,27 +64,39 @@ def _key(stencil, dtype, b, fused, batch):
     return f"{_device_tag()}|p{stencil.order}|{stencil.nx}x{stencil.ny}|{str(dtype)[6:]}|b{b}|f{int(fused)}|n{batch}"
 
 
-def _load_file():
-    global _file_loaded
-    if _file_loaded:
-        return
-    _file_loaded = True
+def _read_file():
     try:
         with open(_cache_path()) as f:
             data = json.load(f)
-        for k, v in data.items():
-            _choices.setdefault(k, (int(v[0]), int(v[1])))
-    except (OSError, ValueError, TypeError, IndexError):
-        pass
+        return {k: (int(v[0]), int(v[1])) for k, v in data.items()}
+    except (OSError, ValueError, TypeError, IndexError, AttributeError):
+        return {}
+
+
+def _load_file(force=False):
+    """Merge the cache file's choices (force: read it again -- the worker processes
+    of a sweep tune shapes concurrently and share them through the file)."""
+    global _file_loaded
+    if _file_loaded and not force:
+        return
+    _file_loaded = True
+    for k, v in _read_file().items():
+        _choices.setdefault(k, v)
+
+
+_untuned = {}
 
 
 def _save_file():
+    """Write this process's tuned choices merged into the file's (atomic replace)."""
     path = _cache_path()
     try:
         os.makedirs(os.path.dirname(path), exist_ok=True)
+        data = _read_file()
+        data.update({k: v for k, v in _choices.items() if k not in _untuned})
         tmp = path + f".tmp{os.getpid()}"
         with open(tmp, "w") as f:
-            json.dump({k: list(v) for k, v in _choices.items()}, f, indent=1, sort_keys=True)
+            json.dump({k: list(v) for k, v in data.items()}, f, indent=1, sort_keys=True)
         os.replace(tmp, path)
     except OSError:
         pass
@@ -103,6 +119,19 @@ def choice_for(stencil, dtype, b, fused, batch, run):
             return _choices[key]
     if os.environ.get("QDSIM_AMD_TUNE", "1") == "0":
         return DEFAULT
+    if int(stencil.nx) * int(stencil.ny) * int(b) * int(batch) < MIN_TUNED_ELEMENTS:
+        # small products: the shapes differ by a few percent and ~100 tuning
+        # launches per shape cost more than they save (remembered in this process
+        # only, so that every later product skips this test)
+        with _lock:
+            _untuned[key] = DEFAULT
+            _choices.setdefault(key, DEFAULT)
+        return DEFAULT
+    with _lock:
+        # another worker process may have tuned this shape meanwhile
+        _load_file(force=True)
+        if key in _choices:
+            return _choices[key]
     st = torch.cuda.current_stream()
     valid = []
     for cand in CANDIDATES:
