@@ -233,9 +233,9 @@ gram_partial(int64_t n, int64_t rows_per_blk, GramShape g, int tpg, int g0, Gram
   }
 }
 
-// grid: one block per 32 outputs; the 8 waves-halves of a block split the partial
-// rows (thread group g sums rows g, g + 8, ...), then combine in a fixed order.
-constexpr int kRedOut = 32;
+// grid: one block per 16 outputs; 16 thread groups of a block split the partial
+// rows (group g sums rows g, g + 16, ...), then combine in a fixed order.
+constexpr int kRedOut = 16;
 constexpr int kRedGroups = kBlock / kRedOut;
 __global__ void __launch_bounds__(kBlock)
 gram_reduce(int nblk, GramShape g, const double* __restrict__ partial, double* __restrict__ C) {
