@@ -56,6 +56,7 @@ struct ResEpilogue {
   void* R;                 // residual block, element size r_es
   int64_t ldr;
   int32_t r_es;
+  int32_t r_vec;           // R and ldr allow one vector store per lane and row
   double* part;            // per lattice block: b sums of r^2, then b sums of (M x)^2
 };
 
@@ -110,6 +111,28 @@ __device__ __forceinline__ void st(T* p, const T (&v)[N]) {
 #pragma unroll
   for (int i = 0; i < N; ++i) tp[i] = v[i];
   *reinterpret_cast<V*>(p) = t;
+}
+
+// N values of type S stored as type D: vector stores of at most 16 bytes (VEC
+// aligned p) or one store per value
+template <typename D, typename S, int N>
+__device__ __forceinline__ void st_cvt(D* p, const S (&v)[N], bool vec) {
+  D t[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) t[i] = (D)v[i];
+  if (!vec) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) p[i] = t[i];
+  } else if constexpr (N * sizeof(D) <= 16) {
+    st<D, N>(p, t);
+  } else {
+    constexpr int H = N / 2;
+    D a[H], b[H];
+#pragma unroll
+    for (int i = 0; i < H; ++i) { a[i] = t[i]; b[i] = t[H + i]; }
+    st<D, H>(p, a);
+    st<D, H>(p + H, b);
+  }
 }
 
 // four consecutive values (one 16-byte LDS read for float, two for double)
@@ -400,15 +423,13 @@ lattice_spmm_kernel(const LatticeArgs a, const T* __restrict__ va, const T* __re
           rsum[v] += (double)rv[v] * (double)rv[v];
           msum[v] += (double)m * (double)m;
         }
-        if (res.r_es == 4) {
-          float* rp = reinterpret_cast<float*>(res.R) + row * res.ldr + c0;
+        T rs[VEC];
 #pragma unroll
-          for (int v = 0; v < VEC; ++v) rp[v] = (float)(rv[v] * rcs[v]);
-        } else {
-          double* rp = reinterpret_cast<double*>(res.R) + row * res.ldr + c0;
-#pragma unroll
-          for (int v = 0; v < VEC; ++v) rp[v] = (double)(rv[v] * rcs[v]);
-        }
+        for (int v = 0; v < VEC; ++v) rs[v] = rv[v] * rcs[v];
+        if (res.r_es == 4)
+          st_cvt<float, T, VEC>(reinterpret_cast<float*>(res.R) + row * res.ldr + c0, rs, res.r_vec != 0);
+        else
+          st_cvt<double, T, VEC>(reinterpret_cast<double*>(res.R) + row * res.ldr + c0, rs, res.r_vec != 0);
       }
     }
   }
@@ -670,7 +691,7 @@ int lattice_spmm_impl(int32_t dtype, const qds_lattice_stencil* s, int32_t b, co
     a.tasks[w][cnt[w]++] = (int8_t)t;
     load[w] += a.len[t / a.nch];
   }
-  const ResEpilogue none{nullptr, nullptr, nullptr, 0, 0, nullptr};
+  const ResEpilogue none{nullptr, nullptr, nullptr, 0, 0, 0, nullptr};
   const ResEpilogue& rs = res ? *res : none;
   size_t lds = block_bytes(m);
   if (res) {
@@ -750,7 +771,12 @@ QDS_API int qds_lattice_spmm_residual(int32_t dtype, const qds_lattice_stencil* 
                 b, (long long)ldr);
   double* part = static_cast<double*>(workspace);
   double* part1 = part + (int64_t)s->cells_y * s->tiles_x * 2 * b;
-  const ResEpilogue res{lam, colscale, R, ldr, r_dtype == qds::F32 ? 4 : 8, part};
+  const int32_t r_es = r_dtype == qds::F32 ? 4 : 8;
+  // a lane stores VEC consecutive values of a row (VEC = 4 for FP32 products, 2 for
+  // FP64): vector stores when R, ldr and the lane's first column keep them aligned
+  const int64_t rvec_elems = dtype == qds::F32 ? 4 : 2;
+  const int32_t r_vec = ((uintptr_t)R % (uintptr_t)(rvec_elems * r_es) == 0) && ldr % rvec_elems == 0;
+  const ResEpilogue res{lam, colscale, R, ldr, r_es, r_vec, part};
   const BatchStrides one{1, 0, 0, 0, 0, 0};
   const Tuning t{tune->values_in_lds != 0, (size_t)tune->lds_budget};
   return lattice_spmm_impl(dtype, s, b, vals_a, vals_b, X, ldx, Y_a, ldy_a, Y_b, ldy_b, one, stream, t, &res, part1,
