@@ -55,14 +55,32 @@ inline void axpy_sub(int n, double a, const double* __restrict__ x, double* __re
   for (int i = 0; i < n; ++i) y[i] -= a * x[i];
 }
 
-// forward substitution L Y = R for a row-major block Y (n x c), in place
+// columns [j0, j0 + W) of row i of the forward substitution: the W running values
+// stay in registers over the k loop (W independent FMA chains, one load per term)
+template <int W>
+inline void lower_solve_cols(int i, int c, int j0, const double* __restrict__ li, double inv,
+                             double* __restrict__ Y) {
+  double acc[W];
+  double* yi = Y + (size_t)i * c + j0;
+  for (int q = 0; q < W; ++q) acc[q] = yi[q];
+  for (int k = 0; k < i; ++k) {
+    const double l = li[k];
+    const double* yk = Y + (size_t)k * c + j0;
+    for (int q = 0; q < W; ++q) acc[q] -= l * yk[q];
+  }
+  for (int q = 0; q < W; ++q) yi[q] = acc[q] * inv;
+}
+
+// forward substitution L Y = R for a row-major block Y (n x c), in place; every
+// entry sees the same operations in the same order as a row-by-row axpy sweep
 void lower_solve_rows(int n, int c, const double* L, double* Y) {
   for (int i = 0; i < n; ++i) {
-    double* yi = Y + (size_t)i * c;
     const double* li = L + (size_t)i * n;
-    for (int k = 0; k < i; ++k) axpy_sub(c, li[k], Y + (size_t)k * c, yi);
     const double inv = 1.0 / li[i];
-    for (int j = 0; j < c; ++j) yi[j] *= inv;
+    int j0 = 0;
+    for (; j0 + 32 <= c; j0 += 32) lower_solve_cols<32>(i, c, j0, li, inv, Y);
+    for (; j0 + 8 <= c; j0 += 8) lower_solve_cols<8>(i, c, j0, li, inv, Y);
+    for (; j0 < c; ++j0) lower_solve_cols<1>(i, c, j0, li, inv, Y);
   }
 }
 
