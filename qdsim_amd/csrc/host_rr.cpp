@@ -302,6 +302,114 @@ struct TriLU {
   }
 };
 
+// Inverse iteration for eigenvalues with no neighbour within the cluster distance:
+// no orthogonalisation, so the vectors are independent and nb of them advance
+// together (rows outer, vectors inner: the division and substitution chains of
+// different vectors overlap).  Every vector sees the operations of TriLU::factor,
+// TriLU::solve, dot and the Rayleigh quotient of the one-vector path, in the same
+// order.  Z: nb vectors interleaved (entry i of vector t at Z[i * nb + t]), already
+// holding the start vectors; s: shifts; rq: Rayleigh quotients out.
+int inverse_iteration_batch(int n, int nb, const double* a, const double* e, const double* s, double tiny,
+                            double* Z, double* rq) {
+  std::vector<double> r((size_t)n * nb), u1((size_t)n * nb), u2((size_t)n * nb), l((size_t)n * nb);
+  std::vector<char> sw((size_t)n * nb);
+  std::vector<double> cd(nb), cu(nb);
+  for (int t = 0; t < nb; ++t) {
+    cd[t] = a[0] - s[t];
+    cu[t] = n > 1 ? e[0] : 0.0;
+  }
+  for (int i = 0; i + 1 < n; ++i) {
+    const double sub = e[i], nu = i + 2 < n ? e[i + 1] : 0.0;
+    for (int t = 0; t < nb; ++t) {
+      const size_t it = (size_t)i * nb + t;
+      const double nd = a[i + 1] - s[t];
+      if (std::fabs(cd[t]) >= std::fabs(sub)) {
+        sw[it] = 0;
+        const double piv = std::fabs(cd[t]) < tiny ? (cd[t] < 0.0 ? -tiny : tiny) : cd[t];
+        l[it] = sub / piv;
+        r[it] = 1.0 / piv;
+        u1[it] = cu[t];
+        u2[it] = 0.0;
+        cd[t] = nd - l[it] * cu[t];
+        cu[t] = nu;
+      } else {
+        sw[it] = 1;
+        l[it] = cd[t] / sub;
+        r[it] = 1.0 / sub;
+        u1[it] = nd;
+        u2[it] = nu;
+        cd[t] = cu[t] - l[it] * nd;
+        cu[t] = -l[it] * nu;
+      }
+    }
+  }
+  for (int t = 0; t < nb; ++t) {
+    const size_t it = (size_t)(n - 1) * nb + t;
+    const double piv = std::fabs(cd[t]) < tiny ? (cd[t] < 0.0 ? -tiny : tiny) : cd[t];
+    r[it] = 1.0 / piv;
+    u1[it] = u2[it] = 0.0;
+  }
+  std::vector<double> p0(nb), p1(nb), p2(nb), p3(nb);
+  const int n4 = n / 4 * 4;
+  for (int iter = 0; iter < 3; ++iter) {
+    for (int i = 0; i + 1 < n; ++i) {
+      double* zi = Z + (size_t)i * nb;
+      double* zn = zi + nb;
+      for (int t = 0; t < nb; ++t) {
+        const size_t it = (size_t)i * nb + t;
+        if (sw[it]) std::swap(zi[t], zn[t]);
+        zn[t] -= l[it] * zi[t];
+      }
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double* zi = Z + (size_t)i * nb;
+      for (int t = 0; t < nb; ++t) {
+        const size_t it = (size_t)i * nb + t;
+        double v = zi[t];
+        if (i + 1 < n) v -= u1[it] * zi[nb + t];
+        if (i + 2 < n) v -= u2[it] * zi[2 * nb + t];
+        zi[t] = v * r[it];
+      }
+    }
+    // norms with dot's four partial sums
+    std::fill(p0.begin(), p0.end(), 0.0);
+    std::fill(p1.begin(), p1.end(), 0.0);
+    std::fill(p2.begin(), p2.end(), 0.0);
+    std::fill(p3.begin(), p3.end(), 0.0);
+    for (int i = 0; i < n4; i += 4)
+      for (int t = 0; t < nb; ++t) {
+        const double z0 = Z[(size_t)i * nb + t], z1 = Z[(size_t)(i + 1) * nb + t];
+        const double z2 = Z[(size_t)(i + 2) * nb + t], z3 = Z[(size_t)(i + 3) * nb + t];
+        p0[t] += z0 * z0;
+        p1[t] += z1 * z1;
+        p2[t] += z2 * z2;
+        p3[t] += z3 * z3;
+      }
+    for (int i = n4; i < n; ++i)
+      for (int t = 0; t < nb; ++t) {
+        const double zi = Z[(size_t)i * nb + t];
+        p0[t] += zi * zi;
+      }
+    for (int t = 0; t < nb; ++t) {
+      const double nrm = std::sqrt((p0[t] + p1[t]) + (p2[t] + p3[t]));
+      if (!(nrm > 0.0) || !std::isfinite(nrm)) return kIllcond;
+      p0[t] = 1.0 / nrm;
+    }
+    for (int i = 0; i < n; ++i)
+      for (int t = 0; t < nb; ++t) Z[(size_t)i * nb + t] *= p0[t];
+  }
+  for (int t = 0; t < nb; ++t) {
+    const double zl = Z[(size_t)(n - 1) * nb + t];
+    rq[t] = a[n - 1] * zl * zl;
+  }
+  for (int i = 0; i + 1 < n; ++i)
+    for (int t = 0; t < nb; ++t) {
+      const double zi = Z[(size_t)i * nb + t], zn = Z[(size_t)(i + 1) * nb + t];
+      rq[t] += zi * (a[i] * zi + 2.0 * e[i] * zn);
+    }
+  return QDS_OK;
+}
+
 }  // namespace
 
 // Lowest b Ritz pairs of the pencil (G[:, :m], G[:, m:2m]) (m x 2m row-major FP64),
@@ -396,10 +504,49 @@ QDS_API int qds_host_rayleigh_ritz(int32_t m, int32_t b, const double* G, double
     TriLU lu(n);
     const double tiny = std::max(kEps * tnorm, 1e-300);
     const double close = 1e-3 * tnorm;
+    // eigenvalues farther than close (with a margin far above the bisection
+    // tolerance, so that the one-vector loop below would also find them alone)
+    // from both neighbours go through the batched path first
+    const double margin = 1e-9 * tnorm;
+    std::vector<char> alone(b, 0);
+    std::vector<int> J;
+    for (int j = 0; j < b; ++j) {
+      const bool lgap = j == 0 || lam[j] - lam[j - 1] > close + margin;
+      const bool rgap = j + 1 == b || lam[j + 1] - lam[j] > close + margin;
+      if (lgap && rgap) {
+        alone[j] = 1;
+        J.push_back(j);
+      }
+    }
+    if (!J.empty()) {
+      const int nb = (int)J.size();
+      std::vector<double> Zb((size_t)n * nb), sb(nb), rqb(nb);
+      for (int t = 0; t < nb; ++t) {
+        const int j = J[t];
+        sb[t] = lam[j];
+        uint32_t seed = 2654435761u * (uint32_t)(j + 1);
+        for (int i = 0; i < n; ++i) {
+          seed = seed * 1664525u + 1013904223u;
+          Zb[(size_t)i * nb + t] = 0.5 + (double)(seed >> 8) / 16777216.0;
+        }
+      }
+      if (inverse_iteration_batch(n, nb, a.data(), e.data(), sb.data(), tiny, Zb.data(), rqb.data()) != QDS_OK)
+        return kIllcond;
+      for (int t = 0; t < nb; ++t) {
+        double* z = Z.data() + (size_t)J[t] * n;
+        for (int i = 0; i < n; ++i) z[i] = Zb[(size_t)i * nb + t];
+        lam[J[t]] = rqb[t];
+      }
+    }
     int cstart = 0;
     double prev = 0.0;
     for (int j = 0; j < b; ++j) {
       double* z = Z.data() + (size_t)j * n;
+      if (alone[j]) {   // a cluster of its own, done above
+        cstart = j;
+        prev = 0.0;
+        continue;
+      }
       if (j > 0 && lam[j] - lam[j - 1] > close) cstart = j;
       double s = lam[j];
       // coincident eigenvalues of a cluster get shifts a few ulps of ||T|| apart
