@@ -513,6 +513,8 @@ def main():
         "assembly_ms": assemble_ms,
         "assembly_kernels": asm,
         "setup_s": setup_s,
+        "setup_note": "one-time host setup (mesh, sparsity pattern and scatter maps, lattice stencil) "
+                      "before the timed steps, not in ms_per_step; the first solve on a new mesh pays it",
         "solver_timings": infos[-1]["timings"],
         "spmm_tuning": {k: v for k, v in __import__("qdsim_amd.autotune", fromlist=["report"]).report().items()
                         if f"|{args.cells * 3 - 1}x{args.cells * 3 - 1}|" in k},
